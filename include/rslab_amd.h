@@ -1,0 +1,155 @@
+/*
+ * rslab_amd -- MI355X (gfx950) numeric factor and solve for rslab's
+ * supernodal left-looking LDL^T with Bunch-Kaufman pivoting.
+ *
+ * C ABI: plain pointers and sizes, no exceptions cross it, every entry point
+ * returns an int status (RSL_OK = 0). Input arrays are borrowed for the
+ * duration of the call; the library owns all device memory; handles are freed
+ * explicitly. A message for the last failure on the calling thread is
+ * available from rsl_last_error().
+ *
+ * The rslab (Rust) entry points each function stands in for are cited as
+ * path:line in the rslab source tree.
+ */
+#ifndef RSLAB_AMD_H
+#define RSLAB_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, mapped from RslabError (src/error.rs:3-43). */
+#define RSL_OK 0
+#define RSL_NUMERICALLY_RANK_DEFICIENT 1 /* RslabError::NumericallyRankDeficient */
+#define RSL_INVALID_INPUT 2              /* RslabError::InvalidInput */
+#define RSL_DIMENSION_MISMATCH 3         /* RslabError::DimensionMismatch */
+#define RSL_DEVICE_ERROR 4               /* InvalidInput("device: ...") in rslab terms */
+#define RSL_OUT_OF_MEMORY 5
+
+/* Fields. */
+#define RSL_F64 0 /* f64, real symmetric */
+#define RSL_C64 1 /* Complex<f64>, complex symmetric (PARDISO mtype 6), interleaved re/im */
+
+/* Orderings (OrderingMethod, src/symbolic/mod.rs:28-138). */
+#define RSL_ORDER_AMD 0
+#define RSL_ORDER_METIS_ND 1
+#define RSL_ORDER_NATURAL 2
+
+/* Pivot policy (ZeroPivotAction, src/numeric/multifrontal_ldlt.rs:165-180). */
+#define RSL_PIVOT_FAIL 0
+#define RSL_PIVOT_PERTURB 1 /* PerturbToEps { abs_floor } */
+#define RSL_PIVOT_FORCE 2   /* ForceAccept */
+
+/* Precision of the Schur / contribution updates; pivot search, 2x2 inverse,
+ * D and residuals always stay FP64. */
+#define RSL_PREC_FP64 0
+#define RSL_PREC_FP32_SCHUR 1
+#define RSL_PREC_BF16_SCHUR 2
+
+/* SolverSettings (src/numeric/multifrontal_ldlt.rs:302-394, defaults :583-614)
+ * restricted to the knobs the device path honours. */
+typedef struct {
+  int32_t ordering;       /* default RSL_ORDER_AMD (OrderingMethod::default) */
+  int32_t nemin;          /* default 16 */
+  int32_t relax;          /* 1 = Some(RelaxAmalgamation), default 1 */
+  int32_t relax_width;    /* default 256 */
+  int32_t relax_rows;     /* default 64 */
+  int32_t panel_nb;       /* default 64 (128 on supernodes >= 512 wide), max 128 */
+  int32_t pivot;          /* default RSL_PIVOT_FAIL */
+  double abs_floor;       /* PerturbToEps floor */
+  int32_t scaling;        /* 1 = OnePassInfNorm (default), 0 = Identity */
+  int32_t precision;      /* default RSL_PREC_FP64 */
+  uint64_t nd_seed;       /* rslab-metis seed, default 1 */
+  double wide_flops;      /* h*w*w above which a supernode's cdiv uses the
+                             multi-workgroup panel path (tuning knob) */
+} rsl_settings;
+
+typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */
+typedef struct rsl_solver rsl_solver;     /* LdltSolver<T>: device factor */
+
+/* Fill *s with the defaults listed above. */
+void rsl_settings_default(rsl_settings *s);
+
+/* LdltSymbolic::analyze_with (src/numeric/sparse_solver.rs:505-515):
+ * lower-triangle CSC, rows sorted (CscMatrix, src/sparse/csc.rs:15-20). */
+int rsl_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
+                const rsl_settings *s, rsl_symbolic **out);
+
+/* Symbolic sizes: out[0]=n out[1]=nsuper out[2]=sum of column counts
+ * out[3]=panel entries out[4]=levels out[5]=children entries
+ * out[6]=full permuted pattern nnz out[7]=stored L entries (padded) and
+ * work figures: flops[0]=factor flops sum (h-k)^2, flops[1]=geom flops
+ * sum h^2 w (MultifrontalSymbolic / estimate_memory,
+ * src/numeric/sparse_solver.rs:556-672). */
+int rsl_symbolic_info(const rsl_symbolic *sym, int64_t out[8], double flops[2]);
+
+/* SymbolicFactorization fields (src/symbolic/mod.rs:333-405): permutation
+ * (new->old), inverse, supernode first column / width / panel height,
+ * children (CSR), full pattern of P A P^T (CSC). Any pointer may be NULL. */
+int rsl_symbolic_export(const rsl_symbolic *sym, int32_t *perm, int32_t *perm_inv,
+                        int32_t *sn_first, int32_t *sn_ncol, int32_t *sn_nrow,
+                        int32_t *ch_ptr, int32_t *ch_idx, int64_t *pp_colp,
+                        int32_t *pp_rowi);
+
+/* LdltSymbolic::factor (src/numeric/sparse_solver.rs:677-711) -> factor_numeric
+ * (src/numeric/multifrontal_ldlt.rs:1851-1893), left-looking branch. values are
+ * the CSC values in the analysed pattern (interleaved for RSL_C64). s may be
+ * NULL (defaults). The symbolic handle must outlive the factor. */
+int rsl_factor(rsl_symbolic *sym, int32_t field, const double *values,
+               const rsl_settings *s, rsl_solver **out);
+
+/* Same-pattern numeric refactorization, reusing plan and device storage
+ * (LdltSymbolic reuse, src/numeric/sparse_solver.rs:457-488). */
+int rsl_refactor(rsl_solver *f, const double *values);
+
+/* LdltSolver::solve_many (src/numeric/sparse_solver.rs:302-327) with
+ * solve_ldlt_many (src/dense/ldlt_generic.rs:564-614): b, x host buffers,
+ * row-major n x nrhs. */
+int rsl_solve(rsl_solver *f, const double *b, int64_t nrhs, double *x);
+
+/* Device-resident variant of rsl_solve: d_b, d_x device pointers, stream may
+ * be NULL (library stream). Asynchronous on that stream. */
+int rsl_solve_device(rsl_solver *f, const void *d_b, int64_t nrhs, void *d_x,
+                     void *stream);
+
+/* inertia[3] = (positive, negative, zero) (LdltSolver::inertia,
+ * src/numeric/sparse_solver.rs:89-91); n_perturbed (:76-78); t_ms = last
+ * numeric factor wall time. Any pointer may be NULL. */
+int rsl_factor_info(const rsl_solver *f, int64_t inertia[3], int64_t *n_perturbed,
+                    double *t_ms);
+
+/* LdltFactors (src/dense/ldlt_generic.rs:31-66) in CSC: first call
+ * rsl_factor_nnz to size l_row_idx / l_values, then rsl_factor_download. */
+int rsl_factor_nnz(rsl_solver *f, int64_t *nnzl);
+int rsl_factor_download(rsl_solver *f, int64_t *l_col_ptr, int32_t *l_row_idx,
+                        double *l_values, double *d_diag, double *d_subdiag,
+                        uint8_t *two_by_two, int32_t *perm);
+
+/* MixedLdltSolver::solve_to (src/numeric/mixed.rs:267-285) around
+ * refine_ladder (:140-221): IR with FP64 residuals against the original
+ * matrix values a_values, escalating to FGMRES-IR. b, x row-major n x nrhs;
+ * info[4*c..4*c+4) = {ir_iters, gmres_iters, backward_error, certified}. */
+int rsl_refine(rsl_solver *f, const double *a_values, const double *b, int64_t nrhs,
+               double target, double *x, double *info);
+
+void rsl_free_factor(rsl_solver *f);
+void rsl_free_symbolic(rsl_symbolic *s);
+const char *rsl_last_error(void);
+
+/* Ordering only: AMD / nested dissection permutation (new->old) of a full
+ * symmetric pattern (rslab_ordering_core::CscPattern contract,
+ * crates/rslab-ordering-core/src/lib.rs:1-75). stats may be NULL;
+ * AMD stats[0..6] = ncmpa, ndense, lnz, ndiv, nms_lu, nms_ldl, dmax. */
+int rsl_order(int32_t method, int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
+              uint64_t seed, int32_t *perm, double *stats);
+
+/* Synthetic inputs: `count` draws of rslab's xorshift64* Rng::range(lo, hi)
+ * from Rng::new(seed) (src/matgen/mod.rs:63-96). */
+void rsl_rng_uniform(uint64_t seed, double lo, double hi, int64_t count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

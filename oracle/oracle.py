@@ -1,0 +1,266 @@
+"""Python handle on the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, by __graft_entry__.smoke() as the checker and by bench.py's
+cpu_baseline leg. The product never imports this module. The C restatement
+and the reference functions it follows are listed in ldlt_oracle.h.
+
+Also holds small numpy restatements used to pin the host symbolic:
+brute-force elimination column counts (the O(n^2) simulation of
+src/symbolic/column_counts.rs:27-77) and the reference's Inertia closed forms.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    return lib
+
+
+class OraSym(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("nsuper", C.c_int32),
+        ("perm", C.POINTER(C.c_int32)), ("perm_inv", C.POINTER(C.c_int32)),
+        ("sn_first", C.POINTER(C.c_int32)), ("sn_ncol", C.POINTER(C.c_int32)),
+        ("ch_ptr", C.POINTER(C.c_int32)), ("ch_idx", C.POINTER(C.c_int32)),
+        ("pp_colp", C.POINTER(C.c_int64)), ("pp_rowi", C.POINTER(C.c_int32)),
+    ]
+
+
+class OraFactors(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("cplx", C.c_int32), ("status", C.c_int32),
+        ("nnzl", C.c_int64), ("n_perturbed", C.c_int64), ("inertia", C.c_int64 * 3),
+        ("l_col_ptr", C.POINTER(C.c_int64)), ("l_row_idx", C.POINTER(C.c_int32)),
+        ("l_values", C.POINTER(C.c_double)), ("d_diag", C.POINTER(C.c_double)),
+        ("d_subdiag", C.POINTER(C.c_double)), ("two_by_two", C.POINTER(C.c_uint8)),
+        ("perm", C.POINTER(C.c_int32)),
+    ]
+
+
+_lib = _load()
+_lib.ora_factor.restype = C.POINTER(OraFactors)
+_lib.ora_factor.argtypes = [C.POINTER(OraSym), C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                            C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_double]
+_lib.ora_free.argtypes = [C.POINTER(OraFactors)]
+_lib.ora_solve_many.restype = C.c_int
+_lib.ora_solve_many.argtypes = [C.POINTER(OraFactors), C.POINTER(C.c_double), C.c_int32,
+                                C.POINTER(C.c_double)]
+_lib.ora_onepass_scale.argtypes = [C.c_int32, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+_lib.ora_symv.argtypes = [C.c_int32, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+_lib.ora_sym_norm_inf.restype = C.c_double
+_lib.ora_sym_norm_inf.argtypes = [C.c_int32, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_double)]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _vals(a):
+    if np.iscomplexobj(a.values):
+        return np.ascontiguousarray(a.values, dtype=np.complex128).view(np.float64), 1
+    return np.ascontiguousarray(a.values, dtype=np.float64), 0
+
+
+@dataclasses.dataclass
+class OracleFactors:
+    n: int
+    l_col_ptr: np.ndarray
+    l_row_idx: np.ndarray
+    l_values: np.ndarray
+    d_diag: np.ndarray
+    d_subdiag: np.ndarray
+    two_by_two: np.ndarray
+    perm: np.ndarray
+    n_perturbed: int
+    inertia: tuple
+    cplx: bool
+    _handle: object = None
+
+    def solve_many(self, b):
+        """solve_ldlt_many on the (already equilibrated) factor; b row-major n x k."""
+        b = np.asarray(b)
+        single = b.ndim == 1
+        B = np.ascontiguousarray(b.reshape(self.n, -1), dtype=np.complex128 if self.cplx else np.float64)
+        X = np.zeros_like(B)
+        bv = B.view(np.float64) if self.cplx else B
+        xv = X.view(np.float64) if self.cplx else X
+        rc = _lib.ora_solve_many(self._handle, _p(bv, C.c_double), B.shape[1], _p(xv, C.c_double))
+        if rc:
+            raise ZeroDivisionError("singular D block")
+        return X.reshape(self.n) if single else X
+
+    def __del__(self):
+        if self._handle is not None:
+            _lib.ora_free(self._handle)
+            self._handle = None
+
+
+class RankDeficient(Exception):
+    pass
+
+
+def onepass_scale(a) -> np.ndarray:
+    v, cp = _vals(a)
+    ri = np.ascontiguousarray(a.row_idx, dtype=np.int32)
+    cpt = np.ascontiguousarray(a.col_ptr, dtype=np.int64)
+    s = np.zeros(max(a.n, 1))
+    _lib.ora_onepass_scale(a.n, cp, _p(cpt, C.c_int64), _p(ri, C.c_int32), _p(v, C.c_double),
+                           _p(s, C.c_double))
+    return s[: a.n]
+
+
+def symv(a, x):
+    v, cp = _vals(a)
+    ri = np.ascontiguousarray(a.row_idx, dtype=np.int32)
+    cpt = np.ascontiguousarray(a.col_ptr, dtype=np.int64)
+    dt = np.complex128 if cp else np.float64
+    x = np.ascontiguousarray(x, dtype=dt)
+    y = np.zeros_like(x)
+    xv = x.view(np.float64) if cp else x
+    yv = y.view(np.float64) if cp else y
+    _lib.ora_symv(a.n, cp, _p(cpt, C.c_int64), _p(ri, C.c_int32), _p(v, C.c_double),
+                  _p(xv, C.c_double), _p(yv, C.c_double))
+    return y
+
+
+def sym_norm_inf(a) -> float:
+    v, cp = _vals(a)
+    ri = np.ascontiguousarray(a.row_idx, dtype=np.int32)
+    cpt = np.ascontiguousarray(a.col_ptr, dtype=np.int64)
+    return float(_lib.ora_sym_norm_inf(a.n, cp, _p(cpt, C.c_int64), _p(ri, C.c_int32),
+                                       _p(v, C.c_double)))
+
+
+def factor(sym, a_scaled, panel_nb=64, pivot=0, abs_floor=0.0, drop_tol=-1.0) -> OracleFactors:
+    """factor_left_looking on an equilibrated matrix given the exported symbolic."""
+    keep = dict(
+        perm=np.ascontiguousarray(sym.perm, dtype=np.int32),
+        perm_inv=np.ascontiguousarray(sym.perm_inv, dtype=np.int32),
+        sn_first=np.ascontiguousarray(sym.sn_first, dtype=np.int32),
+        sn_ncol=np.ascontiguousarray(sym.sn_ncol, dtype=np.int32),
+        ch_ptr=np.ascontiguousarray(sym.ch_ptr, dtype=np.int32),
+        ch_idx=np.ascontiguousarray(sym.ch_idx, dtype=np.int32),
+        pp_colp=np.ascontiguousarray(sym.pp_colp, dtype=np.int64),
+        pp_rowi=np.ascontiguousarray(sym.pp_rowi, dtype=np.int32),
+    )
+    s = OraSym(sym.n, len(sym.sn_first), _p(keep["perm"], C.c_int32), _p(keep["perm_inv"], C.c_int32),
+               _p(keep["sn_first"], C.c_int32), _p(keep["sn_ncol"], C.c_int32),
+               _p(keep["ch_ptr"], C.c_int32), _p(keep["ch_idx"], C.c_int32),
+               _p(keep["pp_colp"], C.c_int64), _p(keep["pp_rowi"], C.c_int32))
+    v, cp = _vals(a_scaled)
+    ri = np.ascontiguousarray(a_scaled.row_idx, dtype=np.int32)
+    cpt = np.ascontiguousarray(a_scaled.col_ptr, dtype=np.int64)
+    h = _lib.ora_factor(C.byref(s), cp, _p(cpt, C.c_int64), _p(ri, C.c_int32), _p(v, C.c_double),
+                        panel_nb, pivot, abs_floor, drop_tol)
+    f = h.contents
+    if f.status != 0:
+        _lib.ora_free(h)
+        raise RankDeficient("numerically rank deficient")
+    n, nnz = f.n, f.nnzl
+    nd = 2 if cp else 1
+
+    def arr(p, k, dt):
+        return np.ctypeslib.as_array(p, shape=(k,)).astype(dt, copy=True)
+
+    lv = arr(f.l_values, nnz * nd, np.float64)
+    dd = arr(f.d_diag, n * nd, np.float64)
+    ds = arr(f.d_subdiag, n * nd, np.float64)
+    if cp:
+        lv, dd, ds = lv.view(np.complex128), dd.view(np.complex128), ds.view(np.complex128)
+    return OracleFactors(n, arr(f.l_col_ptr, n + 1, np.int64), arr(f.l_row_idx, nnz, np.int32), lv, dd,
+                         ds, arr(f.two_by_two, n, np.uint8).astype(bool), arr(f.perm, n, np.int32),
+                         int(f.n_perturbed), tuple(int(x) for x in f.inertia), bool(cp), h)
+
+
+def scaled(a, s):
+    """A_hat = D A D on the same pattern (apply_symmetric_scaling, sparse_solver.rs:380-397)."""
+    cols = np.repeat(np.arange(a.n), np.diff(a.col_ptr))
+    vals = a.values * (s[a.row_idx] * s[cols])
+    return type(a)(a.n, a.col_ptr, a.row_idx, vals)
+
+
+def solve(a, b, sym, panel_nb=64, pivot=0, abs_floor=0.0, scaling=True):
+    """LdltSymbolic::factor + LdltSolver::solve_many on the CPU oracle."""
+    s = onepass_scale(a) if scaling else np.ones(a.n)
+    f = factor(sym, scaled(a, s), panel_nb=panel_nb, pivot=pivot, abs_floor=abs_floor)
+    B = np.asarray(b)
+    single = B.ndim == 1
+    B = B.reshape(a.n, -1)
+    X = f.solve_many(B * s[:, None]) * s[:, None]
+    return (X.reshape(a.n) if single else X), f
+
+
+def backward_error(a, x, b, anorm=None):
+    """Normwise backward error |r|inf / (|A|inf |x|inf + |b|inf) per column (mixed.rs:151-155)."""
+    anorm = sym_norm_inf(a) if anorm is None else anorm
+    X = np.asarray(x).reshape(a.n, -1)
+    B = np.asarray(b).reshape(a.n, -1)
+    out = []
+    for c in range(X.shape[1]):
+        r = B[:, c] - symv(a, X[:, c])
+        den = max(anorm * np.abs(X[:, c]).max() + np.abs(B[:, c]).max(), np.finfo(float).tiny)
+        out.append(float(np.abs(r).max() / den))
+    return np.array(out)
+
+
+# ------------------------------------------------------------ symbolic pins
+def brute_force_col_counts(n, colp, rowi):
+    """Exact column counts of L by symbolic elimination on a full pattern
+    (the O(n^2) simulation of column_counts.rs:27-77)."""
+    cols = [set() for _ in range(n)]
+    for j in range(n):
+        for k in range(colp[j], colp[j + 1]):
+            i = int(rowi[k])
+            if i > j:
+                cols[j].add(i)
+    counts = [1] * n
+    for j in range(n):
+        rows = sorted(cols[j])
+        counts[j] += len(rows)
+        if len(rows) > 1:
+            cols[rows[0]].update(rows[1:])
+    return np.array(counts)
+
+
+def brute_force_etree(n, colp, rowi):
+    """parent[j] = min{ i > j : L(i,j) != 0 } by symbolic elimination."""
+    cols = [set() for _ in range(n)]
+    for j in range(n):
+        for k in range(colp[j], colp[j + 1]):
+            i = int(rowi[k])
+            if i > j:
+                cols[j].add(i)
+    parent = [-1] * n
+    for j in range(n):
+        rows = sorted(cols[j])
+        if rows:
+            parent[j] = rows[0]
+            cols[rows[0]].update(rows[1:])
+    return np.array(parent)
+
+
+def laplacian2d_inertia(m: int, sigma: float):
+    """Closed-form inertia of L - sigma I on an m x m Dirichlet grid:
+    eigenvalues 4 - 2cos(p pi/(m+1)) - 2cos(q pi/(m+1))."""
+    t = 2.0 - 2.0 * np.cos(np.arange(1, m + 1) * np.pi / (m + 1))
+    lam = (t[:, None] + t[None, :]).ravel() - sigma
+    return int((lam > 0).sum()), int((lam < 0).sum()), int((lam == 0).sum()), float(np.abs(lam).min())

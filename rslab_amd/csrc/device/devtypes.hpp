@@ -1,0 +1,60 @@
+// Plain-old-data views of the device plan and factor, passed to kernels by
+// value. Shared by the HIP translation units and the host runtime.
+#pragma once
+#include <cstdint>
+
+namespace rsl {
+
+// Value-free per-pattern arrays (uploaded once per analysis).
+struct DevSym {
+  int32_t n, ns;
+  const int32_t* sn_first;   // first column (permuted numbering)
+  const int32_t* sn_ncol;    // eliminated columns w
+  const int32_t* sn_nrow;    // panel height h = |rs[s]|
+  const int32_t* e_off;      // first elimination position of s
+  const int64_t* panel_off;  // panel offset in the value pool (column-major h x w)
+  const int64_t* rs_off;     // offset of rs[s] in rs_idx
+  const int32_t* rs_idx;     // row structures (global permuted indices)
+  const int32_t* upd_ptr;    // update lists (ns+1)
+  const int32_t* upd_idx;
+  const int32_t* sym_perm;   // permuted -> user index
+  const int64_t* snap_off;   // scratch offset (elements) for per-panel multipliers
+  const int32_t* snap_ld;    // leading dimension of that scratch (min(nb, w))
+  const int64_t* amap;       // user nnz -> panel offset
+  const int32_t* a_row;      // user nnz -> user row
+  const int32_t* a_col;      // user nnz -> user column
+  int64_t nnz;
+};
+
+// Numeric factor storage for field T.
+template <class T>
+struct DevFac {
+  T* pool;            // all panels
+  T* d;               // D diagonal, elimination order
+  T* ds;              // D sub-diagonal (first column of a 2x2 block)
+  uint8_t* two;       // first column of a 2x2 block
+  int32_t* lperm;     // pivoted position -> local column, elimination order
+  int32_t* e_of_g;    // permuted index -> elimination position
+  int32_t* perm_e;    // elimination position -> user index
+  T* snap;            // per-panel time-of-step multipliers (scratch)
+  int32_t* deep;      // per-panel interchange partners (scratch, nb per supernode)
+  unsigned long long* counters;  // [0] pos [1] neg [2] zero [3] perturbed
+  int32_t* status;    // [0] 0 ok / 1 rank deficient, [1] failing supernode
+};
+
+// One cmod work item: a 64x64 tile of target panel s and its task range.
+struct CItem {
+  int32_t s, r0, c0, t_beg, t_end;
+};
+// One contribution of updater k to a tile: off-diagonal row ranges [a0,a1)
+// (tile rows) and [b0,b1) (tile columns), indices into k's off-diagonal rows.
+struct CTask {
+  int32_t k, a0, a1, b0, b1;
+};
+
+struct PivotPolicy {
+  int32_t have_floor;
+  double floor;
+};
+
+}  // namespace rsl

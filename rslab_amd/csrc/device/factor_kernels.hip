@@ -1,0 +1,780 @@
+// Numeric supernodal left-looking LDL^T with Bunch-Kaufman pivoting (gfx950).
+//
+// Device restatement of rslab's factor_left_looking
+// (src/numeric/multifrontal_ldlt.rs:3346-3576):
+//   assembly    ll_factor_node          :2429-2446  -> k_assemble
+//   cmod        ll_factor_node          :2452-2728  -> k_cmod (64x64 target tiles,
+//               updaters applied in ascending order, as update_list)
+//   cdiv        ll_cdiv_emit            :2759-3258  -> k_cdiv_small (one workgroup
+//               per supernode) or k_panel_* (one launch per nb-panel, for wide ones)
+//     getf2     :2846-3016  Bunch-Kaufman on the nb x nb diagonal block in LDS
+//     deep rows apply_bk_panel_trailing :1347-1416
+//     Schur     :3085-3167  lower-trapezoid  L21 D R^T  update
+//   emit        :3196-3256  inertia, elimination maps
+#include <hip/hip_runtime.h>
+
+#include "devtypes.hpp"
+#include "scalar.hpp"
+
+namespace rsl {
+
+constexpr int NT = 256;  // threads per workgroup (4 waves)
+constexpr int TM = 64;   // tile rows
+constexpr int TN = 64;   // tile columns
+constexpr int KC = 16;   // inner chunk
+constexpr int NBMAX = 128;
+
+// ---------------------------------------------------------------------------
+// Tile core: acc[ii][jj] += sum_c  L[ar + i, cb + c] * G[br + j, c]
+// with G = L[:, cb..cb+K) * D  (block diagonal 1x1 / 2x2, symmetric: no
+// conjugation). Thread t owns rows ty + 16*ii, columns tx + 16*jj.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T g_entry(const T* __restrict__ L, int64_t ld, int64_t row, int64_t col,
+                            int c, const T* __restrict__ d, const T* __restrict__ ds,
+                            const uint8_t* __restrict__ two) {
+  // c is the column relative to the D slice.
+  if (two[c]) {
+    T a = L[row + col * ld], b = L[row + (col + 1) * ld];
+    return d[c] * a + ds[c] * b;
+  }
+  if (c > 0 && two[c - 1]) {
+    T a = L[row + (col - 1) * ld], b = L[row + col * ld];
+    return ds[c - 1] * a + d[c] * b;
+  }
+  return L[row + col * ld] * d[c];
+}
+
+template <class T>
+__device__ void ldl_tile_gemm(const T* __restrict__ L, int64_t ld, int64_t cb, int K,
+                              int64_t ar, int ma, int64_t br, int mb,
+                              const T* __restrict__ d, const T* __restrict__ ds,
+                              const uint8_t* __restrict__ two, T (&acc)[4][4], T* As, T* Bs) {
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  for (int c0 = 0; c0 < K; c0 += KC) {
+    const int kc = min(KC, K - c0);
+    for (int e = tid; e < KC * TM; e += NT) {
+      int r = e % TM, c = e / TM;
+      T va = F<T>::zero(), vb = F<T>::zero();
+      if (c < kc) {
+        int64_t col = cb + c0 + c;
+        if (r < ma) va = L[(ar + r) + col * ld];
+        if (r < mb) vb = g_entry(L, ld, br + r, col, c0 + c, d, ds, two);
+      }
+      As[c * TM + r] = va;
+      Bs[c * TN + r] = vb;
+    }
+    __syncthreads();
+    for (int c = 0; c < kc; ++c) {
+      T a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = As[c * TM + ty + 16 * q];
+        b[q] = Bs[c * TN + tx + 16 * q];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + a[i] * b[j];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Assembly: zero-filled pool, then A's equilibrated entries scattered to their
+// panel slots (each lower entry owns exactly one slot).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_assemble(DevSym S, DevFac<T> Fc, const T* __restrict__ vals,
+                           const double* __restrict__ scale) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S.nnz) return;
+  double sc = scale[S.a_row[k]] * scale[S.a_col[k]];
+  Fc.pool[S.amap[k]] = F<T>::scale(vals[k], sc);
+}
+
+template <class T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// cmod: each workgroup owns one 64x64 tile of a target panel (distinct tiles,
+// so no two workgroups write the same entry) and subtracts every relevant
+// updater's  L_k[a] D_k L_k[b]^T  block in ascending updater order.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
+                                             const CItem* __restrict__ items,
+                                             const CTask* __restrict__ tasks) {
+  __shared__ T acc_s[TM * TN];
+  __shared__ T As[KC * TM];
+  __shared__ T Bs[KC * TN];
+  __shared__ int32_t trow[TM], rpos[TM], cpos[TN];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const CItem it = items[blockIdx.x];
+  const int s = it.s;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s];
+  T* P = Fc.pool + S.panel_off[s];
+  const int32_t* rs = S.rs_idx + S.rs_off[s];
+  const int mr = min<int64_t>(TM, h - it.r0), mc = min(TN, w - it.c0);
+  for (int e = tid; e < TM * TN; e += NT) {
+    int i = e % TM, j = e / TM;
+    acc_s[e] = (i < mr && j < mc) ? P[(it.r0 + i) + (int64_t)(it.c0 + j) * h] : F<T>::zero();
+  }
+  for (int i = tid; i < mr; i += NT) trow[i] = rs[it.r0 + i];
+  __syncthreads();
+  for (int t = it.t_beg; t < it.t_end; ++t) {
+    const CTask tk = tasks[t];
+    const int k = tk.k, nck = S.sn_ncol[k], eo = S.e_off[k];
+    const int64_t hk = S.sn_nrow[k];
+    const T* Pk = Fc.pool + S.panel_off[k];
+    const int32_t* okg = S.rs_idx + S.rs_off[k] + nck;
+    const int ma = tk.a1 - tk.a0, mb = tk.b1 - tk.b0;
+    for (int i = tid; i < ma; i += NT) {
+      int g = okg[tk.a0 + i], lo = 0, hi = mr - 1;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (trow[mid] < g) lo = mid + 1;
+        else hi = mid;
+      }
+      rpos[i] = lo;
+    }
+    for (int j = tid; j < mb; j += NT) cpos[j] = okg[tk.b0 + j] - first - it.c0;
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
+    ldl_tile_gemm(Pk, hk, 0, nck, nck + tk.a0, ma, nck + tk.b0, mb, Fc.d + eo, Fc.ds + eo,
+                  Fc.two + eo, acc, As, Bs);
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii) {
+      int i = ty + 16 * ii;
+      if (i >= ma) continue;
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        int j = tx + 16 * jj;
+        if (j < mb && tk.a0 + i >= tk.b0 + j) {
+          int e = rpos[i] + cpos[j] * TM;
+          acc_s[e] = acc_s[e] - acc[ii][jj];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < TM * TN; e += NT) {
+    int i = e % TM, j = e / TM;
+    if (i < mr && j < mc) P[(it.r0 + i) + (int64_t)(it.c0 + j) * h] = acc_s[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cdiv building blocks
+// ---------------------------------------------------------------------------
+template <class T>
+struct SnCtx {
+  T* P;
+  int64_t h;
+  int w, first, eo, nb;
+  T *d, *ds, *snap;
+  uint8_t* two;
+  int32_t* lperm;
+  int32_t* deep;
+  int snap_ld;
+};
+
+template <class T>
+__device__ inline SnCtx<T> make_ctx(const DevSym& S, const DevFac<T>& Fc, int s, int nb) {
+  SnCtx<T> c;
+  c.P = Fc.pool + S.panel_off[s];
+  c.h = S.sn_nrow[s];
+  c.w = S.sn_ncol[s];
+  c.first = S.sn_first[s];
+  c.eo = S.e_off[s];
+  c.nb = nb;
+  c.d = Fc.d + c.eo;
+  c.ds = Fc.ds + c.eo;
+  c.two = Fc.two + c.eo;
+  c.lperm = Fc.lperm + c.eo;
+  c.snap = Fc.snap + S.snap_off[s];
+  c.snap_ld = S.snap_ld[s];
+  c.deep = Fc.deep + (int64_t)s * NBMAX;
+  return c;
+}
+
+__device__ inline int pk_off(int i, int j, int ld) {
+  // column-major packed lower triangle of an ld x ld block
+  return j * ld - (j * (j - 1)) / 2 + (i - j);
+}
+
+template <class T>
+__device__ inline void wave_argmax(double& m, int& idx) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double om = __shfl_xor(m, o, 64);
+    int oi = __shfl_xor(idx, o, 64);
+    if (om > m || (om == m && oi < idx)) {
+      m = om;
+      idx = oi;
+    }
+  }
+}
+
+__device__ inline double wave_max(double m) {
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  return m;
+}
+
+template <class T>
+__device__ inline T perturb_pivot(T dv, double floor) {
+  double mag = F<T>::mag(dv);
+  if (mag >= floor) return dv;
+  if (mag == 0.0) return F<T>::from_real(floor);
+  return dv * F<T>::from_real(floor / mag);
+}
+
+// Initialise the per-supernode D / pivot bookkeeping.
+template <class T>
+__device__ void cdiv_init(const SnCtx<T>& c) {
+  for (int p = threadIdx.x; p < c.w; p += NT) {
+    c.lperm[p] = p;
+    c.two[p] = 0;
+    c.ds[p] = F<T>::zero();
+    c.d[p] = F<T>::zero();
+  }
+}
+
+// Bunch-Kaufman on the diagonal block [kb,ke) held in LDS (packed lower).
+// Candidates, interchanges and updates bounded to the panel rows, as
+// ll_cdiv_emit's getf2 (multifrontal_ldlt.rs:2846-3016). Returns 1 when the
+// pivot policy is Fail and a zero / singular pivot is met (uniform).
+template <class T>
+__device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* l2s,
+                           int32_t* deep_s, PivotPolicy pol, int* npert) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int pw = ke - kb, ld = pw;
+  const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+  const double GROWTH_EPS = 1e-14;
+  T* snap = c.snap;
+  const int sld = c.snap_ld;
+#define A_(i, j) Ab[pk_off((i), (j), ld)]
+  for (int q = tid; q < pw; q += NT) deep_s[q] = -1;
+  __syncthreads();
+  int k = 0;
+  while (k < pw) {
+    // ---- pivot search (every wave computes the same answer) ----
+    const double absakk = F<T>::mag(A_(k, k));
+    double cm = 0.0;
+    int imax = k;
+    for (int i = k + 1 + lane; i < pw; i += 64) {
+      double m = F<T>::magsq(A_(i, k));
+      if (m > cm) {
+        cm = m;
+        imax = i;
+      }
+    }
+    wave_argmax<T>(cm, imax);
+    const double colmax = sqrt(cm);
+    // decision below reads row imax; every wave must finish reading before
+    // any wave starts the interchange / update of this step
+    int kstep, kp;
+    if (fmax(absakk, colmax) == 0.0) {
+      if (!pol.have_floor) return 1;
+      kstep = 1;
+      kp = k;
+    } else if (absakk >= alpha * colmax) {
+      kstep = 1;
+      kp = k;
+    } else {
+      double rm = 0.0;
+      for (int j = k + lane; j < imax; j += 64) rm = fmax(rm, F<T>::magsq(A_(imax, j)));
+      for (int i = imax + 1 + lane; i < pw; i += 64) rm = fmax(rm, F<T>::magsq(A_(i, imax)));
+      rm = wave_max(rm);
+      const double rowmax = sqrt(rm);
+      if (absakk >= alpha * colmax * (colmax / rowmax)) {
+        kstep = 1;
+        kp = k;
+      } else if (F<T>::mag(A_(imax, imax)) >= alpha * rowmax) {
+        kstep = 1;
+        kp = imax;
+      } else {
+        kstep = 2;
+        kp = imax;
+      }
+    }
+    __syncthreads();
+    // ---- symmetric interchange of p < q inside the block ----
+    const int p = (kstep == 1) ? k : k + 1;
+    if (kp != p) {
+      const int q = kp;
+      for (int t = tid; t < pw; t += NT) {
+        int i = q + 1 + t;
+        if (i < pw) {
+          T x = A_(i, p);
+          A_(i, p) = A_(i, q);
+          A_(i, q) = x;
+        }
+        i = p + 1 + t;
+        if (i < q) {
+          T x = A_(i, p);
+          A_(i, p) = A_(q, i);
+          A_(q, i) = x;
+        }
+        if (t < p) {
+          T x = A_(p, t);
+          A_(p, t) = A_(q, t);
+          A_(q, t) = x;
+        }
+        if (t == 0) {
+          T x = A_(p, p);
+          A_(p, p) = A_(q, q);
+          A_(q, q) = x;
+          int32_t y = c.lperm[kb + p];
+          c.lperm[kb + p] = c.lperm[kb + q];
+          c.lperm[kb + q] = y;
+          deep_s[k] = q;
+        }
+      }
+      __syncthreads();
+    }
+    if (kstep == 1) {
+      T dk = A_(k, k);
+      if (pol.have_floor) {
+        if (F<T>::mag(dk) < pol.floor) {
+          dk = perturb_pivot(dk, pol.floor);
+          if (tid == 0) {
+            A_(k, k) = dk;
+            (*npert)++;
+          }
+        }
+      } else if (F<T>::is_zero(dk)) {
+        return 1;
+      }
+      if (tid == 0) c.d[kb + k] = dk;
+      const T dinv = F<T>::recip(dk);
+      // time-of-step multipliers and in-block rank-1 update (rows of this panel)
+      const int nr = pw - k - 1;
+      for (int t = tid; t < nr; t += NT) {
+        int j = k + 1 + t;
+        snap[(int64_t)k * sld + j] = A_(j, k) * dinv;
+      }
+      for (int j = k + 1 + wv; j < pw; j += 4) {
+        const T m = A_(j, k) * dinv;
+        if (F<T>::is_zero(m)) continue;
+        for (int i = j + lane; i < pw; i += 64) A_(i, j) = A_(i, j) - A_(i, k) * m;
+      }
+      __syncthreads();
+      for (int i = k + 1 + tid; i < pw; i += NT) A_(i, k) = A_(i, k) * dinv;
+      __syncthreads();
+      k += 1;
+    } else {
+      T d11 = A_(k, k), d21 = A_(k + 1, k), d22 = A_(k + 1, k + 1);
+      T det = d11 * d22 - d21 * d21;
+      double scale = fmax(fmax(F<T>::mag(d11), F<T>::mag(d22)), F<T>::mag(d21));
+      double growth_floor = GROWTH_EPS * scale * scale;
+      if (pol.have_floor) {
+        double fl = fmax(pol.floor * pol.floor, growth_floor);
+        if (F<T>::mag(det) < fl) {
+          double lift = fmax(pol.floor, scale * sqrt(GROWTH_EPS));
+          d11 = d11 + F<T>::from_real(lift);
+          d22 = d22 + F<T>::from_real(lift);
+          det = d11 * d22 - d21 * d21;
+          if (F<T>::mag(det) < fl) det = det + F<T>::from_real(fl);
+          if (tid == 0) (*npert)++;
+        }
+      } else if (F<T>::mag(det) <= growth_floor) {
+        return 1;
+      }
+      const T detinv = F<T>::recip(det);
+      if (tid == 0) {
+        c.d[kb + k] = d11;
+        c.ds[kb + k] = d21;
+        c.d[kb + k + 1] = d22;
+        c.two[kb + k] = 1;
+      }
+      for (int i = k + 2 + tid; i < pw; i += NT) {
+        T wik = A_(i, k), wik1 = A_(i, k + 1);
+        T a = (d22 * wik - d21 * wik1) * detinv;
+        T b = (d11 * wik1 - d21 * wik) * detinv;
+        l1s[i] = a;
+        l2s[i] = b;
+        snap[(int64_t)k * sld + i] = a;
+        snap[(int64_t)(k + 1) * sld + i] = b;
+      }
+      __syncthreads();
+      for (int j = k + 2 + wv; j < pw; j += 4) {
+        const T m1 = l1s[j], m2 = l2s[j];
+        for (int i = j + lane; i < pw; i += 64)
+          A_(i, j) = A_(i, j) - A_(i, k) * m1 - A_(i, k + 1) * m2;
+      }
+      __syncthreads();
+      for (int i = k + 2 + tid; i < pw; i += NT) {
+        A_(i, k) = l1s[i];
+        A_(i, k + 1) = l2s[i];
+      }
+      __syncthreads();
+      k += 2;
+    }
+  }
+#undef A_
+  return 0;
+}
+
+// Load the lower part of the diagonal block into packed LDS.
+template <class T>
+__device__ void panel_load(const SnCtx<T>& c, int kb, int ke, T* Ab) {
+  const int pw = ke - kb;
+  for (int j = 0; j < pw; ++j)
+    for (int i = j + threadIdx.x; i < pw; i += NT)
+      Ab[pk_off(i, j, pw)] = c.P[(kb + i) + (int64_t)(kb + j) * c.h];
+  __syncthreads();
+}
+
+// Write the factored block back, replay its row interchanges on the already
+// factored columns [0,kb) and publish the interchange partners.
+template <class T>
+__device__ void panel_store(const SnCtx<T>& c, int kb, int ke, const T* Ab,
+                            const int32_t* deep_s) {
+  const int pw = ke - kb;
+  for (int j = 0; j < pw; ++j)
+    for (int i = j + threadIdx.x; i < pw; i += NT)
+      c.P[(kb + i) + (int64_t)(kb + j) * c.h] = Ab[pk_off(i, j, pw)];
+  for (int j = threadIdx.x; j < kb; j += NT) {
+    T* col = c.P + (int64_t)j * c.h;
+    for (int k = 0; k < pw;) {
+      int kp = deep_s[k];
+      bool two = c.two[kb + k];
+      if (kp >= 0) {
+        int src = two ? k + 1 : k;
+        T x = col[kb + src];
+        col[kb + src] = col[kb + kp];
+        col[kb + kp] = x;
+      }
+      k += two ? 2 : 1;
+    }
+  }
+  for (int q = threadIdx.x; q < pw; q += NT) c.deep[q] = deep_s[q];
+  __syncthreads();
+}
+
+// apply_bk_panel_trailing on deep rows i in [r0, r1), one row per thread.
+template <class T>
+__device__ void panel_replay_rows(const SnCtx<T>& c, int kb, int ke, const int32_t* deep,
+                                  int64_t r0, int64_t r1, int64_t tstart, int64_t tstride) {
+  const int64_t h = c.h;
+  const int sld = c.snap_ld;
+  for (int64_t i = r0 + tstart; i < r1; i += tstride) {
+    T* row = c.P + i;
+    int k = kb;
+    while (k < ke) {
+      const int kl = k - kb;
+      const int kp = deep[kl];
+      const bool two = c.two[k];
+      if (kp >= 0) {
+        int src = two ? k + 1 : k;
+        T x = row[(int64_t)src * h];
+        row[(int64_t)src * h] = row[(int64_t)(kb + kp) * h];
+        row[(int64_t)(kb + kp) * h] = x;
+      }
+      if (two) {
+        const T d11 = c.d[k], d21 = c.ds[k], d22 = c.d[k + 1];
+        const T detinv = F<T>::recip(d11 * d22 - d21 * d21);
+        const T wk = row[(int64_t)k * h], wk1 = row[(int64_t)(k + 1) * h];
+        for (int j = k + 2; j < ke; ++j) {
+          const T m1 = c.snap[(int64_t)kl * sld + (j - kb)];
+          const T m2 = c.snap[(int64_t)(kl + 1) * sld + (j - kb)];
+          row[(int64_t)j * h] = row[(int64_t)j * h] - wk * m1 - wk1 * m2;
+        }
+        row[(int64_t)k * h] = (d22 * wk - d21 * wk1) * detinv;
+        row[(int64_t)(k + 1) * h] = (d11 * wk1 - d21 * wk) * detinv;
+        k += 2;
+      } else {
+        const T dinv = F<T>::recip(c.d[k]);
+        const T wk = row[(int64_t)k * h];
+        for (int j = k + 1; j < ke; ++j) {
+          const T m = c.snap[(int64_t)kl * sld + (j - kb)];
+          if (!F<T>::is_zero(m)) row[(int64_t)j * h] = row[(int64_t)j * h] - wk * m;
+        }
+        row[(int64_t)k * h] = wk * dinv;
+        k += 1;
+      }
+    }
+  }
+}
+
+// Deferred Schur update of one 64x64 tile of P[ke.., ke..w).
+template <class T>
+__device__ void panel_schur_tile(const SnCtx<T>& c, int kb, int ke, int rr0, int cc0, T* As,
+                                 T* Bs) {
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int64_t mt = c.h - ke;
+  const int cw = c.w - ke, pw = ke - kb;
+  const int ma = (int)min<int64_t>(TM, mt - rr0), mb = min(TN, cw - cc0);
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
+  ldl_tile_gemm(c.P, c.h, kb, pw, ke + rr0, ma, ke + cc0, mb, c.d + kb, c.ds + kb, c.two + kb,
+                acc, As, Bs);
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii) {
+    int i = ty + 16 * ii;
+    if (i >= ma) continue;
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      int j = tx + 16 * jj;
+      if (j < mb && rr0 + i >= cc0 + j) {
+        T* dst = c.P + (ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h;
+        *dst = *dst - acc[ii][jj];
+      }
+    }
+  }
+}
+
+// Emit maps and inertia for supernode s (multifrontal_ldlt.rs:3196-3256).
+template <class T>
+__device__ void cdiv_emit(const DevSym& S, const DevFac<T>& Fc, const SnCtx<T>& c,
+                          int* cnt /* LDS [4] */, int npert) {
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < c.w; p += NT) {
+    int g = c.first + c.lperm[p];
+    Fc.e_of_g[g] = c.eo + p;
+    Fc.perm_e[c.eo + p] = S.sym_perm[g];
+    const bool second = p > 0 && c.two[p - 1];
+    if (second) continue;
+    if (c.two[p]) {
+      T d1 = c.d[p], d2 = c.d[p + 1], e = c.ds[p];
+      double det_r = F<T>::real(d1 * d2 - e * e);
+      double tr_r = F<T>::real(d1 + d2);
+      if (det_r < 0.0) {
+        atomicAdd(&cnt[0], 1);
+        atomicAdd(&cnt[1], 1);
+      } else if (det_r > 0.0) {
+        atomicAdd(&cnt[tr_r >= 0.0 ? 0 : 1], 2);
+      } else {
+        atomicAdd(&cnt[2], 1);
+        atomicAdd(&cnt[tr_r >= 0.0 ? 0 : 1], 1);
+      }
+    } else {
+      double r = F<T>::real(c.d[p]);
+      atomicAdd(&cnt[r > 0.0 ? 0 : (r < 0.0 ? 1 : 2)], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&Fc.counters[0], (unsigned long long)cnt[0]);
+    atomicAdd(&Fc.counters[1], (unsigned long long)cnt[1]);
+    atomicAdd(&Fc.counters[2], (unsigned long long)cnt[2]);
+    if (npert) atomicAdd(&Fc.counters[3], (unsigned long long)npert);
+  }
+}
+
+template <class T>
+__device__ void flag_failure(const DevFac<T>& Fc, int s) {
+  if (threadIdx.x == 0) {
+    atomicMax(&Fc.status[0], 1);
+    atomicMin(&Fc.status[1], s);
+  }
+}
+
+// The packed diagonal block is live only during getf2, the GEMM staging only
+// during the Schur update, so they share storage (fits 160 KB for c64, nb 128).
+template <class T>
+struct CdivShared {
+  union {
+    T Ab[NBMAX * (NBMAX + 1) / 2];
+    struct {
+      T As[KC * TM];
+      T Bs[KC * TN];
+    } g;
+  } u;
+  T l1s[NBMAX], l2s[NBMAX];
+  int32_t deep_s[NBMAX];
+  int cnt[4];
+  int npert;
+};
+
+// One workgroup factors a whole supernode (all its nb-panels).
+template <class T>
+__global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
+                                                   const int32_t* __restrict__ list, int nb_base,
+                                                   PivotPolicy pol) {
+  extern __shared__ unsigned char smem_raw[];
+  CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
+  const int s = list[blockIdx.x];
+  const int w = S.sn_ncol[s];
+  const int nb = (w >= 512 && nb_base < 128) ? 128 : nb_base;
+  SnCtx<T> c = make_ctx(S, Fc, s, nb);
+  if (threadIdx.x == 0) sh.npert = 0;
+  cdiv_init(c);
+  __syncthreads();
+  for (int kb = 0; kb < w; kb += nb) {
+    const int ke = min(kb + nb, w);
+    panel_load(c, kb, ke, sh.u.Ab);
+    int st = panel_getf2(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.deep_s, pol, &sh.npert);
+    if (st) {
+      flag_failure(Fc, s);
+      return;
+    }
+    panel_store(c, kb, ke, sh.u.Ab, sh.deep_s);
+    panel_replay_rows(c, kb, ke, sh.deep_s, ke, c.h, threadIdx.x, NT);
+    __syncthreads();
+    const int64_t mt = c.h - ke;
+    const int cw = w - ke;
+    if (cw > 0 && mt > 0) {
+      const int nrt = (int)((mt + TM - 1) / TM), nct = (cw + TN - 1) / TN;
+      for (int cb = 0; cb < nct; ++cb)
+        for (int rb = cb; rb < nrt; ++rb) panel_schur_tile(c, kb, ke, rb * TM, cb * TN, sh.u.g.As, sh.u.g.Bs);
+      __syncthreads();
+    }
+  }
+  cdiv_emit(S, Fc, c, sh.cnt, sh.npert);
+}
+
+// ---- wide supernodes: one launch per phase of each nb-panel ----
+template <class T>
+__global__ __launch_bounds__(NT) void k_panel_getf2(DevSym S, DevFac<T> Fc, int s, int kb,
+                                                    int nb, PivotPolicy pol) {
+  extern __shared__ unsigned char smem_raw[];
+  CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
+  SnCtx<T> c = make_ctx(S, Fc, s, nb);
+  if (Fc.status[0]) return;  // an earlier panel of this supernode failed
+  if (threadIdx.x == 0) sh.npert = 0;
+  if (kb == 0) cdiv_init(c);
+  __syncthreads();
+  const int ke = min(kb + nb, c.w);
+  panel_load(c, kb, ke, sh.u.Ab);
+  int st = panel_getf2(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.deep_s, pol, &sh.npert);
+  if (st) {
+    flag_failure(Fc, s);
+    return;
+  }
+  panel_store(c, kb, ke, sh.u.Ab, sh.deep_s);
+  if (threadIdx.x == 0 && sh.npert) atomicAdd(&Fc.counters[3], (unsigned long long)sh.npert);
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_panel_replay(DevSym S, DevFac<T> Fc, int s, int kb,
+                                                     int nb) {
+  __shared__ int32_t deep_s[NBMAX];
+  SnCtx<T> c = make_ctx(S, Fc, s, nb);
+  const int ke = min(kb + nb, c.w);
+  for (int q = threadIdx.x; q < ke - kb; q += NT) deep_s[q] = c.deep[q];
+  __syncthreads();
+  const int64_t r0 = ke + (int64_t)blockIdx.x * NT;
+  const int64_t r1 = min<int64_t>(r0 + NT, c.h);
+  panel_replay_rows(c, kb, ke, deep_s, r0, r1, threadIdx.x, NT);
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_panel_schur(DevSym S, DevFac<T> Fc, int s, int kb,
+                                                    int nb) {
+  __shared__ T As[KC * TM];
+  __shared__ T Bs[KC * TN];
+  const int rb = blockIdx.x, cb = blockIdx.y;
+  if (rb < cb) return;
+  SnCtx<T> c = make_ctx(S, Fc, s, nb);
+  const int ke = min(kb + nb, c.w);
+  panel_schur_tile(c, kb, ke, rb * TM, cb * TN, As, Bs);
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_emit(DevSym S, DevFac<T> Fc, int s, int nb) {
+  __shared__ int cnt[4];
+  SnCtx<T> c = make_ctx(S, Fc, s, nb);
+  cdiv_emit(S, Fc, c, cnt, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Host-callable launchers
+// ---------------------------------------------------------------------------
+template <class T>
+size_t cdiv_smem() {
+  return sizeof(CdivShared<T>);
+}
+
+template <class T>
+void launch_fill(T* p, int64_t n, T v, hipStream_t st) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+}
+
+template <class T>
+void launch_assemble(const DevSym& S, const DevFac<T>& Fc, const T* vals, const double* scale,
+                     hipStream_t st) {
+  if (S.nnz <= 0) return;
+  hipLaunchKernelGGL(k_assemble<T>, dim3((unsigned)((S.nnz + 255) / 256)), dim3(256), 0, st, S,
+                     Fc, vals, scale);
+}
+
+template <class T>
+void launch_cmod(const DevSym& S, const DevFac<T>& Fc, const CItem* items, int nitems,
+                 const CTask* tasks, hipStream_t st) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_cmod<T>, dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks);
+}
+
+template <class T>
+void launch_cdiv_small(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
+                       int nb, PivotPolicy pol, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_cdiv_small<T>, dim3(count), dim3(NT), cdiv_smem<T>(), st, S, Fc, list, nb,
+                     pol);
+}
+
+template <class T>
+void launch_cdiv_wide(const DevSym& S, const DevFac<T>& Fc, int s, int w, int64_t h, int nb_base,
+                      PivotPolicy pol, hipStream_t st) {
+  const int nb = (w >= 512 && nb_base < 128) ? 128 : nb_base;
+  for (int kb = 0; kb < w; kb += nb) {
+    const int ke = kb + nb < w ? kb + nb : w;
+    hipLaunchKernelGGL(k_panel_getf2<T>, dim3(1), dim3(NT), cdiv_smem<T>(), st, S, Fc, s, kb, nb,
+                       pol);
+    const int64_t mt = h - ke;
+    if (mt > 0)
+      hipLaunchKernelGGL(k_panel_replay<T>, dim3((unsigned)((mt + NT - 1) / NT)), dim3(NT), 0, st,
+                         S, Fc, s, kb, nb);
+    const int cw = w - ke;
+    if (cw > 0 && mt > 0) {
+      unsigned nrt = (unsigned)((mt + TM - 1) / TM), nct = (unsigned)((cw + TN - 1) / TN);
+      hipLaunchKernelGGL(k_panel_schur<T>, dim3(nrt, nct), dim3(NT), 0, st, S, Fc, s, kb, nb);
+    }
+  }
+  hipLaunchKernelGGL(k_emit<T>, dim3(1), dim3(NT), 0, st, S, Fc, s, nb);
+}
+
+template <class T>
+hipError_t configure_factor_kernels() {
+  hipError_t e = hipFuncSetAttribute((const void*)k_cdiv_small<T>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)cdiv_smem<T>());
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_panel_getf2<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdiv_smem<T>());
+}
+
+#define RSL_INST(T)                                                                            \
+  template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                   \
+  template void launch_assemble<T>(const DevSym&, const DevFac<T>&, const T*, const double*,   \
+                                   hipStream_t);                                               \
+  template void launch_cmod<T>(const DevSym&, const DevFac<T>&, const CItem*, int,             \
+                               const CTask*, hipStream_t);                                     \
+  template void launch_cdiv_small<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int, \
+                                     PivotPolicy, hipStream_t);                                \
+  template void launch_cdiv_wide<T>(const DevSym&, const DevFac<T>&, int, int, int64_t, int,   \
+                                    PivotPolicy, hipStream_t);                                 \
+  template hipError_t configure_factor_kernels<T>();                                           \
+  template size_t cdiv_smem<T>();
+
+RSL_INST(double)
+RSL_INST(cplx)
+
+}  // namespace rsl

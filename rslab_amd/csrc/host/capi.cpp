@@ -1,0 +1,274 @@
+// C ABI over the host runtime (declarations and reference citations in
+// include/rslab_amd.h).
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../../include/rslab_amd.h"
+#include "runtime.hpp"
+
+struct rsl_symbolic {
+  std::shared_ptr<rsl::DeviceSymbolic> ds;
+};
+struct rsl_solver {
+  std::unique_ptr<rsl::DeviceFactorBase> f;
+  bool have_csc = false;
+  std::vector<int64_t> colp;
+  std::vector<int32_t> rowi;
+  std::vector<double> lval, d, ds;
+  std::vector<uint8_t> two;
+  std::vector<int32_t> perm;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class Fn>
+int guard(Fn&& fn) {
+  try {
+    fn();
+    return RSL_OK;
+  } catch (const rsl::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host out of memory";
+    return RSL_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return RSL_INVALID_INPUT;
+  }
+}
+
+rsl::FactorSettings factor_settings(const rsl_settings& s) {
+  rsl::FactorSettings fs;
+  fs.pivot = s.pivot;
+  fs.abs_floor = s.abs_floor;
+  fs.scaling = s.scaling;
+  fs.precision = static_cast<rsl::Precision>(s.precision);
+  return fs;
+}
+}  // namespace
+
+extern "C" {
+
+void rsl_settings_default(rsl_settings* s) {
+  std::memset(s, 0, sizeof(*s));
+  s->ordering = RSL_ORDER_AMD;
+  s->nemin = 16;
+  s->relax = 1;
+  s->relax_width = 256;
+  s->relax_rows = 64;
+  s->panel_nb = 64;
+  s->pivot = RSL_PIVOT_FAIL;
+  s->abs_floor = 0.0;
+  s->scaling = 1;
+  s->precision = RSL_PREC_FP64;
+  s->nd_seed = 1;
+  s->wide_flops = 4.0e6;
+}
+
+const char* rsl_last_error(void) { return g_err.c_str(); }
+
+int rsl_order(int32_t method, int64_t n, const int64_t* col_ptr, const int64_t* row_idx,
+              uint64_t seed, int32_t* perm, double* stats) {
+  return guard([&] {
+    if (n < 0 || n >= INT32_MAX) throw rsl::Error(RSL_INVALID_INPUT, "dimension out of range");
+    rsl::Pattern p;
+    p.n = (int32_t)n;
+    p.colp.assign(col_ptr, col_ptr + n + 1);
+    p.rowi.resize(col_ptr[n]);
+    for (int64_t k = 0; k < col_ptr[n]; ++k) p.rowi[k] = (int32_t)row_idx[k];
+    std::vector<int32_t> pr;
+    if (method == RSL_ORDER_AMD) {
+      rsl::AmdStats st;
+      pr = rsl::amd_order(p, &st);
+      if (stats) {
+        stats[0] = st.ncmpa;
+        stats[1] = st.ndense;
+        stats[2] = st.lnz;
+        stats[3] = st.ndiv;
+        stats[4] = st.nms_lu;
+        stats[5] = st.nms_ldl;
+        stats[6] = st.dmax;
+      }
+    } else if (method == RSL_ORDER_METIS_ND) {
+      pr = rsl::nd_order(p, seed);
+    } else {
+      throw rsl::Error(RSL_INVALID_INPUT, "unknown ordering");
+    }
+    std::memcpy(perm, pr.data(), sizeof(int32_t) * pr.size());
+  });
+}
+
+int rsl_analyze(int64_t n, const int64_t* col_ptr, const int64_t* row_idx, const rsl_settings* s,
+                rsl_symbolic** out) {
+  *out = nullptr;
+  return guard([&] {
+    rsl_settings st;
+    if (s) st = *s;
+    else rsl_settings_default(&st);
+    if (n < 0 || n >= INT32_MAX) throw rsl::Error(RSL_INVALID_INPUT, "dimension out of range");
+    const int64_t nnz = col_ptr[n];
+    std::vector<int32_t> ri(nnz);
+    for (int64_t k = 0; k < nnz; ++k) {
+      if (row_idx[k] < 0 || row_idx[k] >= n) throw rsl::Error(RSL_INVALID_INPUT, "row index out of bounds");
+      ri[k] = (int32_t)row_idx[k];
+    }
+    std::string v = rsl::validate_lower_csc((int32_t)n, col_ptr, ri.data(), nnz);
+    if (!v.empty()) throw rsl::Error(RSL_INVALID_INPUT, v);
+    if (st.panel_nb > 128) throw rsl::Error(RSL_INVALID_INPUT, "panel_nb above 128 is not supported");
+    rsl::SymbolicParams sp;
+    sp.nemin = st.nemin;
+    sp.relax = st.relax != 0;
+    sp.relax_width = st.relax_width;
+    sp.relax_rows = st.relax_rows;
+    sp.nd_seed = st.nd_seed;
+    rsl::Ordering ord = st.ordering == RSL_ORDER_METIS_ND ? rsl::Ordering::MetisND
+                        : st.ordering == RSL_ORDER_NATURAL ? rsl::Ordering::Natural
+                                                           : rsl::Ordering::Amd;
+    auto ds = std::make_shared<rsl::DeviceSymbolic>();
+    ds->plan.sym = rsl::analyze((int32_t)n, col_ptr, ri.data(), ord, sp);
+    rsl::PlanParams pp;
+    pp.panel_nb = st.panel_nb < 8 ? 8 : st.panel_nb;
+    pp.wide_flops = st.wide_flops > 0 ? st.wide_flops : 4.0e6;
+    rsl::build_plan(ds->plan, col_ptr, ri.data(), pp);
+    ds->h_colp.assign(col_ptr, col_ptr + n + 1);
+    ds->h_rowi = std::move(ri);
+    auto* h = new rsl_symbolic;
+    h->ds = std::move(ds);
+    *out = h;
+  });
+}
+
+int rsl_symbolic_info(const rsl_symbolic* sym, int64_t out[8], double flops[2]) {
+  return guard([&] {
+    const rsl::Plan& P = sym->ds->plan;
+    int64_t nch = 0;
+    for (auto& s : P.sym.snodes) nch += (int64_t)s.children.size();
+    if (out) {
+      out[0] = P.n;
+      out[1] = P.ns;
+      out[2] = P.sym.factor_nnz;
+      out[3] = P.panel_total;
+      out[4] = (int64_t)P.levels.size();
+      out[5] = nch;
+      out[6] = (int64_t)P.sym.permuted.rowi.size();
+      out[7] = P.nnzl_panel;
+    }
+    if (flops) {
+      flops[0] = P.factor_flops;
+      flops[1] = P.geom_flops;
+    }
+  });
+}
+
+int rsl_symbolic_export(const rsl_symbolic* sym, int32_t* perm, int32_t* perm_inv,
+                        int32_t* sn_first, int32_t* sn_ncol, int32_t* sn_nrow, int32_t* ch_ptr,
+                        int32_t* ch_idx, int64_t* pp_colp, int32_t* pp_rowi) {
+  return guard([&] {
+    const rsl::Plan& P = sym->ds->plan;
+    const rsl::Symbolic& S = P.sym;
+    if (perm) std::memcpy(perm, S.perm.data(), sizeof(int32_t) * S.n);
+    if (perm_inv) std::memcpy(perm_inv, S.perm_inv.data(), sizeof(int32_t) * S.n);
+    if (sn_first) std::memcpy(sn_first, P.sn_first.data(), sizeof(int32_t) * P.ns);
+    if (sn_ncol) std::memcpy(sn_ncol, P.sn_ncol.data(), sizeof(int32_t) * P.ns);
+    if (sn_nrow) std::memcpy(sn_nrow, P.sn_nrow.data(), sizeof(int32_t) * P.ns);
+    if (ch_ptr) {
+      int32_t acc = 0;
+      for (int32_t s = 0; s < P.ns; ++s) {
+        ch_ptr[s] = acc;
+        if (ch_idx)
+          for (size_t c = 0; c < S.snodes[s].children.size(); ++c) ch_idx[acc + c] = S.snodes[s].children[c];
+        acc += (int32_t)S.snodes[s].children.size();
+      }
+      ch_ptr[P.ns] = acc;
+    }
+    if (pp_colp) std::memcpy(pp_colp, S.permuted.colp.data(), sizeof(int64_t) * (S.n + 1));
+    if (pp_rowi) std::memcpy(pp_rowi, S.permuted.rowi.data(), sizeof(int32_t) * S.permuted.rowi.size());
+  });
+}
+
+int rsl_factor(rsl_symbolic* sym, int32_t field, const double* values, const rsl_settings* s,
+               rsl_solver** out) {
+  *out = nullptr;
+  return guard([&] {
+    rsl_settings st;
+    if (s) st = *s;
+    else rsl_settings_default(&st);
+    auto f = rsl::make_factor(sym->ds, field, values, factor_settings(st));
+    auto* h = new rsl_solver;
+    h->f = std::move(f);
+    *out = h;
+  });
+}
+
+int rsl_refactor(rsl_solver* f, const double* values) {
+  return guard([&] {
+    f->have_csc = false;
+    f->f->refactor(values);
+  });
+}
+
+int rsl_solve(rsl_solver* f, const double* b, int64_t nrhs, double* x) {
+  return guard([&] {
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->solve_host(b, nrhs, x);
+  });
+}
+
+int rsl_solve_device(rsl_solver* f, const void* d_b, int64_t nrhs, void* d_x, void* stream) {
+  return guard([&] {
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->solve_device(d_b, nrhs, d_x, static_cast<hipStream_t>(stream));
+  });
+}
+
+int rsl_factor_info(const rsl_solver* f, int64_t inertia[3], int64_t* n_perturbed, double* t_ms) {
+  return guard([&] {
+    if (inertia)
+      for (int q = 0; q < 3; ++q) inertia[q] = f->f->stats.inertia[q];
+    if (n_perturbed) *n_perturbed = f->f->stats.n_perturbed;
+    if (t_ms) *t_ms = f->f->stats.t_factor_ms;
+  });
+}
+
+int rsl_factor_nnz(rsl_solver* f, int64_t* nnzl) {
+  return guard([&] {
+    if (!f->have_csc) {
+      f->f->download(f->colp, f->rowi, f->lval, f->d, f->ds, f->two, f->perm);
+      f->have_csc = true;
+    }
+    *nnzl = (int64_t)f->rowi.size();
+  });
+}
+
+int rsl_factor_download(rsl_solver* f, int64_t* l_col_ptr, int32_t* l_row_idx, double* l_values,
+                        double* d_diag, double* d_subdiag, uint8_t* two_by_two, int32_t* perm) {
+  return guard([&] {
+    if (!f->have_csc) {
+      f->f->download(f->colp, f->rowi, f->lval, f->d, f->ds, f->two, f->perm);
+      f->have_csc = true;
+    }
+    if (l_col_ptr) std::memcpy(l_col_ptr, f->colp.data(), sizeof(int64_t) * f->colp.size());
+    if (l_row_idx) std::memcpy(l_row_idx, f->rowi.data(), sizeof(int32_t) * f->rowi.size());
+    if (l_values) std::memcpy(l_values, f->lval.data(), sizeof(double) * f->lval.size());
+    if (d_diag) std::memcpy(d_diag, f->d.data(), sizeof(double) * f->d.size());
+    if (d_subdiag) std::memcpy(d_subdiag, f->ds.data(), sizeof(double) * f->ds.size());
+    if (two_by_two) std::memcpy(two_by_two, f->two.data(), f->two.size());
+    if (perm) std::memcpy(perm, f->perm.data(), sizeof(int32_t) * f->perm.size());
+  });
+}
+
+int rsl_refine(rsl_solver* f, const double* a_values, const double* b, int64_t nrhs, double target,
+               double* x, double* info) {
+  return guard([&] {
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->refine(a_values, b, nrhs, target, x, info);
+  });
+}
+
+void rsl_free_factor(rsl_solver* f) { delete f; }
+void rsl_free_symbolic(rsl_symbolic* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// Execution plan: panel layout, update lists, tree levels, cmod tile tasks and
+// the A -> panel scatter map. Everything here depends on the pattern only, so
+// a plan is reused across numeric refactorizations (LdltSymbolic reuse,
+// src/numeric/sparse_solver.rs:457-488).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace rsl {
+
+namespace {
+constexpr int TILE = 64;
+}
+
+void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanParams& prm) {
+  const Symbolic& S = P.sym;
+  P.params = prm;
+  P.n = S.n;
+  const int32_t ns = (int32_t)S.snodes.size();
+  P.ns = ns;
+  std::vector<std::vector<idx>> rs = supernode_row_structures(S);
+  std::vector<std::vector<idx>> ul = update_lists(S, rs);
+
+  P.sn_first.resize(ns);
+  P.sn_ncol.resize(ns);
+  P.sn_nrow.resize(ns);
+  P.e_off.resize(ns);
+  P.nb_eff.resize(ns);
+  P.rs_off.resize(ns + 1);
+  P.level.assign(ns, 0);
+  int32_t acc = 0, maxlev = 0;
+  int64_t roff = 0;
+  for (int32_t s = 0; s < ns; ++s) {
+    const Supernode& sn = S.snodes[s];
+    P.sn_first[s] = sn.first_col;
+    P.sn_ncol[s] = sn.ncol;
+    P.sn_nrow[s] = (int32_t)rs[s].size();
+    P.e_off[s] = acc;
+    acc += sn.ncol;
+    int nb = prm.panel_nb;
+    if (sn.ncol >= 512 && nb < 128) nb = 128;
+    P.nb_eff[s] = nb;
+    P.rs_off[s] = roff;
+    roff += (int64_t)rs[s].size();
+    int32_t lv = 0;
+    for (idx c : sn.children) lv = std::max(lv, P.level[c] + 1);
+    P.level[s] = lv;
+    maxlev = std::max(maxlev, lv);
+  }
+  P.rs_off[ns] = roff;
+  P.rs_idx.resize(roff);
+  for (int32_t s = 0; s < ns; ++s)
+    std::copy(rs[s].begin(), rs[s].end(), P.rs_idx.begin() + P.rs_off[s]);
+  P.upd_ptr.assign(ns + 1, 0);
+  for (int32_t s = 0; s < ns; ++s) P.upd_ptr[s + 1] = P.upd_ptr[s] + (int32_t)ul[s].size();
+  P.upd_idx.clear();
+  P.upd_idx.reserve(P.upd_ptr[ns]);
+  for (int32_t s = 0; s < ns; ++s) P.upd_idx.insert(P.upd_idx.end(), ul[s].begin(), ul[s].end());
+
+  // levels; panels laid out level by level so a level's panels are contiguous
+  P.levels.assign(ns ? maxlev + 1 : 0, LevelPlan());
+  for (int32_t s = 0; s < ns; ++s) P.levels[P.level[s]].all.push_back(s);
+  P.panel_off.assign(ns, 0);
+  P.snap_off.assign(ns, 0);
+  P.snap_ld.assign(ns, 0);
+  int64_t poff = 0;
+  P.snap_total = 0;
+  P.factor_flops = P.geom_flops = 0;
+  P.nnzl_panel = 0;
+  for (LevelPlan& L : P.levels) {
+    int64_t soff = 0;
+    for (int32_t s : L.all) {
+      const int64_t h = P.sn_nrow[s], w = P.sn_ncol[s];
+      P.panel_off[s] = poff;
+      poff += h * w;
+      int32_t ld = std::min<int32_t>(P.nb_eff[s], (int32_t)w);
+      P.snap_ld[s] = ld;
+      P.snap_off[s] = soff;
+      soff += (int64_t)ld * ld;
+      if ((double)h * w * w > prm.wide_flops && w > 32) L.wide.push_back(s);
+      else L.small.push_back(s);
+      for (int64_t k = 0; k < w; ++k) {
+        P.factor_flops += (double)(h - k) * (double)(h - k);
+        P.nnzl_panel += h - k;
+      }
+      P.geom_flops += (double)h * h * w;
+    }
+    P.snap_total = std::max(P.snap_total, soff);
+  }
+  P.panel_total = poff;
+
+  // cmod tiles and tasks
+  P.items.clear();
+  P.tasks.clear();
+  std::vector<int32_t> gloc(S.n, -1);
+  std::vector<std::vector<CTask>> tt;
+  struct Run {
+    int32_t tile, lo, hi;
+  };
+  std::vector<Run> cruns, rruns;
+  for (LevelPlan& L : P.levels) {
+    L.item_beg = (int64_t)P.items.size();
+    for (int32_t s : L.all) {
+      if (P.upd_ptr[s] == P.upd_ptr[s + 1]) continue;
+      const int32_t h = P.sn_nrow[s], w = P.sn_ncol[s], first = P.sn_first[s];
+      const int32_t* rss = P.rs_idx.data() + P.rs_off[s];
+      for (int32_t i = 0; i < h; ++i) gloc[rss[i]] = i;
+      const int32_t nrt = (h + TILE - 1) / TILE, nct = (w + TILE - 1) / TILE;
+      tt.assign((size_t)nrt * nct, {});
+      for (int32_t u = P.upd_ptr[s]; u < P.upd_ptr[s + 1]; ++u) {
+        const int32_t k = P.upd_idx[u], nck = P.sn_ncol[k];
+        const int32_t* ok = P.rs_idx.data() + P.rs_off[k] + nck;
+        const int32_t nok = P.sn_nrow[k] - nck;
+        const int32_t p0 = (int32_t)(std::lower_bound(ok, ok + nok, first) - ok);
+        const int32_t p1 = (int32_t)(std::lower_bound(ok, ok + nok, first + w) - ok);
+        if (p0 == p1) continue;
+        cruns.clear();
+        for (int32_t b = p0; b < p1; ++b) {
+          int32_t cb = (ok[b] - first) / TILE;
+          if (cruns.empty() || cruns.back().tile != cb) cruns.push_back({cb, b, b + 1});
+          else cruns.back().hi = b + 1;
+        }
+        rruns.clear();
+        for (int32_t a = p0; a < nok; ++a) {
+          int32_t li = gloc[ok[a]];
+          if (li < 0) throw std::runtime_error("plan: updater row outside target structure");
+          int32_t rb = li / TILE;
+          if (rruns.empty() || rruns.back().tile != rb) rruns.push_back({rb, a, a + 1});
+          else rruns.back().hi = a + 1;
+        }
+        for (const Run& c : cruns)
+          for (const Run& r : rruns) {
+            if (r.tile < c.tile) continue;
+            int32_t a0 = std::max(r.lo, c.lo);
+            if (a0 >= r.hi) continue;
+            // the kernel maps these ranges into a 64x64 tile: keep them inside it
+            if (r.hi - a0 > TILE || c.hi - c.lo > TILE ||
+                ok[c.lo] - first < c.tile * TILE || ok[c.hi - 1] - first >= std::min(w, (c.tile + 1) * TILE))
+              throw std::runtime_error("plan: cmod task outside its tile");
+            tt[(size_t)r.tile * nct + c.tile].push_back({k, a0, r.hi, c.lo, c.hi});
+          }
+      }
+      for (int32_t rb = 0; rb < nrt; ++rb)
+        for (int32_t cb = 0; cb < nct; ++cb) {
+          auto& v = tt[(size_t)rb * nct + cb];
+          if (v.empty()) continue;
+          CItem it;
+          it.s = s;
+          it.r0 = rb * TILE;
+          it.c0 = cb * TILE;
+          it.t_beg = (int32_t)P.tasks.size();
+          P.tasks.insert(P.tasks.end(), v.begin(), v.end());
+          it.t_end = (int32_t)P.tasks.size();
+          P.items.push_back(it);
+        }
+      for (int32_t i = 0; i < h; ++i) gloc[rss[i]] = -1;
+    }
+    L.item_end = (int64_t)P.items.size();
+  }
+  if ((int64_t)P.tasks.size() > INT32_MAX) throw std::runtime_error("plan: too many cmod tasks");
+
+  // A scatter map: user lower entry (i >= j) -> slot of P^T A P in its panel
+  P.nnz = colp[S.n];
+  P.amap.resize(P.nnz);
+  P.a_row.resize(P.nnz);
+  P.a_col.resize(P.nnz);
+  std::vector<int32_t> col2s(S.n, 0);
+  for (int32_t s = 0; s < ns; ++s)
+    for (int32_t c = 0; c < P.sn_ncol[s]; ++c) col2s[P.sn_first[s] + c] = s;
+  for (int32_t j = 0; j < S.n; ++j)
+    for (int64_t q = colp[j]; q < colp[j + 1]; ++q) {
+      const int32_t i = rowi[q];
+      const int32_t gi = S.perm_inv[i], gj = S.perm_inv[j];
+      const int32_t r = std::max(gi, gj), c = std::min(gi, gj);
+      const int32_t s = col2s[c];
+      const int32_t* rss = P.rs_idx.data() + P.rs_off[s];
+      const int32_t h = P.sn_nrow[s];
+      const int32_t li = (int32_t)(std::lower_bound(rss, rss + h, r) - rss);
+      if (li >= h || rss[li] != r) throw std::runtime_error("plan: entry outside panel structure");
+      P.amap[q] = P.panel_off[s] + li + (int64_t)(c - P.sn_first[s]) * h;
+      P.a_row[q] = i;
+      P.a_col[q] = j;
+    }
+}
+
+}  // namespace rsl

@@ -1,0 +1,51 @@
+// Device execution plan built once per analysed pattern (value independent).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../device/devtypes.hpp"
+#include "symbolic.hpp"
+
+namespace rsl {
+
+struct PlanParams {
+  int panel_nb = 64;            // SolverSettings::panel_nb default (gemm_tuning.rs:27)
+  double wide_flops = 4.0e6;    // supernodes with h*w*w above this get multi-workgroup cdiv
+};
+
+struct LevelPlan {
+  std::vector<int32_t> all;     // supernodes of the level (solve)
+  std::vector<int32_t> small;   // one workgroup per supernode (cdiv)
+  std::vector<int32_t> wide;    // multi-launch panel path (cdiv)
+  int64_t item_beg = 0, item_end = 0;  // cmod items of the level
+};
+
+struct Plan {
+  Symbolic sym;
+  PlanParams params;
+  int32_t n = 0, ns = 0;
+  std::vector<int32_t> sn_first, sn_ncol, sn_nrow, e_off, nb_eff;
+  std::vector<int64_t> panel_off, rs_off, snap_off;
+  std::vector<int32_t> snap_ld;
+  std::vector<int32_t> rs_idx;
+  std::vector<int32_t> upd_ptr, upd_idx;
+  std::vector<int32_t> level;   // level of each supernode (leaves 0)
+  std::vector<LevelPlan> levels;
+  std::vector<CItem> items;
+  std::vector<CTask> tasks;
+  int64_t panel_total = 0;      // elements in the panel pool
+  int64_t snap_total = 0;       // elements of multiplier scratch
+  // user lower-CSC entries -> panel slot
+  int64_t nnz = 0;
+  std::vector<int64_t> amap;
+  std::vector<int32_t> a_row, a_col;
+  // work figures (real-field flop convention; x4 for complex)
+  double factor_flops = 0;      // sum_s sum_{k<w} (h-k)^2
+  double geom_flops = 0;        // sum_s h^2 w (reference proxy)
+  int64_t nnzl_panel = 0;       // sum_s sum_{k<w} (h-k): stored L incl. diagonal
+};
+
+// Build the plan; colp/rowi is the user lower CSC the symbolic came from.
+void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanParams& prm);
+
+}  // namespace rsl

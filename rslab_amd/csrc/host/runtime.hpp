@@ -1,0 +1,114 @@
+// Host runtime: device plan upload, numeric factor and solve orchestration.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../device/devtypes.hpp"
+#include "../device/scalar.hpp"
+#include "plan.hpp"
+
+namespace rsl {
+
+// Status codes shared with the C ABI (rslab_amd.h).
+enum Status : int {
+  OK = 0,
+  NUMERICALLY_RANK_DEFICIENT = 1,
+  INVALID_INPUT = 2,
+  DIMENSION_MISMATCH = 3,
+  DEVICE_ERROR = 4,
+  OUT_OF_MEMORY = 5,
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// Owning device buffer.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf();
+  void alloc(size_t nbytes);
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+  template <class T>
+  void upload(const std::vector<T>& v, hipStream_t st) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) hip_check(hipMemcpyAsync(p, v.data(), bytes, hipMemcpyHostToDevice, st), "upload");
+  }
+};
+
+enum class Precision : int { FP64 = 0, FP32_SCHUR = 1, BF16_SCHUR = 2 };
+
+struct FactorSettings {
+  int pivot = 0;  // 0 Fail, 1 PerturbToEps, 2 ForceAccept
+  double abs_floor = 0.0;
+  int scaling = 1;  // 1 OnePassInfNorm, 0 Identity
+  Precision precision = Precision::FP64;
+};
+
+// Device-resident plan of one analysed pattern.
+struct DeviceSymbolic {
+  Plan plan;
+  hipStream_t stream = nullptr;
+  DBuf sn_first, sn_ncol, sn_nrow, e_off, panel_off, rs_off, rs_idx, upd_ptr, upd_idx, sym_perm,
+      snap_off, snap_ld, amap, a_row, a_col, items, tasks, level_lists;
+  std::vector<int64_t> level_all_off, level_small_off;  // offsets into level_lists
+  // user CSC (for SpMV in refinement)
+  DBuf u_colp, u_rowi;
+  std::vector<int64_t> h_colp;
+  std::vector<int32_t> h_rowi;
+  DevSym view{};
+  bool uploaded = false;
+  ~DeviceSymbolic();
+  void upload();
+};
+
+struct FactorStats {
+  double t_factor_ms = 0;
+  int64_t inertia[3] = {0, 0, 0};
+  int64_t n_perturbed = 0;
+};
+
+class DeviceFactorBase {
+ public:
+  virtual ~DeviceFactorBase() = default;
+  virtual int field() const = 0;  // 0 f64, 1 c64
+  virtual void refactor(const double* values) = 0;
+  virtual void solve_host(const double* b, int64_t nrhs, double* x) = 0;
+  virtual void solve_device(const void* d_b, int64_t nrhs, void* d_x, hipStream_t st) = 0;
+  virtual void download(std::vector<int64_t>& colp, std::vector<int32_t>& rowi,
+                        std::vector<double>& lval, std::vector<double>& d,
+                        std::vector<double>& ds, std::vector<uint8_t>& two,
+                        std::vector<int32_t>& perm) = 0;
+  // Mixed-precision refinement ladder (IR -> GMRES-IR) against the original
+  // matrix values; b, x row-major n x nrhs host buffers. info: per RHS
+  // {ir_iters, gmres_iters, backward_error, certified}.
+  virtual void refine(const double* a_values, const double* b, int64_t nrhs, double target,
+                      double* x, double* info) = 0;
+  FactorStats stats;
+  FactorSettings settings;
+  std::shared_ptr<DeviceSymbolic> sym;
+};
+
+std::unique_ptr<DeviceFactorBase> make_factor(std::shared_ptr<DeviceSymbolic> sym, int field,
+                                              const double* values, const FactorSettings& fs);
+
+// onepass_scale (src/numeric/sparse_solver.rs:399-418) on the host
+void onepass_scale(int32_t n, int field, const int64_t* colp, const int32_t* rowi,
+                   const double* val, std::vector<double>& scale);
+
+}  // namespace rsl

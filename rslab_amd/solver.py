@@ -1,0 +1,524 @@
+"""Host-side mirror of rslab's symmetric solver surface over the C ABI.
+
+Names, argument meaning and error behaviour follow the Rust API:
+
+* ``CscMatrix``            src/sparse/csc.rs:15-20 (lower triangle, rows sorted)
+* ``SolverSettings``       src/numeric/multifrontal_ldlt.rs:302-394 (+ builders :616-779)
+* ``LdltSymbolic``         src/numeric/sparse_solver.rs:479-712 (analyze / factor)
+* ``LdltSolver``           src/numeric/sparse_solver.rs:40-376 (solve / solve_many / inertia)
+* ``LdltFactors``          src/dense/ldlt_generic.rs:31-66 (downloaded CSC factor)
+* ``MixedLdltSolver``      src/numeric/mixed.rs:225-291 (IR -> GMRES-IR ladder)
+* ``RslabError`` family    src/error.rs:3-43
+
+Every numeric call runs on the GPU through librslab_amd.so; nothing here
+computes a factor or a solve on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import enum
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib, ptr
+
+
+# ---------------------------------------------------------------- errors
+class RslabError(Exception):
+    """Base error (RslabError)."""
+
+
+class NumericallyRankDeficient(RslabError):
+    pass
+
+
+class InvalidInput(RslabError):
+    pass
+
+
+class DimensionMismatch(RslabError):
+    pass
+
+
+class DeviceError(InvalidInput):
+    """rslab has no device variant; it reports InvalidInput("device: ...")."""
+
+
+class OutOfMemory(RslabError):
+    pass
+
+
+_ERR = {
+    _lib.RSL_NUMERICALLY_RANK_DEFICIENT: NumericallyRankDeficient,
+    _lib.RSL_INVALID_INPUT: InvalidInput,
+    _lib.RSL_DIMENSION_MISMATCH: DimensionMismatch,
+    _lib.RSL_DEVICE_ERROR: DeviceError,
+    _lib.RSL_OUT_OF_MEMORY: OutOfMemory,
+}
+
+
+def _check(rc: int):
+    if rc != _lib.RSL_OK:
+        msg = lib.rsl_last_error().decode(errors="replace")
+        raise _ERR.get(rc, RslabError)(msg)
+
+
+# ---------------------------------------------------------------- matrix
+@dataclasses.dataclass
+class CscMatrix:
+    """Lower-triangle CSC of a symmetric (or complex-symmetric) matrix."""
+
+    n: int
+    col_ptr: np.ndarray
+    row_idx: np.ndarray
+    values: np.ndarray
+
+    def __post_init__(self):
+        self.col_ptr = np.ascontiguousarray(self.col_ptr, dtype=np.int64)
+        self.row_idx = np.ascontiguousarray(self.row_idx, dtype=np.int64)
+        self.values = np.ascontiguousarray(self.values)
+
+    @property
+    def is_complex(self) -> bool:
+        return np.iscomplexobj(self.values)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_idx.shape[0])
+
+    @staticmethod
+    def from_triplets(n, rows, cols, vals) -> "CscMatrix":
+        """CscMatrix::from_triplets (csc.rs:41-106): lower entries, duplicates summed."""
+        rows = np.asarray(rows, dtype=np.int64)
+        cols = np.asarray(cols, dtype=np.int64)
+        vals = np.asarray(vals)
+        if not (rows.shape == cols.shape == vals.shape):
+            raise InvalidInput("triplet arrays must have equal length")
+        if rows.size and (cols.max() >= n or rows.max() >= n or cols.min() < 0 or rows.min() < 0):
+            raise InvalidInput("index out of bounds")
+        if np.any(rows < cols):
+            raise InvalidInput("triplet is upper-triangle; CscMatrix stores only the lower triangle")
+        key = cols * n + rows
+        order = np.argsort(key, kind="stable")
+        key = key[order]
+        uniq, start = np.unique(key, return_index=True)
+        summed = np.add.reduceat(vals[order], start) if uniq.size else vals[:0]
+        c = uniq // n
+        r = uniq % n
+        col_ptr = np.zeros(n + 1, dtype=np.int64)
+        np.add.at(col_ptr, c + 1, 1)
+        return CscMatrix(n, np.cumsum(col_ptr), r, summed)
+
+    def with_values(self, values) -> "CscMatrix":
+        return CscMatrix(self.n, self.col_ptr, self.row_idx, values)
+
+    def values_f64(self) -> np.ndarray:
+        """Values as the C ABI takes them (complex as interleaved re/im)."""
+        if self.is_complex:
+            return np.ascontiguousarray(self.values.astype(np.complex128)).view(np.float64)
+        return np.ascontiguousarray(self.values, dtype=np.float64)
+
+    def symv(self, x):
+        """CscMatrix::symv (csc.rs:294-309): y = A x from the stored lower part."""
+        cols = np.repeat(np.arange(self.n), np.diff(self.col_ptr))
+        rows = self.row_idx
+        v = self.values
+        x = np.asarray(x)
+        y = np.zeros((self.n,) + x.shape[1:], dtype=np.result_type(v, x))
+        vv = v.reshape((-1,) + (1,) * (x.ndim - 1))
+        np.add.at(y, rows, vv * x[cols])
+        off = rows != cols
+        np.add.at(y, cols[off], vv[off] * x[rows[off]])
+        return y
+
+    def norm_inf(self) -> float:
+        """sym_norm_inf (mixed.rs:358-371)."""
+        cols = np.repeat(np.arange(self.n), np.diff(self.col_ptr))
+        m = np.abs(self.values)
+        rs = np.zeros(self.n)
+        np.add.at(rs, self.row_idx, m)
+        off = self.row_idx != cols
+        np.add.at(rs, cols[off], m[off])
+        return float(rs.max()) if self.n else 0.0
+
+
+# ---------------------------------------------------------------- settings
+class OrderingMethod(enum.IntEnum):
+    Amd = 0
+    MetisND = 1
+    Natural = 2
+
+
+class Precision(enum.IntEnum):
+    """Precision of the Schur / contribution GEMMs; pivots, D and residuals stay FP64."""
+
+    FP64 = 0
+    FP32_SCHUR = 1
+    BF16_SCHUR = 2
+
+
+@dataclasses.dataclass
+class ZeroPivotAction:
+    """ZeroPivotAction (multifrontal_ldlt.rs:165-180)."""
+
+    kind: int = 0  # 0 Fail, 1 PerturbToEps, 2 ForceAccept
+    abs_floor: float = 0.0
+
+    @staticmethod
+    def Fail():
+        return ZeroPivotAction(0, 0.0)
+
+    @staticmethod
+    def PerturbToEps(abs_floor: float):
+        return ZeroPivotAction(1, float(abs_floor))
+
+    @staticmethod
+    def ForceAccept():
+        return ZeroPivotAction(2, 0.0)
+
+
+@dataclasses.dataclass
+class SolverSettings:
+    """SolverSettings with the defaults of multifrontal_ldlt.rs:583-614."""
+
+    ordering: OrderingMethod = OrderingMethod.Amd
+    nemin: int = 16
+    relax: Optional[tuple] = (256, 64)
+    panel_nb: int = 64
+    on_zero_pivot: ZeroPivotAction = dataclasses.field(default_factory=ZeroPivotAction.Fail)
+    scaling: str = "OnePassInfNorm"
+    precision: Precision = Precision.FP64
+    nd_seed: int = 1
+    wide_flops: float = 4.0e6
+
+    # builders (multifrontal_ldlt.rs:616-779)
+    def with_ordering(self, o):
+        return dataclasses.replace(self, ordering=OrderingMethod(o))
+
+    def with_nemin(self, n):
+        return dataclasses.replace(self, nemin=int(n))
+
+    def with_relax(self, r):
+        return dataclasses.replace(self, relax=r)
+
+    def with_panel_nb(self, nb):
+        return dataclasses.replace(self, panel_nb=int(nb))
+
+    def with_pivot(self, p: ZeroPivotAction):
+        return dataclasses.replace(self, on_zero_pivot=p)
+
+    def with_scaling(self, s: str):
+        return dataclasses.replace(self, scaling=s)
+
+    def with_precision(self, p):
+        return dataclasses.replace(self, precision=Precision(p))
+
+    @staticmethod
+    def preconditioner(abs_floor: float) -> "SolverSettings":
+        return SolverSettings(on_zero_pivot=ZeroPivotAction.PerturbToEps(abs_floor))
+
+    def to_c(self) -> _lib.Settings:
+        s = _lib.Settings()
+        lib.rsl_settings_default(C.byref(s))
+        s.ordering = int(self.ordering)
+        s.nemin = int(self.nemin)
+        if self.relax is None:
+            s.relax = 0
+        else:
+            s.relax = 1
+            s.relax_width, s.relax_rows = int(self.relax[0]), int(self.relax[1])
+        s.panel_nb = int(self.panel_nb)
+        s.pivot = int(self.on_zero_pivot.kind)
+        s.abs_floor = float(self.on_zero_pivot.abs_floor)
+        if self.scaling not in ("OnePassInfNorm", "Identity"):
+            raise InvalidInput(f"scaling strategy {self.scaling!r} is not supported by the device path")
+        s.scaling = 1 if self.scaling == "OnePassInfNorm" else 0
+        s.precision = int(self.precision)
+        s.nd_seed = int(self.nd_seed)
+        s.wide_flops = float(self.wide_flops)
+        return s
+
+
+# ---------------------------------------------------------------- results
+@dataclasses.dataclass
+class Inertia:
+    positive: int
+    negative: int
+    zero: int
+
+    def total(self) -> int:
+        return self.positive + self.negative + self.zero
+
+    def as_tuple(self):
+        return (self.positive, self.negative, self.zero)
+
+
+@dataclasses.dataclass
+class LdltFactors:
+    """LdltFactors (ldlt_generic.rs:31-66), elimination order."""
+
+    n: int
+    l_col_ptr: np.ndarray
+    l_row_idx: np.ndarray
+    l_values: np.ndarray
+    d_diag: np.ndarray
+    d_subdiag: np.ndarray
+    two_by_two: np.ndarray
+    perm: np.ndarray
+    n_perturbed: int
+    inertia: Inertia
+
+
+@dataclasses.dataclass
+class MixedInfo:
+    """MixedInfo (mixed.rs:81-92)."""
+
+    ir_iters: int
+    gmres_iters: int
+    backward_error: float
+    certified: bool
+
+
+@dataclasses.dataclass
+class SymbolicExport:
+    n: int
+    perm: np.ndarray
+    perm_inv: np.ndarray
+    sn_first: np.ndarray
+    sn_ncol: np.ndarray
+    sn_nrow: np.ndarray
+    ch_ptr: np.ndarray
+    ch_idx: np.ndarray
+    pp_colp: np.ndarray
+    pp_rowi: np.ndarray
+
+
+# ---------------------------------------------------------------- symbolic
+class LdltSymbolic:
+    """Phase-1 analysis reused across numeric factorizations."""
+
+    def __init__(self, handle, n: int, nnz: int, settings: SolverSettings):
+        self._h = handle
+        self._n = n
+        self._nnz = nnz
+        self.settings = settings
+
+    @staticmethod
+    def analyze(a: CscMatrix) -> "LdltSymbolic":
+        return LdltSymbolic.analyze_with(a, SolverSettings())
+
+    @staticmethod
+    def analyze_with(a: CscMatrix, opts: SolverSettings) -> "LdltSymbolic":
+        h = C.c_void_p()
+        s = opts.to_c()
+        _check(lib.rsl_analyze(a.n, ptr(a.col_ptr, C.c_int64), ptr(a.row_idx, C.c_int64),
+                               C.byref(s), C.byref(h)))
+        return LdltSymbolic(h, a.n, a.nnz, opts)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.rsl_free_symbolic(h)
+            self._h = None
+
+    def n(self) -> int:
+        return self._n
+
+    def info(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        fl = np.zeros(2, dtype=np.float64)
+        _check(lib.rsl_symbolic_info(self._h, ptr(out, C.c_int64), ptr(fl, C.c_double)))
+        return dict(n=int(out[0]), nsuper=int(out[1]), colcount_nnz=int(out[2]),
+                    panel_entries=int(out[3]), levels=int(out[4]), children=int(out[5]),
+                    pattern_nnz=int(out[6]), nnzl_padded=int(out[7]),
+                    factor_flops=float(fl[0]), geom_flops=float(fl[1]))
+
+    def export(self) -> SymbolicExport:
+        inf = self.info()
+        n, ns = inf["n"], inf["nsuper"]
+        arr = lambda k, dt=np.int32: np.zeros(k, dtype=dt)
+        e = SymbolicExport(n, arr(n), arr(n), arr(ns), arr(ns), arr(ns), arr(ns + 1),
+                           arr(max(inf["children"], 1)), arr(n + 1, np.int64),
+                           arr(max(inf["pattern_nnz"], 1)))
+        _check(lib.rsl_symbolic_export(
+            self._h, ptr(e.perm, C.c_int32), ptr(e.perm_inv, C.c_int32), ptr(e.sn_first, C.c_int32),
+            ptr(e.sn_ncol, C.c_int32), ptr(e.sn_nrow, C.c_int32), ptr(e.ch_ptr, C.c_int32),
+            ptr(e.ch_idx, C.c_int32), ptr(e.pp_colp, C.c_int64), ptr(e.pp_rowi, C.c_int32)))
+        return e
+
+    def factor(self, a: CscMatrix, opts: Optional[SolverSettings] = None) -> "LdltSolver":
+        """LdltSymbolic::factor: equilibrate, then the device numeric factor."""
+        opts = opts or self.settings
+        if a.n != self._n or a.nnz != self._nnz:
+            raise InvalidInput("factor_numeric: matrix does not match the analyzed pattern")
+        field = 1 if a.is_complex else 0
+        h = C.c_void_p()
+        s = opts.to_c()
+        vals = a.values_f64()
+        _check(lib.rsl_factor(self._h, field, ptr(vals, C.c_double), C.byref(s), C.byref(h)))
+        return LdltSolver(h, self, a.n, a.is_complex)
+
+
+# ---------------------------------------------------------------- solver
+class LdltSolver:
+    """A factored matrix on the GPU (drop-in for LdltSolver<T>)."""
+
+    def __init__(self, handle, sym: LdltSymbolic, n: int, cplx: bool):
+        self._h = handle
+        self._sym = sym  # keeps the plan alive
+        self._n = n
+        self._cplx = cplx
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.rsl_free_factor(h)
+            self._h = None
+
+    @staticmethod
+    def factor_with(a: CscMatrix, opts: Optional[SolverSettings] = None) -> "LdltSolver":
+        opts = opts or SolverSettings()
+        return LdltSymbolic.analyze_with(a, opts).factor(a, opts)
+
+    def n(self) -> int:
+        return self._n
+
+    def refactor(self, a: CscMatrix):
+        """Same-pattern numeric refactorization (plan + device storage reused)."""
+        vals = a.values_f64()
+        _check(lib.rsl_refactor(self._h, ptr(vals, C.c_double)))
+
+    def _dtype(self):
+        return np.complex128 if self._cplx else np.float64
+
+    def _as_c(self, b):
+        b = np.ascontiguousarray(np.asarray(b, dtype=self._dtype()))
+        return b, (b.view(np.float64) if self._cplx else b)
+
+    def solve(self, rhs) -> np.ndarray:
+        rhs = np.asarray(rhs)
+        if rhs.ndim != 1 or rhs.shape[0] != self._n:
+            raise DimensionMismatch(f"expected {self._n}, got {rhs.shape}")
+        return self.solve_many(rhs.reshape(self._n, 1), 1).reshape(self._n)
+
+    def solve_many(self, b, nrhs: Optional[int] = None) -> np.ndarray:
+        """b row-major n x nrhs (solve_many, sparse_solver.rs:302-327)."""
+        b = np.asarray(b)
+        if b.ndim == 1:
+            if nrhs is None or b.shape[0] != self._n * nrhs:
+                raise DimensionMismatch(f"expected {self._n * (nrhs or 1)}, got {b.shape[0]}")
+            b = b.reshape(self._n, nrhs)
+        if b.shape[0] != self._n or (nrhs is not None and b.shape[1] != nrhs) or b.shape[1] == 0:
+            raise DimensionMismatch(f"expected {self._n} x {nrhs}, got {b.shape}")
+        k = b.shape[1]
+        bb, bv = self._as_c(b)
+        x = np.zeros_like(bb)
+        xv = x.view(np.float64) if self._cplx else x
+        _check(lib.rsl_solve(self._h, ptr(bv, C.c_double), k, ptr(xv, C.c_double)))
+        return x
+
+    def inertia(self) -> Inertia:
+        i = np.zeros(3, dtype=np.int64)
+        _check(lib.rsl_factor_info(self._h, ptr(i, C.c_int64), None, None))
+        return Inertia(int(i[0]), int(i[1]), int(i[2]))
+
+    def n_perturbed(self) -> int:
+        v = C.c_int64()
+        _check(lib.rsl_factor_info(self._h, None, C.byref(v), None))
+        return int(v.value)
+
+    def factor_time_ms(self) -> float:
+        v = C.c_double()
+        _check(lib.rsl_factor_info(self._h, None, None, C.byref(v)))
+        return float(v.value)
+
+    def factor_nnz(self) -> int:
+        v = C.c_int64()
+        _check(lib.rsl_factor_nnz(self._h, C.byref(v)))
+        return int(v.value)
+
+    def factors(self) -> LdltFactors:
+        nnz = self.factor_nnz()
+        n = self._n
+        nd = 2 if self._cplx else 1
+        colp = np.zeros(n + 1, dtype=np.int64)
+        rowi = np.zeros(max(nnz, 1), dtype=np.int32)
+        lv = np.zeros(max(nnz, 1) * nd, dtype=np.float64)
+        dd = np.zeros(max(n, 1) * nd, dtype=np.float64)
+        ds = np.zeros(max(n, 1) * nd, dtype=np.float64)
+        tw = np.zeros(max(n, 1), dtype=np.uint8)
+        pm = np.zeros(max(n, 1), dtype=np.int32)
+        _check(lib.rsl_factor_download(self._h, ptr(colp, C.c_int64), ptr(rowi, C.c_int32),
+                                       ptr(lv, C.c_double), ptr(dd, C.c_double), ptr(ds, C.c_double),
+                                       ptr(tw, C.c_uint8), ptr(pm, C.c_int32)))
+        cv = (lambda a: a.view(np.complex128)) if self._cplx else (lambda a: a)
+        return LdltFactors(n, colp, rowi[:nnz], cv(lv)[:nnz], cv(dd)[:n], cv(ds)[:n],
+                           tw[:n].astype(bool), pm[:n], self.n_perturbed(), self.inertia())
+
+
+# ---------------------------------------------------------------- mixed
+CERT_FACTOR = 64.0  # mixed.rs:97-105
+EPS_HI = float(np.finfo(np.float64).eps)
+
+
+class MixedLdltSolver:
+    """Reduced-precision Schur factor refined to FP64 against the original A."""
+
+    def __init__(self, lo: LdltSolver, a: CscMatrix):
+        self.lo = lo
+        self._a = a
+        self.anorm_inf = a.norm_inf()
+
+    @staticmethod
+    def factor_with(a: CscMatrix, opts: Optional[SolverSettings] = None) -> "MixedLdltSolver":
+        opts = opts or SolverSettings().with_precision(Precision.FP32_SCHUR)
+        if opts.precision == Precision.FP64:
+            opts = opts.with_precision(Precision.FP32_SCHUR)
+        return MixedLdltSolver(LdltSolver.factor_with(a, opts), a)
+
+    def solve(self, a: CscMatrix, b):
+        return self.solve_to(a, b, CERT_FACTOR * EPS_HI)
+
+    def solve_to(self, a: CscMatrix, b, target_backward_error: float):
+        b = np.asarray(b)
+        single = b.ndim == 1
+        if a.n != self.lo.n() or b.shape[0] != self.lo.n():
+            raise DimensionMismatch(f"expected {self.lo.n()}, got {a.n if a.n != self.lo.n() else b.shape[0]}")
+        B = b.reshape(a.n, -1)
+        k = B.shape[1]
+        bb, bv = self.lo._as_c(B)
+        x = np.zeros_like(bb)
+        xv = x.view(np.float64) if self.lo._cplx else x
+        info = np.zeros(4 * k, dtype=np.float64)
+        vals = a.values_f64()
+        _check(lib.rsl_refine(self.lo._h, ptr(vals, C.c_double), ptr(bv, C.c_double), k,
+                              float(target_backward_error), ptr(xv, C.c_double), ptr(info, C.c_double)))
+        infos = [MixedInfo(int(info[4 * c]), int(info[4 * c + 1]), float(info[4 * c + 2]),
+                           bool(info[4 * c + 3] != 0)) for c in range(k)]
+        if single:
+            return x.reshape(a.n), infos[0]
+        return x, infos
+
+
+def amd_order(n, col_ptr, row_idx):
+    """Stand-alone AMD permutation (new->old) of a full symmetric pattern, plus stats."""
+    col_ptr = np.ascontiguousarray(col_ptr, dtype=np.int64)
+    row_idx = np.ascontiguousarray(row_idx, dtype=np.int64)
+    perm = np.zeros(max(n, 1), dtype=np.int32)
+    st = np.zeros(7, dtype=np.float64)
+    _check(lib.rsl_order(0, n, ptr(col_ptr, C.c_int64), ptr(row_idx, C.c_int64), 0,
+                         ptr(perm, C.c_int32), ptr(st, C.c_double)))
+    keys = ["ncmpa", "ndense", "lnz", "ndiv", "nms_lu", "nms_ldl", "dmax"]
+    return perm[:n], dict(zip(keys, st.tolist()))
+
+
+def nd_order(n, col_ptr, row_idx, seed: int = 1):
+    col_ptr = np.ascontiguousarray(col_ptr, dtype=np.int64)
+    row_idx = np.ascontiguousarray(row_idx, dtype=np.int64)
+    perm = np.zeros(max(n, 1), dtype=np.int32)
+    _check(lib.rsl_order(1, n, ptr(col_ptr, C.c_int64), ptr(row_idx, C.c_int64), seed,
+                         ptr(perm, C.c_int32), None))
+    return perm[:n]
