@@ -20,6 +20,9 @@ struct DevSym {
   const int32_t* sym_perm;   // permuted -> user index
   const int64_t* snap_off;   // scratch offset (elements) for per-panel multipliers
   const int32_t* snap_ld;    // leading dimension of that scratch (min(nb, w))
+  const int64_t* u_off;      // solve scratch row offset of supernode s (h - w rows)
+  const int64_t* gat_ptr;    // per permuted column: scratch rows to gather (ascending updater)
+  const int64_t* gat_idx;
   const int64_t* amap;       // user nnz -> panel offset
   const int32_t* a_row;      // user nnz -> user row
   const int32_t* a_col;      // user nnz -> user column

@@ -659,17 +659,62 @@ __global__ __launch_bounds__(NT) void k_panel_getf2(DevSym S, DevFac<T> Fc, int 
   if (threadIdx.x == 0 && sh.npert) atomicAdd(&Fc.counters[3], (unsigned long long)sh.npert);
 }
 
+// Deep-row replay with the rows staged in LDS: one row per thread (column-major
+// chunk, so a wave's accesses are conflict-free), the reference's per-row step
+// sequence (apply_bk_panel_trailing) replayed on the staged copy.
+constexpr int RR = 64;  // rows per workgroup = threads per workgroup
+
 template <class T>
-__global__ __launch_bounds__(NT) void k_panel_replay(DevSym S, DevFac<T> Fc, int s, int kb,
+__global__ __launch_bounds__(RR) void k_panel_replay(DevSym S, DevFac<T> Fc, int s, int kb,
                                                      int nb) {
+  extern __shared__ unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR]
   __shared__ int32_t deep_s[NBMAX];
   SnCtx<T> c = make_ctx(S, Fc, s, nb);
-  const int ke = min(kb + nb, c.w);
-  for (int q = threadIdx.x; q < ke - kb; q += NT) deep_s[q] = c.deep[q];
+  const int ke = min(kb + nb, c.w), pw = ke - kb;
+  const int t = threadIdx.x;
+  for (int q = t; q < pw; q += RR) deep_s[q] = c.deep[q];
+  const int64_t i = ke + (int64_t)blockIdx.x * RR + t;
+  const bool live = i < c.h;
+  if (live)
+    for (int j = 0; j < pw; ++j) X[j * RR + t] = c.P[i + (int64_t)(kb + j) * c.h];
   __syncthreads();
-  const int64_t r0 = ke + (int64_t)blockIdx.x * NT;
-  const int64_t r1 = min<int64_t>(r0 + NT, c.h);
-  panel_replay_rows(c, kb, ke, deep_s, r0, r1, threadIdx.x, NT);
+  if (live) {
+    const int sld = c.snap_ld;
+    int k = 0;
+    while (k < pw) {
+      const int kp = deep_s[k];
+      const bool two = c.two[kb + k];
+      if (kp >= 0) {
+        const int src = two ? k + 1 : k;
+        T x = X[src * RR + t];
+        X[src * RR + t] = X[kp * RR + t];
+        X[kp * RR + t] = x;
+      }
+      if (two) {
+        const T d11 = c.d[kb + k], d21 = c.ds[kb + k], d22 = c.d[kb + k + 1];
+        const T detinv = F<T>::recip(d11 * d22 - d21 * d21);
+        const T wk = X[k * RR + t], wk1 = X[(k + 1) * RR + t];
+        const T* m1 = c.snap + (int64_t)k * sld;
+        const T* m2 = c.snap + (int64_t)(k + 1) * sld;
+        for (int j = k + 2; j < pw; ++j) X[j * RR + t] = X[j * RR + t] - wk * m1[j] - wk1 * m2[j];
+        X[k * RR + t] = (d22 * wk - d21 * wk1) * detinv;
+        X[(k + 1) * RR + t] = (d11 * wk1 - d21 * wk) * detinv;
+        k += 2;
+      } else {
+        const T dinv = F<T>::recip(c.d[kb + k]);
+        const T wk = X[k * RR + t];
+        const T* m = c.snap + (int64_t)k * sld;
+        for (int j = k + 1; j < pw; ++j) {
+          const T mj = m[j];
+          if (!F<T>::is_zero(mj)) X[j * RR + t] = X[j * RR + t] - wk * mj;
+        }
+        X[k * RR + t] = wk * dinv;
+        k += 1;
+      }
+    }
+    for (int j = 0; j < pw; ++j) c.P[i + (int64_t)(kb + j) * c.h] = X[j * RR + t];
+  }
 }
 
 template <class T>
@@ -740,8 +785,8 @@ void launch_cdiv_wide(const DevSym& S, const DevFac<T>& Fc, int s, int w, int64_
                        pol);
     const int64_t mt = h - ke;
     if (mt > 0)
-      hipLaunchKernelGGL(k_panel_replay<T>, dim3((unsigned)((mt + NT - 1) / NT)), dim3(NT), 0, st,
-                         S, Fc, s, kb, nb);
+      hipLaunchKernelGGL(k_panel_replay<T>, dim3((unsigned)((mt + RR - 1) / RR)), dim3(RR),
+                         sizeof(T) * RR * (ke - kb), st, S, Fc, s, kb, nb);
     const int cw = w - ke;
     if (cw > 0 && mt > 0) {
       unsigned nrt = (unsigned)((mt + TM - 1) / TM), nct = (unsigned)((cw + TN - 1) / TN);
@@ -756,6 +801,9 @@ hipError_t configure_factor_kernels() {
   hipError_t e = hipFuncSetAttribute((const void*)k_cdiv_small<T>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)cdiv_smem<T>());
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_panel_replay<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)(sizeof(T) * RR * NBMAX));
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_panel_getf2<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdiv_smem<T>());

@@ -45,53 +45,230 @@ __global__ void k_scatter_sol(int32_t n, int32_t nrhs, const int32_t* __restrict
 }
 
 template <class T>
-__device__ inline int lower_bound_i32(const int32_t* a, int n, int32_t v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1;
-    else hi = mid;
+__device__ inline T shfl_t(T v, int src) {
+  if constexpr (sizeof(T) == sizeof(double)) {
+    return __shfl(v, src, 64);
+  } else {
+    cplx o;
+    o.re = __shfl(v.re, src, 64);
+    o.im = __shfl(v.im, src, 64);
+    return o;
   }
-  return lo;
 }
 
-// Forward sweep for the supernodes of one level.
 template <class T>
-__global__ __launch_bounds__(SNT) void k_forward(DevSym S, DevFac<T> Fc,
+__device__ inline T shfl_xor_t(T v, int m) {
+  if constexpr (sizeof(T) == sizeof(double)) {
+    return __shfl_xor(v, m, 64);
+  } else {
+    cplx o;
+    o.re = __shfl_xor(v.re, m, 64);
+    o.im = __shfl_xor(v.im, m, 64);
+    return o;
+  }
+}
+
+// L entry of the own block, with the 2x2 coupling row (held in D) read as zero.
+template <class T>
+__device__ inline T l_own(const T* P, int64_t h, const uint8_t* two, int i, int p) {
+  if (i == p + 1 && two[p]) return F<T>::zero();
+  return P[i + (int64_t)p * h];
+}
+
+constexpr int TB = 64;  // triangular block
+
+// Forward, part 1 (one workgroup per supernode of the level): gather the
+// descendants' contributions to this supernode's columns (ascending updater
+// order, precomputed), then solve the unit-lower own block in pivoted order.
+template <class T>
+__global__ __launch_bounds__(SNT) void k_fwd_tri(DevSym S, DevFac<T> Fc,
                                                  const int32_t* __restrict__ list, int32_t nrhs,
-                                                 T* __restrict__ y) {
+                                                 T* __restrict__ y, const T* __restrict__ u) {
+  __shared__ T Ld[TB * TB];
   const int s = list[blockIdx.x];
   const int w = S.sn_ncol[s], first = S.sn_first[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
   const T* P = Fc.pool + S.panel_off[s];
-  // (a) pull the updates of every descendant updater, ascending
-  for (int u = S.upd_ptr[s]; u < S.upd_ptr[s + 1]; ++u) {
-    const int k = S.upd_idx[u], nck = S.sn_ncol[k], eok = S.e_off[k];
-    const int64_t hk = S.sn_nrow[k];
-    const int32_t* okg = S.rs_idx + S.rs_off[k] + nck;
-    const int nok = (int)(hk - nck);
-    const int p0 = lower_bound_i32<T>(okg, nok, first);
-    const int p1 = lower_bound_i32<T>(okg, nok, first + w);
-    const T* Pk = Fc.pool + S.panel_off[k];
-    const int np = p1 - p0;
-    for (int64_t t = threadIdx.x; t < (int64_t)np * nrhs; t += SNT) {
-      const int r = (int)(t % np), c = (int)(t / np);
-      T acc = F<T>::zero();
-      for (int j = 0; j < nck; ++j)
-        acc = acc + Pk[(nck + p0 + r) + (int64_t)j * hk] * y[(int64_t)(eok + j) * nrhs + c];
-      const int e = Fc.e_of_g[okg[p0 + r]];
-      y[(int64_t)e * nrhs + c] = y[(int64_t)e * nrhs + c] - acc;
+  const uint8_t* two = Fc.two + eo;
+  for (int t = threadIdx.x; t < w * nrhs; t += SNT) {
+    const int gl = t % w, c = t / w;
+    const int g = first + gl;
+    const int64_t e = Fc.e_of_g[g];
+    T acc = y[e * nrhs + c];
+    for (int64_t q = S.gat_ptr[g]; q < S.gat_ptr[g + 1]; ++q) acc = acc - u[S.gat_idx[q] * nrhs + c];
+    y[e * nrhs + c] = acc;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < w; j0 += TB) {
+    const int nbk = min(TB, w - j0);
+    for (int t = threadIdx.x; t < TB * TB; t += SNT) {
+      const int i = t % TB, p = t / TB;
+      Ld[t] = (p < nbk && i < nbk && i > p) ? l_own(P, h, two, j0 + i, j0 + p) : F<T>::zero();
+    }
+    __syncthreads();
+    for (int c = wv; c < nrhs; c += SNT / 64) {
+      T yi = lane < nbk ? y[(int64_t)(eo + j0 + lane) * nrhs + c] : F<T>::zero();
+      for (int p = 0; p < nbk; ++p) {
+        const T yp = shfl_t(yi, p);
+        yi = yi - Ld[p * TB + lane] * yp;
+      }
+      if (lane < nbk) y[(int64_t)(eo + j0 + lane) * nrhs + c] = yi;
+    }
+    __syncthreads();
+    // rows below the block inside the own block
+    for (int t = threadIdx.x; t < (w - j0 - nbk) * nrhs; t += SNT) {
+      const int nr = w - j0 - nbk;
+      const int i = j0 + nbk + t % nr, c = t / nr;
+      T acc = y[(int64_t)(eo + i) * nrhs + c];
+      for (int p = 0; p < nbk; ++p)
+        acc = acc - l_own(P, h, two, i, j0 + p) * y[(int64_t)(eo + j0 + p) * nrhs + c];
+      y[(int64_t)(eo + i) * nrhs + c] = acc;
     }
     __syncthreads();
   }
-  // (b) own unit-lower block in pivoted order
-  for (int p = 0; p < w; ++p) {
-    const int i0 = Fc.two[eo + p] ? p + 2 : p + 1;
-    const int nr = w - i0;
-    for (int t = threadIdx.x; t < nr * nrhs; t += SNT) {
-      const int i = i0 + t % nr, c = t / nr;
-      y[(int64_t)(eo + i) * nrhs + c] =
-          y[(int64_t)(eo + i) * nrhs + c] - P[i + (int64_t)p * h] * y[(int64_t)(eo + p) * nrhs + c];
+}
+
+// Forward, part 2: u_s = L21 y_own for 64-row chunks of the off-diagonal rows
+// (items are (supernode, row offset) pairs of one level).
+template <class T>
+__global__ __launch_bounds__(SNT) void k_fwd_update(DevSym S, DevFac<T> Fc,
+                                                    const int2* __restrict__ items, int32_t nrhs,
+                                                    const T* __restrict__ y, T* __restrict__ u) {
+  __shared__ T part[4][64][8];
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, r0 = it.y;
+  const int w = S.sn_ncol[s], eo = S.e_off[s];
+  const int64_t h = S.sn_nrow[s];
+  const T* P = Fc.pool + S.panel_off[s];
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t row = w + r0 + r;
+  const bool live = row < h;
+  const int64_t uo = S.u_off[s] + r0 + r;
+  for (int c0 = 0; c0 < nrhs; c0 += 8) {
+    const int nc = min(8, nrhs - c0);
+    T acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = F<T>::zero();
+    if (live) {
+      for (int p = q; p < w; p += 4) {
+        const T l = P[row + (int64_t)p * h];
+        const T* yp = y + (int64_t)(eo + p) * nrhs + c0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < nc) acc[k] = acc[k] + l * yp[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) part[q][r][k] = acc[k];
+    __syncthreads();
+    if (q == 0 && live) {
+      for (int k = 0; k < nc; ++k) {
+        T v = part[0][r][k] + part[1][r][k];
+        v = v + part[2][r][k];
+        v = v + part[3][r][k];
+        u[uo * nrhs + c0 + k] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Backward, part 1: y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]] for 16-column
+// chunks (items (supernode, column offset)); one wave per column.
+template <class T>
+__global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
+                                                    const int2* __restrict__ items, int32_t nrhs,
+                                                    T* __restrict__ y) {
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, p0 = it.y;
+  const int w = S.sn_ncol[s], eo = S.e_off[s];
+  const int64_t h = S.sn_nrow[s];
+  const T* P = Fc.pool + S.panel_off[s];
+  const int32_t* rs = S.rs_idx + S.rs_off[s];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int p = p0 + wv; p < min(p0 + 16, w); p += SNT / 64) {
+    for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+      const int nc = min(RCH, nrhs - c0);
+      T acc[RCH];
+#pragma unroll
+      for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
+      for (int64_t i = w + lane; i < h; i += 64) {
+        const T l = P[i + (int64_t)p * h];
+        const T* xs = y + (int64_t)Fc.e_of_g[rs[i]] * nrhs + c0;
+#pragma unroll
+        for (int k = 0; k < RCH; ++k)
+          if (k < nc) acc[k] = acc[k] + l * xs[k];
+      }
+#pragma unroll
+      for (int k = 0; k < RCH; ++k) {
+        T v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v = v + shfl_xor_t(v, o);
+        if (lane == 0 && k < nc) {
+          const int64_t o2 = (int64_t)(eo + p) * nrhs + c0 + k;
+          y[o2] = y[o2] - v;
+        }
+      }
+    }
+  }
+}
+
+// Backward, part 2 (one workgroup per supernode): back-substitute the own
+// block, L11^T x = y, from the last 64-block up.
+template <class T>
+__global__ __launch_bounds__(SNT) void k_bwd_tri(DevSym S, DevFac<T> Fc,
+                                                 const int32_t* __restrict__ list, int32_t nrhs,
+                                                 T* __restrict__ y) {
+  __shared__ T Lt[TB * TB];
+  const int s = list[blockIdx.x];
+  const int w = S.sn_ncol[s], eo = S.e_off[s];
+  const int64_t h = S.sn_nrow[s];
+  const T* P = Fc.pool + S.panel_off[s];
+  const uint8_t* two = Fc.two + eo;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nblk = (w + TB - 1) / TB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int j0 = b * TB, nbk = min(TB, w - j0), jend = j0 + nbk;
+    // contributions of the already solved own rows below this block
+    if (jend < w) {
+      for (int p = j0 + wv; p < jend; p += SNT / 64) {
+        for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+          const int nc = min(RCH, nrhs - c0);
+          T acc[RCH];
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
+          for (int i = jend + lane; i < w; i += 64) {
+            const T l = l_own(P, h, two, i, p);
+            const T* xs = y + (int64_t)(eo + i) * nrhs + c0;
+#pragma unroll
+            for (int k = 0; k < RCH; ++k)
+              if (k < nc) acc[k] = acc[k] + l * xs[k];
+          }
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) {
+            T v = acc[k];
+            for (int o = 32; o > 0; o >>= 1) v = v + shfl_xor_t(v, o);
+            if (lane == 0 && k < nc) {
+              const int64_t o2 = (int64_t)(eo + p) * nrhs + c0 + k;
+              y[o2] = y[o2] - v;
+            }
+          }
+        }
+      }
+    }
+    // transposed diagonal block: Lt[p*TB + i] = L[j0+p][j0+i] for p > i
+    for (int t = threadIdx.x; t < TB * TB; t += SNT) {
+      const int i = t % TB, p = t / TB;
+      Lt[t] = (p < nbk && i < nbk && p > i) ? l_own(P, h, two, j0 + p, j0 + i) : F<T>::zero();
+    }
+    __syncthreads();
+    for (int c = wv; c < nrhs; c += SNT / 64) {
+      T yi = lane < nbk ? y[(int64_t)(eo + j0 + lane) * nrhs + c] : F<T>::zero();
+      for (int p = nbk - 1; p >= 0; --p) {
+        const T xp = shfl_t(yi, p);
+        yi = yi - Lt[p * TB + lane] * xp;
+      }
+      if (lane < nbk) y[(int64_t)(eo + j0 + lane) * nrhs + c] = yi;
     }
     __syncthreads();
   }
@@ -115,92 +292,6 @@ __global__ void k_dsolve(int32_t n, int32_t nrhs, const T* __restrict__ d,
   }
 }
 
-// Backward sweep for the supernodes of one level (ancestors already solved).
-template <class T>
-__global__ __launch_bounds__(SNT) void k_backward(DevSym S, DevFac<T> Fc,
-                                                  const int32_t* __restrict__ list,
-                                                  int32_t nrhs, T* __restrict__ y) {
-  __shared__ T red[SNT / 64][RCH];
-  const int s = list[blockIdx.x];
-  const int w = S.sn_ncol[s], eo = S.e_off[s];
-  const int64_t h = S.sn_nrow[s];
-  const T* P = Fc.pool + S.panel_off[s];
-  const int32_t* rs = S.rs_idx + S.rs_off[s];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // (a) off-diagonal rows: y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]]  (wave per p)
-  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
-    const int nc = min(RCH, nrhs - c0);
-    for (int p = wv; p < w; p += SNT / 64) {
-      T acc[RCH];
-#pragma unroll
-      for (int q = 0; q < RCH; ++q) acc[q] = F<T>::zero();
-      for (int64_t i = w + lane; i < h; i += 64) {
-        const T l = P[i + (int64_t)p * h];
-        const int64_t e = Fc.e_of_g[rs[i]];
-#pragma unroll
-        for (int q = 0; q < RCH; ++q)
-          if (q < nc) acc[q] = acc[q] + l * y[e * nrhs + c0 + q];
-      }
-#pragma unroll
-      for (int q = 0; q < RCH; ++q) {
-        T v = acc[q];
-        for (int o = 32; o > 0; o >>= 1) {
-          if constexpr (sizeof(T) == sizeof(double)) {
-            v = v + __shfl_xor(v, o, 64);
-          } else {
-            cplx ov;
-            ov.re = __shfl_xor(v.re, o, 64);
-            ov.im = __shfl_xor(v.im, o, 64);
-            v = v + ov;
-          }
-        }
-        if (lane == 0 && q < nc)
-          y[(int64_t)(eo + p) * nrhs + c0 + q] = y[(int64_t)(eo + p) * nrhs + c0 + q] - v;
-      }
-    }
-  }
-  __syncthreads();
-  // (b) own block: x_p = y_p - sum_{i>p, not the 2x2 coupling row} L[i][p] x_i
-  for (int p = w - 1; p >= 0; --p) {
-    const int i0 = Fc.two[eo + p] ? p + 2 : p + 1;
-    for (int c0 = 0; c0 < nrhs; c0 += RCH) {
-      const int nc = min(RCH, nrhs - c0);
-      T acc[RCH];
-#pragma unroll
-      for (int q = 0; q < RCH; ++q) acc[q] = F<T>::zero();
-      for (int i = i0 + threadIdx.x; i < w; i += SNT) {
-        const T l = P[i + (int64_t)p * h];
-#pragma unroll
-        for (int q = 0; q < RCH; ++q)
-          if (q < nc) acc[q] = acc[q] + l * y[(int64_t)(eo + i) * nrhs + c0 + q];
-      }
-#pragma unroll
-      for (int q = 0; q < RCH; ++q) {
-        T v = acc[q];
-        for (int o = 32; o > 0; o >>= 1) {
-          if constexpr (sizeof(T) == sizeof(double)) {
-            v = v + __shfl_xor(v, o, 64);
-          } else {
-            cplx ov;
-            ov.re = __shfl_xor(v.re, o, 64);
-            ov.im = __shfl_xor(v.im, o, 64);
-            v = v + ov;
-          }
-        }
-        if (lane == 0) red[wv][q] = v;
-      }
-      __syncthreads();
-      if (threadIdx.x < nc) {
-        T v = red[0][threadIdx.x];
-        for (int q = 1; q < SNT / 64; ++q) v = v + red[q][threadIdx.x];
-        const int64_t o = (int64_t)(eo + p) * nrhs + c0 + threadIdx.x;
-        y[o] = y[o] - v;
-      }
-      __syncthreads();
-    }
-  }
-}
-
 // ---- launchers ----
 template <class T>
 void launch_gather_rhs(int32_t n, int32_t nrhs, const int32_t* perm_e, const double* scale,
@@ -219,10 +310,16 @@ void launch_scatter_sol(int32_t n, int32_t nrhs, const int32_t* perm_e, const do
                      nrhs, perm_e, scale, y, x);
 }
 template <class T>
-void launch_forward(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
-                    int32_t nrhs, T* y, hipStream_t st) {
+void launch_fwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
+                    int32_t nrhs, T* y, const T* u, hipStream_t st) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_forward<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y);
+  hipLaunchKernelGGL(k_fwd_tri<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y, u);
+}
+template <class T>
+void launch_fwd_update(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count,
+                       int32_t nrhs, const T* y, T* u, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_fwd_update<T>, dim3(count), dim3(SNT), 0, st, S, Fc, items, nrhs, y, u);
 }
 template <class T>
 void launch_dsolve(int32_t n, int32_t nrhs, const T* d, const T* ds, const uint8_t* two, T* y,
@@ -233,10 +330,16 @@ void launch_dsolve(int32_t n, int32_t nrhs, const T* d, const T* ds, const uint8
                      d, ds, two, y);
 }
 template <class T>
-void launch_backward(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
-                     int32_t nrhs, T* y, hipStream_t st) {
+void launch_bwd_update(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count,
+                       int32_t nrhs, T* y, hipStream_t st) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_backward<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y);
+  hipLaunchKernelGGL(k_bwd_update<T>, dim3(count), dim3(SNT), 0, st, S, Fc, items, nrhs, y);
+}
+template <class T>
+void launch_bwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
+                    int32_t nrhs, T* y, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_bwd_tri<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y);
 }
 
 #define RSL_SINST(T)                                                                           \
@@ -244,12 +347,16 @@ void launch_backward(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, 
                                      T*, hipStream_t);                                         \
   template void launch_scatter_sol<T>(int32_t, int32_t, const int32_t*, const double*,          \
                                       const T*, T*, hipStream_t);                              \
-  template void launch_forward<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
-                                  T*, hipStream_t);                                            \
+  template void launch_fwd_tri<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
+                                  T*, const T*, hipStream_t);                                  \
+  template void launch_fwd_update<T>(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, \
+                                     const T*, T*, hipStream_t);                               \
   template void launch_dsolve<T>(int32_t, int32_t, const T*, const T*, const uint8_t*, T*,      \
                                  hipStream_t);                                                 \
-  template void launch_backward<T>(const DevSym&, const DevFac<T>&, const int32_t*, int,        \
-                                   int32_t, T*, hipStream_t);
+  template void launch_bwd_update<T>(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, \
+                                     T*, hipStream_t);                                         \
+  template void launch_bwd_tri<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
+                                  T*, hipStream_t);
 
 RSL_SINST(double)
 RSL_SINST(cplx)

@@ -160,6 +160,40 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   }
   if ((int64_t)P.tasks.size() > INT32_MAX) throw std::runtime_error("plan: too many cmod tasks");
 
+  // solve plan: scratch rows, gather lists, per-level work items
+  P.u_off.assign(ns, 0);
+  P.u_total = 0;
+  for (int32_t s = 0; s < ns; ++s) {
+    P.u_off[s] = P.u_total;
+    P.u_total += P.sn_nrow[s] - P.sn_ncol[s];
+  }
+  P.gat_ptr.assign((size_t)S.n + 1, 0);
+  for (int32_t k = 0; k < ns; ++k)
+    for (int64_t q = P.rs_off[k] + P.sn_ncol[k]; q < P.rs_off[k + 1]; ++q) P.gat_ptr[P.rs_idx[q] + 1]++;
+  for (int32_t g = 0; g < S.n; ++g) P.gat_ptr[g + 1] += P.gat_ptr[g];
+  P.gat_idx.assign(P.gat_ptr[S.n], 0);
+  {
+    std::vector<int64_t> pos(P.gat_ptr.begin(), P.gat_ptr.end() - 1);
+    for (int32_t k = 0; k < ns; ++k) {
+      const int64_t base = P.rs_off[k] + P.sn_ncol[k];
+      for (int64_t q = base; q < P.rs_off[k + 1]; ++q) P.gat_idx[pos[P.rs_idx[q]]++] = P.u_off[k] + (q - base);
+    }
+  }
+  P.fwd_items.clear();
+  P.bwd_items.clear();
+  for (LevelPlan& L : P.levels) {
+    L.fwd_beg = (int64_t)P.fwd_items.size();
+    L.bwd_beg = (int64_t)P.bwd_items.size();
+    for (int32_t s : L.all) {
+      const int32_t off = P.sn_nrow[s] - P.sn_ncol[s];
+      for (int32_t r0 = 0; r0 < off; r0 += 64) P.fwd_items.push_back({s, r0});
+      if (off > 0)
+        for (int32_t p0 = 0; p0 < P.sn_ncol[s]; p0 += 16) P.bwd_items.push_back({s, p0});
+    }
+    L.fwd_end = (int64_t)P.fwd_items.size();
+    L.bwd_end = (int64_t)P.bwd_items.size();
+  }
+
   // A scatter map: user lower entry (i >= j) -> slot of P^T A P in its panel
   P.nnz = colp[S.n];
   P.amap.resize(P.nnz);

@@ -18,6 +18,12 @@ struct LevelPlan {
   std::vector<int32_t> small;   // one workgroup per supernode (cdiv)
   std::vector<int32_t> wide;    // multi-launch panel path (cdiv)
   int64_t item_beg = 0, item_end = 0;  // cmod items of the level
+  int64_t fwd_beg = 0, fwd_end = 0;    // solve: (supernode, row chunk) L21 y items
+  int64_t bwd_beg = 0, bwd_end = 0;    // solve: (supernode, column chunk) L21^T x items
+};
+
+struct Int2 {
+  int32_t x, y;
 };
 
 struct Plan {
@@ -33,6 +39,11 @@ struct Plan {
   std::vector<LevelPlan> levels;
   std::vector<CItem> items;
   std::vector<CTask> tasks;
+  // solve plan
+  std::vector<int64_t> u_off;   // scratch rows of supernode s (h - w of them)
+  int64_t u_total = 0;
+  std::vector<int64_t> gat_ptr, gat_idx;  // per permuted column, ascending updater
+  std::vector<Int2> fwd_items, bwd_items;
   int64_t panel_total = 0;      // elements in the panel pool
   int64_t snap_total = 0;       // elements of multiplier scratch
   // user lower-CSC entries -> panel slot

@@ -22,9 +22,11 @@ template <class T> void launch_cdiv_wide(const DevSym&, const DevFac<T>&, int, i
 template <class T> hipError_t configure_factor_kernels();
 template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);
 template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);
-template <class T> void launch_forward(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, hipStream_t);
+template <class T> void launch_fwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, const T*, hipStream_t);
+template <class T> void launch_fwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, const T*, T*, hipStream_t);
 template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, const uint8_t*, T*, hipStream_t);
-template <class T> void launch_backward(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, hipStream_t);
+template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, hipStream_t);
+template <class T> void launch_bwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, hipStream_t);
 
 void hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
@@ -92,6 +94,11 @@ void DeviceSymbolic::upload() {
   a_col.upload(P.a_col, stream);
   items.upload(P.items, stream);
   tasks.upload(P.tasks, stream);
+  u_off.upload(P.u_off, stream);
+  gat_ptr.upload(P.gat_ptr, stream);
+  gat_idx.upload(P.gat_idx, stream);
+  fwd_items.upload(P.fwd_items, stream);
+  bwd_items.upload(P.bwd_items, stream);
   std::vector<int32_t> lists;
   level_all_off.clear();
   level_small_off.clear();
@@ -119,6 +126,9 @@ void DeviceSymbolic::upload() {
   view.sym_perm = sym_perm.as<int32_t>();
   view.snap_off = snap_off.as<int64_t>();
   view.snap_ld = snap_ld.as<int32_t>();
+  view.u_off = u_off.as<int64_t>();
+  view.gat_ptr = gat_ptr.as<int64_t>();
+  view.gat_idx = gat_idx.as<int64_t>();
   view.amap = amap.as<int64_t>();
   view.a_row = a_row.as<int32_t>();
   view.a_col = a_col.as<int32_t>();
@@ -256,17 +266,30 @@ class DeviceFactor final : public DeviceFactorBase {
     if (!st) st = sym->stream;
     const size_t need = sizeof(T) * std::max<int64_t>(P.n * nrhs, 1);
     if (work_y.bytes < need) work_y.alloc(need);
+    const size_t needu = sizeof(T) * std::max<int64_t>(P.u_total * nrhs, 1);
+    if (work_b.bytes < needu) work_b.alloc(needu);
     T* y = work_y.as<T>();
+    T* u = work_b.as<T>();
+    const int2* fi = sym->fwd_items.as<int2>();
+    const int2* bi = sym->bwd_items.as<int2>();
     launch_gather_rhs<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(),
                          static_cast<const T*>(d_b), y, st);
     const int32_t* lists = sym->level_lists.as<int32_t>();
-    for (size_t L = 0; L < P.levels.size(); ++L)
-      launch_forward<T>(S, fv, lists + sym->level_all_off[L], (int)P.levels[L].all.size(),
-                        (int32_t)nrhs, y, st);
+    for (size_t L = 0; L < P.levels.size(); ++L) {
+      const LevelPlan& lv = P.levels[L];
+      launch_fwd_tri<T>(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), (int32_t)nrhs, y,
+                        u, st);
+      launch_fwd_update<T>(S, fv, fi + lv.fwd_beg, (int)(lv.fwd_end - lv.fwd_beg), (int32_t)nrhs,
+                           y, u, st);
+    }
     launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st);
-    for (size_t L = P.levels.size(); L-- > 0;)
-      launch_backward<T>(S, fv, lists + sym->level_all_off[L], (int)P.levels[L].all.size(),
-                         (int32_t)nrhs, y, st);
+    for (size_t L = P.levels.size(); L-- > 0;) {
+      const LevelPlan& lv = P.levels[L];
+      launch_bwd_update<T>(S, fv, bi + lv.bwd_beg, (int)(lv.bwd_end - lv.bwd_beg), (int32_t)nrhs, y,
+                           st);
+      launch_bwd_tri<T>(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), (int32_t)nrhs, y,
+                        st);
+    }
     launch_scatter_sol<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(), y,
                           static_cast<T*>(d_x), st);
     hip_check(hipGetLastError(), "solve launch");

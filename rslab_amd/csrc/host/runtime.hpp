@@ -65,7 +65,8 @@ struct DeviceSymbolic {
   Plan plan;
   hipStream_t stream = nullptr;
   DBuf sn_first, sn_ncol, sn_nrow, e_off, panel_off, rs_off, rs_idx, upd_ptr, upd_idx, sym_perm,
-      snap_off, snap_ld, amap, a_row, a_col, items, tasks, level_lists;
+      snap_off, snap_ld, amap, a_row, a_col, items, tasks, level_lists, u_off, gat_ptr, gat_idx,
+      fwd_items, bwd_items;
   std::vector<int64_t> level_all_off, level_small_off;  // offsets into level_lists
   // user CSC (for SpMV in refinement)
   DBuf u_colp, u_rowi;
