@@ -108,6 +108,32 @@ class OracleFactors:
             raise ZeroDivisionError("singular D block")
         return X.reshape(self.n) if single else X
 
+    def solve_arrays(self, b):
+        """solve_ldlt (ldlt_generic.rs:339-424) in numpy on this object's arrays
+        (small cases; lets a test plant a change in L or D and observe it)."""
+        y = np.asarray(b)[self.perm].astype(self.l_values.dtype)
+        n = self.n
+        for j in range(n):
+            s, e = self.l_col_ptr[j], self.l_col_ptr[j + 1]
+            y[self.l_row_idx[s + 1:e]] -= self.l_values[s + 1:e] * y[j]
+        k = 0
+        while k < n:
+            if self.two_by_two[k]:
+                d11, d21, d22 = self.d_diag[k], self.d_subdiag[k], self.d_diag[k + 1]
+                det = d11 * d22 - d21 * d21
+                z0, z1 = y[k], y[k + 1]
+                y[k], y[k + 1] = (d22 * z0 - d21 * z1) / det, (d11 * z1 - d21 * z0) / det
+                k += 2
+            else:
+                y[k] = y[k] / self.d_diag[k]
+                k += 1
+        for j in range(n - 1, -1, -1):
+            s, e = self.l_col_ptr[j], self.l_col_ptr[j + 1]
+            y[j] -= np.dot(self.l_values[s + 1:e], y[self.l_row_idx[s + 1:e]])
+        x = np.empty_like(y)
+        x[self.perm] = y
+        return x
+
     def __del__(self):
         if self._handle is not None:
             _lib.ora_free(self._handle)

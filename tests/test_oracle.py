@@ -1,0 +1,214 @@
+"""Pins for the CPU oracle (oracle/ldlt_oracle.c), independent of the oracle's
+own formulas: the reference's known answers, dense LAPACK Bunch-Kaufman
+(scipy.linalg.ldl, the ?sytrf control flow the reference follows for one
+front), eigenvalue inertia, closed forms, and planted mistakes."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle as O
+import rslab_amd as R
+from rslab_amd import matgen
+
+
+def analyze(a, **kw):
+    st = R.SolverSettings()
+    for k, v in kw.items():
+        st = getattr(st, "with_" + k)(v)
+    return R.LdltSymbolic.analyze_with(a, st).export()
+
+
+def dense(a):
+    cols = np.repeat(np.arange(a.n), np.diff(a.col_ptr))
+    M = np.zeros((a.n, a.n), dtype=a.values.dtype)
+    M[a.row_idx, cols] = a.values
+    M[cols, a.row_idx] = a.values
+    return M
+
+
+def dense_lower_csc(M):
+    n = M.shape[0]
+    r, c = np.tril_indices(n)
+    o = np.lexsort((r, c))
+    r, c = r[o], c[o]
+    cp = np.concatenate([[0], np.cumsum(np.bincount(c, minlength=n))])
+    return R.CscMatrix(n, cp, r, M[r, c])
+
+
+def grid2d(m, diag, off):
+    a = matgen.grid_laplacian((m, m))
+    cols = np.repeat(np.arange(a.n), np.diff(a.col_ptr))
+    dt = np.complex128 if isinstance(diag, complex) or isinstance(off, complex) else np.float64
+    return a.with_values(np.where(a.row_idx == cols, diag, off).astype(dt))
+
+
+def reconstruct(f):
+    n = f.n
+    L = np.zeros((n, n), dtype=f.l_values.dtype)
+    for j in range(n):
+        for k in range(f.l_col_ptr[j], f.l_col_ptr[j + 1]):
+            L[f.l_row_idx[k], j] = f.l_values[k]
+    D = np.diag(f.d_diag).astype(f.l_values.dtype)
+    for k in np.nonzero(f.two_by_two)[0]:
+        D[k + 1, k] = D[k, k + 1] = f.d_subdiag[k]
+    return L @ D @ L.T
+
+
+def test_known_answer_2x2_real_and_complex():
+    # ldlt_generic.rs:836-850 and :896-915
+    a = R.CscMatrix.from_triplets(2, [0, 1, 1], [0, 0, 1], [0.0, 1.0, 0.0])
+    x, f = O.solve(a, np.array([3.0, 5.0]), analyze(a))
+    assert np.allclose(x, [5.0, 3.0], atol=1e-12)
+    assert f.inertia == (1, 1, 0) and f.two_by_two.any()
+    ac = a.with_values(np.array([0, 1, 0], dtype=np.complex128))
+    x, f = O.solve(ac, np.array([1 + 1j, 2 - 1j]), analyze(ac))
+    assert np.abs(x - np.array([2 - 1j, 1 + 1j])).max() < 1e-12
+
+
+def test_diagonal_inertia_and_singular_rejected():
+    # sparse_solver.rs:1006-1021 and ldlt_generic.rs:975-983
+    d = [2.0, -3.0, 4.0, -1.0, 5.0]
+    a = R.CscMatrix.from_triplets(5, range(5), range(5), d)
+    _, f = O.solve(a, np.ones(5), analyze(a))
+    assert f.inertia == (3, 2, 0)
+    z = R.CscMatrix.from_triplets(2, [0, 1], [0, 1], [0.0, 0.0])
+    with pytest.raises(O.RankDeficient):
+        O.solve(z, np.ones(2), analyze(z))
+
+
+def test_perturb_rescues_singular_complex():
+    # multifrontal_ldlt.rs:4186-4221
+    a = R.CscMatrix.from_triplets(3, [0, 2, 1], [0, 0, 1], np.array([2 + 1j, -1 + 0.3j, 0j]))
+    e = analyze(a)
+    with pytest.raises(O.RankDeficient):
+        O.solve(a, np.ones(3), e)
+    x, f = O.solve(a, np.ones(3, dtype=complex), e, pivot=1, abs_floor=1e-8)
+    assert f.n_perturbed >= 1 and np.all(np.isfinite(x))
+
+
+@pytest.mark.parametrize("n,seed", [(40, 0), (64, 1), (150, 2)])
+def test_single_front_matches_lapack_bunch_kaufman(n, seed):
+    """One dense front: within a single nb-panel the reference's getf2 is
+    LAPACK ?sytf2 lower; over several panels it stays a Bunch-Kaufman LDL^T.
+    Pinned by inertia (Sylvester vs eigenvalues), the 2x2 count of LAPACK for
+    n <= nb, and the reconstruction P^T A P = L D L^T."""
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n))
+    M = M + M.T
+    a = dense_lower_csc(M)
+    e = analyze(a, ordering=R.OrderingMethod.Natural)
+    assert len(e.sn_first) == 1
+    f = O.factor(e, a, panel_nb=64)
+    ev = np.linalg.eigvalsh(M)
+    assert f.inertia == (int((ev > 0).sum()), int((ev < 0).sum()), 0)
+    Ae = M[np.ix_(f.perm, f.perm)]
+    assert np.abs(reconstruct(f) - Ae).max() < 1e-10 * np.abs(M).max() * n
+    if n <= 64:
+        lu, d, perm = sla.ldl(M, lower=True)
+        n2_lapack = int(np.count_nonzero(np.abs(np.diag(d, -1)) > 0))
+        assert int(f.two_by_two.sum()) == n2_lapack
+
+
+@pytest.mark.parametrize("case", ["indef10", "cplx9", "spd12", "indef60"])
+def test_reference_ll_test_matrices(case):
+    """multifrontal_ldlt.rs:3866-4013: the left-looking tests' matrices,
+    residual bar 1e-9, inertia against eigenvalues (real cases)."""
+    a = {"indef10": lambda: grid2d(10, 0.5, -1.0), "cplx9": lambda: grid2d(9, 0.4 + 0.3j, -1 + 0.1j),
+         "spd12": lambda: grid2d(12, 8.0, -1.0), "indef60": lambda: grid2d(60, 0.5, -1.0)}[case]()
+    n = a.n
+    b = np.array([(i % 7) - 3.0 for i in range(n)], dtype=a.values.dtype)
+    x, f = O.solve(a, b, analyze(a))
+    assert np.abs(b - a.symv(x)).max() < 1e-9
+    if not np.iscomplexobj(a.values):
+        ev = np.linalg.eigvalsh(dense(a))
+        assert f.inertia == (int((ev > 0).sum()), int((ev < 0).sum()), 0)
+    if case.startswith("indef") or case == "cplx9":
+        assert f.two_by_two.any()
+
+
+def test_multi_panel_dense_fronts():
+    # multifrontal_ldlt.rs:4015-4066
+    n = 100
+    M = np.array([[n + 1.0 if i == j else ((max(i, j) + 2 * min(i, j)) % 5) - 2.0 for j in range(n)] for i in range(n)])
+    a = dense_lower_csc(M)
+    b = np.array([(i % 7) - 3.0 for i in range(n)])
+    x, _ = O.solve(a, b, analyze(a))
+    assert np.abs(b - M @ x).max() < 1e-9
+    n = 90
+    Mc = np.array([[complex(n, 1.0) if i == j else complex(((max(i, j) + 3 * min(i, j)) % 5) - 2.0, 0.2)
+                    for j in range(n)] for i in range(n)])
+    ac = dense_lower_csc(Mc)
+    bc = np.full(n, 1 + 0.5j)
+    x, _ = O.solve(ac, bc, analyze(ac))
+    assert np.abs(bc - Mc @ x).max() < 1e-9
+
+
+def test_config3_like_inertia_within_weyl_bounds():
+    """3D L - 1.5 I + diag(delta): the inertia is bracketed by the eigenvalues
+    of L - 1.5 I shifted by +-0.01 (Weyl), computed in closed form."""
+    m = 12
+    delta = matgen.rng_uniform(3, -0.01, 0.01, m ** 3)
+    a = matgen.grid_laplacian((m, m, m), shift=-1.5, diag_real=delta)
+    x, f = O.solve(a, matgen.rng_uniform(4, -1, 1, a.n), analyze(a, ordering=R.OrderingMethod.MetisND))
+    t = 2 - 2 * np.cos(np.arange(1, m + 1) * np.pi / (m + 1))
+    lam = (t[:, None, None] + t[None, :, None] + t[None, None, :]).ravel() - 1.5
+    lo, hi = int((lam < -0.01).sum()), int((lam < 0.01).sum())
+    assert lo <= f.inertia[1] <= hi
+    ev = np.linalg.eigvalsh(dense(a))
+    assert f.inertia[1] == int((ev < 0).sum())
+
+
+def test_config1_closed_form_inertia():
+    """Config 1's closed form (+8163, -1837, 0) from the 2D Dirichlet spectrum."""
+    p, nneg, z, mn = O.laplacian2d_inertia(100, 2.0)
+    assert (p, nneg, z) == (8163, 1837, 0) and mn > 1e-3
+
+
+def test_solve_many_equals_columnwise():
+    # ldlt_generic.rs:785-832: multi-RHS equals per-column solves
+    a = grid2d(20, 0.5, -1.0)
+    e = analyze(a)
+    B = np.stack([matgen.rng_uniform(9 + c, -1, 1, a.n) for c in range(5)], 1)
+    X, f = O.solve(a, B, e)
+    for c in range(5):
+        xc, _ = O.solve(a, B[:, c], e)
+        assert np.abs(X[:, c] - xc).max() < 1e-12
+
+
+def test_onepass_scale_known_answer():
+    # s_i = 1/sqrt(max_j |A_ij|), guarded to 1 for empty rows
+    a = R.CscMatrix.from_triplets(3, [0, 1, 1], [0, 0, 1], [4.0, -9.0, 1.0])
+    s = O.onepass_scale(a)
+    assert np.allclose(s, [1 / 3.0, 1 / 3.0, 1.0])
+
+
+def test_planted_mistakes_are_caught():
+    """The checks used for the device path catch a dropped term, a wrong sign
+    or a transposed operand when planted in an oracle factor."""
+    a = grid2d(12, 0.5, -1.0)
+    b = np.array([(i % 7) - 3.0 for i in range(a.n)])
+    e = analyze(a)
+    s = O.onepass_scale(a)
+    ah = O.scaled(a, s)
+    good = O.factor(e, ah)
+    xg = good.solve_arrays(b * s) * s
+    assert np.abs(xg - good.solve_many(b * s) * s).max() < 1e-12
+    assert np.abs(b - a.symv(xg)).max() < 1e-9
+    # dropped term in L
+    bad = O.factor(e, ah)
+    k = int(np.argmax(np.abs(bad.l_values) * (bad.l_values != 1.0)))
+    bad.l_values[k] = 0.0
+    x = bad.solve_arrays(b * s) * s
+    assert np.abs(b - a.symv(x)).max() > 1e-6
+    # wrong sign of a 1x1 pivot: residual and inertia both move
+    bad2 = O.factor(e, ah)
+    j = int(np.nonzero(~bad2.two_by_two & ~np.concatenate([[False], bad2.two_by_two[:-1]]))[0][3])
+    bad2.d_diag[j] = -bad2.d_diag[j]
+    x = bad2.solve_arrays(b * s) * s
+    assert np.abs(b - a.symv(x)).max() > 1e-6
+    # transposed 2x2 coupling (d21 applied as -d21)
+    bad3 = O.factor(e, ah)
+    k2 = int(np.nonzero(bad3.two_by_two)[0][0])
+    bad3.d_subdiag[k2] = -bad3.d_subdiag[k2]
+    x = bad3.solve_arrays(b * s) * s
+    assert np.abs(b - a.symv(x)).max() > 1e-6
