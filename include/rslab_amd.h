@@ -109,8 +109,10 @@ int rsl_refactor(rsl_solver *f, const double *values);
  * row-major n x nrhs. */
 int rsl_solve(rsl_solver *f, const double *b, int64_t nrhs, double *x);
 
-/* Device-resident variant of rsl_solve: d_b, d_x device pointers, stream may
- * be NULL (library stream). Asynchronous on that stream. */
+/* Device-resident variant of rsl_solve: d_b, d_x device pointers (row-major
+ * n x nrhs), enqueued on `stream` (a hipStream_t; NULL is the null stream) and
+ * asynchronous with respect to the host. Calls on one factor handle are
+ * serialized by the library (they share its device workspace). */
 int rsl_solve_device(rsl_solver *f, const void *d_b, int64_t nrhs, void *d_x,
                      void *stream);
 
@@ -133,6 +135,17 @@ int rsl_factor_download(rsl_solver *f, int64_t *l_col_ptr, int32_t *l_row_idx,
  * info[4*c..4*c+4) = {ir_iters, gmres_iters, backward_error, certified}. */
 int rsl_refine(rsl_solver *f, const double *a_values, const double *b, int64_t nrhs,
                double target, double *x, double *info);
+
+/* Per-kernel timing (no counterpart in rslab; a measurement aid). While
+ * enabled, every launch of this factor's numeric factor, solve and refinement
+ * is bracketed by HIP events on the stream it runs on. rsl_profile_enable
+ * clears the counters. rsl_profile_read returns, for one launch kind,
+ * out = {total ms, algorithmic flops, algorithmic bytes, launches}; flops are
+ * real-arithmetic (a complex multiply-add counts 8), 2 per multiply-add.
+ * Kinds run from 0 until rsl_profile_kind_name returns NULL. */
+int rsl_profile_enable(rsl_solver *f, int32_t on);
+int rsl_profile_read(rsl_solver *f, int32_t kind, double out[4]);
+const char *rsl_profile_kind_name(int32_t kind);
 
 void rsl_free_factor(rsl_solver *f);
 void rsl_free_symbolic(rsl_symbolic *s);

@@ -26,6 +26,7 @@ struct DevSym {
   const int64_t* amap;       // user nnz -> panel offset
   const int32_t* a_row;      // user nnz -> user row
   const int32_t* a_col;      // user nnz -> user column
+  const int32_t* nb_eff;     // panel width of s (64, or 128 on supernodes >= 512 wide)
   int64_t nnz;
 };
 
@@ -53,6 +54,20 @@ struct CItem {
 // (tile rows) and [b0,b1) (tile columns), indices into k's off-diagonal rows.
 struct CTask {
   int32_t k, a0, a1, b0, b1;
+};
+
+// One unit of the batched wide-supernode path: nb-panel [kb, kb+nb) of
+// supernode s; a = deep-row offset (replay) or row tile (Schur), b = column tile.
+struct WItem {
+  int32_t s, kb, a, b;
+};
+
+// One panel step of a level's wide supernodes, run as three batched launches
+// (getf2 on every supernode that has a p-th panel, then the deep-row replay,
+// then the deferred Schur tiles). Ranges index the WItem array.
+struct WideStep {
+  int64_t g_beg = 0, g_end = 0, r_beg = 0, r_end = 0, t_beg = 0, t_end = 0;
+  double getf2_flops = 0, replay_flops = 0, schur_flops = 0;
 };
 
 struct PivotPolicy {

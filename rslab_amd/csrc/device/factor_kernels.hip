@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "devtypes.hpp"
+#include "prof.hpp"
 #include "scalar.hpp"
 
 namespace rsl {
@@ -637,18 +638,21 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
   cdiv_emit(S, Fc, c, sh.cnt, sh.npert);
 }
 
-// ---- wide supernodes: one launch per phase of each nb-panel ----
+// ---- wide supernodes: batched over the level, one launch per phase of each
+// panel step (every supernode of the level that has a p-th panel) ----
 template <class T>
-__global__ __launch_bounds__(NT) void k_panel_getf2(DevSym S, DevFac<T> Fc, int s, int kb,
-                                                    int nb, PivotPolicy pol) {
+__global__ __launch_bounds__(NT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
+                                                   PivotPolicy pol) {
   extern __shared__ unsigned char smem_raw[];
   CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
-  SnCtx<T> c = make_ctx(S, Fc, s, nb);
-  if (Fc.status[0]) return;  // an earlier panel of this supernode failed
+  const WItem it = items[blockIdx.x];
+  const int s = it.s, kb = it.kb;
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  if (Fc.status[0]) return;  // the factorization has already failed
   if (threadIdx.x == 0) sh.npert = 0;
   if (kb == 0) cdiv_init(c);
   __syncthreads();
-  const int ke = min(kb + nb, c.w);
+  const int ke = min(kb + c.nb, c.w);
   panel_load(c, kb, ke, sh.u.Ab);
   int st = panel_getf2(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.deep_s, pol, &sh.npert);
   if (st) {
@@ -665,16 +669,18 @@ __global__ __launch_bounds__(NT) void k_panel_getf2(DevSym S, DevFac<T> Fc, int 
 constexpr int RR = 64;  // rows per workgroup = threads per workgroup
 
 template <class T>
-__global__ __launch_bounds__(RR) void k_panel_replay(DevSym S, DevFac<T> Fc, int s, int kb,
-                                                     int nb) {
+__global__ __launch_bounds__(RR) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
   extern __shared__ unsigned char smem_raw[];
   T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR]
   __shared__ int32_t deep_s[NBMAX];
-  SnCtx<T> c = make_ctx(S, Fc, s, nb);
-  const int ke = min(kb + nb, c.w), pw = ke - kb;
+  const WItem it = items[blockIdx.x];
+  const int s = it.s, kb = it.kb;
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  if (Fc.status[0]) return;
+  const int ke = min(kb + c.nb, c.w), pw = ke - kb;
   const int t = threadIdx.x;
   for (int q = t; q < pw; q += RR) deep_s[q] = c.deep[q];
-  const int64_t i = ke + (int64_t)blockIdx.x * RR + t;
+  const int64_t i = ke + (int64_t)it.a + t;
   const bool live = i < c.h;
   if (live)
     for (int j = 0; j < pw; ++j) X[j * RR + t] = c.P[i + (int64_t)(kb + j) * c.h];
@@ -718,21 +724,23 @@ __global__ __launch_bounds__(RR) void k_panel_replay(DevSym S, DevFac<T> Fc, int
 }
 
 template <class T>
-__global__ __launch_bounds__(NT) void k_panel_schur(DevSym S, DevFac<T> Fc, int s, int kb,
-                                                    int nb) {
+__global__ __launch_bounds__(NT) void k_wide_schur(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
   __shared__ T As[KC * TM];
   __shared__ T Bs[KC * TN];
-  const int rb = blockIdx.x, cb = blockIdx.y;
-  if (rb < cb) return;
-  SnCtx<T> c = make_ctx(S, Fc, s, nb);
-  const int ke = min(kb + nb, c.w);
-  panel_schur_tile(c, kb, ke, rb * TM, cb * TN, As, Bs);
+  const WItem it = items[blockIdx.x];
+  const int s = it.s, kb = it.kb;
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  if (Fc.status[0]) return;
+  const int ke = min(kb + c.nb, c.w);
+  panel_schur_tile(c, kb, ke, it.a * TM, it.b * TN, As, Bs);
 }
 
 template <class T>
-__global__ __launch_bounds__(NT) void k_emit(DevSym S, DevFac<T> Fc, int s, int nb) {
+__global__ __launch_bounds__(NT) void k_wide_emit(DevSym S, DevFac<T> Fc, const int32_t* __restrict__ list) {
   __shared__ int cnt[4];
-  SnCtx<T> c = make_ctx(S, Fc, s, nb);
+  const int s = list[blockIdx.x];
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  if (Fc.status[0]) return;
   cdiv_emit(S, Fc, c, cnt, 0);
 }
 
@@ -754,46 +762,57 @@ void launch_fill(T* p, int64_t n, T v, hipStream_t st) {
 
 template <class T>
 void launch_assemble(const DevSym& S, const DevFac<T>& Fc, const T* vals, const double* scale,
-                     hipStream_t st) {
+                     hipStream_t st, LaunchProfiler* pf) {
   if (S.nnz <= 0) return;
+  ProfScope ps(pf, LK_ASSEMBLE, 0.0, (double)S.nnz * (2 * sizeof(T) + 24), st);
   hipLaunchKernelGGL(k_assemble<T>, dim3((unsigned)((S.nnz + 255) / 256)), dim3(256), 0, st, S,
                      Fc, vals, scale);
 }
 
 template <class T>
 void launch_cmod(const DevSym& S, const DevFac<T>& Fc, const CItem* items, int nitems,
-                 const CTask* tasks, hipStream_t st) {
+                 const CTask* tasks, double flops, hipStream_t st, LaunchProfiler* pf) {
   if (nitems <= 0) return;
+  ProfScope ps(pf, LK_CMOD, flops, 0.0, st);
   hipLaunchKernelGGL(k_cmod<T>, dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks);
 }
 
 template <class T>
 void launch_cdiv_small(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
-                       int nb, PivotPolicy pol, hipStream_t st) {
+                       int nb, PivotPolicy pol, double flops, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
+  ProfScope ps(pf, LK_CDIV_SMALL, flops, 0.0, st);
   hipLaunchKernelGGL(k_cdiv_small<T>, dim3(count), dim3(NT), cdiv_smem<T>(), st, S, Fc, list, nb,
                      pol);
 }
 
 template <class T>
-void launch_cdiv_wide(const DevSym& S, const DevFac<T>& Fc, int s, int w, int64_t h, int nb_base,
-                      PivotPolicy pol, hipStream_t st) {
-  const int nb = (w >= 512 && nb_base < 128) ? 128 : nb_base;
-  for (int kb = 0; kb < w; kb += nb) {
-    const int ke = kb + nb < w ? kb + nb : w;
-    hipLaunchKernelGGL(k_panel_getf2<T>, dim3(1), dim3(NT), cdiv_smem<T>(), st, S, Fc, s, kb, nb,
-                       pol);
-    const int64_t mt = h - ke;
-    if (mt > 0)
-      hipLaunchKernelGGL(k_panel_replay<T>, dim3((unsigned)((mt + RR - 1) / RR)), dim3(RR),
-                         sizeof(T) * RR * (ke - kb), st, S, Fc, s, kb, nb);
-    const int cw = w - ke;
-    if (cw > 0 && mt > 0) {
-      unsigned nrt = (unsigned)((mt + TM - 1) / TM), nct = (unsigned)((cw + TN - 1) / TN);
-      hipLaunchKernelGGL(k_panel_schur<T>, dim3(nrt, nct), dim3(NT), 0, st, S, Fc, s, kb, nb);
+void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems, const WideStep* steps,
+                       int nsteps, const int32_t* wide_list, int nwide, PivotPolicy pol, hipStream_t st,
+                       LaunchProfiler* pf) {
+  if (nwide <= 0) return;
+  const double fm = sizeof(T) == 16 ? 4.0 : 1.0;
+  const size_t replay_lds = sizeof(T) * RR * NBMAX;
+  for (int p = 0; p < nsteps; ++p) {
+    const WideStep& w = steps[p];
+    if (w.g_end > w.g_beg) {
+      ProfScope ps(pf, LK_PANEL_GETF2, fm * w.getf2_flops, 0.0, st);
+      hipLaunchKernelGGL(k_wide_getf2<T>, dim3((unsigned)(w.g_end - w.g_beg)), dim3(NT), cdiv_smem<T>(), st,
+                         S, Fc, witems + w.g_beg, pol);
+    }
+    if (w.r_end > w.r_beg) {
+      ProfScope ps(pf, LK_PANEL_REPLAY, fm * w.replay_flops, 0.0, st);
+      hipLaunchKernelGGL(k_wide_replay<T>, dim3((unsigned)(w.r_end - w.r_beg)), dim3(RR), replay_lds, st, S,
+                         Fc, witems + w.r_beg);
+    }
+    if (w.t_end > w.t_beg) {
+      ProfScope ps(pf, LK_PANEL_SCHUR, fm * w.schur_flops, 0.0, st);
+      hipLaunchKernelGGL(k_wide_schur<T>, dim3((unsigned)(w.t_end - w.t_beg)), dim3(NT), 0, st, S, Fc,
+                         witems + w.t_beg);
     }
   }
-  hipLaunchKernelGGL(k_emit<T>, dim3(1), dim3(NT), 0, st, S, Fc, s, nb);
+  ProfScope ps(pf, LK_EMIT, 0.0, 0.0, st);
+  hipLaunchKernelGGL(k_wide_emit<T>, dim3((unsigned)nwide), dim3(NT), 0, st, S, Fc, wide_list);
 }
 
 template <class T>
@@ -802,23 +821,24 @@ hipError_t configure_factor_kernels() {
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)cdiv_smem<T>());
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_panel_replay<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  e = hipFuncSetAttribute((const void*)k_wide_replay<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)(sizeof(T) * RR * NBMAX));
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_panel_getf2<T>,
+  return hipFuncSetAttribute((const void*)k_wide_getf2<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdiv_smem<T>());
 }
 
 #define RSL_INST(T)                                                                            \
   template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                   \
   template void launch_assemble<T>(const DevSym&, const DevFac<T>&, const T*, const double*,   \
-                                   hipStream_t);                                               \
+                                   hipStream_t, LaunchProfiler*);                              \
   template void launch_cmod<T>(const DevSym&, const DevFac<T>&, const CItem*, int,             \
-                               const CTask*, hipStream_t);                                     \
+                               const CTask*, double, hipStream_t, LaunchProfiler*);            \
   template void launch_cdiv_small<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int, \
-                                     PivotPolicy, hipStream_t);                                \
-  template void launch_cdiv_wide<T>(const DevSym&, const DevFac<T>&, int, int, int64_t, int,   \
-                                    PivotPolicy, hipStream_t);                                 \
+                                     PivotPolicy, double, hipStream_t, LaunchProfiler*);       \
+  template void launch_wide_level<T>(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, \
+                                     int, const int32_t*, int, PivotPolicy, hipStream_t,         \
+                                     LaunchProfiler*);                                           \
   template hipError_t configure_factor_kernels<T>();                                           \
   template size_t cdiv_smem<T>();
 

@@ -1,0 +1,248 @@
+// FP64 residual and Krylov vector kernels for the refinement ladder
+// (MixedLdltSolver::solve_to, src/numeric/mixed.rs:140-221, and the FGMRES of
+// src/numeric/iterative.rs:822-1021).
+//
+// The matrix is applied as a full symmetric CSR whose values are gathered from
+// the user's lower-triangle CSC values (CscMatrix::symv, src/sparse/csc.rs:294-309).
+// Vectors are row-major n x k blocks (the solve layout) or single columns.
+// Reductions are two-stage with a fixed grid, so results do not depend on
+// scheduling: run-to-run identical.
+#include <hip/hip_runtime.h>
+
+#include "devtypes.hpp"
+#include "prof.hpp"
+#include "scalar.hpp"
+
+namespace rsl {
+
+constexpr int RT = 256;      // threads per block
+constexpr int RED_BLOCKS = 512;  // partials per reduction
+
+template <class T>
+__device__ inline T conj_t(T v);
+template <>
+__device__ inline double conj_t<double>(double v) { return v; }
+template <>
+__device__ inline cplx conj_t<cplx>(cplx v) { return {v.re, -v.im}; }
+
+// full-CSR value q := user lower value src[q]
+template <class T>
+__global__ void k_expand(int64_t nnz_full, const int64_t* __restrict__ src, const T* __restrict__ uv,
+                         T* __restrict__ fv) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nnz_full) fv[q] = uv[src[q]];
+}
+
+// y = b - A x   (b may be null: y = A x), row-major n x k, thread per (row, column)
+template <class T>
+__global__ __launch_bounds__(RT) void k_residual(int32_t n, int32_t k, const int64_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci, const T* __restrict__ av,
+                                                 const T* __restrict__ x, const T* __restrict__ b,
+                                                 T* __restrict__ y) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * k) return;
+  const int64_t i = idx / k, c = idx % k;
+  T acc = F<T>::zero();
+  for (int64_t q = rp[i]; q < rp[i + 1]; ++q) acc = acc + av[q] * x[(int64_t)ci[q] * k + c];
+  y[idx] = b ? b[idx] - acc : acc;
+}
+
+// per-column max |v| of a row-major n x k block; out[c] holds the bit pattern
+// of a non-negative double (ordered like the unsigned integer), pre-zeroed.
+template <class T>
+__global__ __launch_bounds__(RT) void k_colmax(int32_t n, int32_t k, const T* __restrict__ v,
+                                               unsigned long long* __restrict__ out) {
+  const int c = blockIdx.y;
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
+    m = fmax(m, F<T>::mag(v[i * k + c]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  __shared__ double wm[RT / 64];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < RT / 64; ++w) m = fmax(m, wm[w]);
+    atomicMax(out + c, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// x[:, c] += dx[:, c] for columns with mask[c]
+template <class T>
+__global__ void k_update_cols(int32_t n, int32_t k, const uint8_t* __restrict__ mask, T* __restrict__ x,
+                              const T* __restrict__ dx) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * k) return;
+  if (mask[idx % k]) x[idx] = x[idx] + dx[idx];
+}
+
+// dst[:, c] = src[:, c] for columns with mask[c]
+template <class T>
+__global__ void k_copy_cols(int32_t n, int32_t k, const uint8_t* __restrict__ mask, T* __restrict__ dst,
+                            const T* __restrict__ src) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * k) return;
+  if (mask[idx % k]) dst[idx] = src[idx];
+}
+
+// column c of a row-major n x k block <-> contiguous vector
+template <class T>
+__global__ void k_get_col(int32_t n, int32_t k, int32_t c, const T* __restrict__ blk, T* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = blk[i * k + c];
+}
+template <class T>
+__global__ void k_set_col(int32_t n, int32_t k, int32_t c, T* __restrict__ blk, const T* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) blk[i * k + c] = v[i];
+}
+
+// partial[b] = sum over block b's slice of conj(x) * y (x == y gives |x|^2)
+template <class T>
+__global__ __launch_bounds__(RT) void k_dotc_partial(int32_t n, const T* __restrict__ x,
+                                                     const T* __restrict__ y, T* __restrict__ partial) {
+  T acc = F<T>::zero();
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
+    acc = acc + conj_t(x[i]) * y[i];
+  __shared__ T sm[RT];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = RT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+// out[0] = sum of partial[0..m) in a fixed tree order
+template <class T>
+__global__ __launch_bounds__(RT) void k_sum_partials(int32_t m, const T* __restrict__ partial,
+                                                     T* __restrict__ out) {
+  T acc = F<T>::zero();
+  for (int i = threadIdx.x; i < m; i += RT) acc = acc + partial[i];
+  __shared__ T sm[RT];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = RT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+// w -= h[0] * v with h read from device memory (Gram-Schmidt step, no host trip)
+template <class T>
+__global__ void k_axpy_dev(int32_t n, const T* __restrict__ h, const T* __restrict__ v, T* __restrict__ w) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = w[i] - h[0] * v[i];
+}
+
+// y = a * x + y with a host scalar
+template <class T>
+__global__ void k_axpy(int32_t n, T a, const T* __restrict__ x, T* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = y[i] + a * x[i];
+}
+
+// y = a * x with a host real scalar
+template <class T>
+__global__ void k_scal(int32_t n, double a, const T* __restrict__ x, T* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = F<T>::scale(x[i], a);
+}
+
+// ------------------------------------------------------------------ launchers
+namespace {
+inline unsigned blocks_for(int64_t m) { return (unsigned)((m + RT - 1) / RT); }
+}  // namespace
+
+template <class T>
+void launch_expand(int64_t nnz_full, const int64_t* src, const T* uv, T* fv, hipStream_t st) {
+  if (nnz_full <= 0) return;
+  hipLaunchKernelGGL(k_expand<T>, dim3(blocks_for(nnz_full)), dim3(RT), 0, st, nnz_full, src, uv, fv);
+}
+
+template <class T>
+void launch_residual(int32_t n, int32_t k, const int64_t* rp, const int32_t* ci, const T* av, const T* x,
+                     const T* b, T* y, hipStream_t st, LaunchProfiler* pf) {
+  if ((int64_t)n * k <= 0) return;
+  ProfScope ps(pf, LK_RESIDUAL, 0.0, 0.0, st);
+  hipLaunchKernelGGL(k_residual<T>, dim3(blocks_for((int64_t)n * k)), dim3(RT), 0, st, n, k, rp, ci, av, x,
+                     b, y);
+}
+
+template <class T>
+void launch_colmax(int32_t n, int32_t k, const T* v, unsigned long long* out, hipStream_t st) {
+  if (n <= 0 || k <= 0) return;
+  unsigned gx = std::min<unsigned>(blocks_for(n), 256u);
+  hipLaunchKernelGGL(k_colmax<T>, dim3(gx, (unsigned)k), dim3(RT), 0, st, n, k, v, out);
+}
+
+template <class T>
+void launch_update_cols(int32_t n, int32_t k, const uint8_t* mask, T* x, const T* dx, hipStream_t st) {
+  if ((int64_t)n * k <= 0) return;
+  hipLaunchKernelGGL(k_update_cols<T>, dim3(blocks_for((int64_t)n * k)), dim3(RT), 0, st, n, k, mask, x, dx);
+}
+
+template <class T>
+void launch_copy_cols(int32_t n, int32_t k, const uint8_t* mask, T* dst, const T* src, hipStream_t st) {
+  if ((int64_t)n * k <= 0) return;
+  hipLaunchKernelGGL(k_copy_cols<T>, dim3(blocks_for((int64_t)n * k)), dim3(RT), 0, st, n, k, mask, dst, src);
+}
+
+template <class T>
+void launch_get_col(int32_t n, int32_t k, int32_t c, const T* blk, T* v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_get_col<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, k, c, blk, v);
+}
+
+template <class T>
+void launch_set_col(int32_t n, int32_t k, int32_t c, T* blk, const T* v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_set_col<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, k, c, blk, v);
+}
+
+// out[0] = conj(x) . y ; partial must hold RED_BLOCKS entries
+template <class T>
+void launch_dotc(int32_t n, const T* x, const T* y, T* partial, T* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dotc_partial<T>, dim3(RED_BLOCKS), dim3(RT), 0, st, n, x, y, partial);
+  hipLaunchKernelGGL(k_sum_partials<T>, dim3(1), dim3(RT), 0, st, (int32_t)RED_BLOCKS, partial, out);
+}
+
+template <class T>
+void launch_axpy_dev(int32_t n, const T* h, const T* v, T* w, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy_dev<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, h, v, w);
+}
+
+template <class T>
+void launch_axpy(int32_t n, T a, const T* x, T* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, a, x, y);
+}
+
+template <class T>
+void launch_scal(int32_t n, double a, const T* x, T* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scal<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, a, x, y);
+}
+
+int refine_reduction_blocks() { return RED_BLOCKS; }
+
+#define RSL_INST(T)                                                                                     \
+  template void launch_expand<T>(int64_t, const int64_t*, const T*, T*, hipStream_t);                   \
+  template void launch_residual<T>(int32_t, int32_t, const int64_t*, const int32_t*, const T*, const T*, \
+                                   const T*, T*, hipStream_t, LaunchProfiler*);                        \
+  template void launch_colmax<T>(int32_t, int32_t, const T*, unsigned long long*, hipStream_t);         \
+  template void launch_update_cols<T>(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);     \
+  template void launch_copy_cols<T>(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);       \
+  template void launch_get_col<T>(int32_t, int32_t, int32_t, const T*, T*, hipStream_t);                \
+  template void launch_set_col<T>(int32_t, int32_t, int32_t, T*, const T*, hipStream_t);                \
+  template void launch_dotc<T>(int32_t, const T*, const T*, T*, T*, hipStream_t);                       \
+  template void launch_axpy_dev<T>(int32_t, const T*, const T*, T*, hipStream_t);                       \
+  template void launch_axpy<T>(int32_t, T, const T*, T*, hipStream_t);                                  \
+  template void launch_scal<T>(int32_t, double, const T*, T*, hipStream_t);
+RSL_INST(double)
+RSL_INST(cplx)
+#undef RSL_INST
+
+}  // namespace rsl

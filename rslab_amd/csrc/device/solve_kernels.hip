@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "devtypes.hpp"
+#include "prof.hpp"
 #include "scalar.hpp"
 
 namespace rsl {
@@ -295,68 +296,75 @@ __global__ void k_dsolve(int32_t n, int32_t nrhs, const T* __restrict__ d,
 // ---- launchers ----
 template <class T>
 void launch_gather_rhs(int32_t n, int32_t nrhs, const int32_t* perm_e, const double* scale,
-                       const T* b, T* y, hipStream_t st) {
+                       const T* b, T* y, hipStream_t st, LaunchProfiler* pf) {
   int64_t tot = (int64_t)n * nrhs;
   if (tot <= 0) return;
+  ProfScope ps(pf, LK_SOLVE_GATHER, 0.0, 2.0 * sizeof(T) * tot, st);
   hipLaunchKernelGGL(k_gather_rhs<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n,
                      nrhs, perm_e, scale, b, y);
 }
 template <class T>
 void launch_scatter_sol(int32_t n, int32_t nrhs, const int32_t* perm_e, const double* scale,
-                        const T* y, T* x, hipStream_t st) {
+                        const T* y, T* x, hipStream_t st, LaunchProfiler* pf) {
   int64_t tot = (int64_t)n * nrhs;
   if (tot <= 0) return;
+  ProfScope ps(pf, LK_SOLVE_SCATTER, 0.0, 2.0 * sizeof(T) * tot, st);
   hipLaunchKernelGGL(k_scatter_sol<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n,
                      nrhs, perm_e, scale, y, x);
 }
 template <class T>
 void launch_fwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
-                    int32_t nrhs, T* y, const T* u, hipStream_t st) {
+                    int32_t nrhs, T* y, const T* u, double bytes, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
+  ProfScope ps(pf, LK_FWD_TRI, 0.0, bytes, st);
   hipLaunchKernelGGL(k_fwd_tri<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y, u);
 }
 template <class T>
 void launch_fwd_update(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count,
-                       int32_t nrhs, const T* y, T* u, hipStream_t st) {
+                       int32_t nrhs, const T* y, T* u, double bytes, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
+  ProfScope ps(pf, LK_FWD_UPDATE, 0.0, bytes, st);
   hipLaunchKernelGGL(k_fwd_update<T>, dim3(count), dim3(SNT), 0, st, S, Fc, items, nrhs, y, u);
 }
 template <class T>
 void launch_dsolve(int32_t n, int32_t nrhs, const T* d, const T* ds, const uint8_t* two, T* y,
-                   hipStream_t st) {
+                   hipStream_t st, LaunchProfiler* pf) {
   int64_t tot = (int64_t)n * nrhs;
   if (tot <= 0) return;
+  ProfScope ps(pf, LK_DSOLVE, 0.0, 2.0 * sizeof(T) * tot, st);
   hipLaunchKernelGGL(k_dsolve<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, nrhs,
                      d, ds, two, y);
 }
 template <class T>
 void launch_bwd_update(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count,
-                       int32_t nrhs, T* y, hipStream_t st) {
+                       int32_t nrhs, T* y, double bytes, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
+  ProfScope ps(pf, LK_BWD_UPDATE, 0.0, bytes, st);
   hipLaunchKernelGGL(k_bwd_update<T>, dim3(count), dim3(SNT), 0, st, S, Fc, items, nrhs, y);
 }
 template <class T>
 void launch_bwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
-                    int32_t nrhs, T* y, hipStream_t st) {
+                    int32_t nrhs, T* y, double bytes, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
+  ProfScope ps(pf, LK_BWD_TRI, 0.0, bytes, st);
   hipLaunchKernelGGL(k_bwd_tri<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y);
 }
 
 #define RSL_SINST(T)                                                                           \
   template void launch_gather_rhs<T>(int32_t, int32_t, const int32_t*, const double*, const T*, \
-                                     T*, hipStream_t);                                         \
+                                     T*, hipStream_t, LaunchProfiler*);                        \
   template void launch_scatter_sol<T>(int32_t, int32_t, const int32_t*, const double*,          \
-                                      const T*, T*, hipStream_t);                              \
+                                      const T*, T*, hipStream_t, LaunchProfiler*);             \
   template void launch_fwd_tri<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
-                                  T*, const T*, hipStream_t);                                  \
+                                  T*, const T*, double, hipStream_t, LaunchProfiler*);         \
   template void launch_fwd_update<T>(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, \
-                                     const T*, T*, hipStream_t);                               \
+                                     const T*, T*, double, hipStream_t, LaunchProfiler*);      \
   template void launch_dsolve<T>(int32_t, int32_t, const T*, const T*, const uint8_t*, T*,      \
-                                 hipStream_t);                                                 \
+                                 hipStream_t, LaunchProfiler*);                                \
   template void launch_bwd_update<T>(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, \
-                                     T*, hipStream_t);                                         \
+                                     T*, double, hipStream_t, LaunchProfiler*);                \
   template void launch_bwd_tri<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
-                                  T*, hipStream_t);
+                                  T*, double, hipStream_t, LaunchProfiler*);
 
 RSL_SINST(double)
 RSL_SINST(cplx)

@@ -1,6 +1,7 @@
 // C ABI over the host runtime (declarations and reference citations in
 // include/rslab_amd.h).
 #include <cstring>
+#include <mutex>
 #include <memory>
 #include <string>
 
@@ -12,6 +13,7 @@ struct rsl_symbolic {
 };
 struct rsl_solver {
   std::unique_ptr<rsl::DeviceFactorBase> f;
+  std::mutex mu;  // calls on one handle share its device workspace
   bool have_csc = false;
   std::vector<int64_t> colp;
   std::vector<int32_t> rowi;
@@ -45,6 +47,10 @@ rsl::FactorSettings factor_settings(const rsl_settings& s) {
   fs.pivot = s.pivot;
   fs.abs_floor = s.abs_floor;
   fs.scaling = s.scaling;
+  if (s.precision < RSL_PREC_FP64 || s.precision > RSL_PREC_BF16_SCHUR)
+    throw rsl::Error(RSL_INVALID_INPUT, "unknown precision");
+  if (s.precision != RSL_PREC_FP64)
+    throw rsl::Error(RSL_INVALID_INPUT, "reduced-precision Schur updates are not built yet");
   fs.precision = static_cast<rsl::Precision>(s.precision);
   return fs;
 }
@@ -205,6 +211,7 @@ int rsl_factor(rsl_symbolic* sym, int32_t field, const double* values, const rsl
 
 int rsl_refactor(rsl_solver* f, const double* values) {
   return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
     f->have_csc = false;
     f->f->refactor(values);
   });
@@ -212,6 +219,7 @@ int rsl_refactor(rsl_solver* f, const double* values) {
 
 int rsl_solve(rsl_solver* f, const double* b, int64_t nrhs, double* x) {
   return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
     if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
     f->f->solve_host(b, nrhs, x);
   });
@@ -219,6 +227,7 @@ int rsl_solve(rsl_solver* f, const double* b, int64_t nrhs, double* x) {
 
 int rsl_solve_device(rsl_solver* f, const void* d_b, int64_t nrhs, void* d_x, void* stream) {
   return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
     if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
     f->f->solve_device(d_b, nrhs, d_x, static_cast<hipStream_t>(stream));
   });
@@ -233,8 +242,36 @@ int rsl_factor_info(const rsl_solver* f, int64_t inertia[3], int64_t* n_perturbe
   });
 }
 
+int rsl_profile_enable(rsl_solver* f, int32_t on) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    f->f->prof.collect();
+    f->f->prof.reset();
+    f->f->prof_on = on != 0;
+  });
+}
+
+int rsl_profile_read(rsl_solver* f, int32_t kind, double out[4]) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (kind < 0 || kind >= rsl::LK_COUNT) throw rsl::Error(RSL_INVALID_INPUT, "unknown launch kind");
+    f->f->prof.collect();
+    const auto& a = f->f->prof.acc[kind];
+    out[0] = a.ms;
+    out[1] = a.flops;
+    out[2] = a.bytes;
+    out[3] = (double)a.launches;
+  });
+}
+
+const char* rsl_profile_kind_name(int32_t kind) {
+  if (kind < 0 || kind >= rsl::LK_COUNT) return nullptr;
+  return rsl::launch_kind_name(kind);
+}
+
 int rsl_factor_nnz(rsl_solver* f, int64_t* nnzl) {
   return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
     if (!f->have_csc) {
       f->f->download(f->colp, f->rowi, f->lval, f->d, f->ds, f->two, f->perm);
       f->have_csc = true;
@@ -246,6 +283,7 @@ int rsl_factor_nnz(rsl_solver* f, int64_t* nnzl) {
 int rsl_factor_download(rsl_solver* f, int64_t* l_col_ptr, int32_t* l_row_idx, double* l_values,
                         double* d_diag, double* d_subdiag, uint8_t* two_by_two, int32_t* perm) {
   return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
     if (!f->have_csc) {
       f->f->download(f->colp, f->rowi, f->lval, f->d, f->ds, f->two, f->perm);
       f->have_csc = true;
@@ -263,6 +301,7 @@ int rsl_factor_download(rsl_solver* f, int64_t* l_col_ptr, int32_t* l_row_idx, d
 int rsl_refine(rsl_solver* f, const double* a_values, const double* b, int64_t nrhs, double target,
                double* x, double* info) {
   return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
     if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
     f->f->refine(a_values, b, nrhs, target, x, info);
   });

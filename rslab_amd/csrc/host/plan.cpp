@@ -78,8 +78,15 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
       P.snap_ld[s] = ld;
       P.snap_off[s] = soff;
       soff += (int64_t)ld * ld;
-      if ((double)h * w * w > prm.wide_flops && w > 32) L.wide.push_back(s);
-      else L.small.push_back(s);
+      if ((double)h * w * w > prm.wide_flops && w > 32) {
+        L.wide.push_back(s);
+      } else {
+        L.small.push_back(s);
+        for (int64_t k = 0; k < w; ++k) L.small_flops += (double)(h - k) * (h - k) - (double)(h - w) * (h - w);
+      }
+      L.tri_entries += 0.5 * (double)w * (w + 1);
+      L.off_entries += (double)(h - w) * w;
+      L.ncols += w;
       for (int64_t k = 0; k < w; ++k) {
         P.factor_flops += (double)(h - k) * (double)(h - k);
         P.nnzl_panel += h - k;
@@ -139,6 +146,9 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
                 ok[c.lo] - first < c.tile * TILE || ok[c.hi - 1] - first >= std::min(w, (c.tile + 1) * TILE))
               throw std::runtime_error("plan: cmod task outside its tile");
             tt[(size_t)r.tile * nct + c.tile].push_back({k, a0, r.hi, c.lo, c.hi});
+            double pairs = 0;
+            for (int32_t cc = c.lo; cc < c.hi; ++cc) pairs += std::max(0, r.hi - std::max(a0, cc));
+            L.cmod_flops += 2.0 * nck * pairs;
           }
       }
       for (int32_t rb = 0; rb < nrt; ++rb)
@@ -159,6 +169,52 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     L.item_end = (int64_t)P.items.size();
   }
   if ((int64_t)P.tasks.size() > INT32_MAX) throw std::runtime_error("plan: too many cmod tasks");
+
+  // batched wide path: per level, one step per panel index
+  P.witems.clear();
+  for (LevelPlan& L : P.levels) {
+    L.wsteps.clear();
+    int maxp = 0;
+    for (int32_t s : L.wide) maxp = std::max(maxp, (P.sn_ncol[s] + P.nb_eff[s] - 1) / P.nb_eff[s]);
+    for (int p = 0; p < maxp; ++p) {
+      WideStep ws;
+      ws.g_beg = (int64_t)P.witems.size();
+      for (int32_t s : L.wide) {
+        const int32_t kb = p * P.nb_eff[s];
+        if (kb < P.sn_ncol[s]) {
+          P.witems.push_back({s, kb, 0, 0});
+          const double pw = std::min(P.nb_eff[s], P.sn_ncol[s] - kb);
+          ws.getf2_flops += pw * pw * pw / 3.0;
+        }
+      }
+      ws.g_end = ws.r_beg = (int64_t)P.witems.size();
+      for (int32_t s : L.wide) {
+        const int32_t kb = p * P.nb_eff[s];
+        if (kb >= P.sn_ncol[s]) continue;
+        const int32_t ke = std::min(kb + P.nb_eff[s], P.sn_ncol[s]);
+        const int64_t mt = (int64_t)P.sn_nrow[s] - ke;
+        for (int64_t r0 = 0; r0 < mt; r0 += 64) P.witems.push_back({s, kb, (int32_t)r0, 0});
+        const double pw = ke - kb;
+        ws.replay_flops += (double)mt * pw * (pw - 1);
+      }
+      ws.r_end = ws.t_beg = (int64_t)P.witems.size();
+      for (int32_t s : L.wide) {
+        const int32_t kb = p * P.nb_eff[s];
+        if (kb >= P.sn_ncol[s]) continue;
+        const int32_t ke = std::min(kb + P.nb_eff[s], P.sn_ncol[s]);
+        const int64_t mt = (int64_t)P.sn_nrow[s] - ke;
+        const int32_t cw = P.sn_ncol[s] - ke;
+        if (cw <= 0 || mt <= 0) continue;
+        const int32_t nrt = (int32_t)((mt + TILE - 1) / TILE), nct = (cw + TILE - 1) / TILE;
+        for (int32_t cb = 0; cb < nct; ++cb)
+          for (int32_t rb = cb; rb < nrt; ++rb) P.witems.push_back({s, kb, rb, cb});
+        const double pw = ke - kb;
+        ws.schur_flops += 2.0 * pw * ((double)cw * mt - 0.5 * (double)cw * (cw - 1));
+      }
+      ws.t_end = (int64_t)P.witems.size();
+      L.wsteps.push_back(ws);
+    }
+  }
 
   // solve plan: scratch rows, gather lists, per-level work items
   P.u_off.assign(ns, 0);

@@ -20,11 +20,19 @@ struct LevelPlan {
   int64_t item_beg = 0, item_end = 0;  // cmod items of the level
   int64_t fwd_beg = 0, fwd_end = 0;    // solve: (supernode, row chunk) L21 y items
   int64_t bwd_beg = 0, bwd_end = 0;    // solve: (supernode, column chunk) L21^T x items
+  // algorithmic work of the level (real field; 2 flops per multiply-add)
+  double cmod_flops = 0;   // sum over cmod tasks of 2 * nck * (lower-trapezoid pairs)
+  double small_flops = 0;  // cdiv of the small supernodes: sum_k (h-k)^2 - (h-w)^2
+  double tri_entries = 0;  // own-block entries sum w(w+1)/2 (solve)
+  double off_entries = 0;  // off-diagonal entries sum (h-w) w (solve)
+  int64_t ncols = 0;       // columns eliminated in the level
+  std::vector<WideStep> wsteps;
 };
 
 struct Int2 {
   int32_t x, y;
 };
+
 
 struct Plan {
   Symbolic sym;
@@ -39,6 +47,7 @@ struct Plan {
   std::vector<LevelPlan> levels;
   std::vector<CItem> items;
   std::vector<CTask> tasks;
+  std::vector<WItem> witems;    // batched wide-path work units
   // solve plan
   std::vector<int64_t> u_off;   // scratch rows of supernode s (h - w of them)
   int64_t u_total = 0;

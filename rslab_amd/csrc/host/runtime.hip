@@ -15,18 +15,88 @@ namespace rsl {
 
 // kernel launchers (factor_kernels.hip, solve_kernels.hip, refine_kernels.hip)
 template <class T> void launch_fill(T*, int64_t, T, hipStream_t);
-template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t);
-template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, hipStream_t);
-template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, hipStream_t);
-template <class T> void launch_cdiv_wide(const DevSym&, const DevFac<T>&, int, int, int64_t, int, PivotPolicy, hipStream_t);
+template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, hipStream_t, LaunchProfiler*);
 template <class T> hipError_t configure_factor_kernels();
-template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);
-template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);
-template <class T> void launch_fwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, const T*, hipStream_t);
-template <class T> void launch_fwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, const T*, T*, hipStream_t);
-template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, const uint8_t*, T*, hipStream_t);
-template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, hipStream_t);
-template <class T> void launch_bwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, hipStream_t);
+template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_fwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, const T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_fwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, const T*, T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, const uint8_t*, T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_bwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_expand(int64_t, const int64_t*, const T*, T*, hipStream_t);
+template <class T> void launch_residual(int32_t, int32_t, const int64_t*, const int32_t*, const T*, const T*, const T*, T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_colmax(int32_t, int32_t, const T*, unsigned long long*, hipStream_t);
+template <class T> void launch_update_cols(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);
+template <class T> void launch_copy_cols(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);
+template <class T> void launch_get_col(int32_t, int32_t, int32_t, const T*, T*, hipStream_t);
+template <class T> void launch_set_col(int32_t, int32_t, int32_t, T*, const T*, hipStream_t);
+template <class T> void launch_dotc(int32_t, const T*, const T*, T*, T*, hipStream_t);
+template <class T> void launch_axpy_dev(int32_t, const T*, const T*, T*, hipStream_t);
+template <class T> void launch_axpy(int32_t, T, const T*, T*, hipStream_t);
+template <class T> void launch_scal(int32_t, double, const T*, T*, hipStream_t);
+int refine_reduction_blocks();
+
+const char* launch_kind_name(int k) {
+  static const char* names[LK_COUNT] = {"assemble",     "cmod",        "cdiv_small", "panel_getf2",
+                                        "panel_replay", "panel_schur", "emit",       "solve_gather",
+                                        "fwd_tri",      "fwd_update",  "dsolve",     "bwd_update",
+                                        "bwd_tri",      "solve_scatter", "residual"};
+  return (k >= 0 && k < LK_COUNT) ? names[k] : "?";
+}
+
+EventProfiler::~EventProfiler() {
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+hipEvent_t EventProfiler::take() {
+  if (used < pool.size()) return pool[used++];
+  hipEvent_t e;
+  hip_check(hipEventCreate(&e), "hipEventCreate");
+  pool.push_back(e);
+  ++used;
+  return e;
+}
+
+void EventProfiler::begin(int kind, double flops, double bytes, hipStream_t st) {
+  Rec r;
+  r.kind = kind;
+  r.flops = flops * flop_mult;
+  r.bytes = bytes;
+  r.a = take();
+  hip_check(hipEventRecord(r.a, st), "hipEventRecord");
+  recs.push_back(r);
+}
+
+void EventProfiler::end(hipStream_t st) {
+  Rec& r = recs.back();
+  r.b = take();
+  hip_check(hipEventRecord(r.b, st), "hipEventRecord");
+}
+
+void EventProfiler::collect() {
+  for (const Rec& r : recs) {
+    hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+    Acc& a = acc[r.kind];
+    a.ms += ms;
+    a.flops += r.flops;
+    a.bytes += r.bytes;
+    a.launches += 1;
+  }
+  recs.clear();
+  used = 0;
+}
+
+void EventProfiler::reset() {
+  recs.clear();
+  used = 0;
+  for (Acc& a : acc) a = Acc();
+}
 
 void hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
@@ -102,15 +172,18 @@ void DeviceSymbolic::upload() {
   std::vector<int32_t> lists;
   level_all_off.clear();
   level_small_off.clear();
+  level_wide_off.clear();
   for (const LevelPlan& L : P.levels) {
     level_all_off.push_back((int64_t)lists.size());
     lists.insert(lists.end(), L.all.begin(), L.all.end());
     level_small_off.push_back((int64_t)lists.size());
     lists.insert(lists.end(), L.small.begin(), L.small.end());
+    level_wide_off.push_back((int64_t)lists.size());
+    lists.insert(lists.end(), L.wide.begin(), L.wide.end());
   }
   level_lists.upload(lists, stream);
-  u_colp.upload(h_colp, stream);
-  u_rowi.upload(h_rowi, stream);
+  witems.upload(P.witems, stream);
+  nb_eff.upload(P.nb_eff, stream);
   hip_check(hipStreamSynchronize(stream), "plan upload");
   view.n = P.n;
   view.ns = P.ns;
@@ -132,8 +205,43 @@ void DeviceSymbolic::upload() {
   view.amap = amap.as<int64_t>();
   view.a_row = a_row.as<int32_t>();
   view.a_col = a_col.as<int32_t>();
+  view.nb_eff = nb_eff.as<int32_t>();
   view.nnz = P.nnz;
   uploaded = true;
+}
+
+// CscMatrix::symv (src/sparse/csc.rs:294-309) applies the stored lower
+// triangle twice; the device applies the same entries as one full CSR.
+void DeviceSymbolic::ensure_full_csr() {
+  if (full_ready) return;
+  const int32_t n = plan.n;
+  std::vector<int64_t> rp(n + 1, 0);
+  for (int32_t j = 0; j < n; ++j)
+    for (int64_t q = h_colp[j]; q < h_colp[j + 1]; ++q) {
+      const int32_t i = h_rowi[q];
+      rp[i + 1]++;
+      if (i != j) rp[j + 1]++;
+    }
+  for (int32_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+  nnz_full = rp[n];
+  std::vector<int32_t> ci(nnz_full);
+  std::vector<int64_t> src(nnz_full);
+  std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
+  for (int32_t j = 0; j < n; ++j)
+    for (int64_t q = h_colp[j]; q < h_colp[j + 1]; ++q) {
+      const int32_t i = h_rowi[q];
+      ci[pos[i]] = j;
+      src[pos[i]++] = q;
+      if (i != j) {
+        ci[pos[j]] = i;
+        src[pos[j]++] = q;
+      }
+    }
+  fa_rp.upload(rp, stream);
+  fa_ci.upload(ci, stream);
+  fa_src.upload(src, stream);
+  hip_check(hipStreamSynchronize(stream), "full CSR upload");
+  full_ready = true;
 }
 
 namespace {
@@ -163,6 +271,7 @@ class DeviceFactor final : public DeviceFactorBase {
   DeviceFactor(std::shared_ptr<DeviceSymbolic> s, const FactorSettings& fs) {
     sym = std::move(s);
     settings = fs;
+    prof.flop_mult = FieldOf<T>::id ? 4.0 : 1.0;
     if (sym->plan.params.panel_nb > 128)
       throw Error(INVALID_INPUT, "panel_nb above 128 is not supported by the device factor");
     const Plan& P = sym->plan;
@@ -232,17 +341,19 @@ class DeviceFactor final : public DeviceFactorBase {
     hip_check(hipMemsetAsync(counters.p, 0, counters.bytes, st), "memset");
     const int32_t st_init[2] = {0, INT32_MAX};
     hip_check(hipMemcpyAsync(status.p, st_init, sizeof(st_init), hipMemcpyHostToDevice, st), "status");
+    LaunchProfiler* pf = prof_on ? &prof : nullptr;
     launch_fill<T>(fv.pool, P.panel_total, F<T>::zero(), st);
-    launch_assemble<T>(S, fv, vals_d.as<T>(), scale_d.as<double>(), st);
+    launch_assemble<T>(S, fv, vals_d.as<T>(), scale_d.as<double>(), st, pf);
     const int nb = P.params.panel_nb < 8 ? 8 : P.params.panel_nb;
     const int32_t* lists = sym->level_lists.as<int32_t>();
     for (size_t L = 0; L < P.levels.size(); ++L) {
       const LevelPlan& lv = P.levels[L];
       launch_cmod<T>(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
-                     sym->tasks.as<CTask>(), st);
-      launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol, st);
-      for (int32_t s : lv.wide)
-        launch_cdiv_wide<T>(S, fv, s, P.sn_ncol[s], P.sn_nrow[s], nb, pol, st);
+                     sym->tasks.as<CTask>(), lv.cmod_flops, st, pf);
+      launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
+                           lv.small_flops, st, pf);
+      launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
+                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, st, pf);
     }
     hip_check(hipGetLastError(), "factor launch");
     unsigned long long cnt[4];
@@ -263,7 +374,6 @@ class DeviceFactor final : public DeviceFactorBase {
   void solve_device(const void* d_b, int64_t nrhs, void* d_x, hipStream_t st) override {
     const Plan& P = sym->plan;
     const DevSym& S = sym->view;
-    if (!st) st = sym->stream;
     const size_t need = sizeof(T) * std::max<int64_t>(P.n * nrhs, 1);
     if (work_y.bytes < need) work_y.alloc(need);
     const size_t needu = sizeof(T) * std::max<int64_t>(P.u_total * nrhs, 1);
@@ -272,26 +382,28 @@ class DeviceFactor final : public DeviceFactorBase {
     T* u = work_b.as<T>();
     const int2* fi = sym->fwd_items.as<int2>();
     const int2* bi = sym->bwd_items.as<int2>();
+    LaunchProfiler* pf = prof_on ? &prof : nullptr;
+    const double ts = sizeof(T);
     launch_gather_rhs<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(),
-                         static_cast<const T*>(d_b), y, st);
+                         static_cast<const T*>(d_b), y, st, pf);
     const int32_t* lists = sym->level_lists.as<int32_t>();
     for (size_t L = 0; L < P.levels.size(); ++L) {
       const LevelPlan& lv = P.levels[L];
       launch_fwd_tri<T>(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), (int32_t)nrhs, y,
-                        u, st);
+                        u, ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
       launch_fwd_update<T>(S, fv, fi + lv.fwd_beg, (int)(lv.fwd_end - lv.fwd_beg), (int32_t)nrhs,
-                           y, u, st);
+                           y, u, ts * lv.off_entries, st, pf);
     }
-    launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st);
+    launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
     for (size_t L = P.levels.size(); L-- > 0;) {
       const LevelPlan& lv = P.levels[L];
       launch_bwd_update<T>(S, fv, bi + lv.bwd_beg, (int)(lv.bwd_end - lv.bwd_beg), (int32_t)nrhs, y,
-                           st);
+                           ts * lv.off_entries, st, pf);
       launch_bwd_tri<T>(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), (int32_t)nrhs, y,
-                        st);
+                        ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
     }
     launch_scatter_sol<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(), y,
-                          static_cast<T*>(d_x), st);
+                          static_cast<T*>(d_x), st, pf);
     hip_check(hipGetLastError(), "solve launch");
   }
 
@@ -362,11 +474,306 @@ class DeviceFactor final : public DeviceFactorBase {
   }
 
   void refine(const double*, const double*, int64_t, double, double*, double*) override;
+
+ private:
+  void gmres_column(int32_t n, int32_t k, int32_t c, const int64_t* rp, const int32_t* ci, const T* A,
+                    const T* bblk, T* best, double tol, double anorm, double bninf, double& best_be,
+                    int& iters, hipStream_t st);
 };
 
+// ---------------------------------------------------------------- refinement
+// refine_ladder (src/numeric/mixed.rs:140-221) with its constants (:97-105),
+// run on all right-hand sides at once: every column follows the reference's
+// single-vector ladder (IR with an FP64 residual against the original A and a
+// contraction guard, then FGMRES warm-started at the best iterate), the IR
+// steps of the columns still iterating share one multi-RHS solve.
+namespace {
+constexpr int IR_MAX = 8;
+constexpr double IR_CONTRACT = 0.5;
+constexpr int GMRES_RESTART = 30;
+constexpr int GMRES_MAX = 120;
+
+inline double hmag(double v) { return std::fabs(v); }
+inline double hmag(cplx v) { return std::hypot(v.re, v.im); }
+inline double hconj(double v) { return v; }
+inline cplx hconj(cplx v) { return {v.re, -v.im}; }
+inline double hreal(double v) { return v; }
+inline double hreal(cplx v) { return v.re; }
+inline bool hzero(double v) { return v == 0.0; }
+inline bool hzero(cplx v) { return v.re == 0.0 && v.im == 0.0; }
 template <class T>
-void DeviceFactor<T>::refine(const double*, const double*, int64_t, double, double*, double*) {
-  throw Error(INVALID_INPUT, "refinement ladder not available in this build");
+T hfrom(double v) {
+  return F<T>::from_real(v);
+}
+
+inline double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof(d));
+  return d;
+}
+}  // namespace
+
+template <class T>
+void DeviceFactor<T>::refine(const double* a_values, const double* b, int64_t nrhs, double target,
+                             double* x, double* info) {
+  const Plan& P = sym->plan;
+  hipStream_t st = sym->stream;
+  const int32_t n = P.n, k = (int32_t)nrhs;
+  sym->ensure_full_csr();
+  // ||A||_inf of the original matrix (sym_norm_inf, mixed.rs:358-371)
+  std::vector<double> rows(n, 0.0);
+  for (int32_t j = 0; j < n; ++j)
+    for (int64_t q = sym->h_colp[j]; q < sym->h_colp[j + 1]; ++q) {
+      const int32_t i = sym->h_rowi[q];
+      const double m = FieldOf<T>::id ? std::hypot(a_values[2 * q], a_values[2 * q + 1]) : std::fabs(a_values[q]);
+      rows[i] += m;
+      if (i != j) rows[j] += m;
+    }
+  double anorm = 0.0;
+  for (double r : rows) anorm = std::max(anorm, r);
+
+  const size_t blk = sizeof(T) * std::max<int64_t>((int64_t)n * k, 1);
+  DBuf av, afull, db, dx, dbest, dr, ddx, norms, mask_d, mask2_d;
+  av.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
+  afull.alloc(sizeof(T) * std::max<int64_t>(sym->nnz_full, 1));
+  db.alloc(blk);
+  dx.alloc(blk);
+  dbest.alloc(blk);
+  dr.alloc(blk);
+  ddx.alloc(blk);
+  norms.alloc(sizeof(unsigned long long) * 3 * std::max(k, 1));
+  mask_d.alloc(std::max(k, 1));
+  mask2_d.alloc(std::max(k, 1));
+  hip_check(hipMemcpyAsync(av.p, a_values, sizeof(T) * P.nnz, hipMemcpyHostToDevice, st), "upload A");
+  hip_check(hipMemcpyAsync(db.p, b, sizeof(T) * n * k, hipMemcpyHostToDevice, st), "upload b");
+  launch_expand<T>(sym->nnz_full, sym->fa_src.as<int64_t>(), av.as<T>(), afull.as<T>(), st);
+  const int64_t* rp = sym->fa_rp.as<int64_t>();
+  const int32_t* ci = sym->fa_ci.as<int32_t>();
+  const T* A = afull.as<T>();
+  auto* nrm = norms.as<unsigned long long>();
+
+  // x0 = low-precision solve of b; |b|_inf per column
+  solve_device(db.p, k, dx.p, st);
+  hip_check(hipMemsetAsync(norms.p, 0, norms.bytes, st), "memset");
+  launch_colmax<T>(n, k, db.as<T>(), nrm + 2 * k, st);
+
+  std::vector<int> ir(k, 0), phase(k, 0);  // 0 iterating, 1 certified, 2 escalate
+  std::vector<double> best_be(k, INFINITY), prev_rn(k, INFINITY), bn(k, 0.0);
+  std::vector<unsigned long long> hn(3 * k);
+  std::vector<uint8_t> keep(k), step(k);
+  bool first = true;
+  for (;;) {
+    launch_residual<T>(n, k, rp, ci, A, dx.as<T>(), db.as<T>(), dr.as<T>(), st, prof_on ? &prof : nullptr);
+    hip_check(hipMemsetAsync(norms.p, 0, sizeof(unsigned long long) * 2 * k, st), "memset");
+    launch_colmax<T>(n, k, dr.as<T>(), nrm, st);
+    launch_colmax<T>(n, k, dx.as<T>(), nrm + k, st);
+    hip_check(hipMemcpyAsync(hn.data(), norms.p, sizeof(unsigned long long) * 3 * k, hipMemcpyDeviceToHost, st),
+              "norms");
+    hip_check(hipStreamSynchronize(st), "refine norms");
+    if (first) {
+      for (int c = 0; c < k; ++c) bn[c] = bits_to_double(hn[2 * k + c]);
+      first = false;
+    }
+    bool any_step = false;
+    for (int c = 0; c < k; ++c) {
+      keep[c] = step[c] = 0;
+      if (phase[c] != 0) continue;
+      const double rn = bits_to_double(hn[c]), xn = bits_to_double(hn[k + c]);
+      const double be = rn / std::max(anorm * xn + bn[c], 2.2250738585072014e-308);
+      if (be < best_be[c]) {
+        best_be[c] = be;
+        keep[c] = 1;
+      }
+      if (be <= target) {
+        phase[c] = 1;
+        continue;
+      }
+      if (ir[c] >= IR_MAX || (ir[c] > 0 && rn > prev_rn[c] * IR_CONTRACT)) {
+        phase[c] = 2;
+        continue;
+      }
+      prev_rn[c] = rn;
+      step[c] = 1;
+      any_step = true;
+    }
+    hip_check(hipMemcpyAsync(mask_d.p, keep.data(), k, hipMemcpyHostToDevice, st), "mask");
+    launch_copy_cols<T>(n, k, mask_d.as<uint8_t>(), dbest.as<T>(), dx.as<T>(), st);
+    if (!any_step) break;
+    hip_check(hipMemcpyAsync(mask2_d.p, step.data(), k, hipMemcpyHostToDevice, st), "mask");
+    solve_device(dr.p, k, ddx.p, st);
+    launch_update_cols<T>(n, k, mask2_d.as<uint8_t>(), dx.as<T>(), ddx.as<T>(), st);
+    for (int c = 0; c < k; ++c) ir[c] += step[c];
+    // the host vectors keep/step are rewritten next pass: wait for their copies
+    hip_check(hipStreamSynchronize(st), "refine step");
+  }
+
+  std::vector<int> gm_iters(k, 0);
+  for (int c = 0; c < k; ++c) {
+    if (phase[c] != 2) continue;
+    gmres_column(n, k, c, rp, ci, A, db.as<T>(), dbest.as<T>(), target, anorm, bn[c], best_be[c],
+                 gm_iters[c], st);
+  }
+  hip_check(hipMemcpyAsync(x, dbest.p, sizeof(T) * n * k, hipMemcpyDeviceToHost, st), "download x");
+  hip_check(hipStreamSynchronize(st), "refine");
+  for (int c = 0; c < k; ++c) {
+    info[4 * c + 0] = ir[c];
+    info[4 * c + 1] = gm_iters[c];
+    info[4 * c + 2] = best_be[c];
+    info[4 * c + 3] = best_be[c] <= target ? 1.0 : 0.0;
+  }
+}
+
+// gmres (src/numeric/iterative.rs:822-1021): flexible restarted GMRES(30),
+// at most 120 iterations, MGS with DGKS re-orthogonalization (eta = 1/sqrt 2),
+// right preconditioner = the device solve with this factor, warm start at
+// column c of `best`, stop on the true residual relative to ||b||_2. The
+// result replaces column c of `best` when its backward error is smaller.
+template <class T>
+void DeviceFactor<T>::gmres_column(int32_t n, int32_t k, int32_t c, const int64_t* rp, const int32_t* ci,
+                                   const T* A, const T* bblk, T* best, double tol, double anorm, double bninf,
+                                   double& best_be, int& iters, hipStream_t st) {
+  const int m = GMRES_RESTART;
+  const double eta = 0.70710678118654752440;
+  const size_t vb = sizeof(T) * std::max(n, 1);
+  DBuf bv, xv, rv, wv, V, Z, part, hdev, nrm;
+  bv.alloc(vb);
+  xv.alloc(vb);
+  rv.alloc(vb);
+  wv.alloc(vb);
+  V.alloc(vb * (m + 1));
+  Z.alloc(vb * m);
+  part.alloc(sizeof(T) * refine_reduction_blocks());
+  hdev.alloc(sizeof(T) * (2 * (m + 1) + 2));
+  nrm.alloc(sizeof(unsigned long long) * 2);
+  T* b1 = bv.as<T>();
+  T* x1 = xv.as<T>();
+  T* r1 = rv.as<T>();
+  T* w1 = wv.as<T>();
+  T* Vb = V.as<T>();
+  T* Zb = Z.as<T>();
+  T* hd = hdev.as<T>();
+  auto vcol = [&](int i) { return Vb + (size_t)i * n; };
+  auto zcol = [&](int i) { return Zb + (size_t)i * n; };
+  auto dot_to_host = [&](const T* a, const T* bb) {
+    launch_dotc<T>(n, a, bb, part.as<T>(), hd, st);
+    T out;
+    hip_check(hipMemcpyAsync(&out, hd, sizeof(T), hipMemcpyDeviceToHost, st), "dot");
+    hip_check(hipStreamSynchronize(st), "dot");
+    return out;
+  };
+  launch_get_col<T>(n, k, c, bblk, b1, st);
+  launch_get_col<T>(n, k, c, best, x1, st);
+  const double bnorm = std::sqrt(hreal(dot_to_host(b1, b1)));
+  iters = 0;
+  if (bnorm == 0.0) return;  // x = 0 is exact, and best already has berr 0 then
+  std::vector<T> h((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 1), hcol2(m + 1);
+  int total = 0;
+  for (;;) {
+    launch_residual<T>(n, 1, rp, ci, A, x1, b1, r1, st, prof_on ? &prof : nullptr);
+    const double beta = std::sqrt(hreal(dot_to_host(r1, r1)));
+    if (beta / bnorm <= tol || total >= GMRES_MAX) break;
+    launch_scal<T>(n, 1.0 / beta, r1, vcol(0), st);
+    std::fill(h.begin(), h.end(), F<T>::zero());
+    std::fill(cs.begin(), cs.end(), F<T>::zero());
+    std::fill(sn.begin(), sn.end(), F<T>::zero());
+    std::fill(g.begin(), g.end(), F<T>::zero());
+    g[0] = hfrom<T>(beta);
+    int jdim = 0;
+    for (int j = 0; j < m; ++j) {
+      if (total >= GMRES_MAX) break;
+      solve_device(vcol(j), 1, zcol(j), st);
+      launch_residual<T>(n, 1, rp, ci, A, zcol(j), nullptr, w1, st, prof_on ? &prof : nullptr);
+      launch_dotc<T>(n, w1, w1, part.as<T>(), hd + 2 * (m + 1), st);  // ||w0||^2
+      for (int i = 0; i <= j; ++i) {
+        launch_dotc<T>(n, vcol(i), w1, part.as<T>(), hd + i, st);
+        launch_axpy_dev<T>(n, hd + i, vcol(i), w1, st);
+      }
+      launch_dotc<T>(n, w1, w1, part.as<T>(), hd + 2 * (m + 1) + 1, st);
+      std::vector<T> tail(2);
+      hip_check(hipMemcpyAsync(hcol.data(), hd, sizeof(T) * (j + 1), hipMemcpyDeviceToHost, st), "h");
+      hip_check(hipMemcpyAsync(tail.data(), hd + 2 * (m + 1), sizeof(T) * 2, hipMemcpyDeviceToHost, st), "h");
+      hip_check(hipStreamSynchronize(st), "arnoldi");
+      const double wnorm0 = std::sqrt(hreal(tail[0]));
+      double hnrm = std::sqrt(hreal(tail[1]));
+      for (int i = 0; i <= j; ++i) h[(size_t)i * m + j] = hcol[i];
+      if (hnrm < eta * wnorm0) {
+        for (int i = 0; i <= j; ++i) {
+          launch_dotc<T>(n, vcol(i), w1, part.as<T>(), hd + (m + 1) + i, st);
+          launch_axpy_dev<T>(n, hd + (m + 1) + i, vcol(i), w1, st);
+        }
+        launch_dotc<T>(n, w1, w1, part.as<T>(), hd + 2 * (m + 1) + 1, st);
+        hip_check(hipMemcpyAsync(hcol2.data(), hd + (m + 1), sizeof(T) * (j + 1), hipMemcpyDeviceToHost, st),
+                  "h");
+        hip_check(hipMemcpyAsync(tail.data(), hd + 2 * (m + 1), sizeof(T) * 2, hipMemcpyDeviceToHost, st), "h");
+        hip_check(hipStreamSynchronize(st), "arnoldi");
+        for (int i = 0; i <= j; ++i) h[(size_t)i * m + j] = h[(size_t)i * m + j] + hcol2[i];
+        hnrm = std::sqrt(hreal(tail[1]));
+      }
+      h[(size_t)(j + 1) * m + j] = hfrom<T>(hnrm);
+      launch_scal<T>(n, hnrm > 0.0 ? 1.0 / hnrm : 0.0, w1, vcol(j + 1), st);
+      for (int i = 0; i < j; ++i) {
+        const T a = h[(size_t)i * m + j], bb = h[(size_t)(i + 1) * m + j];
+        const T temp = hconj(cs[i]) * a + hconj(sn[i]) * bb;
+        h[(size_t)(i + 1) * m + j] = F<T>::zero() - sn[i] * a + cs[i] * bb;
+        h[(size_t)i * m + j] = temp;
+      }
+      // givens (iterative.rs:599-609)
+      const T f = h[(size_t)j * m + j], gg = h[(size_t)(j + 1) * m + j];
+      T cj, sj;
+      if (hzero(gg)) {
+        cj = F<T>::one();
+        sj = F<T>::zero();
+      } else if (hzero(f)) {
+        cj = F<T>::zero();
+        sj = F<T>::one();
+      } else {
+        const double r = std::sqrt(F<T>::magsq(f) + F<T>::magsq(gg));
+        cj = F<T>::scale(f, 1.0 / r);
+        sj = F<T>::scale(gg, 1.0 / r);
+      }
+      cs[j] = cj;
+      sn[j] = sj;
+      h[(size_t)j * m + j] = hconj(cj) * f + hconj(sj) * gg;
+      h[(size_t)(j + 1) * m + j] = F<T>::zero();
+      const T g_next = F<T>::zero() - sj * g[j];
+      g[j] = hconj(cj) * g[j];
+      g[j + 1] = g_next;
+      ++total;
+      jdim = j + 1;
+      if (hmag(g[j + 1]) / bnorm <= tol) break;
+    }
+    // well_conditioned_dim_flat (iterative.rs:646-661), back substitution, x += Z y
+    double hmax = 0.0;
+    for (int i = 0; i < jdim; ++i) hmax = std::max(hmax, hmag(h[(size_t)i * m + i]));
+    int jd = jdim;
+    for (int i = 0; i < jdim; ++i)
+      if (hmag(h[(size_t)i * m + i]) <= 2.220446049250313e-16 * hmax) {
+        jd = i;
+        break;
+      }
+    for (int i = jd - 1; i >= 0; --i) {
+      T s = g[i];
+      for (int q = i + 1; q < jd; ++q) s = s - h[(size_t)i * m + q] * y[q];
+      y[i] = s * F<T>::recip(h[(size_t)i * m + i]);
+    }
+    for (int i = 0; i < jd; ++i) launch_axpy<T>(n, y[i], zcol(i), x1, st);
+  }
+  iters = total;
+  // backward error of the GMRES result; keep whichever iterate is better
+  launch_residual<T>(n, 1, rp, ci, A, x1, b1, r1, st, prof_on ? &prof : nullptr);
+  hip_check(hipMemsetAsync(nrm.p, 0, nrm.bytes, st), "memset");
+  launch_colmax<T>(n, 1, r1, nrm.as<unsigned long long>(), st);
+  launch_colmax<T>(n, 1, x1, nrm.as<unsigned long long>() + 1, st);
+  unsigned long long hb[2];
+  hip_check(hipMemcpyAsync(hb, nrm.p, sizeof(hb), hipMemcpyDeviceToHost, st), "norms");
+  hip_check(hipStreamSynchronize(st), "gmres");
+  const double be2 =
+      bits_to_double(hb[0]) / std::max(anorm * bits_to_double(hb[1]) + bninf, 2.2250738585072014e-308);
+  if (be2 < best_be) {
+    best_be = be2;
+    launch_set_col<T>(n, k, c, best, x1, st);
+    hip_check(hipStreamSynchronize(st), "gmres");
+  }
 }
 
 }  // namespace

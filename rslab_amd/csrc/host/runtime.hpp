@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../device/devtypes.hpp"
+#include "../device/prof.hpp"
 #include "../device/scalar.hpp"
 #include "plan.hpp"
 
@@ -66,17 +67,48 @@ struct DeviceSymbolic {
   hipStream_t stream = nullptr;
   DBuf sn_first, sn_ncol, sn_nrow, e_off, panel_off, rs_off, rs_idx, upd_ptr, upd_idx, sym_perm,
       snap_off, snap_ld, amap, a_row, a_col, items, tasks, level_lists, u_off, gat_ptr, gat_idx,
+      witems, nb_eff,
       fwd_items, bwd_items;
-  std::vector<int64_t> level_all_off, level_small_off;  // offsets into level_lists
-  // user CSC (for SpMV in refinement)
-  DBuf u_colp, u_rowi;
+  std::vector<int64_t> level_all_off, level_small_off, level_wide_off;  // offsets into level_lists
+  // user lower CSC pattern (host) and, built on first refinement, the full
+  // symmetric CSR of A with each entry's source index in the CSC values
   std::vector<int64_t> h_colp;
   std::vector<int32_t> h_rowi;
+  DBuf fa_rp, fa_ci, fa_src;
+  int64_t nnz_full = 0;
+  bool full_ready = false;
   DevSym view{};
   bool uploaded = false;
   ~DeviceSymbolic();
   void upload();
+  void ensure_full_csr();
 };
+
+// Per-launch event timing, attached to the launchers while enabled.
+struct EventProfiler final : LaunchProfiler {
+  struct Rec {
+    int kind;
+    double flops, bytes;
+    hipEvent_t a, b;
+  };
+  struct Acc {
+    double ms = 0, flops = 0, bytes = 0;
+    int64_t launches = 0;
+  };
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<Rec> recs;
+  Acc acc[LK_COUNT];
+  double flop_mult = 1.0;  // 4 for the complex field (4 real products per complex one)
+  ~EventProfiler() override;
+  void begin(int kind, double flops, double bytes, hipStream_t st) override;
+  void end(hipStream_t st) override;
+  void collect();  // waits for the recorded events, folds them into acc
+  void reset();
+ private:
+  hipEvent_t take();
+};
+const char* launch_kind_name(int k);
 
 struct FactorStats {
   double t_factor_ms = 0;
@@ -102,6 +134,8 @@ class DeviceFactorBase {
                       double* x, double* info) = 0;
   FactorStats stats;
   FactorSettings settings;
+  EventProfiler prof;
+  bool prof_on = false;
   std::shared_ptr<DeviceSymbolic> sym;
 };
 

@@ -420,6 +420,34 @@ class LdltSolver:
         _check(lib.rsl_solve(self._h, ptr(bv, C.c_double), k, ptr(xv, C.c_double)))
         return x
 
+    def solve_device(self, d_b: int, nrhs: int, d_x: int, stream: int = 0):
+        """Device-resident solve: d_b, d_x are device addresses of row-major
+        n x nrhs arrays; enqueued on `stream` (0 = the library stream)."""
+        if nrhs <= 0:
+            raise DimensionMismatch("nrhs must be positive")
+        _check(lib.rsl_solve_device(self._h, C.c_void_p(d_b), int(nrhs), C.c_void_p(d_x),
+                                    C.c_void_p(stream) if stream else None))
+
+    def profile(self, on: bool = True):
+        """Enable (and clear) or disable per-kernel event timing."""
+        _check(lib.rsl_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        """{kind: {ms, flops, bytes, launches}} for every launch kind seen."""
+        out = {}
+        k = 0
+        while True:
+            name = lib.rsl_profile_kind_name(k)
+            if name is None:
+                break
+            v = np.zeros(4)
+            _check(lib.rsl_profile_read(self._h, k, ptr(v, C.c_double)))
+            if v[3] > 0:
+                out[name.decode()] = dict(ms=float(v[0]), flops=float(v[1]), bytes=float(v[2]),
+                                          launches=int(v[3]))
+            k += 1
+        return out
+
     def inertia(self) -> Inertia:
         i = np.zeros(3, dtype=np.int64)
         _check(lib.rsl_factor_info(self._h, ptr(i, C.c_int64), None, None))

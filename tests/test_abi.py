@@ -1,0 +1,116 @@
+"""C ABI checks that need no GPU: every function declared in include/rslab_amd.h
+is exported by librslab_amd.so and bound by the Python mirror, the settings
+defaults equal SolverSettings::default (multifrontal_ldlt.rs:583-614), and the
+host-side validation reports the reference's error kinds."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import rslab_amd as R
+from rslab_amd import matgen
+from rslab_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rslab_amd.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(rsl_[a-z_0-9]+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for must in ("rsl_analyze", "rsl_factor", "rsl_refactor", "rsl_solve", "rsl_solve_device",
+                 "rsl_factor_download", "rsl_refine", "rsl_free_factor", "rsl_free_symbolic"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    # and the ctypes mirror binds each of them
+    assert set(declared_functions()) <= set(_lib.SIGNATURES)
+    lib = C.CDLL(_lib.LIB_PATH)
+    for n in declared_functions():
+        assert getattr(lib, n) is not None
+
+
+def test_settings_defaults_match_reference():
+    s = _lib.Settings()
+    _lib.lib.rsl_settings_default(C.byref(s))
+    assert (s.ordering, s.nemin, s.relax, s.relax_width, s.relax_rows) == (0, 16, 1, 256, 64)
+    assert (s.panel_nb, s.pivot, s.scaling, s.precision, s.nd_seed) == (64, 0, 1, 0, 1)
+    py = R.SolverSettings().to_c()
+    for f, _ in _lib.Settings._fields_:
+        assert getattr(py, f) == getattr(s, f), f
+
+
+def lower(n, cols, rows):
+    return R.CscMatrix(n, np.asarray(cols), np.asarray(rows), np.ones(len(rows)))
+
+
+@pytest.mark.parametrize("case,err", [
+    ("upper", R.InvalidInput),          # entry above the diagonal
+    ("unsorted", R.InvalidInput),       # rows not strictly increasing
+    ("oob", R.InvalidInput),            # row index >= n
+    ("colptr", R.InvalidInput),         # decreasing column pointers
+    ("panel_nb", R.InvalidInput),       # panel width beyond the device limit
+])
+def test_analyze_rejects_invalid_input(case, err):
+    st = R.SolverSettings()
+    if case == "upper":
+        a = lower(3, [0, 1, 3, 4], [0, 0, 1, 2])
+    elif case == "unsorted":
+        a = lower(3, [0, 2, 3, 4], [1, 0, 1, 2])
+    elif case == "oob":
+        a = lower(3, [0, 1, 2, 3], [0, 1, 5])
+    elif case == "colptr":
+        a = lower(3, [0, 2, 1, 3], [0, 1, 2])
+    else:
+        a = lower(3, [0, 1, 2, 3], [0, 1, 2])
+        st = st.with_panel_nb(256)
+    with pytest.raises(err):
+        R.LdltSymbolic.analyze_with(a, st)
+
+
+def test_factor_rejects_pattern_mismatch_before_device_work():
+    a = R.CscMatrix.from_triplets(3, [0, 1, 2], [0, 1, 2], [1.0, 2.0, 3.0])
+    sym = R.LdltSymbolic.analyze(a)
+    b = R.CscMatrix.from_triplets(4, [0, 1, 2, 3], [0, 1, 2, 3], [1.0, 2.0, 3.0, 4.0])
+    with pytest.raises(R.InvalidInput):
+        sym.factor(b)
+
+
+def test_from_triplets_sums_duplicates_and_rejects_upper():
+    # CscMatrix::from_triplets (csc.rs:41-106)
+    a = R.CscMatrix.from_triplets(3, [0, 2, 2, 1], [0, 0, 0, 1], [1.0, 2.0, 3.0, 4.0])
+    assert a.col_ptr.tolist() == [0, 2, 3, 3] and a.row_idx.tolist() == [0, 2, 1]
+    assert a.values.tolist() == [1.0, 5.0, 4.0]
+    with pytest.raises(R.InvalidInput):
+        R.CscMatrix.from_triplets(3, [0], [1], [1.0])
+
+
+def test_rng_matches_reference_xorshift():
+    """Rng (src/matgen/mod.rs:63-96): state = (seed ^ 0x9E3779B97F4A7C15) | 1,
+    xorshift64* (12, 25, 27, * 0x2545F4914F6CDD1D), uniform = (x >> 11) / 2^53."""
+    st = (5 ^ 0x9E3779B97F4A7C15) | 1
+    want = []
+    M = (1 << 64) - 1
+    for _ in range(4):
+        st ^= st >> 12
+        st ^= (st << 25) & M
+        st ^= st >> 27
+        v = (st * 0x2545F4914F6CDD1D) & M
+        want.append(-2.0 + 5.0 * ((v >> 11) / float(1 << 53)))
+    got = matgen.rng_uniform(5, -2.0, 3.0, 4)
+    assert np.array_equal(got, np.array(want))
