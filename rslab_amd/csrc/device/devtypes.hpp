@@ -70,6 +70,19 @@ struct WideStep {
   double getf2_flops = 0, replay_flops = 0, schur_flops = 0;
 };
 
+// Solve work unit on a wide supernode's own block: diagonal block j0 of
+// supernode s; for an update unit r0 = first own row of a 64-row chunk, for a
+// diagonal unit (r0, x) = (count, first index) of its partial sums.
+struct SItem {
+  int32_t s, j0, r0, x;
+};
+
+// One 64-column block step of a level's wide supernodes in the solve:
+// diagonal units [d_beg, d_end), update units [u_beg, u_end) (of sitems).
+struct SolveStep {
+  int64_t d_beg = 0, d_end = 0, u_beg = 0, u_end = 0;
+};
+
 struct PivotPolicy {
   int32_t have_floor;
   double floor;

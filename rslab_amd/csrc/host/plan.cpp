@@ -11,6 +11,7 @@ namespace rsl {
 
 namespace {
 constexpr int TILE = 64;
+constexpr int SOLVE_WIDE = 128;  // own blocks wider than this are solved in 64-column steps
 }
 
 void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanParams& prm) {
@@ -237,17 +238,54 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   }
   P.fwd_items.clear();
   P.bwd_items.clear();
+  P.gat_items.clear();
+  P.sitems.clear();
+  P.max_step_upd = 0;
   for (LevelPlan& L : P.levels) {
     L.fwd_beg = (int64_t)P.fwd_items.size();
     L.bwd_beg = (int64_t)P.bwd_items.size();
+    L.gat_beg = (int64_t)P.gat_items.size();
+    L.snarrow.clear();
+    L.swide.clear();
+    L.ssteps.clear();
     for (int32_t s : L.all) {
       const int32_t off = P.sn_nrow[s] - P.sn_ncol[s];
       for (int32_t r0 = 0; r0 < off; r0 += 64) P.fwd_items.push_back({s, r0});
       if (off > 0)
         for (int32_t p0 = 0; p0 < P.sn_ncol[s]; p0 += 16) P.bwd_items.push_back({s, p0});
+      bool any = false;
+      for (int32_t c = 0; c < P.sn_ncol[s] && !any; ++c)
+        any = P.gat_ptr[P.sn_first[s] + c + 1] > P.gat_ptr[P.sn_first[s] + c];
+      if (any)
+        for (int32_t r0 = 0; r0 < P.sn_ncol[s]; r0 += 64) P.gat_items.push_back({s, r0});
+      if (P.sn_ncol[s] > SOLVE_WIDE) L.swide.push_back(s);
+      else L.snarrow.push_back(s);
     }
     L.fwd_end = (int64_t)P.fwd_items.size();
     L.bwd_end = (int64_t)P.bwd_items.size();
+    L.gat_end = (int64_t)P.gat_items.size();
+    int32_t maxb = 0;
+    for (int32_t s : L.swide) maxb = std::max(maxb, (P.sn_ncol[s] + 63) / 64);
+    for (int32_t b = 0; b < maxb; ++b) {
+      SolveStep st;
+      st.u_beg = (int64_t)P.sitems.size();
+      std::vector<std::pair<int32_t, int32_t>> span;  // per wide supernode: (first unit, count)
+      for (int32_t s : L.swide) {
+        const int32_t j0 = b * 64, w = P.sn_ncol[s];
+        const int32_t first = (int32_t)((int64_t)P.sitems.size() - st.u_beg);
+        if (j0 < w)
+          for (int32_t r0 = j0 + 64; r0 < w; r0 += 64) P.sitems.push_back({s, j0, r0, 0});
+        span.push_back({first, (int32_t)((int64_t)P.sitems.size() - st.u_beg) - first});
+      }
+      st.u_end = st.d_beg = (int64_t)P.sitems.size();
+      for (size_t q = 0; q < L.swide.size(); ++q) {
+        const int32_t s = L.swide[q], j0 = b * 64;
+        if (j0 < P.sn_ncol[s]) P.sitems.push_back({s, j0, span[q].second, span[q].first});
+      }
+      st.d_end = (int64_t)P.sitems.size();
+      P.max_step_upd = std::max(P.max_step_upd, st.u_end - st.u_beg);
+      L.ssteps.push_back(st);
+    }
   }
 
   // A scatter map: user lower entry (i >= j) -> slot of P^T A P in its panel

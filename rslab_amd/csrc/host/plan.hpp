@@ -20,6 +20,9 @@ struct LevelPlan {
   int64_t item_beg = 0, item_end = 0;  // cmod items of the level
   int64_t fwd_beg = 0, fwd_end = 0;    // solve: (supernode, row chunk) L21 y items
   int64_t bwd_beg = 0, bwd_end = 0;    // solve: (supernode, column chunk) L21^T x items
+  int64_t gat_beg = 0, gat_end = 0;    // solve: (supernode, own-row chunk) gather items
+  std::vector<int32_t> snarrow, swide; // solve: own block in one workgroup / in block steps
+  std::vector<SolveStep> ssteps;       // solve: block steps of the wide ones
   // algorithmic work of the level (real field; 2 flops per multiply-add)
   double cmod_flops = 0;   // sum over cmod tasks of 2 * nck * (lower-trapezoid pairs)
   double small_flops = 0;  // cdiv of the small supernodes: sum_k (h-k)^2 - (h-w)^2
@@ -52,7 +55,9 @@ struct Plan {
   std::vector<int64_t> u_off;   // scratch rows of supernode s (h - w of them)
   int64_t u_total = 0;
   std::vector<int64_t> gat_ptr, gat_idx;  // per permuted column, ascending updater
-  std::vector<Int2> fwd_items, bwd_items;
+  std::vector<Int2> fwd_items, bwd_items, gat_items;
+  std::vector<SItem> sitems;
+  int64_t max_step_upd = 0;     // largest number of update units in one solve step
   int64_t panel_total = 0;      // elements in the panel pool
   int64_t snap_total = 0;       // elements of multiplier scratch
   // user lower-CSC entries -> panel slot

@@ -22,7 +22,10 @@ template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const
 template <class T> hipError_t configure_factor_kernels();
 template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
-template <class T> void launch_fwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, const T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_fgather(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, const T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_fwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_ftri_step(const DevSym&, const DevFac<T>&, const SItem*, const SolveStep&, int32_t, T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_btri_step(const DevSym&, const DevFac<T>&, const SItem*, const SolveStep&, int32_t, T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_fwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, const T*, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, const uint8_t*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
@@ -173,6 +176,7 @@ void DeviceSymbolic::upload() {
   level_all_off.clear();
   level_small_off.clear();
   level_wide_off.clear();
+  level_snarrow_off.clear();
   for (const LevelPlan& L : P.levels) {
     level_all_off.push_back((int64_t)lists.size());
     lists.insert(lists.end(), L.all.begin(), L.all.end());
@@ -180,9 +184,13 @@ void DeviceSymbolic::upload() {
     lists.insert(lists.end(), L.small.begin(), L.small.end());
     level_wide_off.push_back((int64_t)lists.size());
     lists.insert(lists.end(), L.wide.begin(), L.wide.end());
+    level_snarrow_off.push_back((int64_t)lists.size());
+    lists.insert(lists.end(), L.snarrow.begin(), L.snarrow.end());
   }
   level_lists.upload(lists, stream);
   witems.upload(P.witems, stream);
+  gat_items.upload(P.gat_items, stream);
+  sitems.upload(P.sitems, stream);
   nb_eff.upload(P.nb_eff, stream);
   hip_check(hipStreamSynchronize(stream), "plan upload");
   view.n = P.n;
@@ -263,7 +271,7 @@ template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, snap, deep, counters, status, scale_d, vals_d,
-      work_y, work_b;
+      work_y, work_b, work_p;
   DevFac<T> fv{};
   std::vector<double> scale;
   bool configured = false;
@@ -384,13 +392,20 @@ class DeviceFactor final : public DeviceFactorBase {
     const int2* bi = sym->bwd_items.as<int2>();
     LaunchProfiler* pf = prof_on ? &prof : nullptr;
     const double ts = sizeof(T);
+    const size_t needp = sizeof(T) * std::max<int64_t>(P.max_step_upd * 64 * nrhs, 1);
+    if (work_p.bytes < needp) work_p.alloc(needp);
+    T* part = work_p.as<T>();
+    const SItem* si = sym->sitems.as<SItem>();
+    const int2* gi = sym->gat_items.as<int2>();
     launch_gather_rhs<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(),
                          static_cast<const T*>(d_b), y, st, pf);
     const int32_t* lists = sym->level_lists.as<int32_t>();
     for (size_t L = 0; L < P.levels.size(); ++L) {
       const LevelPlan& lv = P.levels[L];
-      launch_fwd_tri<T>(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), (int32_t)nrhs, y,
-                        u, ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
+      launch_fgather<T>(S, fv, gi + lv.gat_beg, (int)(lv.gat_end - lv.gat_beg), (int32_t)nrhs, y, u, st, pf);
+      launch_fwd_tri<T>(S, fv, lists + sym->level_snarrow_off[L], (int)lv.snarrow.size(), (int32_t)nrhs, y,
+                        ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
+      for (const SolveStep& stp : lv.ssteps) launch_ftri_step<T>(S, fv, si, stp, (int32_t)nrhs, y, st, pf);
       launch_fwd_update<T>(S, fv, fi + lv.fwd_beg, (int)(lv.fwd_end - lv.fwd_beg), (int32_t)nrhs,
                            y, u, ts * lv.off_entries, st, pf);
     }
@@ -399,8 +414,10 @@ class DeviceFactor final : public DeviceFactorBase {
       const LevelPlan& lv = P.levels[L];
       launch_bwd_update<T>(S, fv, bi + lv.bwd_beg, (int)(lv.bwd_end - lv.bwd_beg), (int32_t)nrhs, y,
                            ts * lv.off_entries, st, pf);
-      launch_bwd_tri<T>(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), (int32_t)nrhs, y,
+      launch_bwd_tri<T>(S, fv, lists + sym->level_snarrow_off[L], (int)lv.snarrow.size(), (int32_t)nrhs, y,
                         ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
+      for (size_t q = lv.ssteps.size(); q-- > 0;)
+        launch_btri_step<T>(S, fv, si, lv.ssteps[q], (int32_t)nrhs, y, part, st, pf);
     }
     launch_scatter_sol<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(), y,
                           static_cast<T*>(d_x), st, pf);
