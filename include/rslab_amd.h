@@ -77,6 +77,18 @@ void rsl_settings_default(rsl_settings *s);
 int rsl_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
                 const rsl_settings *s, rsl_symbolic **out);
 
+/* Batch of independent systems (no counterpart in rslab, which factors one
+ * matrix per call; for callers with many small systems of uneven size):
+ * each of the `count` lower-CSC matrices is analysed on its own exactly as by
+ * rsl_analyze, and they are laid out as one block-diagonal system
+ * diag(A_0, ..., A_count-1). rsl_factor, rsl_refactor, rsl_solve and
+ * rsl_refine then act on the whole batch in one launch sequence (values, b and
+ * x concatenated in system order); rsl_factor_inertia_batch gives each
+ * system's inertia. */
+int rsl_analyze_batch(int32_t count, const int64_t *n, const int64_t *const *col_ptr,
+                      const int64_t *const *row_idx, const rsl_settings *s, rsl_symbolic **out);
+int rsl_factor_inertia_batch(rsl_solver *f, int32_t count, int64_t *inertia);
+
 /* Symbolic sizes: out[0]=n out[1]=nsuper out[2]=sum of column counts
  * out[3]=panel entries out[4]=levels out[5]=children entries
  * out[6]=full permuted pattern nnz out[7]=stored L entries (padded) and

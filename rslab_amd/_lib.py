@@ -65,6 +65,9 @@ SIGNATURES = {
     "rsl_order": (C.c_int, [C.c_int32, C.c_int64, _I64P, _I64P, C.c_uint64, _I32P, _DP]),
     "rsl_rng_uniform": (None, [C.c_uint64, C.c_double, C.c_double, C.c_int64, _DP]),
     "rsl_profile_enable": (C.c_int, [_P, C.c_int32]),
+    "rsl_analyze_batch": (C.c_int, [C.c_int32, _I64P, C.POINTER(_I64P), C.POINTER(_I64P), C.POINTER(Settings),
+                                    C.POINTER(_P)]),
+    "rsl_factor_inertia_batch": (C.c_int, [_P, C.c_int32, _I64P]),
     "rsl_profile_read": (C.c_int, [_P, C.c_int32, _DP]),
     "rsl_profile_kind_name": (C.c_char_p, [C.c_int32]),
 }

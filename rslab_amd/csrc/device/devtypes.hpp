@@ -43,6 +43,7 @@ struct DevFac {
   T* snap;            // per-panel time-of-step multipliers (scratch)
   int32_t* deep;      // per-panel interchange partners (scratch, nb per supernode)
   unsigned long long* counters;  // [0] pos [1] neg [2] zero [3] perturbed
+  int32_t* sn_inertia;          // per supernode: pos, neg, zero
   int32_t* status;    // [0] 0 ok / 1 rank deficient, [1] failing supernode
 };
 
@@ -68,6 +69,7 @@ struct WItem {
 struct WideStep {
   int64_t g_beg = 0, g_end = 0, r_beg = 0, r_end = 0, t_beg = 0, t_end = 0;
   double getf2_flops = 0, replay_flops = 0, schur_flops = 0;
+  int32_t max_kb = 0;  // largest panel start of the step (columns whose rows are interchanged)
 };
 
 // Solve work unit on a wide supernode's own block: diagonal block j0 of

@@ -13,9 +13,12 @@
 //   emit        :3196-3256  inertia, elimination maps
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "devtypes.hpp"
 #include "prof.hpp"
 #include "scalar.hpp"
+#include "tile_mfma.hpp"
 
 namespace rsl {
 
@@ -24,6 +27,10 @@ constexpr int TM = 64;   // tile rows
 constexpr int TN = 64;   // tile columns
 constexpr int KC = 16;   // inner chunk
 constexpr int NBMAX = 128;
+constexpr int GT = 64;   // threads of the wide-path getf2 (one wave: its barriers are free)
+constexpr int STG = 1152;  // staging elements per operand (fits every tile core's chunk)
+static_assert(STG >= KC * TM && STG >= tile_stage_elems<TP_F64>() && STG * 8 >= tile_stage_elems<TP_BF16>() * 2,
+              "staging buffer too small");
 
 // ---------------------------------------------------------------------------
 // Tile core: acc[ii][jj] += sum_c  L[ar + i, cb + c] * G[br + j, c]
@@ -52,20 +59,33 @@ __device__ void ldl_tile_gemm(const T* __restrict__ L, int64_t ld, int64_t cb, i
                               const T* __restrict__ d, const T* __restrict__ ds,
                               const uint8_t* __restrict__ two, T (&acc)[4][4], T* As, T* Bs) {
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  constexpr int PER = KC * TM / NT;
+  T va[PER], vb[PER];
+  auto fetch = [&](int c0) {
+    const int kc = min(KC, K - c0);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * NT, r = e % TM, c = e / TM;
+      va[u] = F<T>::zero();
+      vb[u] = F<T>::zero();
+      if (c < kc) {
+        const int64_t col = cb + c0 + c;
+        if (r < ma) va[u] = L[(ar + r) + col * ld];
+        if (r < mb) vb[u] = g_entry(L, ld, br + r, col, c0 + c, d, ds, two);
+      }
+    }
+  };
+  fetch(0);
   for (int c0 = 0; c0 < K; c0 += KC) {
     const int kc = min(KC, K - c0);
-    for (int e = tid; e < KC * TM; e += NT) {
-      int r = e % TM, c = e / TM;
-      T va = F<T>::zero(), vb = F<T>::zero();
-      if (c < kc) {
-        int64_t col = cb + c0 + c;
-        if (r < ma) va = L[(ar + r) + col * ld];
-        if (r < mb) vb = g_entry(L, ld, br + r, col, c0 + c, d, ds, two);
-      }
-      As[c * TM + r] = va;
-      Bs[c * TN + r] = vb;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * NT, r = e % TM, c = e / TM;
+      As[c * TM + r] = va[u];
+      Bs[c * TN + r] = vb[u];
     }
     __syncthreads();
+    if (c0 + KC < K) fetch(c0 + KC);
     for (int c = 0; c < kc; ++c) {
       T a[4], b[4];
 #pragma unroll
@@ -107,13 +127,12 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
 // so no two workgroups write the same entry) and subtracts every relevant
 // updater's  L_k[a] D_k L_k[b]^T  block in ascending updater order.
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, int PR>
 __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
                                              const CItem* __restrict__ items,
                                              const CTask* __restrict__ tasks) {
   __shared__ T acc_s[TM * TN];
-  __shared__ T As[KC * TM];
-  __shared__ T Bs[KC * TN];
+  __shared__ T stage[2 * STG];
   __shared__ int32_t trow[TM], rpos[TM], cpos[TN];
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const CItem it = items[blockIdx.x];
@@ -123,9 +142,16 @@ __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
   T* P = Fc.pool + S.panel_off[s];
   const int32_t* rs = S.rs_idx + S.rs_off[s];
   const int mr = min<int64_t>(TM, h - it.r0), mc = min(TN, w - it.c0);
-  for (int e = tid; e < TM * TN; e += NT) {
-    int i = e % TM, j = e / TM;
-    acc_s[e] = (i < mr && j < mc) ? P[(it.r0 + i) + (int64_t)(it.c0 + j) * h] : F<T>::zero();
+  {
+    // the target tile, 16 independent loads per thread before the LDS stores
+    T v[TM * TN / NT];
+#pragma unroll
+    for (int u = 0; u < TM * TN / NT; ++u) {
+      const int e = tid + u * NT, i = e % TM, j = e / TM;
+      v[u] = (i < mr && j < mc) ? P[(it.r0 + i) + (int64_t)(it.c0 + j) * h] : F<T>::zero();
+    }
+#pragma unroll
+    for (int u = 0; u < TM * TN / NT; ++u) acc_s[tid + u * NT] = v[u];
   }
   for (int i = tid; i < mr; i += NT) trow[i] = rs[it.r0 + i];
   __syncthreads();
@@ -146,23 +172,37 @@ __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
       rpos[i] = lo;
     }
     for (int j = tid; j < mb; j += NT) cpos[j] = okg[tk.b0 + j] - first - it.c0;
-    T acc[4][4];
+    if constexpr (std::is_same<T, double>::value) {
+      typedef typename TileCfg<PR>::S Sx;
+      TileAcc<PR> acc;
+      tile_zero(acc);
+      ldl_tile_mfma<PR>(Pk, hk, 0, nck, nck + tk.a0, ma, nck + tk.b0, mb, Fc.d + eo, Fc.ds + eo, Fc.two + eo, acc,
+                       reinterpret_cast<Sx*>(stage), reinterpret_cast<Sx*>(stage + STG));
+      tile_for_each<PR>(acc, [&](int i, int j, double v) {
+        if (i < ma && j < mb && tk.a0 + i >= tk.b0 + j) {
+          const int e = rpos[i] + cpos[j] * TM;
+          acc_s[e] = acc_s[e] - v;
+        }
+      });
+    } else {
+      T acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
-    ldl_tile_gemm(Pk, hk, 0, nck, nck + tk.a0, ma, nck + tk.b0, mb, Fc.d + eo, Fc.ds + eo,
-                  Fc.two + eo, acc, As, Bs);
+        for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
+      ldl_tile_gemm(Pk, hk, 0, nck, nck + tk.a0, ma, nck + tk.b0, mb, Fc.d + eo, Fc.ds + eo,
+                    Fc.two + eo, acc, stage, stage + STG);
 #pragma unroll
-    for (int ii = 0; ii < 4; ++ii) {
-      int i = ty + 16 * ii;
-      if (i >= ma) continue;
+      for (int ii = 0; ii < 4; ++ii) {
+        int i = ty + 16 * ii;
+        if (i >= ma) continue;
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) {
-        int j = tx + 16 * jj;
-        if (j < mb && tk.a0 + i >= tk.b0 + j) {
-          int e = rpos[i] + cpos[j] * TM;
-          acc_s[e] = acc_s[e] - acc[ii][jj];
+        for (int jj = 0; jj < 4; ++jj) {
+          int j = tx + 16 * jj;
+          if (j < mb && tk.a0 + i >= tk.b0 + j) {
+            int e = rpos[i] + cpos[j] * TM;
+            acc_s[e] = acc_s[e] - acc[ii][jj];
+          }
         }
       }
     }
@@ -179,6 +219,7 @@ __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
 // ---------------------------------------------------------------------------
 template <class T>
 struct SnCtx {
+  int s;
   T* P;
   int64_t h;
   int w, first, eo, nb;
@@ -192,6 +233,7 @@ struct SnCtx {
 template <class T>
 __device__ inline SnCtx<T> make_ctx(const DevSym& S, const DevFac<T>& Fc, int s, int nb) {
   SnCtx<T> c;
+  c.s = s;
   c.P = Fc.pool + S.panel_off[s];
   c.h = S.sn_nrow[s];
   c.w = S.sn_ncol[s];
@@ -204,7 +246,7 @@ __device__ inline SnCtx<T> make_ctx(const DevSym& S, const DevFac<T>& Fc, int s,
   c.lperm = Fc.lperm + c.eo;
   c.snap = Fc.snap + S.snap_off[s];
   c.snap_ld = S.snap_ld[s];
-  c.deep = Fc.deep + (int64_t)s * NBMAX;
+  c.deep = Fc.deep + (int64_t)s * 2 * NBMAX;
   return c;
 }
 
@@ -239,9 +281,9 @@ __device__ inline T perturb_pivot(T dv, double floor) {
 }
 
 // Initialise the per-supernode D / pivot bookkeeping.
-template <class T>
+template <int NTG, class T>
 __device__ void cdiv_init(const SnCtx<T>& c) {
-  for (int p = threadIdx.x; p < c.w; p += NT) {
+  for (int p = threadIdx.x; p < c.w; p += NTG) {
     c.lperm[p] = p;
     c.two[p] = 0;
     c.ds[p] = F<T>::zero();
@@ -253,17 +295,24 @@ __device__ void cdiv_init(const SnCtx<T>& c) {
 // Candidates, interchanges and updates bounded to the panel rows, as
 // ll_cdiv_emit's getf2 (multifrontal_ldlt.rs:2846-3016). Returns 1 when the
 // pivot policy is Fail and a zero / singular pivot is met (uniform).
-template <class T>
-__device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* l2s,
-                           int32_t* deep_s, PivotPolicy pol, int* npert) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+template <int NTG, class T>
+__device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* l2s, T* w1s, T* w2s,
+                           int32_t* deep_s, int32_t* two_s, int32_t* lperm_s, PivotPolicy pol, int* npert,
+                           bool write_snap) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  constexpr int RWP = NTG < NBMAX ? NTG : NBMAX;  // trailing rows per pass
+  constexpr int CS = NTG / RWP;                   // column stride of a thread
   const int pw = ke - kb, ld = pw;
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   const double GROWTH_EPS = 1e-14;
   T* snap = c.snap;
   const int sld = c.snap_ld;
 #define A_(i, j) Ab[pk_off((i), (j), ld)]
-  for (int q = tid; q < pw; q += NT) deep_s[q] = -1;
+  for (int q = tid; q < pw; q += NTG) {
+    deep_s[q] = -1;
+    two_s[q] = 0;
+    lperm_s[q] = c.lperm[kb + q];
+  }
   __syncthreads();
   int k = 0;
   while (k < pw) {
@@ -312,7 +361,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
     const int p = (kstep == 1) ? k : k + 1;
     if (kp != p) {
       const int q = kp;
-      for (int t = tid; t < pw; t += NT) {
+      for (int t = tid; t < pw; t += NTG) {
         int i = q + 1 + t;
         if (i < pw) {
           T x = A_(i, p);
@@ -334,9 +383,9 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
           T x = A_(p, p);
           A_(p, p) = A_(q, q);
           A_(q, q) = x;
-          int32_t y = c.lperm[kb + p];
-          c.lperm[kb + p] = c.lperm[kb + q];
-          c.lperm[kb + q] = y;
+          int32_t y = lperm_s[p];
+          lperm_s[p] = lperm_s[q];
+          lperm_s[q] = y;
           deep_s[k] = q;
         }
       }
@@ -358,18 +407,41 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       if (tid == 0) c.d[kb + k] = dk;
       const T dinv = F<T>::recip(dk);
       // time-of-step multipliers and in-block rank-1 update (rows of this panel)
+      // column k (w1s) and its multipliers (l1s), then the rank-1 update of the
+      // trailing block row by row (two threads per row, alternate columns)
+      // with column k scaled in the same pass
       const int nr = pw - k - 1;
-      for (int t = tid; t < nr; t += NT) {
-        int j = k + 1 + t;
-        snap[(int64_t)k * sld + j] = A_(j, k) * dinv;
-      }
-      for (int j = k + 1 + wv; j < pw; j += 4) {
-        const T m = A_(j, k) * dinv;
-        if (F<T>::is_zero(m)) continue;
-        for (int i = j + lane; i < pw; i += 64) A_(i, j) = A_(i, j) - A_(i, k) * m;
+      for (int t = tid; t < nr; t += NTG) {
+        const int j = k + 1 + t;
+        const T a = A_(j, k);
+        const T m = a * dinv;
+        w1s[j] = a;
+        l1s[j] = m;
+        if (write_snap) snap[(int64_t)k * sld + j] = m;
       }
       __syncthreads();
-      for (int i = k + 1 + tid; i < pw; i += NT) A_(i, k) = A_(i, k) * dinv;
+      for (int i = k + 1 + (tid % RWP); i < pw; i += RWP) {
+        const T a = w1s[i];
+        int j = k + 1 + tid / RWP;
+        // batches of 8 columns: all loads issued before the stores
+        for (; j + 7 * CS <= i; j += 8 * CS) {
+          T m[8], x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            m[u] = l1s[j + CS * u];
+            x[u] = A_(i, j + CS * u);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[u] = F<T>::is_zero(m[u]) ? x[u] : x[u] - a * m[u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) A_(i, j + CS * u) = x[u];
+        }
+        for (; j <= i; j += CS) {
+          const T m = l1s[j];
+          if (!F<T>::is_zero(m)) A_(i, j) = A_(i, j) - a * m;
+        }
+        if (tid < RWP) A_(i, k) = l1s[i];
+      }
       __syncthreads();
       k += 1;
     } else {
@@ -396,26 +468,41 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         c.ds[kb + k] = d21;
         c.d[kb + k + 1] = d22;
         c.two[kb + k] = 1;
+        two_s[k] = 1;
       }
-      for (int i = k + 2 + tid; i < pw; i += NT) {
+      for (int i = k + 2 + tid; i < pw; i += NTG) {
         T wik = A_(i, k), wik1 = A_(i, k + 1);
         T a = (d22 * wik - d21 * wik1) * detinv;
         T b = (d11 * wik1 - d21 * wik) * detinv;
         l1s[i] = a;
         l2s[i] = b;
-        snap[(int64_t)k * sld + i] = a;
-        snap[(int64_t)(k + 1) * sld + i] = b;
+        w1s[i] = wik;
+        w2s[i] = wik1;
+        if (write_snap) {
+          snap[(int64_t)k * sld + i] = a;
+          snap[(int64_t)(k + 1) * sld + i] = b;
+        }
       }
       __syncthreads();
-      for (int j = k + 2 + wv; j < pw; j += 4) {
-        const T m1 = l1s[j], m2 = l2s[j];
-        for (int i = j + lane; i < pw; i += 64)
-          A_(i, j) = A_(i, j) - A_(i, k) * m1 - A_(i, k + 1) * m2;
-      }
-      __syncthreads();
-      for (int i = k + 2 + tid; i < pw; i += NT) {
-        A_(i, k) = l1s[i];
-        A_(i, k + 1) = l2s[i];
+      for (int i = k + 2 + (tid % RWP); i < pw; i += RWP) {
+        const T a = w1s[i], b = w2s[i];
+        int j = k + 2 + tid / RWP;
+        for (; j + 7 * CS <= i; j += 8 * CS) {
+          T m1[8], m2[8], x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            m1[u] = l1s[j + CS * u];
+            m2[u] = l2s[j + CS * u];
+            x[u] = A_(i, j + CS * u);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) A_(i, j + CS * u) = x[u] - a * m1[u] - b * m2[u];
+        }
+        for (; j <= i; j += CS) A_(i, j) = A_(i, j) - a * l1s[j] - b * l2s[j];
+        if (tid < RWP) {
+          A_(i, k) = l1s[i];
+          A_(i, k + 1) = l2s[i];
+        }
       }
       __syncthreads();
       k += 2;
@@ -426,45 +513,94 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
 }
 
 // Load the lower part of the diagonal block into packed LDS.
-template <class T>
+template <int NTG, class T>
 __device__ void panel_load(const SnCtx<T>& c, int kb, int ke, T* Ab) {
-  const int pw = ke - kb;
-  for (int j = 0; j < pw; ++j)
-    for (int i = j + threadIdx.x; i < pw; i += NT)
-      Ab[pk_off(i, j, pw)] = c.P[(kb + i) + (int64_t)(kb + j) * c.h];
+  const int pw = ke - kb, n = pw * pw;
+  // batches of 8 independent loads per thread before their LDS stores
+  for (int e0 = threadIdx.x; e0 < n; e0 += 8 * NTG) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * NTG;
+      const int i = e % pw, j = e / pw;
+      v[u] = (e < n && i >= j) ? c.P[(kb + i) + (int64_t)(kb + j) * c.h] : F<T>::zero();
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * NTG;
+      const int i = e % pw, j = e / pw;
+      if (e < n && i >= j) Ab[pk_off(i, j, pw)] = v[u];
+    }
+  }
   __syncthreads();
 }
 
-// Write the factored block back, replay its row interchanges on the already
-// factored columns [0,kb) and publish the interchange partners.
-template <class T>
-__device__ void panel_store(const SnCtx<T>& c, int kb, int ke, const T* Ab,
-                            const int32_t* deep_s) {
+// Write the factored block back, apply its row interchanges to the already
+// factored columns [0,kb) of the panel rows (staged 64 columns at a time in
+// the block's LDS, which is free once written back), and publish the
+// interchange partners and the composite permutation (position q holds the
+// row / column that sat at position cperm[q] before this panel's steps).
+template <int NTG, bool SWAP_COLS, class T>
+__device__ void panel_store(const SnCtx<T>& c, int kb, int ke, T* Ab, const int32_t* deep_s,
+                            const int32_t* two_s, const int32_t* lperm_s, int32_t* cperm_s, int* any_s) {
   const int pw = ke - kb;
-  for (int j = 0; j < pw; ++j)
-    for (int i = j + threadIdx.x; i < pw; i += NT)
-      c.P[(kb + i) + (int64_t)(kb + j) * c.h] = Ab[pk_off(i, j, pw)];
-  for (int j = threadIdx.x; j < kb; j += NT) {
-    T* col = c.P + (int64_t)j * c.h;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < pw * pw; e += NTG) {
+    const int i = e % pw, j = e / pw;
+    if (i >= j) c.P[(kb + i) + (int64_t)(kb + j) * c.h] = Ab[pk_off(i, j, pw)];
+  }
+  for (int q = threadIdx.x; q < pw; q += NTG) c.lperm[kb + q] = lperm_s[q];
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < pw; ++q) cperm_s[q] = q;
+    int any = 0;
     for (int k = 0; k < pw;) {
-      int kp = deep_s[k];
-      bool two = c.two[kb + k];
+      const int kp = deep_s[k];
       if (kp >= 0) {
-        int src = two ? k + 1 : k;
-        T x = col[kb + src];
-        col[kb + src] = col[kb + kp];
-        col[kb + kp] = x;
+        const int src = two_s[k] ? k + 1 : k;
+        const int t = cperm_s[src];
+        cperm_s[src] = cperm_s[kp];
+        cperm_s[kp] = t;
+        any = 1;
       }
-      k += two ? 2 : 1;
+      k += two_s[k] ? 2 : 1;
+    }
+    *any_s = any;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < pw; q += NTG) {
+    c.deep[q] = deep_s[q];
+    c.deep[NBMAX + q] = cperm_s[q];
+  }
+  if (SWAP_COLS && *any_s && kb > 0) {
+    // CB columns at a time: every thread gathers its (up to 8) permuted
+    // sources into registers, the workgroup synchronizes, then they are stored
+    constexpr int PER = 8;
+    constexpr int CB = PER * NTG / NBMAX;
+    for (int j0 = 0; j0 < kb; j0 += CB) {
+      const int nc = min(CB, kb - j0), n = pw * nc;
+      T v[PER];
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * NTG;
+        const int i = e % pw, j = e / pw;
+        v[u] = e < n ? c.P[(kb + cperm_s[i]) + (int64_t)(j0 + j) * c.h] : F<T>::zero();
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * NTG;
+        const int i = e % pw, j = e / pw;
+        if (e < n) c.P[(kb + i) + (int64_t)(j0 + j) * c.h] = v[u];
+      }
+      __syncthreads();
     }
   }
-  for (int q = threadIdx.x; q < pw; q += NT) c.deep[q] = deep_s[q];
   __syncthreads();
 }
 
 // apply_bk_panel_trailing on deep rows i in [r0, r1), one row per thread.
 template <class T>
-__device__ void panel_replay_rows(const SnCtx<T>& c, int kb, int ke, const int32_t* deep,
+__device__ void panel_replay_rows(const SnCtx<T>& c, int kb, int ke, const int32_t* deep, const int32_t* two_s,
                                   int64_t r0, int64_t r1, int64_t tstart, int64_t tstride) {
   const int64_t h = c.h;
   const int sld = c.snap_ld;
@@ -474,7 +610,7 @@ __device__ void panel_replay_rows(const SnCtx<T>& c, int kb, int ke, const int32
     while (k < ke) {
       const int kl = k - kb;
       const int kp = deep[kl];
-      const bool two = c.two[k];
+      const bool two = two_s[kl] != 0;
       if (kp >= 0) {
         int src = two ? k + 1 : k;
         T x = row[(int64_t)src * h];
@@ -508,33 +644,54 @@ __device__ void panel_replay_rows(const SnCtx<T>& c, int kb, int ke, const int32
 }
 
 // Deferred Schur update of one 64x64 tile of P[ke.., ke..w).
-template <class T>
-__device__ void panel_schur_tile(const SnCtx<T>& c, int kb, int ke, int rr0, int cc0, T* As,
-                                 T* Bs) {
+template <int PR, class T>
+__device__ void panel_schur_tile(const SnCtx<T>& c, int kb, int ke, int rr0, int cc0, T* stage) {
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int64_t mt = c.h - ke;
   const int cw = c.w - ke, pw = ke - kb;
   const int ma = (int)min<int64_t>(TM, mt - rr0), mb = min(TN, cw - cc0);
-  T acc[4][4];
+  if constexpr (std::is_same<T, double>::value) {
+    typedef typename TileCfg<PR>::S Sx;
+    TileAcc<PR> acc;
+    tile_zero(acc);
+    ldl_tile_mfma<PR>(c.P, c.h, kb, pw, ke + rr0, ma, ke + cc0, mb, c.d + kb, c.ds + kb, c.two + kb, acc,
+                     reinterpret_cast<Sx*>(stage), reinterpret_cast<Sx*>(stage + STG));
+    tile_for_each<PR>(acc, [&](int i, int j, double v) {
+      if (i < ma && j < mb && rr0 + i >= cc0 + j) {
+        double* dst = c.P + (ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h;
+        *dst = *dst - v;
+      }
+    });
+  } else {
+    T acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
-  ldl_tile_gemm(c.P, c.h, kb, pw, ke + rr0, ma, ke + cc0, mb, c.d + kb, c.ds + kb, c.two + kb,
-                acc, As, Bs);
+      for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
+    ldl_tile_gemm(c.P, c.h, kb, pw, ke + rr0, ma, ke + cc0, mb, c.d + kb, c.ds + kb, c.two + kb,
+                  acc, stage, stage + STG);
 #pragma unroll
-  for (int ii = 0; ii < 4; ++ii) {
-    int i = ty + 16 * ii;
-    if (i >= ma) continue;
+    for (int ii = 0; ii < 4; ++ii) {
+      int i = ty + 16 * ii;
+      if (i >= ma) continue;
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      int j = tx + 16 * jj;
-      if (j < mb && rr0 + i >= cc0 + j) {
-        T* dst = c.P + (ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h;
-        *dst = *dst - acc[ii][jj];
+      for (int jj = 0; jj < 4; ++jj) {
+        int j = tx + 16 * jj;
+        if (j < mb && rr0 + i >= cc0 + j) {
+          T* dst = c.P + (ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h;
+          *dst = *dst - acc[ii][jj];
+        }
       }
     }
   }
+}
+
+// Out-of-line copy for the one-workgroup supernode kernel, so the tile core's
+// registers are not kept live across its getf2 and replay phases.
+template <int PR, class T>
+__device__ __attribute__((noinline)) void panel_schur_tile_call(const SnCtx<T>& c, int kb, int ke, int rr0, int cc0,
+                                                                T* stage) {
+  panel_schur_tile<PR>(c, kb, ke, rr0, cc0, stage);
 }
 
 // Emit maps and inertia for supernode s (multifrontal_ldlt.rs:3196-3256).
@@ -568,6 +725,7 @@ __device__ void cdiv_emit(const DevSym& S, const DevFac<T>& Fc, const SnCtx<T>& 
     }
   }
   __syncthreads();
+  if (threadIdx.x < 3) Fc.sn_inertia[3 * (int64_t)c.s + threadIdx.x] = cnt[threadIdx.x];
   if (threadIdx.x == 0) {
     atomicAdd(&Fc.counters[0], (unsigned long long)cnt[0]);
     atomicAdd(&Fc.counters[1], (unsigned long long)cnt[1]);
@@ -590,19 +748,17 @@ template <class T>
 struct CdivShared {
   union {
     T Ab[NBMAX * (NBMAX + 1) / 2];
-    struct {
-      T As[KC * TM];
-      T Bs[KC * TN];
-    } g;
+    T stage[2 * STG];
   } u;
-  T l1s[NBMAX], l2s[NBMAX];
-  int32_t deep_s[NBMAX];
+  T l1s[NBMAX], l2s[NBMAX], w1s[NBMAX], w2s[NBMAX];
+  int32_t deep_s[NBMAX], two_s[NBMAX], cperm_s[NBMAX], lperm_s[NBMAX];
   int cnt[4];
   int npert;
+  int any_swap;
 };
 
 // One workgroup factors a whole supernode (all its nb-panels).
-template <class T>
+template <class T, int PR>
 __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
                                                    const int32_t* __restrict__ list, int nb_base,
                                                    PivotPolicy pol) {
@@ -613,25 +769,25 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
   const int nb = (w >= 512 && nb_base < 128) ? 128 : nb_base;
   SnCtx<T> c = make_ctx(S, Fc, s, nb);
   if (threadIdx.x == 0) sh.npert = 0;
-  cdiv_init(c);
+  cdiv_init<NT>(c);
   __syncthreads();
   for (int kb = 0; kb < w; kb += nb) {
     const int ke = min(kb + nb, w);
-    panel_load(c, kb, ke, sh.u.Ab);
-    int st = panel_getf2(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.deep_s, pol, &sh.npert);
+    panel_load<NT>(c, kb, ke, sh.u.Ab);
+    int st = panel_getf2<NT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert, true);
     if (st) {
       flag_failure(Fc, s);
       return;
     }
-    panel_store(c, kb, ke, sh.u.Ab, sh.deep_s);
-    panel_replay_rows(c, kb, ke, sh.deep_s, ke, c.h, threadIdx.x, NT);
+    panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
+    panel_replay_rows(c, kb, ke, sh.deep_s, sh.two_s, ke, c.h, threadIdx.x, NT);
     __syncthreads();
     const int64_t mt = c.h - ke;
     const int cw = w - ke;
     if (cw > 0 && mt > 0) {
       const int nrt = (int)((mt + TM - 1) / TM), nct = (cw + TN - 1) / TN;
       for (int cb = 0; cb < nct; ++cb)
-        for (int rb = cb; rb < nrt; ++rb) panel_schur_tile(c, kb, ke, rb * TM, cb * TN, sh.u.g.As, sh.u.g.Bs);
+        for (int rb = cb; rb < nrt; ++rb) panel_schur_tile_call<PR>(c, kb, ke, rb * TM, cb * TN, sh.u.stage);
       __syncthreads();
     }
   }
@@ -641,7 +797,7 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
 // ---- wide supernodes: batched over the level, one launch per phase of each
 // panel step (every supernode of the level that has a p-th panel) ----
 template <class T>
-__global__ __launch_bounds__(NT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
+__global__ __launch_bounds__(GT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
                                                    PivotPolicy pol) {
   extern __shared__ unsigned char smem_raw[];
   CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
@@ -650,89 +806,180 @@ __global__ __launch_bounds__(NT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const
   SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
   if (Fc.status[0]) return;  // the factorization has already failed
   if (threadIdx.x == 0) sh.npert = 0;
-  if (kb == 0) cdiv_init(c);
+  if (kb == 0) cdiv_init<GT>(c);
   __syncthreads();
   const int ke = min(kb + c.nb, c.w);
-  panel_load(c, kb, ke, sh.u.Ab);
-  int st = panel_getf2(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.deep_s, pol, &sh.npert);
+  panel_load<GT>(c, kb, ke, sh.u.Ab);
+  int st = panel_getf2<GT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert, false);
   if (st) {
     flag_failure(Fc, s);
     return;
   }
-  panel_store(c, kb, ke, sh.u.Ab, sh.deep_s);
+  panel_store<GT, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
   if (threadIdx.x == 0 && sh.npert) atomicAdd(&Fc.counters[3], (unsigned long long)sh.npert);
 }
 
-// Deep-row replay with the rows staged in LDS: one row per thread (column-major
-// chunk, so a wave's accesses are conflict-free), the reference's per-row step
-// sequence (apply_bk_panel_trailing) replayed on the staged copy.
-constexpr int RR = 64;  // rows per workgroup = threads per workgroup
+// The panel's row interchanges applied to the already factored columns
+// [0, kb) of its rows: blockIdx.y = the step's getf2 unit, blockIdx.x = a
+// block of 64 columns. Every thread gathers its permuted sources into
+// registers, the workgroup synchronizes, then they are stored.
+template <class T>
+__global__ __launch_bounds__(NT) void k_wide_swapcols(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
+  __shared__ int32_t cperm[NBMAX];
+  __shared__ int ident;
+  const WItem it = items[blockIdx.y];
+  const int s = it.s, kb = it.kb, j0 = 64 * blockIdx.x;
+  if (j0 >= kb || Fc.status[0]) return;
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  const int pw = min(kb + c.nb, c.w) - kb;
+  if (threadIdx.x == 0) ident = 1;
+  __syncthreads();
+  for (int q = threadIdx.x; q < pw; q += NT) {
+    cperm[q] = c.deep[NBMAX + q];
+    if (cperm[q] != q) ident = 0;
+  }
+  __syncthreads();
+  if (ident) return;
+  constexpr int PER = NBMAX * 64 / NT;
+  const int nc = min(64, kb - j0), n = pw * nc;
+  T v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = threadIdx.x + u * NT;
+    const int i = e % pw, j = e / pw;
+    v[u] = e < n ? c.P[(kb + cperm[i]) + (int64_t)(j0 + j) * c.h] : F<T>::zero();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = threadIdx.x + u * NT;
+    const int i = e % pw, j = e / pw;
+    if (e < n) c.P[(kb + i) + (int64_t)(j0 + j) * c.h] = v[u];
+  }
+}
+
+// Deep-row replay (apply_bk_panel_trailing, multifrontal_ldlt.rs:1347-1416) of
+// 64 rows [ke + a, ke + a + 64): the rows are staged in LDS with the panel's
+// composite column permutation applied up front, then the panel's steps are
+// replayed in order with the final multipliers L11 (a content's multiplier
+// moves with it under the later symmetric interchanges, so this is the same
+// sequence of operations), all 256 threads sharing each step's update, and
+// finally scaled by the 1x1 / 2x2 blocks of D.
+constexpr int RR = 64;  // deep rows per workgroup
+// L11 columns staged per chunk (LDS: (RR + RCW + 1) * NBMAX elements)
+template <class T>
+constexpr int replay_chunk() {
+  return sizeof(T) == 16 ? 8 : 32;
+}
 
 template <class T>
-__global__ __launch_bounds__(RR) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
+__global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
   extern __shared__ unsigned char smem_raw[];
-  T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR]
-  __shared__ int32_t deep_s[NBMAX];
+  T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
+  constexpr int RCW = replay_chunk<T>();
+  T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
+  __shared__ int32_t cperm[NBMAX];
+  __shared__ int32_t two_s[NBMAX];
   const WItem it = items[blockIdx.x];
   const int s = it.s, kb = it.kb;
   SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
   if (Fc.status[0]) return;
   const int ke = min(kb + c.nb, c.w), pw = ke - kb;
-  const int t = threadIdx.x;
-  for (int q = t; q < pw; q += RR) deep_s[q] = c.deep[q];
-  const int64_t i = ke + (int64_t)it.a + t;
-  const bool live = i < c.h;
-  if (live)
-    for (int j = 0; j < pw; ++j) X[j * RR + t] = c.P[i + (int64_t)(kb + j) * c.h];
+  const int64_t row0 = ke + (int64_t)it.a;
+  const int nr = (int)min<int64_t>(RR, c.h - row0);
+  const int tid = threadIdx.x, r = tid & (RR - 1), q = tid / RR;
+  for (int j = tid; j < pw; j += NT) {
+    cperm[j] = c.deep[NBMAX + j];
+    two_s[j] = c.two[kb + j];
+  }
   __syncthreads();
-  if (live) {
-    const int sld = c.snap_ld;
-    int k = 0;
-    while (k < pw) {
-      const int kp = deep_s[k];
-      const bool two = c.two[kb + k];
-      if (kp >= 0) {
-        const int src = two ? k + 1 : k;
-        T x = X[src * RR + t];
-        X[src * RR + t] = X[kp * RR + t];
-        X[kp * RR + t] = x;
-      }
-      if (two) {
-        const T d11 = c.d[kb + k], d21 = c.ds[kb + k], d22 = c.d[kb + k + 1];
-        const T detinv = F<T>::recip(d11 * d22 - d21 * d21);
-        const T wk = X[k * RR + t], wk1 = X[(k + 1) * RR + t];
-        const T* m1 = c.snap + (int64_t)k * sld;
-        const T* m2 = c.snap + (int64_t)(k + 1) * sld;
-        for (int j = k + 2; j < pw; ++j) X[j * RR + t] = X[j * RR + t] - wk * m1[j] - wk1 * m2[j];
-        X[k * RR + t] = (d22 * wk - d21 * wk1) * detinv;
-        X[(k + 1) * RR + t] = (d11 * wk1 - d21 * wk) * detinv;
+  for (int e = tid; e < pw * RR; e += NT) {
+    const int j = e / RR, rr = e % RR;
+    X[e] = rr < nr ? c.P[row0 + rr + (int64_t)(kb + cperm[j]) * c.h] : F<T>::zero();
+  }
+  const T* L = c.P + kb + (int64_t)kb * c.h;  // L11(j, k) = L[j + k * h]
+  int k = 0;
+  for (int k0 = 0; k0 < pw; k0 += RCW) {
+    // stage L11 columns [k0, k0 + RCW) (one more when a 2x2 block straddles the chunk end)
+    const int kc = min(RCW + 1, pw - k0);
+    __syncthreads();
+    for (int e = tid; e < kc * NBMAX; e += NT) {
+      const int j = e % NBMAX, kk = e / NBMAX;
+      Lc[e] = j < pw ? L[j + (int64_t)(k0 + kk) * c.h] : F<T>::zero();
+    }
+    __syncthreads();
+    const int kend = min(k0 + RCW, pw);
+    while (k < kend) {
+      const T* m1 = Lc + (k - k0) * NBMAX;
+      constexpr int G = NT / RR;  // column stride of a thread
+      if (two_s[k]) {
+        const T* m2 = m1 + NBMAX;
+        const T wk = X[k * RR + r], wk1 = X[(k + 1) * RR + r];
+        int j = k + 2 + q;
+        for (; j + 7 * G < pw; j += 8 * G) {
+          T a1[8], a2[8], x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            a1[u] = m1[j + u * G];
+            a2[u] = m2[j + u * G];
+            x[u] = X[(j + u * G) * RR + r];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) X[(j + u * G) * RR + r] = x[u] - wk * a1[u] - wk1 * a2[u];
+        }
+        for (; j < pw; j += G) X[j * RR + r] = X[j * RR + r] - wk * m1[j] - wk1 * m2[j];
         k += 2;
       } else {
-        const T dinv = F<T>::recip(c.d[kb + k]);
-        const T wk = X[k * RR + t];
-        const T* m = c.snap + (int64_t)k * sld;
-        for (int j = k + 1; j < pw; ++j) {
-          const T mj = m[j];
-          if (!F<T>::is_zero(mj)) X[j * RR + t] = X[j * RR + t] - wk * mj;
+        const T wk = X[k * RR + r];
+        int j = k + 1 + q;
+        for (; j + 7 * G < pw; j += 8 * G) {
+          T a1[8], x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            a1[u] = m1[j + u * G];
+            x[u] = X[(j + u * G) * RR + r];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            X[(j + u * G) * RR + r] = F<T>::is_zero(a1[u]) ? x[u] : x[u] - wk * a1[u];
         }
-        X[k * RR + t] = wk * dinv;
+        for (; j < pw; j += G) {
+          const T m = m1[j];
+          if (!F<T>::is_zero(m)) X[j * RR + r] = X[j * RR + r] - wk * m;
+        }
         k += 1;
       }
+      __syncthreads();
     }
-    for (int j = 0; j < pw; ++j) c.P[i + (int64_t)(kb + j) * c.h] = X[j * RR + t];
+  }
+  for (int kk = q; kk < pw; kk += NT / RR) {
+    if (kk > 0 && two_s[kk - 1]) continue;  // second column of a 2x2: done with the first
+    if (two_s[kk]) {
+      const T d11 = c.d[kb + kk], d21 = c.ds[kb + kk], d22 = c.d[kb + kk + 1];
+      const T detinv = F<T>::recip(d11 * d22 - d21 * d21);
+      const T wk = X[kk * RR + r], wk1 = X[(kk + 1) * RR + r];
+      X[kk * RR + r] = (d22 * wk - d21 * wk1) * detinv;
+      X[(kk + 1) * RR + r] = (d11 * wk1 - d21 * wk) * detinv;
+    } else {
+      X[kk * RR + r] = X[kk * RR + r] * F<T>::recip(c.d[kb + kk]);
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < pw * RR; e += NT) {
+    const int j = e / RR, rr = e % RR;
+    if (rr < nr) c.P[row0 + rr + (int64_t)(kb + j) * c.h] = X[e];
   }
 }
 
-template <class T>
+template <class T, int PR>
 __global__ __launch_bounds__(NT) void k_wide_schur(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
-  __shared__ T As[KC * TM];
-  __shared__ T Bs[KC * TN];
+  __shared__ T stage[2 * STG];
   const WItem it = items[blockIdx.x];
   const int s = it.s, kb = it.kb;
   SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
   if (Fc.status[0]) return;
   const int ke = min(kb + c.nb, c.w);
-  panel_schur_tile(c, kb, ke, it.a * TM, it.b * TN, As, Bs);
+  panel_schur_tile<PR>(c, kb, ke, it.a * TM, it.b * TN, stage);
 }
 
 template <class T>
@@ -769,46 +1016,65 @@ void launch_assemble(const DevSym& S, const DevFac<T>& Fc, const T* vals, const 
                      Fc, vals, scale);
 }
 
+// Reduced precisions apply to the real field only; the complex field always
+// runs its FP64 (VALU) tile core.
+#define RSL_PREC_SWITCH(T, prec, STMT)           \
+  do {                                           \
+    if (sizeof(T) == 16 || (prec) == TP_F64) {   \
+      constexpr int P_ = TP_F64;                 \
+      STMT;                                      \
+    } else if ((prec) == TP_F32) {               \
+      constexpr int P_ = TP_F32;                 \
+      STMT;                                      \
+    } else {                                     \
+      constexpr int P_ = TP_BF16;                \
+      STMT;                                      \
+    }                                            \
+  } while (0)
+
 template <class T>
 void launch_cmod(const DevSym& S, const DevFac<T>& Fc, const CItem* items, int nitems,
-                 const CTask* tasks, double flops, hipStream_t st, LaunchProfiler* pf) {
+                 const CTask* tasks, double flops, int prec, hipStream_t st, LaunchProfiler* pf) {
   if (nitems <= 0) return;
   ProfScope ps(pf, LK_CMOD, flops, 0.0, st);
-  hipLaunchKernelGGL(k_cmod<T>, dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks);
+  RSL_PREC_SWITCH(T, prec, hipLaunchKernelGGL((k_cmod<T, P_>), dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks));
 }
 
 template <class T>
 void launch_cdiv_small(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, int count,
-                       int nb, PivotPolicy pol, double flops, hipStream_t st, LaunchProfiler* pf) {
+                       int nb, PivotPolicy pol, double flops, int prec, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
   ProfScope ps(pf, LK_CDIV_SMALL, flops, 0.0, st);
-  hipLaunchKernelGGL(k_cdiv_small<T>, dim3(count), dim3(NT), cdiv_smem<T>(), st, S, Fc, list, nb,
-                     pol);
+  RSL_PREC_SWITCH(T, prec, hipLaunchKernelGGL((k_cdiv_small<T, P_>), dim3(count), dim3(NT), cdiv_smem<T>(), st, S,
+                                              Fc, list, nb, pol));
 }
 
 template <class T>
 void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems, const WideStep* steps,
-                       int nsteps, const int32_t* wide_list, int nwide, PivotPolicy pol, hipStream_t st,
+                       int nsteps, const int32_t* wide_list, int nwide, PivotPolicy pol, int prec, hipStream_t st,
                        LaunchProfiler* pf) {
   if (nwide <= 0) return;
   const double fm = sizeof(T) == 16 ? 4.0 : 1.0;
-  const size_t replay_lds = sizeof(T) * RR * NBMAX;
+  const size_t replay_lds = sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX;
   for (int p = 0; p < nsteps; ++p) {
     const WideStep& w = steps[p];
     if (w.g_end > w.g_beg) {
       ProfScope ps(pf, LK_PANEL_GETF2, fm * w.getf2_flops, 0.0, st);
-      hipLaunchKernelGGL(k_wide_getf2<T>, dim3((unsigned)(w.g_end - w.g_beg)), dim3(NT), cdiv_smem<T>(), st,
+      hipLaunchKernelGGL(k_wide_getf2<T>, dim3((unsigned)(w.g_end - w.g_beg)), dim3(GT), cdiv_smem<T>(), st,
                          S, Fc, witems + w.g_beg, pol);
+      if (w.max_kb > 0)
+        hipLaunchKernelGGL(k_wide_swapcols<T>, dim3((unsigned)((w.max_kb + 63) / 64), (unsigned)(w.g_end - w.g_beg)),
+                           dim3(NT), 0, st, S, Fc, witems + w.g_beg);
     }
     if (w.r_end > w.r_beg) {
       ProfScope ps(pf, LK_PANEL_REPLAY, fm * w.replay_flops, 0.0, st);
-      hipLaunchKernelGGL(k_wide_replay<T>, dim3((unsigned)(w.r_end - w.r_beg)), dim3(RR), replay_lds, st, S,
+      hipLaunchKernelGGL(k_wide_replay<T>, dim3((unsigned)(w.r_end - w.r_beg)), dim3(NT), replay_lds, st, S,
                          Fc, witems + w.r_beg);
     }
     if (w.t_end > w.t_beg) {
       ProfScope ps(pf, LK_PANEL_SCHUR, fm * w.schur_flops, 0.0, st);
-      hipLaunchKernelGGL(k_wide_schur<T>, dim3((unsigned)(w.t_end - w.t_beg)), dim3(NT), 0, st, S, Fc,
-                         witems + w.t_beg);
+      RSL_PREC_SWITCH(T, prec, hipLaunchKernelGGL((k_wide_schur<T, P_>), dim3((unsigned)(w.t_end - w.t_beg)),
+                                                  dim3(NT), 0, st, S, Fc, witems + w.t_beg));
     }
   }
   ProfScope ps(pf, LK_EMIT, 0.0, 0.0, st);
@@ -817,12 +1083,14 @@ void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems
 
 template <class T>
 hipError_t configure_factor_kernels() {
-  hipError_t e = hipFuncSetAttribute((const void*)k_cdiv_small<T>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)cdiv_smem<T>());
+  hipError_t e = hipSuccess;
+  for (int prec = TP_F64; prec <= TP_BF16 && e == hipSuccess; ++prec)
+    RSL_PREC_SWITCH(T, prec, e = hipFuncSetAttribute((const void*)k_cdiv_small<T, P_>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)cdiv_smem<T>()));
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)k_wide_replay<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)(sizeof(T) * RR * NBMAX));
+                          (int)(sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX));
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_wide_getf2<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdiv_smem<T>());
@@ -833,11 +1101,11 @@ hipError_t configure_factor_kernels() {
   template void launch_assemble<T>(const DevSym&, const DevFac<T>&, const T*, const double*,   \
                                    hipStream_t, LaunchProfiler*);                              \
   template void launch_cmod<T>(const DevSym&, const DevFac<T>&, const CItem*, int,             \
-                               const CTask*, double, hipStream_t, LaunchProfiler*);            \
+                               const CTask*, double, int, hipStream_t, LaunchProfiler*);       \
   template void launch_cdiv_small<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int, \
-                                     PivotPolicy, double, hipStream_t, LaunchProfiler*);       \
+                                     PivotPolicy, double, int, hipStream_t, LaunchProfiler*);  \
   template void launch_wide_level<T>(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, \
-                                     int, const int32_t*, int, PivotPolicy, hipStream_t,         \
+                                     int, const int32_t*, int, PivotPolicy, int, hipStream_t,    \
                                      LaunchProfiler*);                                           \
   template hipError_t configure_factor_kernels<T>();                                           \
   template size_t cdiv_smem<T>();

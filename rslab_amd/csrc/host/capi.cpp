@@ -49,10 +49,31 @@ rsl::FactorSettings factor_settings(const rsl_settings& s) {
   fs.scaling = s.scaling;
   if (s.precision < RSL_PREC_FP64 || s.precision > RSL_PREC_BF16_SCHUR)
     throw rsl::Error(RSL_INVALID_INPUT, "unknown precision");
-  if (s.precision != RSL_PREC_FP64)
-    throw rsl::Error(RSL_INVALID_INPUT, "reduced-precision Schur updates are not built yet");
   fs.precision = static_cast<rsl::Precision>(s.precision);
   return fs;
+}
+}  // namespace
+
+namespace {
+rsl::SymbolicParams symbolic_params(const rsl_settings& st) {
+  rsl::SymbolicParams sp;
+  sp.nemin = st.nemin;
+  sp.relax = st.relax != 0;
+  sp.relax_width = st.relax_width;
+  sp.relax_rows = st.relax_rows;
+  sp.nd_seed = st.nd_seed;
+  return sp;
+}
+rsl::Ordering ordering_of(const rsl_settings& st) {
+  return st.ordering == RSL_ORDER_METIS_ND ? rsl::Ordering::MetisND
+         : st.ordering == RSL_ORDER_NATURAL ? rsl::Ordering::Natural
+                                            : rsl::Ordering::Amd;
+}
+rsl::PlanParams plan_params(const rsl_settings& st) {
+  rsl::PlanParams pp;
+  pp.panel_nb = st.panel_nb < 8 ? 8 : st.panel_nb;
+  pp.wide_flops = st.wide_flops > 0 ? st.wide_flops : 4.0e6;
+  return pp;
 }
 }  // namespace
 
@@ -124,23 +145,58 @@ int rsl_analyze(int64_t n, const int64_t* col_ptr, const int64_t* row_idx, const
     std::string v = rsl::validate_lower_csc((int32_t)n, col_ptr, ri.data(), nnz);
     if (!v.empty()) throw rsl::Error(RSL_INVALID_INPUT, v);
     if (st.panel_nb > 128) throw rsl::Error(RSL_INVALID_INPUT, "panel_nb above 128 is not supported");
-    rsl::SymbolicParams sp;
-    sp.nemin = st.nemin;
-    sp.relax = st.relax != 0;
-    sp.relax_width = st.relax_width;
-    sp.relax_rows = st.relax_rows;
-    sp.nd_seed = st.nd_seed;
-    rsl::Ordering ord = st.ordering == RSL_ORDER_METIS_ND ? rsl::Ordering::MetisND
-                        : st.ordering == RSL_ORDER_NATURAL ? rsl::Ordering::Natural
-                                                           : rsl::Ordering::Amd;
     auto ds = std::make_shared<rsl::DeviceSymbolic>();
-    ds->plan.sym = rsl::analyze((int32_t)n, col_ptr, ri.data(), ord, sp);
-    rsl::PlanParams pp;
-    pp.panel_nb = st.panel_nb < 8 ? 8 : st.panel_nb;
-    pp.wide_flops = st.wide_flops > 0 ? st.wide_flops : 4.0e6;
-    rsl::build_plan(ds->plan, col_ptr, ri.data(), pp);
+    ds->plan.sym = rsl::analyze((int32_t)n, col_ptr, ri.data(), ordering_of(st), symbolic_params(st));
+    rsl::build_plan(ds->plan, col_ptr, ri.data(), plan_params(st));
     ds->h_colp.assign(col_ptr, col_ptr + n + 1);
     ds->h_rowi = std::move(ri);
+    auto* h = new rsl_symbolic;
+    h->ds = std::move(ds);
+    *out = h;
+  });
+}
+
+
+int rsl_analyze_batch(int32_t count, const int64_t* n, const int64_t* const* col_ptr,
+                      const int64_t* const* row_idx, const rsl_settings* s, rsl_symbolic** out) {
+  *out = nullptr;
+  return guard([&] {
+    rsl_settings st;
+    if (s) st = *s;
+    else rsl_settings_default(&st);
+    if (count <= 0) throw rsl::Error(RSL_INVALID_INPUT, "batch must hold at least one system");
+    if (st.panel_nb > 128) throw rsl::Error(RSL_INVALID_INPUT, "panel_nb above 128 is not supported");
+    std::vector<rsl::Symbolic> parts;
+    parts.reserve(count);
+    std::vector<int64_t> colp(1, 0), off(1, 0);
+    std::vector<int32_t> rows;
+    for (int32_t q = 0; q < count; ++q) {
+      const int64_t nq = n[q];
+      if (nq < 0 || nq >= INT32_MAX) throw rsl::Error(RSL_INVALID_INPUT, "dimension out of range");
+      const int64_t nnz = col_ptr[q][nq];
+      std::vector<int32_t> ri(nnz);
+      for (int64_t k = 0; k < nnz; ++k) {
+        if (row_idx[q][k] < 0 || row_idx[q][k] >= nq)
+          throw rsl::Error(RSL_INVALID_INPUT, "row index out of bounds in system " + std::to_string(q));
+        ri[k] = (int32_t)row_idx[q][k];
+      }
+      std::string v = rsl::validate_lower_csc((int32_t)nq, col_ptr[q], ri.data(), nnz);
+      if (!v.empty()) throw rsl::Error(RSL_INVALID_INPUT, "system " + std::to_string(q) + ": " + v);
+      parts.push_back(rsl::analyze((int32_t)nq, col_ptr[q], ri.data(), ordering_of(st), symbolic_params(st)));
+      const int64_t cb = colp.back(), rb = off.back();
+      for (int64_t j = 0; j < nq; ++j) colp.push_back(cb + col_ptr[q][j + 1]);
+      for (int32_t r : ri) rows.push_back((int32_t)(r + rb));
+      off.push_back(rb + nq);
+    }
+    if (off.back() >= INT32_MAX) throw rsl::Error(RSL_INVALID_INPUT, "batch dimension out of range");
+    auto ds = std::make_shared<rsl::DeviceSymbolic>();
+    ds->plan.sym = rsl::compose_block_diagonal(parts);
+    rsl::build_plan(ds->plan, colp.data(), rows.data(), plan_params(st));
+    ds->h_colp = std::move(colp);
+    ds->h_rowi = std::move(rows);
+    ds->batch_off = std::move(off);
+    ds->batch_snode_off.assign(1, 0);
+    for (const rsl::Symbolic& q : parts) ds->batch_snode_off.push_back(ds->batch_snode_off.back() + (int64_t)q.snodes.size());
     auto* h = new rsl_symbolic;
     h->ds = std::move(ds);
     *out = h;
@@ -202,6 +258,8 @@ int rsl_factor(rsl_symbolic* sym, int32_t field, const double* values, const rsl
     rsl_settings st;
     if (s) st = *s;
     else rsl_settings_default(&st);
+    if (field == RSL_C64 && st.precision != RSL_PREC_FP64)
+      throw rsl::Error(RSL_INVALID_INPUT, "reduced-precision Schur updates are built for the real field only");
     auto f = rsl::make_factor(sym->ds, field, values, factor_settings(st));
     auto* h = new rsl_solver;
     h->f = std::move(f);
@@ -304,6 +362,23 @@ int rsl_refine(rsl_solver* f, const double* a_values, const double* b, int64_t n
     std::lock_guard<std::mutex> lk(f->mu);
     if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
     f->f->refine(a_values, b, nrhs, target, x, info);
+  });
+}
+
+int rsl_factor_inertia_batch(rsl_solver* f, int32_t count, int64_t* inertia) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    const auto& soff = f->f->sym->batch_snode_off;
+    if (soff.empty() || (int64_t)count != (int64_t)soff.size() - 1)
+      throw rsl::Error(RSL_DIMENSION_MISMATCH, "factor is not a batch of that many systems");
+    std::vector<int32_t> sn;
+    f->f->supernode_inertia(sn);
+    for (int32_t q = 0; q < count; ++q) {
+      int64_t c[3] = {0, 0, 0};
+      for (int64_t s = soff[q]; s < soff[q + 1]; ++s)
+        for (int k = 0; k < 3; ++k) c[k] += sn[3 * s + k];
+      for (int k = 0; k < 3; ++k) inertia[3 * q + k] = c[k];
+    }
   });
 }
 

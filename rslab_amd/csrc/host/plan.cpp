@@ -184,6 +184,7 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
         const int32_t kb = p * P.nb_eff[s];
         if (kb < P.sn_ncol[s]) {
           P.witems.push_back({s, kb, 0, 0});
+          ws.max_kb = std::max(ws.max_kb, kb);
           const double pw = std::min(P.nb_eff[s], P.sn_ncol[s] - kb);
           ws.getf2_flops += pw * pw * pw / 3.0;
         }

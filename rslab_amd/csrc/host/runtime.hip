@@ -16,9 +16,9 @@ namespace rsl {
 // kernel launchers (factor_kernels.hip, solve_kernels.hip, refine_kernels.hip)
 template <class T> void launch_fill(T*, int64_t, T, hipStream_t);
 template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t, LaunchProfiler*);
-template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, hipStream_t, LaunchProfiler*);
-template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, hipStream_t, LaunchProfiler*);
-template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, hipStream_t, LaunchProfiler*);
+template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, int, hipStream_t, LaunchProfiler*);
+template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, int, hipStream_t, LaunchProfiler*);
+template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, int, hipStream_t, LaunchProfiler*);
 template <class T> hipError_t configure_factor_kernels();
 template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
@@ -270,7 +270,7 @@ struct FieldOf<cplx> {
 template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
-  DBuf pool, d, ds, two, lperm, e_of_g, perm_e, snap, deep, counters, status, scale_d, vals_d,
+  DBuf pool, d, ds, two, lperm, e_of_g, perm_e, snap, deep, counters, status, scale_d, vals_d, sn_inert,
       work_y, work_b, work_p;
   DevFac<T> fv{};
   std::vector<double> scale;
@@ -292,8 +292,9 @@ class DeviceFactor final : public DeviceFactorBase {
     e_of_g.alloc(sizeof(int32_t) * std::max<int64_t>(n, 1));
     perm_e.alloc(sizeof(int32_t) * std::max<int64_t>(n, 1));
     snap.alloc(sizeof(T) * std::max<int64_t>(P.snap_total, 1));
-    deep.alloc(sizeof(int32_t) * 128 * std::max<int64_t>(P.ns, 1));
+    deep.alloc(sizeof(int32_t) * 2 * 128 * std::max<int64_t>(P.ns, 1));
     counters.alloc(sizeof(unsigned long long) * 4);
+    sn_inert.alloc(sizeof(int32_t) * 3 * std::max<int64_t>(P.ns, 1));
     status.alloc(sizeof(int32_t) * 2);
     scale_d.alloc(sizeof(double) * std::max<int64_t>(n, 1));
     vals_d.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
@@ -307,6 +308,7 @@ class DeviceFactor final : public DeviceFactorBase {
     fv.snap = snap.as<T>();
     fv.deep = deep.as<int32_t>();
     fv.counters = counters.as<unsigned long long>();
+    fv.sn_inertia = sn_inert.as<int32_t>();
     fv.status = status.as<int32_t>();
   }
 
@@ -353,15 +355,16 @@ class DeviceFactor final : public DeviceFactorBase {
     launch_fill<T>(fv.pool, P.panel_total, F<T>::zero(), st);
     launch_assemble<T>(S, fv, vals_d.as<T>(), scale_d.as<double>(), st, pf);
     const int nb = P.params.panel_nb < 8 ? 8 : P.params.panel_nb;
+    const int prec = static_cast<int>(settings.precision);
     const int32_t* lists = sym->level_lists.as<int32_t>();
     for (size_t L = 0; L < P.levels.size(); ++L) {
       const LevelPlan& lv = P.levels[L];
       launch_cmod<T>(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
-                     sym->tasks.as<CTask>(), lv.cmod_flops, st, pf);
+                     sym->tasks.as<CTask>(), lv.cmod_flops, prec, st, pf);
       launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
-                           lv.small_flops, st, pf);
+                           lv.small_flops, prec, st, pf);
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
-                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, st, pf);
+                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, st, pf);
     }
     hip_check(hipGetLastError(), "factor launch");
     unsigned long long cnt[4];
@@ -491,6 +494,14 @@ class DeviceFactor final : public DeviceFactorBase {
   }
 
   void refine(const double*, const double*, int64_t, double, double*, double*) override;
+
+  void supernode_inertia(std::vector<int32_t>& out) override {
+    const Plan& P = sym->plan;
+    out.assign(3 * (size_t)P.ns, 0);
+    hip_check(hipMemcpyAsync(out.data(), sn_inert.p, sizeof(int32_t) * 3 * P.ns, hipMemcpyDeviceToHost,
+                             sym->stream), "inertia download");
+    hip_check(hipStreamSynchronize(sym->stream), "inertia download");
+  }
 
  private:
   void gmres_column(int32_t n, int32_t k, int32_t c, const int64_t* rp, const int32_t* ci, const T* A,

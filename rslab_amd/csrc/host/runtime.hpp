@@ -74,6 +74,8 @@ struct DeviceSymbolic {
   // symmetric CSR of A with each entry's source index in the CSC values
   std::vector<int64_t> h_colp;
   std::vector<int32_t> h_rowi;
+  std::vector<int64_t> batch_off;        // first unknown of each system of a batch (count+1), or empty
+  std::vector<int64_t> batch_snode_off;  // first supernode of each system of a batch (count+1)
   DBuf fa_rp, fa_ci, fa_src;
   int64_t nnz_full = 0;
   bool full_ready = false;
@@ -132,6 +134,8 @@ class DeviceFactorBase {
   // {ir_iters, gmres_iters, backward_error, certified}.
   virtual void refine(const double* a_values, const double* b, int64_t nrhs, double target,
                       double* x, double* info) = 0;
+  // per-supernode (positive, negative, zero) pivot counts of the last factor
+  virtual void supernode_inertia(std::vector<int32_t>& out) = 0;
   FactorStats stats;
   FactorSettings settings;
   EventProfiler prof;

@@ -2,6 +2,7 @@
 #include "symbolic.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <numeric>
 #include <stdexcept>
 
@@ -459,6 +460,44 @@ Symbolic analyze(idx n, const off* colp, const idx* rowi, Ordering method,
   S.factor_nnz = 0;
   for (idx c : S.col_counts) S.factor_nnz += c;
   S.snodes = find_supernodes(S.parent, S.col_counts, prm, strat);
+  return S;
+}
+
+Symbolic compose_block_diagonal(const std::vector<Symbolic>& parts) {
+  Symbolic S;
+  int64_t n = 0;
+  for (const Symbolic& q : parts) n += q.n;
+  if (n >= INT32_MAX) throw std::runtime_error("batch dimension out of range");
+  S.n = (idx)n;
+  S.method = parts.empty() ? Ordering::Amd : parts[0].method;
+  S.resolved_amalgamation = parts.empty() ? Amalgamation::Adjacency : parts[0].resolved_amalgamation;
+  S.perm.reserve(n);
+  S.perm_inv.reserve(n);
+  S.parent.reserve(n);
+  S.col_counts.reserve(n);
+  S.permuted.n = S.n;
+  S.permuted.colp.assign(1, 0);
+  idx base = 0, sbase = 0;
+  for (const Symbolic& q : parts) {
+    for (idx k = 0; k < q.n; ++k) {
+      S.perm.push_back(q.perm[k] + base);
+      S.perm_inv.push_back(q.perm_inv[k] + base);
+      S.parent.push_back(q.parent[k] >= 0 ? q.parent[k] + base : -1);
+      S.col_counts.push_back(q.col_counts[k]);
+    }
+    const off pb = S.permuted.colp.back();
+    for (idx k = 0; k < q.n; ++k) S.permuted.colp.push_back(pb + q.permuted.colp[k + 1]);
+    for (idx r : q.permuted.rowi) S.permuted.rowi.push_back(r + base);
+    for (const Supernode& sn : q.snodes) {
+      Supernode t = sn;
+      t.first_col += base;
+      for (idx& c : t.children) c += sbase;
+      S.snodes.push_back(std::move(t));
+    }
+    S.factor_nnz += q.factor_nnz;
+    base += q.n;
+    sbase += (idx)q.snodes.size();
+  }
   return S;
 }
 

@@ -100,6 +100,11 @@ std::vector<Supernode> find_supernodes(const std::vector<idx>& parent,
 Symbolic analyze(idx n, const off* colp, const idx* rowi, Ordering method,
                  const SymbolicParams& p);
 
+// Symbolic of the block-diagonal matrix diag(A_0, A_1, ...) whose blocks were
+// analysed one by one: permutations, trees and supernodes side by side, so
+// every block keeps exactly its own analysis.
+Symbolic compose_block_diagonal(const std::vector<Symbolic>& parts);
+
 // Filled row structure per supernode: own columns, then sorted trailing rows.
 std::vector<std::vector<idx>> supernode_row_structures(const Symbolic& s);
 // update_list[s]: descendants k whose off-diagonal rows hit s's columns,

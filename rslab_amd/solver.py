@@ -112,6 +112,20 @@ class CscMatrix:
         np.add.at(col_ptr, c + 1, 1)
         return CscMatrix(n, np.cumsum(col_ptr), r, summed)
 
+    @staticmethod
+    def block_diag(mats) -> "CscMatrix":
+        """diag(A_0, A_1, ...) in lower CSC (the layout rsl_analyze_batch uses)."""
+        n = sum(m.n for m in mats)
+        cps, ris, vs = [np.zeros(1, dtype=np.int64)], [], []
+        cbase, rbase = 0, 0
+        for m in mats:
+            cps.append(m.col_ptr[1:] + cbase)
+            ris.append(m.row_idx + rbase)
+            vs.append(m.values)
+            cbase += m.nnz
+            rbase += m.n
+        return CscMatrix(n, np.concatenate(cps), np.concatenate(ris), np.concatenate(vs))
+
     def with_values(self, values) -> "CscMatrix":
         return CscMatrix(self.n, self.col_ptr, self.row_idx, values)
 
@@ -318,6 +332,25 @@ class LdltSymbolic:
                                C.byref(s), C.byref(h)))
         return LdltSymbolic(h, a.n, a.nnz, opts)
 
+    @staticmethod
+    def analyze_batch(mats, opts: Optional[SolverSettings] = None) -> "LdltSymbolic":
+        """Independent systems analysed one by one and laid out as one
+        block-diagonal system; factor / solve / refine then take the
+        concatenated values and right-hand sides (`CscMatrix.block_diag`)."""
+        opts = opts or SolverSettings()
+        cnt = len(mats)
+        ns = np.array([m.n for m in mats], dtype=np.int64)
+        keep = [(np.ascontiguousarray(m.col_ptr, dtype=np.int64), np.ascontiguousarray(m.row_idx, dtype=np.int64))
+                for m in mats]
+        cps = (C.POINTER(C.c_int64) * cnt)(*[ptr(k[0], C.c_int64) for k in keep])
+        ris = (C.POINTER(C.c_int64) * cnt)(*[ptr(k[1], C.c_int64) for k in keep])
+        h = C.c_void_p()
+        s = opts.to_c()
+        _check(lib.rsl_analyze_batch(cnt, ptr(ns, C.c_int64), cps, ris, C.byref(s), C.byref(h)))
+        sym = LdltSymbolic(h, int(ns.sum()), int(sum(m.nnz for m in mats)), opts)
+        sym.batch = cnt
+        return sym
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -453,6 +486,12 @@ class LdltSolver:
         _check(lib.rsl_factor_info(self._h, ptr(i, C.c_int64), None, None))
         return Inertia(int(i[0]), int(i[1]), int(i[2]))
 
+    def inertia_batch(self, count: int):
+        """Per-system inertia of a factor of a batch (rsl_analyze_batch)."""
+        out = np.zeros(3 * count, dtype=np.int64)
+        _check(lib.rsl_factor_inertia_batch(self._h, count, ptr(out, C.c_int64)))
+        return [Inertia(*map(int, out[3 * q:3 * q + 3])) for q in range(count)]
+
     def n_perturbed(self) -> int:
         v = C.c_int64()
         _check(lib.rsl_factor_info(self._h, None, C.byref(v), None))
@@ -502,9 +541,10 @@ class MixedLdltSolver:
 
     @staticmethod
     def factor_with(a: CscMatrix, opts: Optional[SolverSettings] = None) -> "MixedLdltSolver":
+        """Factor with reduced-precision Schur products (FP32 unless `opts`
+        names another precision; FP64 gives the plain factor under the same
+        ladder)."""
         opts = opts or SolverSettings().with_precision(Precision.FP32_SCHUR)
-        if opts.precision == Precision.FP64:
-            opts = opts.with_precision(Precision.FP32_SCHUR)
         return MixedLdltSolver(LdltSolver.factor_with(a, opts), a)
 
     def solve(self, a: CscMatrix, b):

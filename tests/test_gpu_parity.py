@@ -321,3 +321,100 @@ def test_config3_full_size_properties():
     assert 7171 <= neg <= 7358 and f.inertia().zero == 0
     X = f.solve_many(B)
     assert O.backward_error(a, X, B).max() < 1e-12
+
+
+# ------------------------------------------------------------ refinement ladder, reduced-precision Schur
+def lap2d_ref(k):
+    """mixed.rs:389-412: 2D 5-point Laplacian, diagonal 4, couplings -1."""
+    return matgen.grid_laplacian((k, k))
+
+
+@pytest.mark.parametrize("prec", [R.Precision.FP64, R.Precision.FP32_SCHUR, R.Precision.BF16_SCHUR])
+def test_mixed_ldlt_certifies_on_laplacian(prec):
+    # mixed.rs:415-439: certified, and equal to the FP64 direct solve to 1e-10
+    a = lap2d_ref(40)
+    b = np.array([(i % 13) - 6.0 for i in range(a.n)])
+    m = R.MixedLdltSolver.factor_with(a, R.SolverSettings().with_precision(prec))
+    x, info = m.solve(a, b)
+    assert info.certified, info
+    xd = R.LdltSolver.factor_with(a).solve(b)
+    assert np.abs(x - xd).max() <= 1e-10 * max(np.abs(xd).max(), 1.0)
+
+
+@pytest.mark.parametrize("prec", [R.Precision.FP32_SCHUR, R.Precision.BF16_SCHUR])
+def test_reduced_precision_schur_refines_config3_shape(prec):
+    """Schur / cmod products in FP32 (or BF16 inputs, FP32 accumulation),
+    pivots and D in FP64: the ladder reaches backward error <= 1e-14 on every
+    column against the FP64 matrix, and the solution equals the FP64 one to
+    what the conditioning allows."""
+    a, B = matgen.config3(m=24, nrhs=8)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
+    sym = R.LdltSymbolic.analyze_with(a, st)
+    lo = sym.factor(a, st.with_precision(prec))
+    X, infos = R.MixedLdltSolver(lo, a).solve_to(a, B, 1e-14)
+    assert all(i.certified and i.backward_error <= 1e-14 for i in infos), infos
+    hi = sym.factor(a)
+    Xd, infos_d = R.MixedLdltSolver(hi, a).solve_to(a, B, 1e-14)
+    assert all(i.certified for i in infos_d)
+    assert np.abs(X - Xd).max() <= 1e-8 * np.abs(Xd).max()
+    # the reduced product really is reduced: the raw factors differ
+    fl, fh = lo.factors(), hi.factors()
+    if np.array_equal(fl.perm, fh.perm):
+        d = np.abs(csc(fl) - csc(fh)).max()
+        assert d > 0.0
+
+
+def test_fp64_ladder_certifies_config3_shape_and_matches_oracle_solution():
+    a, B = matgen.config3(m=24, nrhs=4)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
+    sym, fg, fo, s = factor_pair(a, st)
+    X, infos = R.MixedLdltSolver(fg, a).solve_to(a, B, 1e-14)
+    assert all(i.certified for i in infos)
+    Xo = fo.solve_many(B * s[:, None]) * s[:, None]
+    assert np.abs(X - Xo).max() <= 1e-9 * np.abs(Xo).max()
+    assert O.backward_error(a, X, B).max() <= 1e-14
+
+
+# ------------------------------------------------------------ batched independent systems (config 4 shape)
+def test_batch_factor_equals_per_system_factors():
+    """A batch of uneven shifted 2D Laplacians in one launch sequence: each
+    system's inertia equals its own factor's and the closed form
+    #{lambda_pq < sigma}; solutions equal the per-system solves."""
+    systems = matgen.config4(count=24)
+    mats = [a for a, _, _ in systems]
+    big = R.CscMatrix.block_diag(mats)
+    b = np.concatenate([bb for _, bb, _ in systems])
+    sym = R.LdltSymbolic.analyze_batch(mats)
+    f = sym.factor(big)
+    inert = f.inertia_batch(len(mats))
+    x = f.solve(b)
+    off = 0
+    for (a, bb, sg), ib in zip(systems, inert):
+        m = int(round(np.sqrt(a.n)))
+        t = 2 - 2 * np.cos(np.arange(1, m + 1) * np.pi / (m + 1))
+        lam = (t[:, None] + t[None, :]).ravel() - sg
+        fi = R.LdltSolver.factor_with(a)
+        assert ib.as_tuple() == fi.inertia().as_tuple()
+        if np.abs(lam).min() > 1e-8:
+            assert ib.negative == int((lam < 0).sum())
+        xi = fi.solve(bb)
+        assert np.abs(x[off:off + a.n] - xi).max() <= 1e-12 * max(np.abs(xi).max(), 1.0)
+        off += a.n
+    assert sum(i.total() for i in inert) == big.n
+
+
+@pytest.mark.parametrize("prec", [R.Precision.FP64, R.Precision.FP32_SCHUR])
+def test_batch_refines_to_fp64(prec):
+    systems = matgen.config4(count=16)
+    mats = [a for a, _, _ in systems]
+    big = R.CscMatrix.block_diag(mats)
+    b = np.concatenate([bb for _, bb, _ in systems])
+    sym = R.LdltSymbolic.analyze_batch(mats)
+    f = sym.factor(big, sym.settings.with_precision(prec))
+    x, info = R.MixedLdltSolver(f, big).solve_to(big, b, 1e-14)
+    assert info.certified
+    off = 0
+    for a, bb, _ in systems:
+        xi = x[off:off + a.n]
+        assert O.backward_error(a, xi, bb)[0] <= 1e-13
+        off += a.n

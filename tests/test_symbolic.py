@@ -228,3 +228,21 @@ def test_config_pattern_sizes():
     a2, b2 = matgen.config2(m=40, nrhs=16)
     assert (a2.n, a2.nnz, b2.shape) == (64000, 251200, (64000, 16))
     assert np.iscomplexobj(a2.values)
+
+
+def test_batch_symbolic_is_per_system_symbolic():
+    """rsl_analyze_batch lays the systems' own analyses side by side."""
+    systems = matgen.config4(count=6)
+    mats = [a for a, _, _ in systems]
+    eb = R.LdltSymbolic.analyze_batch(mats).export()
+    off = soff = 0
+    for m in mats:
+        ei = R.LdltSymbolic.analyze(m).export()
+        ns = len(ei.sn_first)
+        assert np.array_equal(eb.perm[off:off + m.n] - off, ei.perm)
+        assert np.array_equal(eb.sn_first[soff:soff + ns] - off, ei.sn_first)
+        assert np.array_equal(eb.sn_ncol[soff:soff + ns], ei.sn_ncol)
+        assert np.array_equal(eb.sn_nrow[soff:soff + ns], ei.sn_nrow)
+        off += m.n
+        soff += ns
+    assert off == eb.n and soff == len(eb.sn_first)
