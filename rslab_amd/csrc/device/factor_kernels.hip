@@ -19,6 +19,7 @@
 #include "prof.hpp"
 #include "scalar.hpp"
 #include "tile_mfma.hpp"
+#include "wave.hpp"
 
 namespace rsl {
 
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
                                              const CItem* __restrict__ items,
                                              const CTask* __restrict__ tasks) {
   __shared__ T acc_s[TM * TN];
-  __shared__ T stage[2 * STG];
+  __shared__ __align__(16) T stage[2 * STG];
   __shared__ int32_t trow[TM], rpos[TM], cpos[TN];
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const CItem it = items[blockIdx.x];
@@ -327,7 +328,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         imax = i;
       }
     }
-    wave_argmax<T>(cm, imax);
+    wave_argmax_dpp(cm, imax, k + 1);  // pw <= 128: at most two candidate rows per lane
     const double colmax = sqrt(cm);
     // decision below reads row imax; every wave must finish reading before
     // any wave starts the interchange / update of this step
@@ -343,7 +344,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       double rm = 0.0;
       for (int j = k + lane; j < imax; j += 64) rm = fmax(rm, F<T>::magsq(A_(imax, j)));
       for (int i = imax + 1 + lane; i < pw; i += 64) rm = fmax(rm, F<T>::magsq(A_(i, imax)));
-      rm = wave_max(rm);
+      rm = wave_max_dpp(rm);
       const double rowmax = sqrt(rm);
       if (absakk >= alpha * colmax * (colmax / rowmax)) {
         kstep = 1;
@@ -701,7 +702,12 @@ __device__ void cdiv_emit(const DevSym& S, const DevFac<T>& Fc, const SnCtx<T>& 
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
   __syncthreads();
   for (int p = threadIdx.x; p < c.w; p += NT) {
-    int g = c.first + c.lperm[p];
+    const int lp = c.lperm[p];
+    if (lp < 0 || lp >= c.w) {  // a pivot map outside the supernode: report, never write through it
+      atomicMax(&Fc.status[0], 2);
+      continue;
+    }
+    int g = c.first + lp;
     Fc.e_of_g[g] = c.eo + p;
     Fc.perm_e[c.eo + p] = S.sym_perm[g];
     const bool second = p > 0 && c.two[p - 1];
@@ -746,7 +752,7 @@ __device__ void flag_failure(const DevFac<T>& Fc, int s) {
 // during the Schur update, so they share storage (fits 160 KB for c64, nb 128).
 template <class T>
 struct CdivShared {
-  union {
+  union alignas(16) {
     T Ab[NBMAX * (NBMAX + 1) / 2];
     T stage[2 * STG];
   } u;
@@ -973,7 +979,7 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
 
 template <class T, int PR>
 __global__ __launch_bounds__(NT) void k_wide_schur(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
-  __shared__ T stage[2 * STG];
+  __shared__ __align__(16) T stage[2 * STG];
   const WItem it = items[blockIdx.x];
   const int s = it.s, kb = it.kb;
   SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);

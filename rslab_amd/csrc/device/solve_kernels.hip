@@ -408,7 +408,7 @@ __global__ void k_dsolve(int32_t n, int32_t nrhs, const T* __restrict__ d,
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n * nrhs) return;
   int32_t e = (int32_t)(idx / nrhs), c = (int32_t)(idx % nrhs);
-  if (two[e]) {
+  if (two[e] && e + 1 < n) {
     T d11 = d[e], d21 = ds[e], d22 = d[e + 1];
     T detinv = F<T>::recip(d11 * d22 - d21 * d21);
     T z0 = y[(int64_t)e * nrhs + c], z1 = y[(int64_t)(e + 1) * nrhs + c];

@@ -374,6 +374,7 @@ class DeviceFactor final : public DeviceFactorBase {
     hip_check(hipStreamSynchronize(st), "factor");
     stats.t_factor_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stat[0] == 2) throw Error(DEVICE_ERROR, "device: inconsistent pivot map in the factor");
     if (stat[0] != 0) throw Error(NUMERICALLY_RANK_DEFICIENT, "numerically rank deficient");
     stats.inertia[0] = (int64_t)cnt[0];
     stats.inertia[1] = (int64_t)cnt[1];

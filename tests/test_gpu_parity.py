@@ -418,3 +418,32 @@ def test_batch_refines_to_fp64(prec):
         xi = x[off:off + a.n]
         assert O.backward_error(a, xi, bb)[0] <= 1e-13
         off += a.n
+
+
+# ------------------------------------------------------------ same-pattern refactor sweep (config 5 shape)
+def test_refactor_sweep_vector_stencil():
+    """3 dof per node, 27-point coupling, shifted by -omega^2 I for several
+    omega^2 with the analysis and device storage reused: every refactor
+    equals a fresh factor of that matrix (inertia = the oracle's), and the
+    FP32-Schur factor refined reaches backward error <= 1e-12 (32 RHS)."""
+    om = matgen.config5_omegas(20)[::6]
+    base = matgen.vector_stencil27(m=8, omega2=om[0])
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
+    sym = R.LdltSymbolic.analyze_with(base, st)
+    e = sym.export()
+    f64 = sym.factor(base)
+    f32 = sym.factor(base, st.with_precision(R.Precision.FP32_SCHUR))
+    for k, w2 in enumerate(om):
+        a = matgen.vector_stencil27(m=8, omega2=w2)
+        f64.refactor(a)
+        s = O.onepass_scale(a)
+        fo = O.factor(e, O.scaled(a, s))
+        assert f64.inertia().as_tuple() == fo.inertia
+        B = matgen.config5_rhs(a.n, k, nrhs=32)
+        X = f64.solve_many(B)
+        Xo = fo.solve_many(B * s[:, None]) * s[:, None]
+        assert np.abs(X - Xo).max() <= 1e-9 * np.abs(Xo).max()
+        f32.refactor(a)
+        X32, infos = R.MixedLdltSolver(f32, a).solve_to(a, B, 1e-12)
+        assert all(i.certified for i in infos), infos
+        assert O.backward_error(a, X32, B).max() <= 1e-12
