@@ -39,6 +39,7 @@ struct DevFac {
   uint8_t* two;       // first column of a 2x2 block
   int32_t* lperm;     // pivoted position -> local column, elimination order
   int32_t* e_of_g;    // permuted index -> elimination position
+  int32_t* rs_e;      // per panel row (rs_idx order): its elimination position
   int32_t* perm_e;    // elimination position -> user index
   T* snap;            // per-panel time-of-step multipliers (scratch)
   int32_t* deep;      // per-panel interchange partners (scratch, nb per supernode)
@@ -54,7 +55,9 @@ struct CItem {
 // One contribution of updater k to a tile: off-diagonal row ranges [a0,a1)
 // (tile rows) and [b0,b1) (tile columns), indices into k's off-diagonal rows.
 struct CTask {
-  int32_t k, a0, a1, b0, b1;
+  int32_t k, a0, a1, b0, b1, pad;
+  uint64_t rmask;  // tile rows hit by rows a0..a1-1 (in order)
+  uint64_t cmask;  // tile columns hit by rows b0..b1-1 (in order)
 };
 
 // One unit of the batched wide-supernode path: nb-panel [kb, kb+nb) of

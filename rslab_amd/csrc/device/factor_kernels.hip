@@ -28,7 +28,7 @@ constexpr int TM = 64;   // tile rows
 constexpr int TN = 64;   // tile columns
 constexpr int KC = 16;   // inner chunk
 constexpr int NBMAX = 128;
-constexpr int GT = 64;   // threads of the wide-path getf2 (one wave: its barriers are free)
+constexpr int GT = 256;  // threads of the wide-path getf2
 constexpr int STG = 1152;  // staging elements per operand (fits every tile core's chunk)
 static_assert(STG >= KC * TM && STG >= tile_stage_elems<TP_F64>() && STG * 8 >= tile_stage_elems<TP_BF16>() * 2,
               "staging buffer too small");
@@ -114,6 +114,13 @@ __global__ void k_assemble(DevSym S, DevFac<T> Fc, const T* __restrict__ vals,
   if (k >= S.nnz) return;
   double sc = scale[S.a_row[k]] * scale[S.a_col[k]];
   Fc.pool[S.amap[k]] = F<T>::scale(vals[k], sc);
+}
+
+// Elimination position of every panel row (the solve's backward gather).
+template <class T>
+__global__ void k_rows_e(DevSym S, DevFac<T> Fc, int64_t nrows) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nrows) Fc.rs_e[q] = Fc.e_of_g[S.rs_idx[q]];
 }
 
 template <class T>
@@ -212,6 +219,109 @@ __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
   for (int e = tid; e < TM * TN; e += NT) {
     int i = e % TM, j = e / TM;
     if (i < mr && j < mc) P[(it.r0 + i) + (int64_t)(it.c0 + j) * h] = acc_s[e];
+  }
+}
+
+// Real-field cmod on matrix cores. Every updater k of a 64x64 target tile
+// contributes L_k[rows] D_k L_k[cols]^T; its rows and columns are gathered
+// straight into tile coordinates (the plan gives each task the tile rows /
+// columns it hits as bit masks, so a tile row finds its source row by a
+// population count), and all updaters' products accumulate in the matrix
+// cores' registers over one long K loop, 32 columns per chunk, the next
+// chunk (possibly the next updater's first) loading into registers while
+// the current one multiplies. The tile is then reduced once, on the entries
+// of the panel's lower triangle, the only ones the reference updates.
+template <int PR>
+__global__ __launch_bounds__(NT) void k_cmod_mfma(DevSym S, DevFac<double> Fc, const CItem* __restrict__ items,
+                                                  const CTask* __restrict__ tasks) {
+  typedef TileCfgW<PR> C;
+  typedef typename C::S Sx;
+  constexpr int KCW = C::KC, PER = KCW * 64 / NT;
+  __shared__ __align__(16) Sx As[C::ELEMS];
+  __shared__ __align__(16) Sx Bs[C::ELEMS];
+  const int tid = threadIdx.x;
+  const CItem it = items[blockIdx.x];
+  double va[PER], vb[PER];
+  // chunk [cc, cc + KCW) of task tt gathered into tile rows / columns
+  auto fetch = [&](int tt, int cc) {
+    const CTask tk = tasks[tt];
+    const int k = tk.k, nck = S.sn_ncol[k], eo = S.e_off[k];
+    const int64_t hk = S.sn_nrow[k];
+    const double* Pk = Fc.pool + S.panel_off[k];
+    const int kc = min(KCW, nck - cc);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + NT * u, r = e & 63, c = e >> 6;
+      const uint64_t below = (1ull << r) - 1ull;
+      va[u] = 0.0;
+      vb[u] = 0.0;
+      if (c < kc) {
+        const int64_t col = cc + c;
+        if ((tk.rmask >> r) & 1ull) va[u] = Pk[(nck + tk.a0 + __popcll(tk.rmask & below)) + col * hk];
+        if ((tk.cmask >> r) & 1ull)
+          vb[u] = g_entry_d(Pk, hk, nck + tk.b0 + __popcll(tk.cmask & below), col, cc + c, Fc.d + eo, Fc.ds + eo,
+                            Fc.two + eo);
+      }
+    }
+  };
+  const int t_beg = it.t_beg, t_end = it.t_end;
+  int t = t_beg, c0 = 0;
+  int nck_t = S.sn_ncol[tasks[t].k];
+  fetch(t, 0);
+  TileAcc<PR> acc;
+  tile_zero(acc);
+  double acc64[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc64[q] = 0.0;
+  for (;;) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + NT * u, r = e & 63, c = e >> 6;
+      stage_store_w<PR>(As, r, c, va[u]);
+      stage_store_w<PR>(Bs, r, c, vb[u]);
+    }
+    __syncthreads();
+    int tn = t, cn = c0 + KCW, nck_n = nck_t;
+    if (cn >= nck_t) {
+      tn = t + 1;
+      cn = 0;
+      if (tn < t_end) nck_n = S.sn_ncol[tasks[tn].k];
+    }
+    if (tn < t_end) fetch(tn, cn);
+    tile_mma_w<PR>(As, Bs, acc);
+    if constexpr (PR != TP_F64) {
+      // reduced-precision products: each updater's sum is carried into FP64
+      if (tn != t) {
+        fold_into(acc64, acc);
+        tile_zero(acc);
+      }
+    }
+    __syncthreads();
+    if (tn >= t_end) break;
+    t = tn;
+    c0 = cn;
+    nck_t = nck_n;
+  }
+  const int s = it.s;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], gcol0 = S.sn_first[s] + it.c0;
+  double* P = Fc.pool + S.panel_off[s];
+  const int32_t* rs = S.rs_idx + S.rs_off[s] + it.r0;
+  const int mr = min<int64_t>(TM, h - it.r0), mc = min(TN, w - it.c0);
+  if constexpr (PR != TP_F64) {
+    tile_for_each_d<PR>(acc64, [&](int i, int j, double v) {
+      if (i < mr && j < mc && rs[i] >= gcol0 + j) {
+        double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * h;
+        *dst = *dst - v;
+      }
+    });
+  } else {
+    tile_for_each<PR>(acc, [&](int i, int j, double v) {
+      if (i < mr && j < mc && rs[i] >= gcol0 + j) {
+        double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * h;
+        *dst = *dst - v;
+      }
+    });
   }
 }
 
@@ -1014,6 +1124,12 @@ void launch_fill(T* p, int64_t n, T v, hipStream_t st) {
 }
 
 template <class T>
+void launch_rows_e(const DevSym& S, const DevFac<T>& Fc, int64_t nrows, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_rows_e<T>, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, S, Fc, nrows);
+}
+
+template <class T>
 void launch_assemble(const DevSym& S, const DevFac<T>& Fc, const T* vals, const double* scale,
                      hipStream_t st, LaunchProfiler* pf) {
   if (S.nnz <= 0) return;
@@ -1043,7 +1159,11 @@ void launch_cmod(const DevSym& S, const DevFac<T>& Fc, const CItem* items, int n
                  const CTask* tasks, double flops, int prec, hipStream_t st, LaunchProfiler* pf) {
   if (nitems <= 0) return;
   ProfScope ps(pf, LK_CMOD, flops, 0.0, st);
-  RSL_PREC_SWITCH(T, prec, hipLaunchKernelGGL((k_cmod<T, P_>), dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks));
+  if constexpr (std::is_same<T, double>::value) {
+    RSL_PREC_SWITCH(T, prec, hipLaunchKernelGGL((k_cmod_mfma<P_>), dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks));
+  } else {
+    hipLaunchKernelGGL((k_cmod<T, TP_F64>), dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks);
+  }
 }
 
 template <class T>
@@ -1104,6 +1224,7 @@ hipError_t configure_factor_kernels() {
 
 #define RSL_INST(T)                                                                            \
   template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                   \
+  template void launch_rows_e<T>(const DevSym&, const DevFac<T>&, int64_t, hipStream_t);        \
   template void launch_assemble<T>(const DevSym&, const DevFac<T>&, const T*, const double*,   \
                                    hipStream_t, LaunchProfiler*);                              \
   template void launch_cmod<T>(const DevSym&, const DevFac<T>&, const CItem*, int,             \

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
   const T* P = Fc.pool + S.panel_off[s];
-  const int32_t* rs = S.rs_idx + S.rs_off[s];
+  const int32_t* re = Fc.rs_e + S.rs_off[s];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int p = p0 + wv; p < min(p0 + 16, w); p += SNT / 64) {
     for (int c0 = 0; c0 < nrhs; c0 += RCH) {
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
       for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
       for (int64_t i = w + lane; i < h; i += 64) {
         const T l = P[i + (int64_t)p * h];
-        const T* xs = y + (int64_t)Fc.e_of_g[rs[i]] * nrhs + c0;
+        const T* xs = y + (int64_t)re[i] * nrhs + c0;
 #pragma unroll
         for (int k = 0; k < RCH; ++k)
           if (k < nc) acc[k] = acc[k] + l * xs[k];

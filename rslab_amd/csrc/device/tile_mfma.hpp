@@ -48,12 +48,91 @@ struct TileCfg<TP_BF16> {
   static constexpr int LDA = 32 + 8;  // row-major: [row][k], 8 consecutive k per lane load
 };
 
+// Staging for the gathered cmod: 32 columns of K per chunk (64 for bf16).
+template <int P>
+struct TileCfgW;
+template <>
+struct TileCfgW<TP_F64> {
+  typedef double S;
+  static constexpr int KC = 32;
+  static constexpr int LDA = 64 + 4;
+  static constexpr int ELEMS = KC * LDA;
+};
+template <>
+struct TileCfgW<TP_F32> {
+  typedef float S;
+  static constexpr int KC = 32;
+  static constexpr int LDA = 64 + 4;
+  static constexpr int ELEMS = KC * LDA;
+};
+template <>
+struct TileCfgW<TP_BF16> {
+  typedef __bf16 S;
+  static constexpr int KC = 64;
+  static constexpr int LDA = 64 + 8;  // row-major [row][k]
+  static constexpr int ELEMS = 64 * LDA;
+};
+
+template <int P>
+__device__ inline void stage_store_w(typename TileCfgW<P>::S* As, int r, int c, double v) {
+  typedef typename TileCfgW<P>::S S;
+  if constexpr (P == TP_BF16)
+    As[r * TileCfgW<P>::LDA + c] = (S)(float)v;
+  else
+    As[c * TileCfgW<P>::LDA + r] = (S)v;
+}
+
+// acc += A(64 x KC) * B(64 x KC)^T from the staged chunk (all 256 threads,
+// each wave its 32x32 quadrant).
+template <int P, class Acc>
+__device__ inline void tile_mma_w(const typename TileCfgW<P>::S* As, const typename TileCfgW<P>::S* Bs, Acc& acc) {
+  typedef TileCfgW<P> C;
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  if constexpr (P == TP_BF16) {
+#pragma unroll
+    for (int kk = 0; kk < C::KC; kk += 32) {
+      mf_bf8 a[2], b[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        a[q] = *reinterpret_cast<const mf_bf8*>(As + (wr + q * 16 + (l & 15)) * C::LDA + kk + 8 * (l >> 4));
+        b[q] = *reinterpret_cast<const mf_bf8*>(Bs + (wc + q * 16 + (l & 15)) * C::LDA + kk + 8 * (l >> 4));
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+          acc.v[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[bi], b[bj], acc.v[bi][bj], 0, 0, 0);
+    }
+  } else {
+#pragma unroll 4
+    for (int kk = 0; kk < C::KC; kk += 4) {
+      typename C::S a[2], b[2];
+      const int k = kk + (l >> 4);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        a[q] = As[k * C::LDA + wr + q * 16 + (l & 15)];
+        b[q] = Bs[k * C::LDA + wc + q * 16 + (l & 15)];
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          if constexpr (P == TP_F64)
+            acc.v[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc.v[bi][bj], 0, 0, 0);
+          else
+            acc.v[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[bi], b[bj], acc.v[bi][bj], 0, 0, 0);
+        }
+    }
+  }
+}
+
 template <int P>
 struct TileAcc {
   typename TileCfg<P>::V v[2][2];
 };
 
-// LDS bytes for one operand of a chunk
+// LDS elements for one operand of a chunk
 template <int P>
 constexpr int tile_stage_elems() {
   return P == TP_BF16 ? 64 * TileCfg<P>::LDA : TileCfg<P>::KC * TileCfg<P>::LDA;
@@ -177,6 +256,31 @@ __device__ inline void tile_for_each(const TileAcc<P>& acc, Fn&& fn) {
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         fn(wr + tacc_row<P>(bi, l, r), wc + bj * 16 + (l & 15), (double)acc.v[bi][bj][r]);
+}
+
+// acc64 (16 doubles per thread, the accumulator's element order) += acc
+template <int P>
+__device__ inline void fold_into(double (&acc64)[16], const TileAcc<P>& acc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[(bi * 2 + bj) * 4 + r] += (double)acc.v[bi][bj][r];
+}
+
+// tile_for_each over an FP64 copy of the accumulator
+template <int P, class Fn>
+__device__ inline void tile_for_each_d(const double (&acc64)[16], Fn&& fn) {
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        fn(wr + tacc_row<P>(bi, l, r), wc + bj * 16 + (l & 15), acc64[(bi * 2 + bj) * 4 + r]);
 }
 
 }  // namespace rsl

@@ -146,7 +146,10 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
             if (r.hi - a0 > TILE || c.hi - c.lo > TILE ||
                 ok[c.lo] - first < c.tile * TILE || ok[c.hi - 1] - first >= std::min(w, (c.tile + 1) * TILE))
               throw std::runtime_error("plan: cmod task outside its tile");
-            tt[(size_t)r.tile * nct + c.tile].push_back({k, a0, r.hi, c.lo, c.hi});
+            CTask tk{k, a0, r.hi, c.lo, c.hi, 0, 0, 0};
+            for (int32_t a = a0; a < r.hi; ++a) tk.rmask |= 1ull << (gloc[ok[a]] - r.tile * TILE);
+            for (int32_t b = c.lo; b < c.hi; ++b) tk.cmask |= 1ull << (ok[b] - first - c.tile * TILE);
+            tt[(size_t)r.tile * nct + c.tile].push_back(tk);
             double pairs = 0;
             for (int32_t cc = c.lo; cc < c.hi; ++cc) pairs += std::max(0, r.hi - std::max(a0, cc));
             L.cmod_flops += 2.0 * nck * pairs;

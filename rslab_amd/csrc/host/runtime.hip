@@ -15,6 +15,7 @@ namespace rsl {
 
 // kernel launchers (factor_kernels.hip, solve_kernels.hip, refine_kernels.hip)
 template <class T> void launch_fill(T*, int64_t, T, hipStream_t);
+template <class T> void launch_rows_e(const DevSym&, const DevFac<T>&, int64_t, hipStream_t);
 template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, int, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, int, hipStream_t, LaunchProfiler*);
@@ -270,7 +271,7 @@ struct FieldOf<cplx> {
 template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
-  DBuf pool, d, ds, two, lperm, e_of_g, perm_e, snap, deep, counters, status, scale_d, vals_d, sn_inert,
+  DBuf pool, d, ds, two, lperm, e_of_g, perm_e, snap, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
       work_y, work_b, work_p;
   DevFac<T> fv{};
   std::vector<double> scale;
@@ -295,6 +296,11 @@ class DeviceFactor final : public DeviceFactorBase {
     deep.alloc(sizeof(int32_t) * 2 * 128 * std::max<int64_t>(P.ns, 1));
     counters.alloc(sizeof(unsigned long long) * 4);
     sn_inert.alloc(sizeof(int32_t) * 3 * std::max<int64_t>(P.ns, 1));
+    rows_e.alloc(sizeof(int32_t) * std::max<int64_t>(P.rs_off[P.ns], 1));
+    // index maps start at 0 so no later read can leave the vectors
+    hip_check(hipMemset(e_of_g.p, 0, e_of_g.bytes), "memset");
+    hip_check(hipMemset(perm_e.p, 0, perm_e.bytes), "memset");
+    hip_check(hipMemset(rows_e.p, 0, rows_e.bytes), "memset");
     status.alloc(sizeof(int32_t) * 2);
     scale_d.alloc(sizeof(double) * std::max<int64_t>(n, 1));
     vals_d.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
@@ -309,6 +315,7 @@ class DeviceFactor final : public DeviceFactorBase {
     fv.deep = deep.as<int32_t>();
     fv.counters = counters.as<unsigned long long>();
     fv.sn_inertia = sn_inert.as<int32_t>();
+    fv.rs_e = rows_e.as<int32_t>();
     fv.status = status.as<int32_t>();
   }
 
@@ -323,6 +330,7 @@ class DeviceFactor final : public DeviceFactorBase {
       configured = true;
     }
     auto t0 = std::chrono::steady_clock::now();
+    valid = false;
     if (settings.scaling == 1)
       onepass_scale(P.n, FieldOf<T>::id, sym->h_colp.data(), sym->h_rowi.data(), values, scale);
     else
@@ -366,6 +374,7 @@ class DeviceFactor final : public DeviceFactorBase {
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
                            lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, st, pf);
     }
+    launch_rows_e<T>(S, fv, P.rs_off[P.ns], st);
     hip_check(hipGetLastError(), "factor launch");
     unsigned long long cnt[4];
     int32_t stat[2];
@@ -380,10 +389,12 @@ class DeviceFactor final : public DeviceFactorBase {
     stats.inertia[1] = (int64_t)cnt[1];
     stats.inertia[2] = (int64_t)cnt[2];
     stats.n_perturbed = (int64_t)cnt[3];
+    valid = true;
   }
 
   // y (device, e-order) := solve in place; b/x device user-order buffers
   void solve_device(const void* d_b, int64_t nrhs, void* d_x, hipStream_t st) override {
+    if (!valid) throw Error(INVALID_INPUT, "the last factorization of this handle failed; refactor first");
     const Plan& P = sym->plan;
     const DevSym& S = sym->view;
     const size_t need = sizeof(T) * std::max<int64_t>(P.n * nrhs, 1);

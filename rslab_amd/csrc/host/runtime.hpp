@@ -140,6 +140,7 @@ class DeviceFactorBase {
   FactorSettings settings;
   EventProfiler prof;
   bool prof_on = false;
+  bool valid = false;  // the last numeric factorization succeeded
   std::shared_ptr<DeviceSymbolic> sym;
 };
 

@@ -138,6 +138,18 @@ def test_singular_is_rank_deficient():
         R.LdltSolver.factor_with(a)
 
 
+def test_failed_refactor_blocks_solves_until_refactored():
+    a = R.CscMatrix.from_triplets(2, [0, 1], [0, 1], [2.0, 3.0])
+    z = a.with_values(np.array([0.0, 0.0]))
+    f = R.LdltSolver.factor_with(a)
+    with pytest.raises(R.NumericallyRankDeficient):
+        f.refactor(z)
+    with pytest.raises(R.InvalidInput):
+        f.solve(np.ones(2))
+    f.refactor(a)
+    assert np.allclose(f.solve(np.array([2.0, 3.0])), [1.0, 1.0], atol=1e-15)
+
+
 def test_perturb_to_eps_rescues_singular_complex():
     # multifrontal_ldlt.rs:4186-4221
     a = R.CscMatrix.from_triplets(3, [0, 2, 1], [0, 0, 1], np.array([2 + 1j, -1 + 0.3j, 0j]))
