@@ -54,10 +54,22 @@ struct CItem {
 };
 // One contribution of updater k to a tile: off-diagonal row ranges [a0,a1)
 // (tile rows) and [b0,b1) (tile columns), indices into k's off-diagonal rows.
+// When the level runs the two-phase cmod, entry (a, b) of the updater's
+// product block for this target sits at cbuf[cbase + a * ldc + b].
 struct CTask {
-  int32_t k, a0, a1, b0, b1, pad;
+  int32_t k, a0, a1, b0, b1, ldc;
   uint64_t rmask;  // tile rows hit by rows a0..a1-1 (in order)
   uint64_t cmask;  // tile columns hit by rows b0..b1-1 (in order)
+  int64_t cbase;
+};
+
+// One 64x64 tile of a product block  C = L_k[p0.., :] D_k L_k[p0..p0+N, :]^T
+// (rows p0..p0+M of k's off-diagonal rows against the N of them that fall in
+// the target's columns), stored row-major with leading dimension N from
+// element c_off of the level's product buffer.
+struct PTile {
+  int32_t k, p0, m, n, mt, nt;
+  int64_t c_off;
 };
 
 // One unit of the batched wide-supernode path: nb-panel [kb, kb+nb) of

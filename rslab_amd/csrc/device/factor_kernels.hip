@@ -325,6 +325,98 @@ __global__ __launch_bounds__(NT) void k_cmod_mfma(DevSym S, DevFac<double> Fc, c
   }
 }
 
+// Two-phase real-field cmod. Phase one computes every (updater, target)
+// product block densely on matrix cores,
+//   C = L_k[p0.., :] D_k L_k[p0..p0+N, :]^T,
+// one 64x64 tile per workgroup with contiguous operand rows (no gathers, no
+// masks), and stores it in the level's product buffer: FP64 for the FP64
+// path, FP32 for the reduced-precision ones (their products accumulate in
+// FP32 per updater, as in the gathered kernel). Phase two gives each target
+// tile one workgroup that sums its updaters' entries in FP64, in ascending
+// updater order, and subtracts the sum from the panel's lower triangle:
+// every thread owns fixed tile entries, so no two threads touch one entry and
+// the result does not depend on scheduling.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs; the product tiles
+// are renumbered so that each XCD runs one contiguous run of them, keeping a
+// pair's operand columns within one L2.
+__device__ inline int64_t xcd_contiguous(int64_t b, int64_t n) {
+  const int64_t x = b & 7, q = n >> 3, r = n & 7;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (b >> 3);
+}
+
+template <int PR>
+__global__ __launch_bounds__(NT) void k_cmod_prod(DevSym S, DevFac<double> Fc, const PTile* __restrict__ pt,
+                                                  int64_t npt, void* __restrict__ cbuf) {
+  typedef typename TileCfg<PR>::S Sx;
+  __shared__ __align__(16) Sx As[tile_stage_elems<PR>()];
+  __shared__ __align__(16) Sx Bs[tile_stage_elems<PR>()];
+  const PTile p = pt[xcd_contiguous(blockIdx.x, npt)];
+  const int k = p.k, nck = S.sn_ncol[k], eo = S.e_off[k];
+  const int64_t hk = S.sn_nrow[k];
+  const double* Pk = Fc.pool + S.panel_off[k];
+  const int ma = min(TM, p.m - p.mt * TM), mb = min(TN, p.n - p.nt * TN);
+  TileAcc<PR> acc;
+  tile_zero(acc);
+  ldl_tile_mfma<PR>(Pk, hk, 0, nck, nck + p.p0 + p.mt * TM, ma, nck + p.p0 + p.nt * TN, mb, Fc.d + eo, Fc.ds + eo,
+                    Fc.two + eo, acc, As, Bs);
+  const int64_t ldc = p.n, r0 = (int64_t)p.mt * TM, c0 = (int64_t)p.nt * TN;
+  if constexpr (PR == TP_F64) {
+    double* C = static_cast<double*>(cbuf) + p.c_off;
+    tile_for_each<PR>(acc, [&](int i, int j, double v) {
+      if (i < ma && j < mb) C[(r0 + i) * ldc + c0 + j] = v;
+    });
+  } else {
+    float* C = static_cast<float*>(cbuf) + p.c_off;
+    tile_for_each<PR>(acc, [&](int i, int j, double v) {
+      if (i < ma && j < mb) C[(r0 + i) * ldc + c0 + j] = (float)v;
+    });
+  }
+}
+
+template <class CT>
+__global__ __launch_bounds__(NT) void k_cmod_sum(DevSym S, DevFac<double> Fc, const CItem* __restrict__ items,
+                                                 const CTask* __restrict__ tasks, const CT* __restrict__ cbuf) {
+  __shared__ double tile[TN * (TM + 1)];
+  const int tid = threadIdx.x, j = tid & 63, i0 = tid >> 6;
+  const CItem it = items[blockIdx.x];
+  const uint64_t jbelow = (1ull << j) - 1ull;
+  // this thread's entries: column j, rows i0 + 4u
+  double acc[TM / 4];
+#pragma unroll
+  for (int u = 0; u < TM / 4; ++u) acc[u] = 0.0;
+  for (int t = it.t_beg; t < it.t_end; ++t) {
+    const CTask tk = tasks[t];
+    if (!((tk.cmask >> j) & 1ull)) continue;
+    const CT* cp = cbuf + tk.cbase + tk.b0 + __popcll(tk.cmask & jbelow);
+    CT v[TM / 4];
+#pragma unroll
+    for (int u = 0; u < TM / 4; ++u) {
+      const int i = i0 + 4 * u;
+      v[u] = 0;
+      if ((tk.rmask >> i) & 1ull) v[u] = cp[(int64_t)(tk.a0 + __popcll(tk.rmask & ((1ull << i) - 1ull))) * tk.ldc];
+    }
+#pragma unroll
+    for (int u = 0; u < TM / 4; ++u) acc[u] += (double)v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < TM / 4; ++u) tile[j * (TM + 1) + i0 + 4 * u] = acc[u];
+  __syncthreads();
+  const int s = it.s;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], gcol0 = S.sn_first[s] + it.c0;
+  double* P = Fc.pool + S.panel_off[s];
+  const int32_t* rs = S.rs_idx + S.rs_off[s] + it.r0;
+  const int mr = min<int64_t>(TM, h - it.r0), mc = min(TN, w - it.c0);
+  for (int e = tid; e < TM * TN; e += NT) {
+    const int i = e & 63, jj = e >> 6;
+    if (i < mr && jj < mc && rs[i] >= gcol0 + jj) {
+      double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + jj) * h;
+      *dst = *dst - tile[jj * (TM + 1) + i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // cdiv building blocks
 // ---------------------------------------------------------------------------
@@ -1164,6 +1256,25 @@ void launch_cmod(const DevSym& S, const DevFac<T>& Fc, const CItem* items, int n
   } else {
     hipLaunchKernelGGL((k_cmod<T, TP_F64>), dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks);
   }
+}
+
+void launch_cmod_two_phase(const DevSym& S, const DevFac<double>& Fc, const CItem* items, int nitems,
+                           const CTask* tasks, const PTile* ptiles, int64_t nptiles, void* cbuf, int64_t c_elems,
+                           double flops, int prec, hipStream_t st, LaunchProfiler* pf) {
+  if (nitems <= 0 || nptiles <= 0) return;
+  const size_t esz = prec == 0 ? sizeof(double) : sizeof(float);
+  {
+    ProfScope ps(pf, LK_CMOD, flops, 0.0, st);
+    RSL_PREC_SWITCH(double, prec, hipLaunchKernelGGL((k_cmod_prod<P_>), dim3((unsigned)nptiles), dim3(NT), 0, st, S,
+                                                     Fc, ptiles, nptiles, cbuf));
+  }
+  ProfScope ps(pf, LK_CMOD_SUM, 0.0, (double)c_elems * esz, st);
+  if (prec == 0)
+    hipLaunchKernelGGL((k_cmod_sum<double>), dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks,
+                       static_cast<const double*>(cbuf));
+  else
+    hipLaunchKernelGGL((k_cmod_sum<float>), dim3(nitems), dim3(NT), 0, st, S, Fc, items, tasks,
+                       static_cast<const float*>(cbuf));
 }
 
 template <class T>

@@ -23,6 +23,7 @@ enum LaunchKind : int {
   LK_BWD_TRI,
   LK_SOLVE_SCATTER,
   LK_RESIDUAL,
+  LK_CMOD_SUM,
   LK_COUNT
 };
 

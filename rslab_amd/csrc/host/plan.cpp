@@ -101,6 +101,8 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   // cmod tiles and tasks
   P.items.clear();
   P.tasks.clear();
+  P.ptiles.clear();
+  P.c_elems_max = 0;
   std::vector<int32_t> gloc(S.n, -1);
   std::vector<std::vector<CTask>> tt;
   struct Run {
@@ -109,6 +111,8 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   std::vector<Run> cruns, rruns;
   for (LevelPlan& L : P.levels) {
     L.item_beg = (int64_t)P.items.size();
+    L.ptile_beg = (int64_t)P.ptiles.size();
+    int64_t coff = 0;  // product blocks of the level, one after the other
     for (int32_t s : L.all) {
       if (P.upd_ptr[s] == P.upd_ptr[s + 1]) continue;
       const int32_t h = P.sn_nrow[s], w = P.sn_ncol[s], first = P.sn_first[s];
@@ -123,6 +127,13 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
         const int32_t p0 = (int32_t)(std::lower_bound(ok, ok + nok, first) - ok);
         const int32_t p1 = (int32_t)(std::lower_bound(ok, ok + nok, first + w) - ok);
         if (p0 == p1) continue;
+        // product block of the pair: rows p0..nok, columns p0..p1 (row-major);
+        // tiles wholly above its diagonal are never read and not computed
+        const int32_t pm = nok - p0, pn = p1 - p0;
+        for (int32_t nt = 0; nt * TILE < pn; ++nt)
+          for (int32_t mt = nt; mt * TILE < pm; ++mt) P.ptiles.push_back({k, p0, pm, pn, mt, nt, coff});
+        const int64_t cbase = coff - (int64_t)p0 * pn - p0;
+        coff += (int64_t)pm * pn;
         cruns.clear();
         for (int32_t b = p0; b < p1; ++b) {
           int32_t cb = (ok[b] - first) / TILE;
@@ -146,7 +157,7 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
             if (r.hi - a0 > TILE || c.hi - c.lo > TILE ||
                 ok[c.lo] - first < c.tile * TILE || ok[c.hi - 1] - first >= std::min(w, (c.tile + 1) * TILE))
               throw std::runtime_error("plan: cmod task outside its tile");
-            CTask tk{k, a0, r.hi, c.lo, c.hi, 0, 0, 0};
+            CTask tk{k, a0, r.hi, c.lo, c.hi, pn, 0, 0, cbase};
             for (int32_t a = a0; a < r.hi; ++a) tk.rmask |= 1ull << (gloc[ok[a]] - r.tile * TILE);
             for (int32_t b = c.lo; b < c.hi; ++b) tk.cmask |= 1ull << (ok[b] - first - c.tile * TILE);
             tt[(size_t)r.tile * nct + c.tile].push_back(tk);
@@ -171,6 +182,10 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
       for (int32_t i = 0; i < h; ++i) gloc[rss[i]] = -1;
     }
     L.item_end = (int64_t)P.items.size();
+    L.ptile_end = (int64_t)P.ptiles.size();
+    if (L.ptile_end - L.ptile_beg > INT32_MAX) throw std::runtime_error("plan: too many cmod product tiles");
+    L.c_elems = coff;
+    P.c_elems_max = std::max(P.c_elems_max, coff);
   }
   if ((int64_t)P.tasks.size() > INT32_MAX) throw std::runtime_error("plan: too many cmod tasks");
 

@@ -18,6 +18,8 @@ struct LevelPlan {
   std::vector<int32_t> small;   // one workgroup per supernode (cdiv)
   std::vector<int32_t> wide;    // multi-launch panel path (cdiv)
   int64_t item_beg = 0, item_end = 0;  // cmod items of the level
+  int64_t ptile_beg = 0, ptile_end = 0;  // two-phase cmod: product tiles of the level
+  int64_t c_elems = 0;                   // product-buffer elements the level needs
   int64_t fwd_beg = 0, fwd_end = 0;    // solve: (supernode, row chunk) L21 y items
   int64_t bwd_beg = 0, bwd_end = 0;    // solve: (supernode, column chunk) L21^T x items
   int64_t gat_beg = 0, gat_end = 0;    // solve: (supernode, own-row chunk) gather items
@@ -50,6 +52,8 @@ struct Plan {
   std::vector<LevelPlan> levels;
   std::vector<CItem> items;
   std::vector<CTask> tasks;
+  std::vector<PTile> ptiles;
+  int64_t c_elems_max = 0;      // largest c_elems over the levels
   std::vector<WItem> witems;    // batched wide-path work units
   // solve plan
   std::vector<int64_t> u_off;   // scratch rows of supernode s (h - w of them)

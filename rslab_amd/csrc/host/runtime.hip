@@ -18,6 +18,8 @@ template <class T> void launch_fill(T*, int64_t, T, hipStream_t);
 template <class T> void launch_rows_e(const DevSym&, const DevFac<T>&, int64_t, hipStream_t);
 template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, int, hipStream_t, LaunchProfiler*);
+void launch_cmod_two_phase(const DevSym&, const DevFac<double>&, const CItem*, int, const CTask*, const PTile*,
+                           int64_t, void*, int64_t, double, int, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, int, hipStream_t, LaunchProfiler*);
 template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, int, hipStream_t, LaunchProfiler*);
 template <class T> hipError_t configure_factor_kernels();
@@ -48,7 +50,7 @@ const char* launch_kind_name(int k) {
   static const char* names[LK_COUNT] = {"assemble",     "cmod",        "cdiv_small", "panel_getf2",
                                         "panel_replay", "panel_schur", "emit",       "solve_gather",
                                         "fwd_tri",      "fwd_update",  "dsolve",     "bwd_update",
-                                        "bwd_tri",      "solve_scatter", "residual"};
+                                        "bwd_tri",      "solve_scatter", "residual",   "cmod_sum"};
   return (k >= 0 && k < LK_COUNT) ? names[k] : "?";
 }
 
@@ -168,6 +170,7 @@ void DeviceSymbolic::upload() {
   a_col.upload(P.a_col, stream);
   items.upload(P.items, stream);
   tasks.upload(P.tasks, stream);
+  ptiles.upload(P.ptiles, stream);
   u_off.upload(P.u_off, stream);
   gat_ptr.upload(P.gat_ptr, stream);
   gat_idx.upload(P.gat_idx, stream);
@@ -272,7 +275,8 @@ template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, snap, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
-      work_y, work_b, work_p;
+      work_y, work_b, work_p, cbuf;
+  int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
   DevFac<T> fv{};
   std::vector<double> scale;
   bool configured = false;
@@ -301,6 +305,14 @@ class DeviceFactor final : public DeviceFactorBase {
     hip_check(hipMemset(e_of_g.p, 0, e_of_g.bytes), "memset");
     hip_check(hipMemset(perm_e.p, 0, perm_e.bytes), "memset");
     hip_check(hipMemset(rows_e.p, 0, rows_e.bytes), "memset");
+    if (FieldOf<T>::id == 0 && P.c_elems_max > 0) {
+      // two-phase cmod product buffer, at most a quarter of the free memory
+      const size_t esz = fs.precision == Precision::FP64 ? sizeof(double) : sizeof(float);
+      size_t fr = 0, tot = 0;
+      hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+      c_cap = std::min<int64_t>(P.c_elems_max, (int64_t)(fr / 4 / esz));
+      if (c_cap > 0) cbuf.alloc(esz * (size_t)c_cap);
+    }
     status.alloc(sizeof(int32_t) * 2);
     scale_d.alloc(sizeof(double) * std::max<int64_t>(n, 1));
     vals_d.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
@@ -367,8 +379,18 @@ class DeviceFactor final : public DeviceFactorBase {
     const int32_t* lists = sym->level_lists.as<int32_t>();
     for (size_t L = 0; L < P.levels.size(); ++L) {
       const LevelPlan& lv = P.levels[L];
-      launch_cmod<T>(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
-                     sym->tasks.as<CTask>(), lv.cmod_flops, prec, st, pf);
+      bool done = false;
+      if constexpr (std::is_same<T, double>::value) {
+        if (lv.c_elems <= c_cap) {
+          launch_cmod_two_phase(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
+                                sym->tasks.as<CTask>(), sym->ptiles.as<PTile>() + lv.ptile_beg,
+                                lv.ptile_end - lv.ptile_beg, cbuf.p, lv.c_elems, lv.cmod_flops, prec, st, pf);
+          done = true;
+        }
+      }
+      if (!done)
+        launch_cmod<T>(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
+                       sym->tasks.as<CTask>(), lv.cmod_flops, prec, st, pf);
       launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
                            lv.small_flops, prec, st, pf);
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
