@@ -69,14 +69,225 @@ __device__ inline T shfl_xor_t(T v, int m) {
   }
 }
 
-// L entry of the own block, with the 2x2 coupling row (held in D) read as zero.
+// ---------------------------------------------------------------------------
+// Own-block and update building blocks. Every loop that reads global memory
+// issues its loads as one batch into registers before using them, so a
+// workgroup keeps many loads in flight instead of waiting on one at a time.
+// ---------------------------------------------------------------------------
+constexpr int TB = 64;   // triangular block
+constexpr int NWV = SNT / 64;  // waves per workgroup
+
+// lane p's value, made wave-uniform (p wave-uniform)
+__device__ inline double lane_bcast(double v, int p) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), p);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), p);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline cplx lane_bcast(cplx v, int p) { return {lane_bcast(v.re, p), lane_bcast(v.im, p)}; }
+
+// sum over the 64 lanes (DPP partial sums inside rows of 16, then across the
+// rows); the total lands in lane 63. Fixed order: run-to-run identical.
+template <int CTRL, int RM>
+__device__ inline double dpp_take(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, RM, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, RM, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double wave_sum_to63(double v) {
+  v = v + dpp_take<0x111, 0xF>(v);
+  v = v + dpp_take<0x112, 0xF>(v);
+  v = v + dpp_take<0x114, 0xF>(v);
+  v = v + dpp_take<0x118, 0xF>(v);
+  v = v + dpp_take<0x142, 0xA>(v);
+  v = v + dpp_take<0x143, 0xC>(v);
+  return v;
+}
+__device__ inline cplx wave_sum_to63(cplx v) { return {wave_sum_to63(v.re), wave_sum_to63(v.im)}; }
+
+// Stage the strictly lower part of the own block's 64-block at j0 in LDS:
+// Ld[p * TB + i] = L[j0+i][j0+p] for i > p, zero elsewhere and on the 2x2
+// coupling entries (held in D, not in L).
 template <class T>
-__device__ inline T l_own(const T* P, int64_t h, const uint8_t* two, int i, int p) {
-  if (i == p + 1 && two[p]) return F<T>::zero();
-  return P[i + (int64_t)p * h];
+__device__ void stage_block_lower(const T* P, int64_t h, const uint8_t* two, int j0, int nbk, T* Ld) {
+  constexpr int PER = TB * TB / SNT;
+  T v[PER];
+  uint8_t tw[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * SNT, i = t % TB, p = t / TB;
+    const bool in = p < nbk && i < nbk && i > p;
+    v[u] = in ? P[(int64_t)(j0 + i) + (int64_t)(j0 + p) * h] : F<T>::zero();
+    tw[u] = (in && i == p + 1) ? two[j0 + p] : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) Ld[threadIdx.x + u * SNT] = tw[u] ? F<T>::zero() : v[u];
+  __syncthreads();
 }
 
-constexpr int TB = 64;  // triangular block
+// Transposed: Lt[p * (TB+1) + i] = L[j0+p][j0+i] for p > i (read along the
+// panel's columns, padded against LDS bank conflicts).
+template <class T>
+__device__ void stage_block_upper(const T* P, int64_t h, const uint8_t* two, int j0, int nbk, T* Lt) {
+  constexpr int PER = TB * TB / SNT;
+  T v[PER];
+  uint8_t tw[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * SNT, p = t % TB, i = t / TB;
+    const bool in = p < nbk && i < nbk && p > i;
+    v[u] = in ? P[(int64_t)(j0 + p) + (int64_t)(j0 + i) * h] : F<T>::zero();
+    tw[u] = (in && p == i + 1) ? two[j0 + i] : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * SNT, p = t % TB, i = t / TB;
+    Lt[p * (TB + 1) + i] = tw[u] ? F<T>::zero() : v[u];
+  }
+  __syncthreads();
+}
+
+// Forward substitution with the staged block: one wave per right-hand side,
+// two right-hand sides interleaved per wave; lane i holds y_i and takes
+// y_p from lane p once it is final.
+template <class T>
+__device__ void fwd_block_solve(const T* Ld, int eo, int j0, int nbk, int32_t nrhs, T* y) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t row = eo + j0 + lane;
+  for (int c = wv; c < nrhs; c += 2 * NWV) {
+    const int c2 = c + NWV;
+    const bool two_rhs = c2 < nrhs;
+    T y0 = lane < nbk ? y[row * nrhs + c] : F<T>::zero();
+    T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c2] : F<T>::zero();
+    for (int p = 0; p + 1 < nbk; ++p) {
+      const T l = Ld[p * TB + lane];
+      const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
+      y0 = y0 - l * a0;
+      y1 = y1 - l * a1;
+    }
+    if (lane < nbk) {
+      y[row * nrhs + c] = y0;
+      if (two_rhs) y[row * nrhs + c2] = y1;
+    }
+  }
+}
+
+// Back substitution with L^T of the staged block (Lt from stage_block_upper).
+template <class T>
+__device__ void bwd_block_solve(const T* Lt, int eo, int j0, int nbk, int32_t nrhs, T* y) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t row = eo + j0 + lane;
+  for (int c = wv; c < nrhs; c += 2 * NWV) {
+    const int c2 = c + NWV;
+    const bool two_rhs = c2 < nrhs;
+    T y0 = lane < nbk ? y[row * nrhs + c] : F<T>::zero();
+    T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c2] : F<T>::zero();
+    for (int p = nbk - 1; p > 0; --p) {
+      const T l = Lt[p * (TB + 1) + lane];
+      const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
+      y0 = y0 - l * a0;
+      y1 = y1 - l * a1;
+    }
+    if (lane < nbk) {
+      y[row * nrhs + c] = y0;
+      if (two_rhs) y[row * nrhs + c2] = y1;
+    }
+  }
+}
+
+// red[r][k] = sum_{p in [c_lo, c_hi)} L[row0 + r][p] * y[(eo + p) * nrhs + c0 + k]
+// for r < nr (<= 64), k < nc (<= RCH). Thread (r, q): columns c_lo + q + 4m,
+// 8 of them per batch of 32; the batch's y rows are staged in LDS. zr/zc name one
+// entry (relative row, absolute column) read as zero: a 2x2 coupling entry
+// that falls outside the diagonal block, or -1.
+template <class T>
+__device__ void rows_times_y(const T* P, int64_t h, int64_t row0, int nr, int c_lo, int c_hi, int eo,
+                             int32_t nrhs, int c0, int nc, const T* __restrict__ y, int zr, int zc, T* ys,
+                             T* red) {
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  constexpr int CB = 32;  // columns per batch
+  T acc[RCH];
+#pragma unroll
+  for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
+  for (int pb = c_lo; pb < c_hi; pb += CB) {
+    const int ncb = min(CB, c_hi - pb);
+    // y rows of the batch: CB x RCH values, one per thread
+    {
+      const int t = threadIdx.x, p = t / RCH, k = t % RCH;
+      ys[t] = (p < ncb && k < nc) ? y[(int64_t)(eo + pb + p) * nrhs + c0 + k] : F<T>::zero();
+    }
+    T l[CB / NWV];
+#pragma unroll
+    for (int m = 0; m < CB / NWV; ++m) {
+      const int p = q + NWV * m;
+      l[m] = (r < nr && p < ncb) ? P[(row0 + r) + (int64_t)(pb + p) * h] : F<T>::zero();
+      if (r == zr && pb + p == zc) l[m] = F<T>::zero();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < CB / NWV; ++m) {
+      const T* yp = ys + (q + NWV * m) * RCH;
+#pragma unroll
+      for (int k = 0; k < RCH; ++k) acc[k] = acc[k] + l[m] * yp[k];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < RCH; ++k) red[(q * 64 + r) * RCH + k] = acc[k];
+  __syncthreads();
+  // fixed-order sum over the four column groups into group 0's slots
+  for (int t = threadIdx.x; t < 64 * RCH; t += SNT) {
+    T v = red[t];
+    v = v + red[64 * RCH + t];
+    v = v + red[2 * 64 * RCH + t];
+    v = v + red[3 * 64 * RCH + t];
+    red[t] = v;
+  }
+  __syncthreads();
+}
+
+// part[p][k] = sum_{i < nr} L[r0 + i][j0 + p] * x[(eo + r0 + i) * nrhs + c0 + k]
+// (transposed product of a 64-row chunk of the own block), p < nbk, k < nc:
+// L chunk and x rows staged in LDS, thread (p, k-pair) sums 64 terms.
+template <class T>
+__device__ void chunk_t_times_x(const T* P, int64_t h, const uint8_t* two, int eo, int j0, int nbk, int r0, int nr,
+                                int32_t nrhs, int c0, int nc, const T* __restrict__ y, T* Ls, T* xs, T* out) {
+  constexpr int PER = TB * TB / SNT;
+  T v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * SNT, i = t % TB, p = t / TB;
+    v[u] = (p < nbk && i < nr) ? P[(int64_t)(r0 + i) + (int64_t)(j0 + p) * h] : F<T>::zero();
+  }
+  T xv[TB * RCH / SNT];
+#pragma unroll
+  for (int u = 0; u < TB * RCH / SNT; ++u) {
+    const int t = threadIdx.x + u * SNT, i = t / RCH, k = t % RCH;
+    xv[u] = (i < nr && k < nc) ? y[(int64_t)(eo + r0 + i) * nrhs + c0 + k] : F<T>::zero();
+  }
+  // a 2x2 coupling entry (j0+p+1, j0+p) can only sit in this chunk at its
+  // first row, when the chunk starts right below the block
+  const bool zc = r0 == j0 + nbk && nbk > 0 && two[j0 + nbk - 1];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * SNT, i = t % TB, p = t / TB;
+    Ls[i * (TB + 1) + p] = (zc && i == 0 && p == nbk - 1) ? F<T>::zero() : v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < TB * RCH / SNT; ++u) xs[threadIdx.x + u * SNT] = xv[u];
+  __syncthreads();
+  const int p = threadIdx.x & 63, kq = threadIdx.x >> 6;
+  T a0 = F<T>::zero(), a1 = F<T>::zero();
+#pragma unroll 8
+  for (int i = 0; i < TB; ++i) {
+    const T l = Ls[i * (TB + 1) + p];
+    a0 = a0 + l * xs[i * RCH + kq];
+    a1 = a1 + l * xs[i * RCH + kq + NWV];
+  }
+  out[p * RCH + kq] = a0;
+  out[p * RCH + kq + NWV] = a1;
+  __syncthreads();
+}
 
 // Forward, gather: own rows of a level's supernodes take the contributions of
 // their descendants (ascending updater order, precomputed); 64-row chunks.
@@ -92,58 +303,20 @@ __global__ __launch_bounds__(SNT) void k_fgather(DevSym S, DevFac<T> Fc, const i
     const int g = first + gl;
     const int64_t e = Fc.e_of_g[g];
     T acc = y[e * nrhs + c];
-    for (int64_t q = S.gat_ptr[g]; q < S.gat_ptr[g + 1]; ++q) acc = acc - u[S.gat_idx[q] * nrhs + c];
+    int64_t q = S.gat_ptr[g];
+    const int64_t qe = S.gat_ptr[g + 1];
+    for (; q + 4 <= qe; q += 4) {
+      int64_t ix[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) ix[m] = S.gat_idx[q + m];
+      T v[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) v[m] = u[ix[m] * nrhs + c];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc = acc - v[m];
+    }
+    for (; q < qe; ++q) acc = acc - u[S.gat_idx[q] * nrhs + c];
     y[e * nrhs + c] = acc;
-  }
-}
-
-// Solve the unit-lower 64-block at j0 in place for all right-hand sides: one
-// wave per right-hand side, the block's strict lower part staged in LDS.
-template <class T>
-__device__ void fwd_diag_block(const T* P, int64_t h, const uint8_t* two, int eo, int j0, int nbk, int32_t nrhs,
-                               T* y, T* Ld) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int t = threadIdx.x; t < TB * TB; t += SNT) {
-    const int i = t % TB, p = t / TB;
-    Ld[t] = (p < nbk && i < nbk && i > p) ? l_own(P, h, two, j0 + i, j0 + p) : F<T>::zero();
-  }
-  __syncthreads();
-  for (int c = wv; c < nrhs; c += SNT / 64) {
-    T yi = lane < nbk ? y[(int64_t)(eo + j0 + lane) * nrhs + c] : F<T>::zero();
-    for (int p = 0; p < nbk; ++p) {
-      const T yp = shfl_t(yi, p);
-      yi = yi - Ld[p * TB + lane] * yp;
-    }
-    if (lane < nbk) y[(int64_t)(eo + j0 + lane) * nrhs + c] = yi;
-  }
-  __syncthreads();
-}
-
-// y_i -= sum_{p<nbk} L[i][j0+p] y_{j0+p} for own rows i in [r0, r1): thread per
-// row, right-hand sides in chunks of RCH, the solved block's y staged in LDS.
-template <class T>
-__device__ void fwd_rows_update(const T* P, int64_t h, const uint8_t* two, int eo, int j0, int nbk, int r0,
-                                int r1, int32_t nrhs, T* y, T* yb) {
-  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
-    const int nc = min(RCH, nrhs - c0);
-    for (int t = threadIdx.x; t < nbk * RCH; t += SNT) {
-      const int p = t / RCH, k = t % RCH;
-      yb[t] = k < nc ? y[(int64_t)(eo + j0 + p) * nrhs + c0 + k] : F<T>::zero();
-    }
-    __syncthreads();
-    for (int i = r0 + threadIdx.x; i < r1; i += SNT) {
-      T acc[RCH];
-#pragma unroll
-      for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
-      for (int p = 0; p < nbk; ++p) {
-        const T l = l_own(P, h, two, i, j0 + p);
-#pragma unroll
-        for (int k = 0; k < RCH; ++k) acc[k] = acc[k] + l * yb[p * RCH + k];
-      }
-      T* yi = y + (int64_t)(eo + i) * nrhs + c0;
-      for (int k = 0; k < nc; ++k) yi[k] = yi[k] - acc[k];
-    }
-    __syncthreads();
   }
 }
 
@@ -153,7 +326,8 @@ __global__ __launch_bounds__(SNT) void k_fwd_tri(DevSym S, DevFac<T> Fc,
                                                  const int32_t* __restrict__ list, int32_t nrhs,
                                                  T* __restrict__ y) {
   __shared__ T Ld[TB * TB];
-  __shared__ T yb[TB * RCH];
+  __shared__ T ys[64 * RCH];
+  __shared__ T red[4 * 64 * RCH];
   const int s = list[blockIdx.x];
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
@@ -161,8 +335,26 @@ __global__ __launch_bounds__(SNT) void k_fwd_tri(DevSym S, DevFac<T> Fc,
   const uint8_t* two = Fc.two + eo;
   for (int j0 = 0; j0 < w; j0 += TB) {
     const int nbk = min(TB, w - j0);
-    fwd_diag_block(P, h, two, eo, j0, nbk, nrhs, y, Ld);
-    if (j0 + nbk < w) fwd_rows_update(P, h, two, eo, j0, nbk, j0 + nbk, w, nrhs, y, yb);
+    stage_block_lower(P, h, two, j0, nbk, Ld);
+    fwd_block_solve(Ld, eo, j0, nbk, nrhs, y);
+    __syncthreads();
+    const int rb = j0 + nbk;
+    for (int r0 = rb; r0 < w; r0 += 64) {
+      const int nr = min(64, w - r0);
+      const int zr = (r0 == rb && two[rb - 1]) ? 0 : -1;
+      for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+        const int nc = min(RCH, nrhs - c0);
+        rows_times_y(P, h, r0, nr, j0, rb, eo, nrhs, c0, nc, y, zr, rb - 1, ys, red);
+        for (int t = threadIdx.x; t < nr * RCH; t += SNT) {
+          const int rr = t / RCH, k = t % RCH;
+          if (k < nc) {
+            const int64_t o = (int64_t)(eo + r0 + rr) * nrhs + c0 + k;
+            y[o] = y[o] - red[rr * RCH + k];
+          }
+        }
+        __syncthreads();
+      }
+    }
   }
 }
 
@@ -174,20 +366,35 @@ __global__ __launch_bounds__(SNT) void k_ftri_diag(DevSym S, DevFac<T> Fc, const
   const SItem it = items[blockIdx.x];
   const int s = it.s, j0 = it.j0;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
-  fwd_diag_block(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, eo, j0, min(TB, w - j0), nrhs,
-                 y, Ld);
+  const int nbk = min(TB, w - j0);
+  stage_block_lower(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, j0, nbk, Ld);
+  fwd_block_solve(Ld, eo, j0, nbk, nrhs, y);
 }
 
 // ... then the 64-row chunks below it inside the own block
 template <class T>
 __global__ __launch_bounds__(SNT) void k_ftri_upd(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items,
                                                   int32_t nrhs, T* __restrict__ y) {
-  __shared__ T yb[TB * RCH];
+  __shared__ T ys[64 * RCH];
+  __shared__ T red[4 * 64 * RCH];
   const SItem it = items[blockIdx.x];
-  const int s = it.s, j0 = it.j0;
+  const int s = it.s, j0 = it.j0, r0 = it.r0;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
-  fwd_rows_update(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, eo, j0, min(TB, w - j0), it.r0,
-                  min(it.r0 + 64, w), nrhs, y, yb);
+  const int rb = j0 + min(TB, w - j0), nr = min(64, w - r0);
+  const int zr = (r0 == rb && Fc.two[eo + rb - 1]) ? 0 : -1;
+  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+    const int nc = min(RCH, nrhs - c0);
+    rows_times_y(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], r0, nr, j0, rb, eo, nrhs, c0, nc, y, zr, rb - 1,
+                 ys, red);
+    for (int t = threadIdx.x; t < nr * RCH; t += SNT) {
+      const int rr = t / RCH, k = t % RCH;
+      if (k < nc) {
+        const int64_t o = (int64_t)(eo + r0 + rr) * nrhs + c0 + k;
+        y[o] = y[o] - red[rr * RCH + k];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Forward, part 2: u_s = L21 y_own for 64-row chunks of the off-diagonal rows
@@ -196,51 +403,35 @@ template <class T>
 __global__ __launch_bounds__(SNT) void k_fwd_update(DevSym S, DevFac<T> Fc,
                                                     const int2* __restrict__ items, int32_t nrhs,
                                                     const T* __restrict__ y, T* __restrict__ u) {
-  __shared__ T part[4][64][8];
+  __shared__ T ys[64 * RCH];
+  __shared__ T red[4 * 64 * RCH];
   const int2 it = items[blockIdx.x];
   const int s = it.x, r0 = it.y;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
-  const T* P = Fc.pool + S.panel_off[s];
-  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int64_t row = w + r0 + r;
-  const bool live = row < h;
-  const int64_t uo = S.u_off[s] + r0 + r;
-  for (int c0 = 0; c0 < nrhs; c0 += 8) {
-    const int nc = min(8, nrhs - c0);
-    T acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = F<T>::zero();
-    if (live) {
-      for (int p = q; p < w; p += 4) {
-        const T l = P[row + (int64_t)p * h];
-        const T* yp = y + (int64_t)(eo + p) * nrhs + c0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (k < nc) acc[k] = acc[k] + l * yp[k];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) part[q][r][k] = acc[k];
-    __syncthreads();
-    if (q == 0 && live) {
-      for (int k = 0; k < nc; ++k) {
-        T v = part[0][r][k] + part[1][r][k];
-        v = v + part[2][r][k];
-        v = v + part[3][r][k];
-        u[uo * nrhs + c0 + k] = v;
-      }
+  const int nr = (int)min<int64_t>(64, h - w - r0);
+  const int64_t uo = S.u_off[s] + r0;
+  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+    const int nc = min(RCH, nrhs - c0);
+    rows_times_y(Fc.pool + S.panel_off[s], h, (int64_t)w + r0, nr, 0, w, eo, nrhs, c0, nc, y, -1, -1, ys, red);
+    for (int t = threadIdx.x; t < nr * RCH; t += SNT) {
+      const int rr = t / RCH, k = t % RCH;
+      if (k < nc) u[(uo + rr) * nrhs + c0 + k] = red[rr * RCH + k];
     }
     __syncthreads();
   }
 }
 
 // Backward, part 1: y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]] for 16-column
-// chunks (items (supernode, column offset)); one wave per column.
+// chunks (items (supernode, column offset)). Lanes run over rows, 256 rows
+// per pass (their x rows gathered into LDS), each wave owns four columns;
+// the lane partial sums are reduced once at the end.
 template <class T>
 __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
                                                     const int2* __restrict__ items, int32_t nrhs,
                                                     T* __restrict__ y) {
+  constexpr int RB = 256, CW = 4;  // rows per pass, columns per wave
+  __shared__ T xs[RB * RCH];
   const int2 it = items[blockIdx.x];
   const int s = it.x, p0 = it.y;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
@@ -248,52 +439,63 @@ __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
   const T* P = Fc.pool + S.panel_off[s];
   const int32_t* re = Fc.rs_e + S.rs_off[s];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int p = p0 + wv; p < min(p0 + 16, w); p += SNT / 64) {
-    for (int c0 = 0; c0 < nrhs; c0 += RCH) {
-      const int nc = min(RCH, nrhs - c0);
-      T acc[RCH];
+  const int pw = p0 + wv * CW;  // this wave's first column
+  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+    const int nc = min(RCH, nrhs - c0);
+    T acc[CW][RCH];
 #pragma unroll
-      for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
-      for (int64_t i = w + lane; i < h; i += 64) {
-        const T l = P[i + (int64_t)p * h];
-        const T* xs = y + (int64_t)re[i] * nrhs + c0;
+    for (int a = 0; a < CW; ++a)
 #pragma unroll
-        for (int k = 0; k < RCH; ++k)
-          if (k < nc) acc[k] = acc[k] + l * xs[k];
+      for (int k = 0; k < RCH; ++k) acc[a][k] = F<T>::zero();
+    for (int64_t ib = w; ib < h; ib += RB) {
+      const int nr = (int)min<int64_t>(RB, h - ib);
+      // x rows of the pass: thread t gathers row t's RCH values
+      {
+        const int t = threadIdx.x;
+        T xv[RCH];
+        if (t < nr) {
+          const int64_t e = re[ib + t];
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) xv[k] = k < nc ? y[e * nrhs + c0 + k] : F<T>::zero();
+        } else {
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) xv[k] = F<T>::zero();
+        }
+#pragma unroll
+        for (int k = 0; k < RCH; ++k) xs[k * RB + t] = xv[k];  // [k][row]: lanes read consecutive words
       }
+      T l[RB / 64][CW];
+#pragma unroll
+      for (int sb = 0; sb < RB / 64; ++sb)
+#pragma unroll
+        for (int a = 0; a < CW; ++a) {
+          const int ir = sb * 64 + lane, p = pw + a;
+          l[sb][a] = (ir < nr && p < w) ? P[(ib + ir) + (int64_t)p * h] : F<T>::zero();
+        }
+      __syncthreads();
+#pragma unroll
+      for (int sb = 0; sb < RB / 64; ++sb) {
+        T xr[RCH];
+#pragma unroll
+        for (int k = 0; k < RCH; ++k) xr[k] = xs[k * RB + sb * 64 + lane];
+#pragma unroll
+        for (int a = 0; a < CW; ++a)
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) acc[a][k] = acc[a][k] + l[sb][a] * xr[k];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < CW; ++a)
 #pragma unroll
       for (int k = 0; k < RCH; ++k) {
-        T v = acc[k];
-        for (int o = 32; o > 0; o >>= 1) v = v + shfl_xor_t(v, o);
-        if (lane == 0 && k < nc) {
-          const int64_t o2 = (int64_t)(eo + p) * nrhs + c0 + k;
+        const T v = wave_sum_to63(acc[a][k]);
+        if (lane == 63 && k < nc && pw + a < w && pw + a < p0 + 16) {
+          const int64_t o2 = (int64_t)(eo + pw + a) * nrhs + c0 + k;
           y[o2] = y[o2] - v;
         }
       }
-    }
   }
-}
-
-// Back-substitute the transposed unit-lower 64-block at j0 in place, one
-// wave per right-hand side (L^T of the block staged in LDS).
-template <class T>
-__device__ void bwd_diag_block(const T* P, int64_t h, const uint8_t* two, int eo, int j0, int nbk, int32_t nrhs,
-                               T* y, T* Lt) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int t = threadIdx.x; t < TB * TB; t += SNT) {
-    const int i = t % TB, p = t / TB;
-    Lt[t] = (p < nbk && i < nbk && p > i) ? l_own(P, h, two, j0 + p, j0 + i) : F<T>::zero();
-  }
-  __syncthreads();
-  for (int c = wv; c < nrhs; c += SNT / 64) {
-    T yi = lane < nbk ? y[(int64_t)(eo + j0 + lane) * nrhs + c] : F<T>::zero();
-    for (int p = nbk - 1; p >= 0; --p) {
-      const T xp = shfl_t(yi, p);
-      yi = yi - Lt[p * TB + lane] * xp;
-    }
-    if (lane < nbk) y[(int64_t)(eo + j0 + lane) * nrhs + c] = yi;
-  }
-  __syncthreads();
 }
 
 // Backward own-block solve of a narrow supernode (w <= 128): one workgroup
@@ -302,45 +504,36 @@ template <class T>
 __global__ __launch_bounds__(SNT) void k_bwd_tri(DevSym S, DevFac<T> Fc,
                                                  const int32_t* __restrict__ list, int32_t nrhs,
                                                  T* __restrict__ y) {
-  __shared__ T Lt[TB * TB];
+  __shared__ T Lt[TB * (TB + 1)];
+  __shared__ T xs[TB * RCH];
+  __shared__ T out[TB * RCH];
   const int s = list[blockIdx.x];
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
   const T* P = Fc.pool + S.panel_off[s];
   const uint8_t* two = Fc.two + eo;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nblk = (w + TB - 1) / TB;
   for (int b = nblk - 1; b >= 0; --b) {
     const int j0 = b * TB, nbk = min(TB, w - j0), jend = j0 + nbk;
     // contributions of the already solved own rows below this block
-    if (jend < w) {
-      for (int p = j0 + wv; p < jend; p += SNT / 64) {
-        for (int c0 = 0; c0 < nrhs; c0 += RCH) {
-          const int nc = min(RCH, nrhs - c0);
-          T acc[RCH];
-#pragma unroll
-          for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
-          for (int i = jend + lane; i < w; i += 64) {
-            const T l = l_own(P, h, two, i, p);
-            const T* xs = y + (int64_t)(eo + i) * nrhs + c0;
-#pragma unroll
-            for (int k = 0; k < RCH; ++k)
-              if (k < nc) acc[k] = acc[k] + l * xs[k];
-          }
-#pragma unroll
-          for (int k = 0; k < RCH; ++k) {
-            T v = acc[k];
-            for (int o = 32; o > 0; o >>= 1) v = v + shfl_xor_t(v, o);
-            if (lane == 0 && k < nc) {
-              const int64_t o2 = (int64_t)(eo + p) * nrhs + c0 + k;
-              y[o2] = y[o2] - v;
-            }
+    for (int r0 = jend; r0 < w; r0 += 64) {
+      const int nr = min(64, w - r0);
+      for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+        const int nc = min(RCH, nrhs - c0);
+        chunk_t_times_x(P, h, two, eo, j0, nbk, r0, nr, nrhs, c0, nc, y, Lt, xs, out);
+        for (int t = threadIdx.x; t < nbk * RCH; t += SNT) {
+          const int p = t / RCH, k = t % RCH;
+          if (k < nc) {
+            const int64_t o = (int64_t)(eo + j0 + p) * nrhs + c0 + k;
+            y[o] = y[o] - out[t];
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-    bwd_diag_block(P, h, two, eo, j0, nbk, nrhs, y, Lt);
+    stage_block_upper(P, h, two, j0, nbk, Lt);
+    bwd_block_solve(Lt, eo, j0, nbk, nrhs, y);
+    __syncthreads();
   }
 }
 
@@ -351,30 +544,19 @@ __global__ __launch_bounds__(SNT) void k_btri_part(DevSym S, DevFac<T> Fc, const
                                                    int32_t nrhs, const T* __restrict__ y, T* __restrict__ part) {
   __shared__ T Ls[TB * (TB + 1)];
   __shared__ T xs[TB * RCH];
+  __shared__ T out[TB * RCH];
   const SItem it = items[blockIdx.x];
   const int s = it.s, j0 = it.j0;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
-  const int64_t h = S.sn_nrow[s];
-  const T* P = Fc.pool + S.panel_off[s];
-  const uint8_t* two = Fc.two + eo;
   const int nbk = min(TB, w - j0), r0 = it.r0, nr = min(64, w - r0);
-  for (int t = threadIdx.x; t < TB * TB; t += SNT) {
-    const int i = t % TB, p = t / TB;
-    Ls[i * (TB + 1) + p] = (p < nbk && i < nr) ? l_own(P, h, two, r0 + i, j0 + p) : F<T>::zero();
-  }
-  T* out = part + (int64_t)blockIdx.x * TB * nrhs;
+  T* dst = part + (int64_t)blockIdx.x * TB * nrhs;
   for (int c0 = 0; c0 < nrhs; c0 += RCH) {
     const int nc = min(RCH, nrhs - c0);
-    for (int t = threadIdx.x; t < TB * RCH; t += SNT) {
-      const int i = t / RCH, k = t % RCH;
-      xs[t] = (i < nr && k < nc) ? y[(int64_t)(eo + r0 + i) * nrhs + c0 + k] : F<T>::zero();
-    }
-    __syncthreads();
+    chunk_t_times_x(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, eo, j0, nbk, r0, nr, nrhs, c0,
+                    nc, y, Ls, xs, out);
     for (int t = threadIdx.x; t < TB * RCH; t += SNT) {
       const int p = t / RCH, k = t % RCH;
-      T acc = F<T>::zero();
-      for (int i = 0; i < nr; ++i) acc = acc + Ls[i * (TB + 1) + p] * xs[i * RCH + k];
-      if (k < nc) out[(int64_t)p * nrhs + c0 + k] = acc;
+      if (k < nc) dst[(int64_t)p * nrhs + c0 + k] = out[t];
     }
     __syncthreads();
   }
@@ -385,7 +567,7 @@ __global__ __launch_bounds__(SNT) void k_btri_part(DevSym S, DevFac<T> Fc, const
 template <class T>
 __global__ __launch_bounds__(SNT) void k_btri_diag(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items,
                                                    int32_t nrhs, T* __restrict__ y, const T* __restrict__ part) {
-  __shared__ T Lt[TB * TB];
+  __shared__ T Lt[TB * (TB + 1)];
   const SItem it = items[blockIdx.x];
   const int s = it.s, j0 = it.j0;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
@@ -393,12 +575,20 @@ __global__ __launch_bounds__(SNT) void k_btri_diag(DevSym S, DevFac<T> Fc, const
   for (int t = threadIdx.x; t < nbk * nrhs; t += SNT) {
     const int p = t / nrhs, c = t % nrhs;
     T acc = F<T>::zero();
-    for (int q = 0; q < it.r0; ++q) acc = acc + part[((int64_t)(it.x + q) * TB + p) * nrhs + c];
+    int q = 0;
+    for (; q + 4 <= it.r0; q += 4) {
+      T v[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) v[m] = part[((int64_t)(it.x + q + m) * TB + p) * nrhs + c];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc = acc + v[m];
+    }
+    for (; q < it.r0; ++q) acc = acc + part[((int64_t)(it.x + q) * TB + p) * nrhs + c];
     const int64_t o = (int64_t)(eo + j0 + p) * nrhs + c;
     y[o] = y[o] - acc;
   }
-  __syncthreads();
-  bwd_diag_block(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, eo, j0, nbk, nrhs, y, Lt);
+  stage_block_upper(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, j0, nbk, Lt);
+  bwd_block_solve(Lt, eo, j0, nbk, nrhs, y);
 }
 
 template <class T>
