@@ -44,8 +44,13 @@ def run_path(R, sym, a, B, prec, steps, warmup, torch):
         f.refactor(a)
         return mixed.solve_to(a, B, TARGET)
 
+    infos = None
     for _ in range(warmup):
-        step()
+        _, infos = step()
+    # a path that does not reach the target runs its ladder to the iteration
+    # cap every step: one timed step is enough to report it
+    if infos is not None and not all(i.certified for i in infos):
+        steps = 1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -59,6 +64,7 @@ def run_path(R, sym, a, B, prec, steps, warmup, torch):
     f.profile(False)
     res = dict(
         s_per_step=dt,
+        timed_steps=steps,
         factor_ms=f.factor_time_ms(),
         berr_max=max(i.backward_error for i in infos),
         certified=all(i.certified for i in infos),

@@ -494,6 +494,68 @@ __device__ void cdiv_init(const SnCtx<T>& c) {
   }
 }
 
+// Column of packed offset e in a pw x pw column-major packed lower triangle:
+// the largest j with S(j) = j*pw - j(j-1)/2 <= e, from the quadratic's root
+// in float and one integer correction each way (pw <= 128: all exact in float
+// up to the square root's last bit).
+__device__ inline int packed_col(int e, int pw) {
+  const float b = (float)(2 * pw + 1);
+  int j = (int)((b - sqrtf(b * b - 8.0f * (float)e)) * 0.5f);
+  if (j * pw - (j * (j - 1)) / 2 > e) --j;
+  if ((j + 1) * pw - ((j + 1) * j) / 2 <= e) ++j;
+  return j;
+}
+
+// Trailing update of one Bunch-Kaufman step on the packed block, element-
+// parallel: the columns k0..pw-1 form one contiguous run of the packed
+// triangle, swept in storage order (each thread every NTG-th entry, B of them
+// in flight), so every thread gets the same share and LDS accesses are
+// consecutive. Entry (i, j):
+//   1x1:  x -= w1[i] * l1[j]                      (skipped when l1[j] == 0)
+//   2x2:  x  = x - w1[i] * l1[j] - w2[i] * l2[j]
+template <int NTG, bool TWO, class T>
+__device__ inline void packed_trailing_update(T* Ab, int pw, int k0, const T* w1s, const T* w2s, const T* l1s,
+                                              const T* l2s) {
+  constexpr int B = 4;
+  const int base = pk_off(k0, k0, pw), tot = pw * (pw + 1) / 2;
+  for (int e0 = base + (int)threadIdx.x; e0 < tot; e0 += NTG * B) {
+    int ii[B], jj[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int e = e0 + NTG * u;
+      ii[u] = -1;
+      jj[u] = 0;
+      if (e < tot) {
+        const int j = packed_col(e, pw);
+        jj[u] = j;
+        ii[u] = j + (e - (j * pw - (j * (j - 1)) / 2));
+      }
+    }
+    T x[B], a[B], m[B], b[B], m2[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (ii[u] >= 0) {
+        x[u] = Ab[e0 + NTG * u];
+        a[u] = w1s[ii[u]];
+        m[u] = l1s[jj[u]];
+        if constexpr (TWO) {
+          b[u] = w2s[ii[u]];
+          m2[u] = l2s[jj[u]];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (ii[u] >= 0) {
+        if constexpr (TWO)
+          Ab[e0 + NTG * u] = x[u] - a[u] * m[u] - b[u] * m2[u];
+        else if (!F<T>::is_zero(m[u]))
+          Ab[e0 + NTG * u] = x[u] - a[u] * m[u];
+      }
+    }
+  }
+}
+
 // Bunch-Kaufman on the diagonal block [kb,ke) held in LDS (packed lower).
 // Candidates, interchanges and updates bounded to the panel rows, as
 // ll_cdiv_emit's getf2 (multifrontal_ldlt.rs:2846-3016). Returns 1 when the
@@ -503,8 +565,6 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
                            int32_t* deep_s, int32_t* two_s, int32_t* lperm_s, PivotPolicy pol, int* npert,
                            bool write_snap) {
   const int tid = threadIdx.x, lane = tid & 63;
-  constexpr int RWP = NTG < NBMAX ? NTG : NBMAX;  // trailing rows per pass
-  constexpr int CS = NTG / RWP;                   // column stride of a thread
   const int pw = ke - kb, ld = pw;
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   const double GROWTH_EPS = 1e-14;
@@ -609,10 +669,9 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       }
       if (tid == 0) c.d[kb + k] = dk;
       const T dinv = F<T>::recip(dk);
-      // time-of-step multipliers and in-block rank-1 update (rows of this panel)
-      // column k (w1s) and its multipliers (l1s), then the rank-1 update of the
-      // trailing block row by row (two threads per row, alternate columns)
-      // with column k scaled in the same pass
+      // time-of-step multipliers and in-block rank-1 update (rows of this panel):
+      // column k (w1s) and its multipliers (l1s), then the trailing block and
+      // column k replaced by its multipliers
       const int nr = pw - k - 1;
       for (int t = tid; t < nr; t += NTG) {
         const int j = k + 1 + t;
@@ -623,28 +682,8 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         if (write_snap) snap[(int64_t)k * sld + j] = m;
       }
       __syncthreads();
-      for (int i = k + 1 + (tid % RWP); i < pw; i += RWP) {
-        const T a = w1s[i];
-        int j = k + 1 + tid / RWP;
-        // batches of 8 columns: all loads issued before the stores
-        for (; j + 7 * CS <= i; j += 8 * CS) {
-          T m[8], x[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            m[u] = l1s[j + CS * u];
-            x[u] = A_(i, j + CS * u);
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) x[u] = F<T>::is_zero(m[u]) ? x[u] : x[u] - a * m[u];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) A_(i, j + CS * u) = x[u];
-        }
-        for (; j <= i; j += CS) {
-          const T m = l1s[j];
-          if (!F<T>::is_zero(m)) A_(i, j) = A_(i, j) - a * m;
-        }
-        if (tid < RWP) A_(i, k) = l1s[i];
-      }
+      packed_trailing_update<NTG, false>(Ab, pw, k + 1, w1s, w2s, l1s, l2s);
+      for (int i = k + 1 + tid; i < pw; i += NTG) A_(i, k) = l1s[i];
       __syncthreads();
       k += 1;
     } else {
@@ -687,25 +726,10 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         }
       }
       __syncthreads();
-      for (int i = k + 2 + (tid % RWP); i < pw; i += RWP) {
-        const T a = w1s[i], b = w2s[i];
-        int j = k + 2 + tid / RWP;
-        for (; j + 7 * CS <= i; j += 8 * CS) {
-          T m1[8], m2[8], x[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            m1[u] = l1s[j + CS * u];
-            m2[u] = l2s[j + CS * u];
-            x[u] = A_(i, j + CS * u);
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) A_(i, j + CS * u) = x[u] - a * m1[u] - b * m2[u];
-        }
-        for (; j <= i; j += CS) A_(i, j) = A_(i, j) - a * l1s[j] - b * l2s[j];
-        if (tid < RWP) {
-          A_(i, k) = l1s[i];
-          A_(i, k + 1) = l2s[i];
-        }
+      packed_trailing_update<NTG, true>(Ab, pw, k + 2, w1s, w2s, l1s, l2s);
+      for (int i = k + 2 + tid; i < pw; i += NTG) {
+        A_(i, k) = l1s[i];
+        A_(i, k + 1) = l2s[i];
       }
       __syncthreads();
       k += 2;
