@@ -18,8 +18,6 @@ struct DevSym {
   const int32_t* upd_ptr;    // update lists (ns+1)
   const int32_t* upd_idx;
   const int32_t* sym_perm;   // permuted -> user index
-  const int64_t* snap_off;   // scratch offset (elements) for per-panel multipliers
-  const int32_t* snap_ld;    // leading dimension of that scratch (min(nb, w))
   const int64_t* u_off;      // solve scratch row offset of supernode s (h - w rows)
   const int64_t* gat_ptr;    // per permuted column: scratch rows to gather (ascending updater)
   const int64_t* gat_idx;
@@ -41,7 +39,6 @@ struct DevFac {
   int32_t* e_of_g;    // permuted index -> elimination position
   int32_t* rs_e;      // per panel row (rs_idx order): its elimination position
   int32_t* perm_e;    // elimination position -> user index
-  T* snap;            // per-panel time-of-step multipliers (scratch)
   int32_t* deep;      // per-panel interchange partners (scratch, nb per supernode)
   unsigned long long* counters;  // [0] pos [1] neg [2] zero [3] perturbed
   int32_t* sn_inertia;          // per supernode: pos, neg, zero

@@ -426,11 +426,10 @@ struct SnCtx {
   T* P;
   int64_t h;
   int w, first, eo, nb;
-  T *d, *ds, *snap;
+  T *d, *ds;
   uint8_t* two;
   int32_t* lperm;
   int32_t* deep;
-  int snap_ld;
 };
 
 template <class T>
@@ -447,8 +446,6 @@ __device__ inline SnCtx<T> make_ctx(const DevSym& S, const DevFac<T>& Fc, int s,
   c.ds = Fc.ds + c.eo;
   c.two = Fc.two + c.eo;
   c.lperm = Fc.lperm + c.eo;
-  c.snap = Fc.snap + S.snap_off[s];
-  c.snap_ld = S.snap_ld[s];
   c.deep = Fc.deep + (int64_t)s * 2 * NBMAX;
   return c;
 }
@@ -562,14 +559,11 @@ __device__ inline void packed_trailing_update(T* Ab, int pw, int k0, const T* w1
 // pivot policy is Fail and a zero / singular pivot is met (uniform).
 template <int NTG, class T>
 __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* l2s, T* w1s, T* w2s,
-                           int32_t* deep_s, int32_t* two_s, int32_t* lperm_s, PivotPolicy pol, int* npert,
-                           bool write_snap) {
+                           int32_t* deep_s, int32_t* two_s, int32_t* lperm_s, PivotPolicy pol, int* npert) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int pw = ke - kb, ld = pw;
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   const double GROWTH_EPS = 1e-14;
-  T* snap = c.snap;
-  const int sld = c.snap_ld;
 #define A_(i, j) Ab[pk_off((i), (j), ld)]
   for (int q = tid; q < pw; q += NTG) {
     deep_s[q] = -1;
@@ -669,9 +663,9 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       }
       if (tid == 0) c.d[kb + k] = dk;
       const T dinv = F<T>::recip(dk);
-      // time-of-step multipliers and in-block rank-1 update (rows of this panel):
-      // column k (w1s) and its multipliers (l1s), then the trailing block and
-      // column k replaced by its multipliers
+      // in-block rank-1 update (rows of this panel): column k (w1s) and its
+      // multipliers (l1s), then the trailing block and column k replaced by
+      // its multipliers
       const int nr = pw - k - 1;
       for (int t = tid; t < nr; t += NTG) {
         const int j = k + 1 + t;
@@ -679,7 +673,6 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         const T m = a * dinv;
         w1s[j] = a;
         l1s[j] = m;
-        if (write_snap) snap[(int64_t)k * sld + j] = m;
       }
       __syncthreads();
       packed_trailing_update<NTG, false>(Ab, pw, k + 1, w1s, w2s, l1s, l2s);
@@ -720,10 +713,6 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         l2s[i] = b;
         w1s[i] = wik;
         w2s[i] = wik1;
-        if (write_snap) {
-          snap[(int64_t)k * sld + i] = a;
-          snap[(int64_t)(k + 1) * sld + i] = b;
-        }
       }
       __syncthreads();
       packed_trailing_update<NTG, true>(Ab, pw, k + 2, w1s, w2s, l1s, l2s);
@@ -823,51 +812,6 @@ __device__ void panel_store(const SnCtx<T>& c, int kb, int ke, T* Ab, const int3
     }
   }
   __syncthreads();
-}
-
-// apply_bk_panel_trailing on deep rows i in [r0, r1), one row per thread.
-template <class T>
-__device__ void panel_replay_rows(const SnCtx<T>& c, int kb, int ke, const int32_t* deep, const int32_t* two_s,
-                                  int64_t r0, int64_t r1, int64_t tstart, int64_t tstride) {
-  const int64_t h = c.h;
-  const int sld = c.snap_ld;
-  for (int64_t i = r0 + tstart; i < r1; i += tstride) {
-    T* row = c.P + i;
-    int k = kb;
-    while (k < ke) {
-      const int kl = k - kb;
-      const int kp = deep[kl];
-      const bool two = two_s[kl] != 0;
-      if (kp >= 0) {
-        int src = two ? k + 1 : k;
-        T x = row[(int64_t)src * h];
-        row[(int64_t)src * h] = row[(int64_t)(kb + kp) * h];
-        row[(int64_t)(kb + kp) * h] = x;
-      }
-      if (two) {
-        const T d11 = c.d[k], d21 = c.ds[k], d22 = c.d[k + 1];
-        const T detinv = F<T>::recip(d11 * d22 - d21 * d21);
-        const T wk = row[(int64_t)k * h], wk1 = row[(int64_t)(k + 1) * h];
-        for (int j = k + 2; j < ke; ++j) {
-          const T m1 = c.snap[(int64_t)kl * sld + (j - kb)];
-          const T m2 = c.snap[(int64_t)(kl + 1) * sld + (j - kb)];
-          row[(int64_t)j * h] = row[(int64_t)j * h] - wk * m1 - wk1 * m2;
-        }
-        row[(int64_t)k * h] = (d22 * wk - d21 * wk1) * detinv;
-        row[(int64_t)(k + 1) * h] = (d11 * wk1 - d21 * wk) * detinv;
-        k += 2;
-      } else {
-        const T dinv = F<T>::recip(c.d[k]);
-        const T wk = row[(int64_t)k * h];
-        for (int j = k + 1; j < ke; ++j) {
-          const T m = c.snap[(int64_t)kl * sld + (j - kb)];
-          if (!F<T>::is_zero(m)) row[(int64_t)j * h] = row[(int64_t)j * h] - wk * m;
-        }
-        row[(int64_t)k * h] = wk * dinv;
-        k += 1;
-      }
-    }
-  }
 }
 
 // Deferred Schur update of one 64x64 tile of P[ke.., ke..w).
@@ -976,128 +920,16 @@ __device__ void flag_failure(const DevFac<T>& Fc, int s) {
 
 // The packed diagonal block is live only during getf2, the GEMM staging only
 // during the Schur update, so they share storage (fits 160 KB for c64, nb 128).
-template <class T>
-struct CdivShared {
-  union alignas(16) {
-    T Ab[NBMAX * (NBMAX + 1) / 2];
-    T stage[2 * STG];
-  } u;
-  T l1s[NBMAX], l2s[NBMAX], w1s[NBMAX], w2s[NBMAX];
-  int32_t deep_s[NBMAX], two_s[NBMAX], cperm_s[NBMAX], lperm_s[NBMAX];
-  int cnt[4];
-  int npert;
-  int any_swap;
-};
-
-// One workgroup factors a whole supernode (all its nb-panels).
-template <class T, int PR>
-__global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
-                                                   const int32_t* __restrict__ list, int nb_base,
-                                                   PivotPolicy pol) {
-  extern __shared__ unsigned char smem_raw[];
-  CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
-  const int s = list[blockIdx.x];
-  const int w = S.sn_ncol[s];
-  const int nb = (w >= 512 && nb_base < 128) ? 128 : nb_base;
-  SnCtx<T> c = make_ctx(S, Fc, s, nb);
-  if (threadIdx.x == 0) sh.npert = 0;
-  cdiv_init<NT>(c);
-  __syncthreads();
-  for (int kb = 0; kb < w; kb += nb) {
-    const int ke = min(kb + nb, w);
-    panel_load<NT>(c, kb, ke, sh.u.Ab);
-    int st = panel_getf2<NT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert, true);
-    if (st) {
-      flag_failure(Fc, s);
-      return;
-    }
-    panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
-    panel_replay_rows(c, kb, ke, sh.deep_s, sh.two_s, ke, c.h, threadIdx.x, NT);
-    __syncthreads();
-    const int64_t mt = c.h - ke;
-    const int cw = w - ke;
-    if (cw > 0 && mt > 0) {
-      const int nrt = (int)((mt + TM - 1) / TM), nct = (cw + TN - 1) / TN;
-      for (int cb = 0; cb < nct; ++cb)
-        for (int rb = cb; rb < nrt; ++rb) panel_schur_tile_call<PR>(c, kb, ke, rb * TM, cb * TN, sh.u.stage);
-      __syncthreads();
-    }
-  }
-  cdiv_emit(S, Fc, c, sh.cnt, sh.npert);
-}
-
-// ---- wide supernodes: batched over the level, one launch per phase of each
-// panel step (every supernode of the level that has a p-th panel) ----
-template <class T>
-__global__ __launch_bounds__(GT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
-                                                   PivotPolicy pol) {
-  extern __shared__ unsigned char smem_raw[];
-  CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
-  const WItem it = items[blockIdx.x];
-  const int s = it.s, kb = it.kb;
-  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
-  if (Fc.status[0]) return;  // the factorization has already failed
-  if (threadIdx.x == 0) sh.npert = 0;
-  if (kb == 0) cdiv_init<GT>(c);
-  __syncthreads();
-  const int ke = min(kb + c.nb, c.w);
-  panel_load<GT>(c, kb, ke, sh.u.Ab);
-  int st = panel_getf2<GT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert, false);
-  if (st) {
-    flag_failure(Fc, s);
-    return;
-  }
-  panel_store<GT, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
-  if (threadIdx.x == 0 && sh.npert) atomicAdd(&Fc.counters[3], (unsigned long long)sh.npert);
-}
-
-// The panel's row interchanges applied to the already factored columns
-// [0, kb) of its rows: blockIdx.y = the step's getf2 unit, blockIdx.x = a
-// block of 64 columns. Every thread gathers its permuted sources into
-// registers, the workgroup synchronizes, then they are stored.
-template <class T>
-__global__ __launch_bounds__(NT) void k_wide_swapcols(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
-  __shared__ int32_t cperm[NBMAX];
-  __shared__ int ident;
-  const WItem it = items[blockIdx.y];
-  const int s = it.s, kb = it.kb, j0 = 64 * blockIdx.x;
-  if (j0 >= kb || Fc.status[0]) return;
-  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
-  const int pw = min(kb + c.nb, c.w) - kb;
-  if (threadIdx.x == 0) ident = 1;
-  __syncthreads();
-  for (int q = threadIdx.x; q < pw; q += NT) {
-    cperm[q] = c.deep[NBMAX + q];
-    if (cperm[q] != q) ident = 0;
-  }
-  __syncthreads();
-  if (ident) return;
-  constexpr int PER = NBMAX * 64 / NT;
-  const int nc = min(64, kb - j0), n = pw * nc;
-  T v[PER];
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int e = threadIdx.x + u * NT;
-    const int i = e % pw, j = e / pw;
-    v[u] = e < n ? c.P[(kb + cperm[i]) + (int64_t)(j0 + j) * c.h] : F<T>::zero();
-  }
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int e = threadIdx.x + u * NT;
-    const int i = e % pw, j = e / pw;
-    if (e < n) c.P[(kb + i) + (int64_t)(j0 + j) * c.h] = v[u];
-  }
-}
-
 // Deep-row replay (apply_bk_panel_trailing, multifrontal_ldlt.rs:1347-1416) of
-// 64 rows [ke + a, ke + a + 64): the rows are staged in LDS with the panel's
-// composite column permutation applied up front, then the panel's steps are
-// replayed in order with the final multipliers L11 (a content's multiplier
-// moves with it under the later symmetric interchanges, so this is the same
-// sequence of operations), all 256 threads sharing each step's update, and
-// finally scaled by the 1x1 / 2x2 blocks of D.
-constexpr int RR = 64;  // deep rows per workgroup
+// up to RR rows [row0, row0 + nr) below a factored panel: the rows are staged
+// in LDS with the panel's composite column permutation applied up front, then
+// the panel's steps are replayed in order with the final multipliers L11 (a
+// content's multiplier moves with it under the later symmetric interchanges,
+// so this is the same sequence of operations), all threads sharing each
+// step's update, and finally scaled by the 1x1 / 2x2 blocks of D.
+// X holds RR * NBMAX elements, Lc (replay_chunk + 1) * NBMAX; cperm / two_s
+// are the panel's composite permutation and 2x2 flags (pw entries).
+constexpr int RR = 64;  // deep rows per pass
 // L11 columns staged per chunk (LDS: (RR + RCW + 1) * NBMAX elements)
 template <class T>
 constexpr int replay_chunk() {
@@ -1105,39 +937,39 @@ constexpr int replay_chunk() {
 }
 
 template <class T>
-__global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
-  extern __shared__ unsigned char smem_raw[];
-  T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
+__device__ void replay_rows_block(const SnCtx<T>& c, int kb, int ke, int64_t row0, int nr, T* X, T* Lc,
+                                  const int32_t* cperm, const int32_t* two_s) {
   constexpr int RCW = replay_chunk<T>();
-  T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
-  __shared__ int32_t cperm[NBMAX];
-  __shared__ int32_t two_s[NBMAX];
-  const WItem it = items[blockIdx.x];
-  const int s = it.s, kb = it.kb;
-  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
-  if (Fc.status[0]) return;
-  const int ke = min(kb + c.nb, c.w), pw = ke - kb;
-  const int64_t row0 = ke + (int64_t)it.a;
-  const int nr = (int)min<int64_t>(RR, c.h - row0);
+  constexpr int SB = 8;  // loads in flight per thread while staging
+  const int pw = ke - kb, n = pw * RR;
   const int tid = threadIdx.x, r = tid & (RR - 1), q = tid / RR;
-  for (int j = tid; j < pw; j += NT) {
-    cperm[j] = c.deep[NBMAX + j];
-    two_s[j] = c.two[kb + j];
-  }
-  __syncthreads();
-  for (int e = tid; e < pw * RR; e += NT) {
-    const int j = e / RR, rr = e % RR;
-    X[e] = rr < nr ? c.P[row0 + rr + (int64_t)(kb + cperm[j]) * c.h] : F<T>::zero();
+  for (int e0 = tid; e0 < n; e0 += NT * SB) {
+    T v[SB];
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      const int e = e0 + u * NT, j = e / RR, rr = e % RR;
+      v[u] = (e < n && rr < nr) ? c.P[row0 + rr + (int64_t)(kb + cperm[j]) * c.h] : F<T>::zero();
+    }
+#pragma unroll
+    for (int u = 0; u < SB; ++u)
+      if (e0 + u * NT < n) X[e0 + u * NT] = v[u];
   }
   const T* L = c.P + kb + (int64_t)kb * c.h;  // L11(j, k) = L[j + k * h]
   int k = 0;
   for (int k0 = 0; k0 < pw; k0 += RCW) {
     // stage L11 columns [k0, k0 + RCW) (one more when a 2x2 block straddles the chunk end)
-    const int kc = min(RCW + 1, pw - k0);
+    const int kc = min(RCW + 1, pw - k0), nl = kc * NBMAX;
     __syncthreads();
-    for (int e = tid; e < kc * NBMAX; e += NT) {
-      const int j = e % NBMAX, kk = e / NBMAX;
-      Lc[e] = j < pw ? L[j + (int64_t)(k0 + kk) * c.h] : F<T>::zero();
+    for (int e0 = tid; e0 < nl; e0 += NT * SB) {
+      T v[SB];
+#pragma unroll
+      for (int u = 0; u < SB; ++u) {
+        const int e = e0 + u * NT, j = e % NBMAX, kk = e / NBMAX;
+        v[u] = (e < nl && j < pw) ? L[j + (int64_t)(k0 + kk) * c.h] : F<T>::zero();
+      }
+#pragma unroll
+      for (int u = 0; u < SB; ++u)
+        if (e0 + u * NT < nl) Lc[e0 + u * NT] = v[u];
     }
     __syncthreads();
     const int kend = min(k0 + RCW, pw);
@@ -1197,10 +1029,149 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
     }
   }
   __syncthreads();
-  for (int e = tid; e < pw * RR; e += NT) {
+  for (int e = tid; e < n; e += NT) {
     const int j = e / RR, rr = e % RR;
     if (rr < nr) c.P[row0 + rr + (int64_t)(kb + j) * c.h] = X[e];
   }
+  __syncthreads();
+}
+
+template <class T>
+struct CdivShared {
+  union alignas(16) {
+    T Ab[NBMAX * (NBMAX + 1) / 2];
+    T stage[2 * STG];
+    T rp[(RR + replay_chunk<T>() + 1) * NBMAX];  // replay: X then Lc
+  } u;
+  T l1s[NBMAX], l2s[NBMAX], w1s[NBMAX], w2s[NBMAX];
+  int32_t deep_s[NBMAX], two_s[NBMAX], cperm_s[NBMAX], lperm_s[NBMAX];
+  int cnt[4];
+  int npert;
+  int any_swap;
+};
+
+// One workgroup factors a whole supernode (all its nb-panels).
+template <class T, int PR>
+__global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
+                                                   const int32_t* __restrict__ list, int nb_base,
+                                                   PivotPolicy pol) {
+  extern __shared__ unsigned char smem_raw[];
+  CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
+  const int s = list[blockIdx.x];
+  const int w = S.sn_ncol[s];
+  const int nb = (w >= 512 && nb_base < 128) ? 128 : nb_base;
+  SnCtx<T> c = make_ctx(S, Fc, s, nb);
+  if (threadIdx.x == 0) sh.npert = 0;
+  cdiv_init<NT>(c);
+  __syncthreads();
+  for (int kb = 0; kb < w; kb += nb) {
+    const int ke = min(kb + nb, w);
+    panel_load<NT>(c, kb, ke, sh.u.Ab);
+    int st = panel_getf2<NT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert);
+    if (st) {
+      flag_failure(Fc, s);
+      return;
+    }
+    panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
+    for (int64_t row0 = ke; row0 < c.h; row0 += RR)
+      replay_rows_block(c, kb, ke, row0, (int)min<int64_t>(RR, c.h - row0), sh.u.rp, sh.u.rp + RR * NBMAX,
+                        sh.cperm_s, sh.two_s);
+    const int64_t mt = c.h - ke;
+    const int cw = w - ke;
+    if (cw > 0 && mt > 0) {
+      const int nrt = (int)((mt + TM - 1) / TM), nct = (cw + TN - 1) / TN;
+      for (int cb = 0; cb < nct; ++cb)
+        for (int rb = cb; rb < nrt; ++rb) panel_schur_tile_call<PR>(c, kb, ke, rb * TM, cb * TN, sh.u.stage);
+      __syncthreads();
+    }
+  }
+  cdiv_emit(S, Fc, c, sh.cnt, sh.npert);
+}
+
+// ---- wide supernodes: batched over the level, one launch per phase of each
+// panel step (every supernode of the level that has a p-th panel) ----
+template <class T>
+__global__ __launch_bounds__(GT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
+                                                   PivotPolicy pol) {
+  extern __shared__ unsigned char smem_raw[];
+  CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
+  const WItem it = items[blockIdx.x];
+  const int s = it.s, kb = it.kb;
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  if (Fc.status[0]) return;  // the factorization has already failed
+  if (threadIdx.x == 0) sh.npert = 0;
+  if (kb == 0) cdiv_init<GT>(c);
+  __syncthreads();
+  const int ke = min(kb + c.nb, c.w);
+  panel_load<GT>(c, kb, ke, sh.u.Ab);
+  int st = panel_getf2<GT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert);
+  if (st) {
+    flag_failure(Fc, s);
+    return;
+  }
+  panel_store<GT, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
+  if (threadIdx.x == 0 && sh.npert) atomicAdd(&Fc.counters[3], (unsigned long long)sh.npert);
+}
+
+// The panel's row interchanges applied to the already factored columns
+// [0, kb) of its rows: blockIdx.y = the step's getf2 unit, blockIdx.x = a
+// block of 64 columns. Every thread gathers its permuted sources into
+// registers, the workgroup synchronizes, then they are stored.
+template <class T>
+__global__ __launch_bounds__(NT) void k_wide_swapcols(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
+  __shared__ int32_t cperm[NBMAX];
+  __shared__ int ident;
+  const WItem it = items[blockIdx.y];
+  const int s = it.s, kb = it.kb, j0 = 64 * blockIdx.x;
+  if (j0 >= kb || Fc.status[0]) return;
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  const int pw = min(kb + c.nb, c.w) - kb;
+  if (threadIdx.x == 0) ident = 1;
+  __syncthreads();
+  for (int q = threadIdx.x; q < pw; q += NT) {
+    cperm[q] = c.deep[NBMAX + q];
+    if (cperm[q] != q) ident = 0;
+  }
+  __syncthreads();
+  if (ident) return;
+  constexpr int PER = NBMAX * 64 / NT;
+  const int nc = min(64, kb - j0), n = pw * nc;
+  T v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = threadIdx.x + u * NT;
+    const int i = e % pw, j = e / pw;
+    v[u] = e < n ? c.P[(kb + cperm[i]) + (int64_t)(j0 + j) * c.h] : F<T>::zero();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = threadIdx.x + u * NT;
+    const int i = e % pw, j = e / pw;
+    if (e < n) c.P[(kb + i) + (int64_t)(j0 + j) * c.h] = v[u];
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
+  extern __shared__ unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
+  T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
+  __shared__ int32_t cperm[NBMAX];
+  __shared__ int32_t two_s[NBMAX];
+  const WItem it = items[blockIdx.x];
+  const int s = it.s, kb = it.kb;
+  SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
+  if (Fc.status[0]) return;
+  const int ke = min(kb + c.nb, c.w), pw = ke - kb;
+  const int64_t row0 = ke + (int64_t)it.a;
+  const int nr = (int)min<int64_t>(RR, c.h - row0);
+  for (int j = threadIdx.x; j < pw; j += NT) {
+    cperm[j] = c.deep[NBMAX + j];
+    two_s[j] = c.two[kb + j];
+  }
+  __syncthreads();
+  replay_rows_block(c, kb, ke, row0, nr, X, Lc, cperm, two_s);
 }
 
 template <class T, int PR>

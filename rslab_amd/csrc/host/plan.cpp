@@ -63,22 +63,14 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   P.levels.assign(ns ? maxlev + 1 : 0, LevelPlan());
   for (int32_t s = 0; s < ns; ++s) P.levels[P.level[s]].all.push_back(s);
   P.panel_off.assign(ns, 0);
-  P.snap_off.assign(ns, 0);
-  P.snap_ld.assign(ns, 0);
   int64_t poff = 0;
-  P.snap_total = 0;
   P.factor_flops = P.geom_flops = 0;
   P.nnzl_panel = 0;
   for (LevelPlan& L : P.levels) {
-    int64_t soff = 0;
     for (int32_t s : L.all) {
       const int64_t h = P.sn_nrow[s], w = P.sn_ncol[s];
       P.panel_off[s] = poff;
       poff += h * w;
-      int32_t ld = std::min<int32_t>(P.nb_eff[s], (int32_t)w);
-      P.snap_ld[s] = ld;
-      P.snap_off[s] = soff;
-      soff += (int64_t)ld * ld;
       if ((double)h * w * w > prm.wide_flops && w > 32) {
         L.wide.push_back(s);
       } else {
@@ -94,7 +86,6 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
       }
       P.geom_flops += (double)h * h * w;
     }
-    P.snap_total = std::max(P.snap_total, soff);
   }
   P.panel_total = poff;
 

@@ -44,8 +44,7 @@ struct Plan {
   PlanParams params;
   int32_t n = 0, ns = 0;
   std::vector<int32_t> sn_first, sn_ncol, sn_nrow, e_off, nb_eff;
-  std::vector<int64_t> panel_off, rs_off, snap_off;
-  std::vector<int32_t> snap_ld;
+  std::vector<int64_t> panel_off, rs_off;
   std::vector<int32_t> rs_idx;
   std::vector<int32_t> upd_ptr, upd_idx;
   std::vector<int32_t> level;   // level of each supernode (leaves 0)
@@ -63,7 +62,6 @@ struct Plan {
   std::vector<SItem> sitems;
   int64_t max_step_upd = 0;     // largest number of update units in one solve step
   int64_t panel_total = 0;      // elements in the panel pool
-  int64_t snap_total = 0;       // elements of multiplier scratch
   // user lower-CSC entries -> panel slot
   int64_t nnz = 0;
   std::vector<int64_t> amap;

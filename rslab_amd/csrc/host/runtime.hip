@@ -163,8 +163,6 @@ void DeviceSymbolic::upload() {
   upd_ptr.upload(P.upd_ptr, stream);
   upd_idx.upload(P.upd_idx, stream);
   sym_perm.upload(P.sym.perm, stream);
-  snap_off.upload(P.snap_off, stream);
-  snap_ld.upload(P.snap_ld, stream);
   amap.upload(P.amap, stream);
   a_row.upload(P.a_row, stream);
   a_col.upload(P.a_col, stream);
@@ -209,8 +207,6 @@ void DeviceSymbolic::upload() {
   view.upd_ptr = upd_ptr.as<int32_t>();
   view.upd_idx = upd_idx.as<int32_t>();
   view.sym_perm = sym_perm.as<int32_t>();
-  view.snap_off = snap_off.as<int64_t>();
-  view.snap_ld = snap_ld.as<int32_t>();
   view.u_off = u_off.as<int64_t>();
   view.gat_ptr = gat_ptr.as<int64_t>();
   view.gat_idx = gat_idx.as<int64_t>();
@@ -274,7 +270,7 @@ struct FieldOf<cplx> {
 template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
-  DBuf pool, d, ds, two, lperm, e_of_g, perm_e, snap, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
+  DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
       work_y, work_b, work_p, cbuf;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
   DevFac<T> fv{};
@@ -296,7 +292,6 @@ class DeviceFactor final : public DeviceFactorBase {
     lperm.alloc(sizeof(int32_t) * std::max<int64_t>(n, 1));
     e_of_g.alloc(sizeof(int32_t) * std::max<int64_t>(n, 1));
     perm_e.alloc(sizeof(int32_t) * std::max<int64_t>(n, 1));
-    snap.alloc(sizeof(T) * std::max<int64_t>(P.snap_total, 1));
     deep.alloc(sizeof(int32_t) * 2 * 128 * std::max<int64_t>(P.ns, 1));
     counters.alloc(sizeof(unsigned long long) * 4);
     sn_inert.alloc(sizeof(int32_t) * 3 * std::max<int64_t>(P.ns, 1));
@@ -323,7 +318,6 @@ class DeviceFactor final : public DeviceFactorBase {
     fv.lperm = lperm.as<int32_t>();
     fv.e_of_g = e_of_g.as<int32_t>();
     fv.perm_e = perm_e.as<int32_t>();
-    fv.snap = snap.as<T>();
     fv.deep = deep.as<int32_t>();
     fv.counters = counters.as<unsigned long long>();
     fv.sn_inertia = sn_inert.as<int32_t>();

@@ -66,7 +66,7 @@ struct DeviceSymbolic {
   Plan plan;
   hipStream_t stream = nullptr;
   DBuf sn_first, sn_ncol, sn_nrow, e_off, panel_off, rs_off, rs_idx, upd_ptr, upd_idx, sym_perm,
-      snap_off, snap_ld, amap, a_row, a_col, items, tasks, level_lists, u_off, gat_ptr, gat_idx,
+      amap, a_row, a_col, items, tasks, level_lists, u_off, gat_ptr, gat_idx,
       witems, nb_eff, gat_items, sitems,
       fwd_items, bwd_items, ptiles;
   std::vector<int64_t> level_all_off, level_small_off, level_wide_off, level_snarrow_off;  // into level_lists
