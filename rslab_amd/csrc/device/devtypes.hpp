@@ -78,6 +78,9 @@ struct WItem {
 // One panel step of a level's wide supernodes, run as three batched launches
 // (getf2 on every supernode that has a p-th panel, then the deep-row replay,
 // then the deferred Schur tiles). Ranges index the WItem array.
+// Deep rows per replay work unit of the wide path.
+constexpr int REPLAY_ROWS = 256;
+
 struct WideStep {
   int64_t g_beg = 0, g_end = 0, r_beg = 0, r_end = 0, t_beg = 0, t_end = 0;
   double getf2_flops = 0, replay_flops = 0, schur_flops = 0;

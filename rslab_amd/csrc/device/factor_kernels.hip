@@ -1036,6 +1036,142 @@ __device__ void replay_rows_block(const SnCtx<T>& c, int kb, int ke, int64_t row
   __syncthreads();
 }
 
+// Real field: one thread per deep row, its panel columns held in registers,
+// RH at a time, so the replay needs neither per-step barriers nor LDS traffic
+// for the rows. L11 is staged in LDS by RH x RH blocks as the unit-lower
+// factor of the replay (2x2 coupling entries zero: those sit in D), and step
+// k subtracts x_k L11(j, k) from every later column j in ascending k, the
+// reference's order (a 2x2 step's two subtractions included). For a panel
+// wider than RH, the second RH columns are first brought up to date with the
+// finished first RH (kept in a scratch of RH x REPLAY_ROWS values per unit,
+// column-major by row so the threads' accesses coalesce), then solved the
+// same way.
+constexpr int RH = 64;
+
+// Lb[k * RH + j] = L11(j0 + j, k0 + k) below the diagonal, zero elsewhere.
+__device__ void stage_l11_block(const SnCtx<double>& c, int kb, int pw, int j0, int k0, const int32_t* two_s,
+                                double* Lb) {
+  constexpr int SB = 8;
+  const double* L = c.P + kb + (int64_t)kb * c.h;
+  for (int e0 = threadIdx.x; e0 < RH * RH; e0 += NT * SB) {
+    double v[SB];
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      const int e = e0 + u * NT, gj = j0 + e % RH, gk = k0 + e / RH;
+      const bool in = e < RH * RH && gj < pw && gk < pw && gj > gk && !(gj == gk + 1 && two_s[gk]);
+      v[u] = in ? L[gj + (int64_t)gk * c.h] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SB; ++u)
+      if (e0 + u * NT < RH * RH) Lb[e0 + u * NT] = v[u];
+  }
+}
+
+// x[j] -= x[k] * L(j, k) for k < n - 1 < RHN, j in (k, RHN); columns past n
+// see zero multipliers.
+template <int RHN>
+__device__ __forceinline__ void reg_unit_solve(double (&x)[RH], const double* Lb, int n) {
+#pragma unroll
+  for (int k = 0; k < RHN - 1; ++k) {
+    // keep each step's multiplier loads in the step: hoisting them all ahead
+    // would hold more values than the row itself and spill it
+    __builtin_amdgcn_sched_barrier(0);
+    if (k + 1 < n) {
+      const double xk = x[k];
+#pragma unroll
+      for (int j = k + 1; j < RHN; ++j) x[j] = x[j] - xk * Lb[k * RH + j];
+    }
+  }
+}
+
+__device__ __forceinline__ void reg_unit_solve_n(double (&x)[RH], const double* Lb, int n) {
+  if (n <= 16) reg_unit_solve<16>(x, Lb, n);
+  else if (n <= 32) reg_unit_solve<32>(x, Lb, n);
+  else reg_unit_solve<RH>(x, Lb, n);
+}
+
+// Scale columns [base, base + n) by the 1x1 / 2x2 blocks of D (a 2x2 pair
+// starting at base + n - 1 is left to the caller) and store them unpermuted.
+__device__ __forceinline__ void reg_scale_store(const SnCtx<double>& c, int kb, int base, int n, double (&x)[RH],
+                                       const int32_t* two_s, bool live, int64_t row, bool skip_first) {
+#pragma unroll
+  for (int kk = 0; kk < RH; ++kk) {
+    const int g = base + kk;
+    const bool second = (kk == 0) ? skip_first : (kk < n && two_s[g - 1] != 0);
+    if (kk < n && !second) {
+      if (two_s[g]) {
+        if (kk + 1 < RH && kk + 1 < n) {
+          const double d11 = c.d[kb + g], d21 = c.ds[kb + g], d22 = c.d[kb + g + 1];
+          const double detinv = 1.0 / (d11 * d22 - d21 * d21);
+          const double wk = x[kk], wk1 = x[kk + 1];
+          x[kk] = (d22 * wk - d21 * wk1) * detinv;
+          x[kk + 1] = (d11 * wk1 - d21 * wk) * detinv;
+        }
+      } else {
+        x[kk] = x[kk] * (1.0 / c.d[kb + g]);
+      }
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int kk = 0; kk < RH; ++kk)
+      if (kk < n && !(kk == 0 && skip_first)) c.P[row + (int64_t)(kb + base + kk) * c.h] = x[kk];
+  }
+}
+
+// Rows [row0, row0 + nr), nr <= NT: La and Lm hold RH * RH doubles each (LDS);
+// ys (global, RH * NT doubles) is needed only when the panel is wider than RH.
+__device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, int ke, int64_t row0, int nr, double* La,
+                                double* Lm, const int32_t* cperm, const int32_t* two_s, double* ys) {
+  const int pw = ke - kb, n1 = min(pw, RH), n2 = pw - n1;
+  const int tid = threadIdx.x;
+  const bool live = tid < nr;
+  const int64_t row = row0 + tid;
+  stage_l11_block(c, kb, pw, 0, 0, two_s, La);
+  if (n2 > 0) stage_l11_block(c, kb, pw, RH, 0, two_s, Lm);  // Lm[k * RH + j] = L11(RH + j, k)
+  double x[RH];
+#pragma unroll
+  for (int j = 0; j < RH; ++j) x[j] = (live && j < n1) ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : 0.0;
+  __syncthreads();
+  reg_unit_solve_n(x, La, n1);
+  bool straddle = false;
+  double s_last = 0.0, s_first = 0.0;  // scaled columns RH-1, RH when a 2x2 pair straddles the passes
+  if (n2 > 0) {
+    // every source column of this row is read before any of its columns is written
+    for (int j = 0; j < n2; ++j) {
+      double y = live ? c.P[row + (int64_t)(kb + cperm[RH + j]) * c.h] : 0.0;
+#pragma unroll
+      for (int k = 0; k < RH; ++k) {
+        if ((k & 7) == 0) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight (see above)
+        y = y - x[k] * Lm[k * RH + j];
+      }
+      if (j == 0 && two_s[RH - 1]) {
+        // pair (RH-1, RH): both columns are final now
+        straddle = true;
+        const double d11 = c.d[kb + RH - 1], d21 = c.ds[kb + RH - 1], d22 = c.d[kb + RH];
+        const double detinv = 1.0 / (d11 * d22 - d21 * d21);
+        const double wk = x[RH - 1];
+        s_last = (d22 * wk - d21 * y) * detinv;
+        s_first = (d11 * y - d21 * wk) * detinv;
+      }
+      if (live) ys[(int64_t)j * NT + tid] = y;
+    }
+  }
+  reg_scale_store(c, kb, 0, straddle ? RH - 1 : n1, x, two_s, live, row, false);
+  if (straddle && live) c.P[row + (int64_t)(kb + RH - 1) * c.h] = s_last;
+  if (n2 > 0) {
+    __syncthreads();
+    stage_l11_block(c, kb, pw, RH, RH, two_s, La);
+#pragma unroll
+    for (int j = 0; j < RH; ++j) x[j] = (live && j < n2) ? ys[(int64_t)j * NT + tid] : 0.0;
+    __syncthreads();
+    reg_unit_solve_n(x, La, n2);
+    reg_scale_store(c, kb, RH, n2, x, two_s, live, row, straddle);
+    if (straddle && live) c.P[row + (int64_t)(kb + RH) * c.h] = s_first;
+  }
+  __syncthreads();
+}
+
 template <class T>
 struct CdivShared {
   union alignas(16) {
@@ -1073,9 +1209,19 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
       return;
     }
     panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
-    for (int64_t row0 = ke; row0 < c.h; row0 += RR)
-      replay_rows_block(c, kb, ke, row0, (int)min<int64_t>(RR, c.h - row0), sh.u.rp, sh.u.rp + RR * NBMAX,
-                        sh.cperm_s, sh.two_s);
+    bool done = false;
+    if constexpr (std::is_same<T, double>::value) {
+      if (ke - kb <= RH) {
+        for (int64_t row0 = ke; row0 < c.h; row0 += NT)
+          replay_rows_reg(c, kb, ke, row0, (int)min<int64_t>(NT, c.h - row0), sh.u.rp, nullptr, sh.cperm_s,
+                          sh.two_s, nullptr);
+        done = true;
+      }
+    }
+    if (!done)
+      for (int64_t row0 = ke; row0 < c.h; row0 += RR)
+        replay_rows_block(c, kb, ke, row0, (int)min<int64_t>(RR, c.h - row0), sh.u.rp, sh.u.rp + RR * NBMAX,
+                          sh.cperm_s, sh.two_s);
     const int64_t mt = c.h - ke;
     const int cw = w - ke;
     if (cw > 0 && mt > 0) {
@@ -1153,10 +1299,9 @@ __global__ __launch_bounds__(NT) void k_wide_swapcols(DevSym S, DevFac<T> Fc, co
 }
 
 template <class T>
-__global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
+__global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
+                                                   double* __restrict__ scratch) {
   extern __shared__ unsigned char smem_raw[];
-  T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
-  T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
   __shared__ int32_t cperm[NBMAX];
   __shared__ int32_t two_s[NBMAX];
   const WItem it = items[blockIdx.x];
@@ -1165,13 +1310,29 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
   if (Fc.status[0]) return;
   const int ke = min(kb + c.nb, c.w), pw = ke - kb;
   const int64_t row0 = ke + (int64_t)it.a;
-  const int nr = (int)min<int64_t>(RR, c.h - row0);
+  const int nr = (int)min<int64_t>(REPLAY_ROWS, c.h - row0);
   for (int j = threadIdx.x; j < pw; j += NT) {
     cperm[j] = c.deep[NBMAX + j];
     two_s[j] = c.two[kb + j];
   }
   __syncthreads();
-  replay_rows_block(c, kb, ke, row0, nr, X, Lc, cperm, two_s);
+  if constexpr (std::is_same<T, double>::value) {
+    static_assert(REPLAY_ROWS == NT, "one thread per deep row");
+    double* La = reinterpret_cast<double*>(smem_raw);
+    replay_rows_reg(c, kb, ke, row0, nr, La, La + RH * RH, cperm, two_s,
+                    scratch + (int64_t)blockIdx.x * RH * REPLAY_ROWS);
+  } else {
+    T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
+    T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
+    for (int r0 = 0; r0 < nr; r0 += RR)
+      replay_rows_block(c, kb, ke, row0 + r0, min(RR, nr - r0), X, Lc, cperm, two_s);
+  }
+}
+
+template <class T>
+constexpr size_t replay_smem() {
+  return std::is_same<T, double>::value ? sizeof(double) * 2 * RH * RH
+                                        : sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX;
 }
 
 template <class T, int PR>
@@ -1283,11 +1444,11 @@ void launch_cdiv_small(const DevSym& S, const DevFac<T>& Fc, const int32_t* list
 
 template <class T>
 void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems, const WideStep* steps,
-                       int nsteps, const int32_t* wide_list, int nwide, PivotPolicy pol, int prec, hipStream_t st,
-                       LaunchProfiler* pf) {
+                       int nsteps, const int32_t* wide_list, int nwide, PivotPolicy pol, int prec, double* rscratch,
+                       hipStream_t st, LaunchProfiler* pf) {
   if (nwide <= 0) return;
   const double fm = sizeof(T) == 16 ? 4.0 : 1.0;
-  const size_t replay_lds = sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX;
+  const size_t replay_lds = replay_smem<T>();
   for (int p = 0; p < nsteps; ++p) {
     const WideStep& w = steps[p];
     if (w.g_end > w.g_beg) {
@@ -1301,7 +1462,7 @@ void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems
     if (w.r_end > w.r_beg) {
       ProfScope ps(pf, LK_PANEL_REPLAY, fm * w.replay_flops, 0.0, st);
       hipLaunchKernelGGL(k_wide_replay<T>, dim3((unsigned)(w.r_end - w.r_beg)), dim3(NT), replay_lds, st, S,
-                         Fc, witems + w.r_beg);
+                         Fc, witems + w.r_beg, rscratch);
     }
     if (w.t_end > w.t_beg) {
       ProfScope ps(pf, LK_PANEL_SCHUR, fm * w.schur_flops, 0.0, st);
@@ -1322,7 +1483,7 @@ hipError_t configure_factor_kernels() {
                                                      (int)cdiv_smem<T>()));
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)k_wide_replay<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)(sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX));
+                          (int)replay_smem<T>());
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_wide_getf2<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdiv_smem<T>());
@@ -1338,8 +1499,8 @@ hipError_t configure_factor_kernels() {
   template void launch_cdiv_small<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int, \
                                      PivotPolicy, double, int, hipStream_t, LaunchProfiler*);  \
   template void launch_wide_level<T>(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, \
-                                     int, const int32_t*, int, PivotPolicy, int, hipStream_t,    \
-                                     LaunchProfiler*);                                           \
+                                     int, const int32_t*, int, PivotPolicy, int, double*,        \
+                                     hipStream_t, LaunchProfiler*);                              \
   template hipError_t configure_factor_kernels<T>();                                           \
   template size_t cdiv_smem<T>();
 

@@ -182,6 +182,7 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
 
   // batched wide path: per level, one step per panel index
   P.witems.clear();
+  P.max_replay_items = 0;
   for (LevelPlan& L : P.levels) {
     L.wsteps.clear();
     int maxp = 0;
@@ -204,11 +205,12 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
         if (kb >= P.sn_ncol[s]) continue;
         const int32_t ke = std::min(kb + P.nb_eff[s], P.sn_ncol[s]);
         const int64_t mt = (int64_t)P.sn_nrow[s] - ke;
-        for (int64_t r0 = 0; r0 < mt; r0 += 64) P.witems.push_back({s, kb, (int32_t)r0, 0});
+        for (int64_t r0 = 0; r0 < mt; r0 += REPLAY_ROWS) P.witems.push_back({s, kb, (int32_t)r0, 0});
         const double pw = ke - kb;
         ws.replay_flops += (double)mt * pw * (pw - 1);
       }
       ws.r_end = ws.t_beg = (int64_t)P.witems.size();
+      P.max_replay_items = std::max(P.max_replay_items, ws.r_end - ws.r_beg);
       for (int32_t s : L.wide) {
         const int32_t kb = p * P.nb_eff[s];
         if (kb >= P.sn_ncol[s]) continue;

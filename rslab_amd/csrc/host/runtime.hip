@@ -21,7 +21,7 @@ template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem
 void launch_cmod_two_phase(const DevSym&, const DevFac<double>&, const CItem*, int, const CTask*, const PTile*,
                            int64_t, void*, int64_t, double, int, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, int, hipStream_t, LaunchProfiler*);
-template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, int, hipStream_t, LaunchProfiler*);
+template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, int, double*, hipStream_t, LaunchProfiler*);
 template <class T> hipError_t configure_factor_kernels();
 template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
@@ -271,7 +271,7 @@ template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
-      work_y, work_b, work_p, cbuf;
+      work_y, work_b, work_p, cbuf, rscratch;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
   DevFac<T> fv{};
   std::vector<double> scale;
@@ -308,6 +308,8 @@ class DeviceFactor final : public DeviceFactorBase {
       c_cap = std::min<int64_t>(P.c_elems_max, (int64_t)(fr / 4 / esz));
       if (c_cap > 0) cbuf.alloc(esz * (size_t)c_cap);
     }
+    // register replay of the wide path: RH x REPLAY_ROWS doubles per unit of a step
+    if (FieldOf<T>::id == 0) rscratch.alloc(sizeof(double) * 64 * REPLAY_ROWS * std::max<int64_t>(P.max_replay_items, 1));
     status.alloc(sizeof(int32_t) * 2);
     scale_d.alloc(sizeof(double) * std::max<int64_t>(n, 1));
     vals_d.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
@@ -388,7 +390,7 @@ class DeviceFactor final : public DeviceFactorBase {
       launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
                            lv.small_flops, prec, st, pf);
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
-                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, st, pf);
+                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, rscratch.as<double>(), st, pf);
     }
     launch_rows_e<T>(S, fv, P.rs_off[P.ns], st);
     hip_check(hipGetLastError(), "factor launch");
