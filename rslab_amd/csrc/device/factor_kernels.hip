@@ -348,28 +348,61 @@ __device__ inline int64_t xcd_contiguous(int64_t b, int64_t n) {
 template <int PR>
 __global__ __launch_bounds__(NT) void k_cmod_prod(DevSym S, DevFac<double> Fc, const PTile* __restrict__ pt,
                                                   int64_t npt, void* __restrict__ cbuf) {
-  typedef typename TileCfg<PR>::S Sx;
-  __shared__ __align__(16) Sx As[tile_stage_elems<PR>()];
-  __shared__ __align__(16) Sx Bs[tile_stage_elems<PR>()];
+  typedef TileCfgW<PR> C;
+  typedef typename C::S Sx;
+  constexpr int KCW = C::KC, PER = KCW * 64 / NT;
+  __shared__ __align__(16) Sx As[C::ELEMS];
+  __shared__ __align__(16) Sx Bs[C::ELEMS];
   const PTile p = pt[xcd_contiguous(blockIdx.x, npt)];
   const int k = p.k, nck = S.sn_ncol[k], eo = S.e_off[k];
   const int64_t hk = S.sn_nrow[k];
   const double* Pk = Fc.pool + S.panel_off[k];
+  const double *d = Fc.d + eo, *ds = Fc.ds + eo;
+  const uint8_t* two = Fc.two + eo;
   const int ma = min(TM, p.m - p.mt * TM), mb = min(TN, p.n - p.nt * TN);
+  const int64_t ar = nck + p.p0 + p.mt * TM, br = nck + p.p0 + p.nt * TN;
+  const int tid = threadIdx.x;
+  double va[PER], vb[PER];
+  // chunk [c0, c0 + KCW): operand rows straight from the updater's panel
+  auto fetch = [&](int c0) {
+    const int kc = min(KCW, nck - c0);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + NT * u, r = e & 63, c = e >> 6;
+      va[u] = 0.0;
+      vb[u] = 0.0;
+      if (c < kc) {
+        const int64_t col = c0 + c;
+        if (r < ma) va[u] = Pk[(ar + r) + col * hk];
+        if (r < mb) vb[u] = g_entry_d(Pk, hk, br + r, col, c0 + c, d, ds, two);
+      }
+    }
+  };
   TileAcc<PR> acc;
   tile_zero(acc);
-  ldl_tile_mfma<PR>(Pk, hk, 0, nck, nck + p.p0 + p.mt * TM, ma, nck + p.p0 + p.nt * TN, mb, Fc.d + eo, Fc.ds + eo,
-                    Fc.two + eo, acc, As, Bs);
+  fetch(0);
+  for (int c0 = 0; c0 < nck; c0 += KCW) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + NT * u, r = e & 63, c = e >> 6;
+      stage_store_w<PR>(As, r, c, va[u]);
+      stage_store_w<PR>(Bs, r, c, vb[u]);
+    }
+    __syncthreads();
+    if (c0 + KCW < nck) fetch(c0 + KCW);
+    tile_mma_w<PR>(As, Bs, acc);
+    __syncthreads();
+  }
   const int64_t ldc = p.n, r0 = (int64_t)p.mt * TM, c0 = (int64_t)p.nt * TN;
   if constexpr (PR == TP_F64) {
-    double* C = static_cast<double*>(cbuf) + p.c_off;
+    double* Cp = static_cast<double*>(cbuf) + p.c_off;
     tile_for_each<PR>(acc, [&](int i, int j, double v) {
-      if (i < ma && j < mb) C[(r0 + i) * ldc + c0 + j] = v;
+      if (i < ma && j < mb) Cp[(r0 + i) * ldc + c0 + j] = v;
     });
   } else {
-    float* C = static_cast<float*>(cbuf) + p.c_off;
+    float* Cp = static_cast<float*>(cbuf) + p.c_off;
     tile_for_each<PR>(acc, [&](int i, int j, double v) {
-      if (i < ma && j < mb) C[(r0 + i) * ldc + c0 + j] = (float)v;
+      if (i < ma && j < mb) Cp[(r0 + i) * ldc + c0 + j] = (float)v;
     });
   }
 }
