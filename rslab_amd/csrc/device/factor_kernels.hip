@@ -28,7 +28,7 @@ constexpr int TM = 64;   // tile rows
 constexpr int TN = 64;   // tile columns
 constexpr int KC = 16;   // inner chunk
 constexpr int NBMAX = 128;
-constexpr int GT = 256;  // threads of the wide-path getf2
+constexpr int GT = 512;  // threads of the wide-path getf2
 constexpr int STG = 1152;  // staging elements per operand (fits every tile core's chunk)
 static_assert(STG >= KC * TM && STG >= tile_stage_elems<TP_F64>() && STG * 8 >= tile_stage_elems<TP_BF16>() * 2,
               "staging buffer too small");
@@ -586,6 +586,46 @@ __device__ inline void packed_trailing_update(T* Ab, int pw, int k0, const T* w1
   }
 }
 
+// The 1x1 step's update reading column k in place: entry (i, j) of the
+// trailing block takes w = A(i, k) and the multiplier A(j, k) * dinv (the
+// same product the staged form stores), skipped when that multiplier is zero.
+template <int NTG, class T>
+__device__ inline void packed_trailing_update_1x1(T* Ab, int pw, int k, T dinv) {
+  constexpr int B = 4;
+  const int k0 = k + 1, base = pk_off(k0, k0, pw), tot = pw * (pw + 1) / 2;
+  const T* colk = Ab + pk_off(k, k, pw) - k;  // colk[i] = A(i, k)
+  for (int e0 = base + (int)threadIdx.x; e0 < tot; e0 += NTG * B) {
+    int ii[B], jj[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int e = e0 + NTG * u;
+      ii[u] = -1;
+      jj[u] = 0;
+      if (e < tot) {
+        const int j = packed_col(e, pw);
+        jj[u] = j;
+        ii[u] = j + (e - (j * pw - (j * (j - 1)) / 2));
+      }
+    }
+    T x[B], a[B], m[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (ii[u] >= 0) {
+        x[u] = Ab[e0 + NTG * u];
+        a[u] = colk[ii[u]];
+        m[u] = colk[jj[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (ii[u] >= 0) {
+        const T mu = m[u] * dinv;
+        if (!F<T>::is_zero(mu)) Ab[e0 + NTG * u] = x[u] - a[u] * mu;
+      }
+    }
+  }
+}
+
 // Bunch-Kaufman on the diagonal block [kb,ke) held in LDS (packed lower).
 // Candidates, interchanges and updates bounded to the panel rows, as
 // ll_cdiv_emit's getf2 (multifrontal_ldlt.rs:2846-3016). Returns 1 when the
@@ -696,21 +736,13 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       }
       if (tid == 0) c.d[kb + k] = dk;
       const T dinv = F<T>::recip(dk);
-      // in-block rank-1 update (rows of this panel): column k (w1s) and its
-      // multipliers (l1s), then the trailing block and column k replaced by
-      // its multipliers
-      const int nr = pw - k - 1;
-      for (int t = tid; t < nr; t += NTG) {
-        const int j = k + 1 + t;
-        const T a = A_(j, k);
-        const T m = a * dinv;
-        w1s[j] = a;
-        l1s[j] = m;
-      }
+      // in-block rank-1 update (rows of this panel) straight from column k,
+      // which is replaced by its multipliers once every thread is past the
+      // update (the next step reads column k again only after its first
+      // barrier, and scaling a column commutes with its interchanges)
+      packed_trailing_update_1x1<NTG>(Ab, pw, k, dinv);
       __syncthreads();
-      packed_trailing_update<NTG, false>(Ab, pw, k + 1, w1s, w2s, l1s, l2s);
-      for (int i = k + 1 + tid; i < pw; i += NTG) A_(i, k) = l1s[i];
-      __syncthreads();
+      for (int i = k + 1 + tid; i < pw; i += NTG) A_(i, k) = A_(i, k) * dinv;
       k += 1;
     } else {
       T d11 = A_(k, k), d21 = A_(k + 1, k), d22 = A_(k + 1, k + 1);
@@ -757,6 +789,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       k += 2;
     }
   }
+  __syncthreads();  // the last step's multipliers are in place
 #undef A_
   return 0;
 }
