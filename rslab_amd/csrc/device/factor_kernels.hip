@@ -1104,138 +1104,152 @@ __device__ void replay_rows_block(const SnCtx<T>& c, int kb, int ke, int64_t row
 
 // Real field: one thread per deep row, its panel columns held in registers,
 // RH at a time, so the replay needs neither per-step barriers nor LDS traffic
-// for the rows. L11 is staged in LDS by RH x RH blocks as the unit-lower
-// factor of the replay (2x2 coupling entries zero: those sit in D), and step
-// k subtracts x_k L11(j, k) from every later column j in ascending k, the
-// reference's order (a 2x2 step's two subtractions included). For a panel
-// wider than RH, the second RH columns are first brought up to date with the
-// finished first RH (kept in a scratch of RH x REPLAY_ROWS values per unit,
-// column-major by row so the threads' accesses coalesce), then solved the
-// same way.
-constexpr int RH = 64;
+// for the rows. Pass p takes columns [RH p, RH p + RH): L11's column panel for
+// those columns is staged in LDS as the unit-lower factor of the replay (2x2
+// coupling entries zero: those sit in D); step k subtracts x_k L11(j, k) from
+// every later column j in ascending k, the reference's order (a 2x2 step's
+// two subtractions included): inside the pass in registers, and for the
+// columns of later passes as running values kept in a scratch of
+// pw x NT values (column-major by row, so the threads' accesses coalesce).
+// RH = 32 leaves the registers the compiler needs to keep the multiplier
+// loads in flight.
+constexpr int RH = 32;
 
-// Lb[k * RH + j] = L11(j0 + j, k0 + k) below the diagonal, zero elsewhere.
-__device__ void stage_l11_block(const SnCtx<double>& c, int kb, int pw, int j0, int k0, const int32_t* two_s,
-                                double* Lb) {
+// Lp[k * LPW + jj] = L11(RH p + jj, RH p + k) below the diagonal, zero
+// elsewhere; dq[kk] = the pass's D coefficients: {1/d} for a 1x1 column,
+// {d11, d21, d22, 1/det} for the first column of a 2x2 pair.
+constexpr int LPW = NBMAX;
+__device__ void stage_replay_pass(const SnCtx<double>& c, int kb, int pw, int p0, const int32_t* two_s, double* Lp,
+                                  double4* dq) {
   constexpr int SB = 8;
   const double* L = c.P + kb + (int64_t)kb * c.h;
-  for (int e0 = threadIdx.x; e0 < RH * RH; e0 += NT * SB) {
+  const int nj = pw - p0, n = RH * LPW;
+  for (int e0 = threadIdx.x; e0 < n; e0 += NT * SB) {
     double v[SB];
 #pragma unroll
     for (int u = 0; u < SB; ++u) {
-      const int e = e0 + u * NT, gj = j0 + e % RH, gk = k0 + e / RH;
-      const bool in = e < RH * RH && gj < pw && gk < pw && gj > gk && !(gj == gk + 1 && two_s[gk]);
+      const int e = e0 + u * NT, jj = e % LPW, k = e / LPW, gj = p0 + jj, gk = p0 + k;
+      const bool in = e < n && jj < nj && gk < pw && gj > gk && !(gj == gk + 1 && two_s[gk]);
       v[u] = in ? L[gj + (int64_t)gk * c.h] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < SB; ++u)
-      if (e0 + u * NT < RH * RH) Lb[e0 + u * NT] = v[u];
+      if (e0 + u * NT < n) Lp[e0 + u * NT] = v[u];
   }
-}
-
-// x[j] -= x[k] * L(j, k) for k < n - 1 < RHN, j in (k, RHN); columns past n
-// see zero multipliers.
-template <int RHN>
-__device__ __forceinline__ void reg_unit_solve(double (&x)[RH], const double* Lb, int n) {
-#pragma unroll
-  for (int k = 0; k < RHN - 1; ++k) {
-    // keep each step's multiplier loads in the step: hoisting them all ahead
-    // would hold more values than the row itself and spill it
-    __builtin_amdgcn_sched_barrier(0);
-    if (k + 1 < n) {
-      const double xk = x[k];
-#pragma unroll
-      for (int j = k + 1; j < RHN; ++j) x[j] = x[j] - xk * Lb[k * RH + j];
+  if (threadIdx.x < RH) {
+    const int g = p0 + threadIdx.x;
+    double4 q = make_double4(1.0, 0.0, 0.0, 1.0);
+    if (g < pw) {
+      if (two_s[g]) {
+        const double d11 = c.d[kb + g], d21 = c.ds[kb + g], d22 = c.d[kb + g + 1];
+        q = make_double4(d11, d21, d22, 1.0 / (d11 * d22 - d21 * d21));
+      } else if (!(g > 0 && two_s[g - 1])) {
+        q.x = 1.0 / c.d[kb + g];
+      }
     }
+    dq[threadIdx.x] = q;
   }
 }
 
-__device__ __forceinline__ void reg_unit_solve_n(double (&x)[RH], const double* Lb, int n) {
-  if (n <= 16) reg_unit_solve<16>(x, Lb, n);
-  else if (n <= 32) reg_unit_solve<32>(x, Lb, n);
-  else reg_unit_solve<RH>(x, Lb, n);
-}
-
-// Scale columns [base, base + n) by the 1x1 / 2x2 blocks of D (a 2x2 pair
-// starting at base + n - 1 is left to the caller) and store them unpermuted.
-__device__ __forceinline__ void reg_scale_store(const SnCtx<double>& c, int kb, int base, int n, double (&x)[RH],
-                                       const int32_t* two_s, bool live, int64_t row, bool skip_first) {
+// D scaling of the pass's columns and their store (unpermuted). A pair whose
+// second column opens the next pass is scaled by the caller.
+__device__ __forceinline__ void replay_pass_store(const SnCtx<double>& c, int kb, int p0, int n, double (&x)[RH],
+                                                  const int32_t* two_s, const double4* dq, bool live, int64_t row,
+                                                  bool first_done, bool last_deferred) {
 #pragma unroll
   for (int kk = 0; kk < RH; ++kk) {
-    const int g = base + kk;
-    const bool second = (kk == 0) ? skip_first : (kk < n && two_s[g - 1] != 0);
-    if (kk < n && !second) {
-      if (two_s[g]) {
-        if (kk + 1 < RH && kk + 1 < n) {
-          const double d11 = c.d[kb + g], d21 = c.ds[kb + g], d22 = c.d[kb + g + 1];
-          const double detinv = 1.0 / (d11 * d22 - d21 * d21);
-          const double wk = x[kk], wk1 = x[kk + 1];
-          x[kk] = (d22 * wk - d21 * wk1) * detinv;
-          x[kk + 1] = (d11 * wk1 - d21 * wk) * detinv;
-        }
+    const int g = p0 + kk;
+    const bool second = (kk == 0) ? first_done : (kk < n && two_s[g - 1] != 0);
+    if (kk < n && !second && !(kk == n - 1 && last_deferred)) {
+      const double4 q = dq[kk];
+      if (kk + 1 < RH && two_s[g]) {
+        const double wk = x[kk], wk1 = x[kk + 1];
+        x[kk] = (q.z * wk - q.y * wk1) * q.w;
+        x[kk + 1] = (q.x * wk1 - q.y * wk) * q.w;
       } else {
-        x[kk] = x[kk] * (1.0 / c.d[kb + g]);
+        x[kk] = x[kk] * q.x;
       }
     }
   }
   if (live) {
 #pragma unroll
     for (int kk = 0; kk < RH; ++kk)
-      if (kk < n && !(kk == 0 && skip_first)) c.P[row + (int64_t)(kb + base + kk) * c.h] = x[kk];
+      if (kk < n && !(kk == 0 && first_done) && !(kk == n - 1 && last_deferred))
+        c.P[row + (int64_t)(kb + p0 + kk) * c.h] = x[kk];
   }
 }
 
-// Rows [row0, row0 + nr), nr <= NT: La and Lm hold RH * RH doubles each (LDS);
-// ys (global, RH * NT doubles) is needed only when the panel is wider than RH.
-__device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, int ke, int64_t row0, int nr, double* La,
-                                double* Lm, const int32_t* cperm, const int32_t* two_s, double* ys) {
-  const int pw = ke - kb, n1 = min(pw, RH), n2 = pw - n1;
+// Rows [row0, row0 + nr), nr <= NT. Lp: RH * LPW doubles of LDS, dq: RH
+// entries of LDS; ys: (pw - RH) * NT doubles (scratch for the columns past the
+// first pass; ys[(j - RH) * NT + t] for column j of thread t's row).
+__device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, int ke, int64_t row0, int nr,
+                                                double* Lp, double4* dq, const int32_t* cperm, const int32_t* two_s,
+                                                double* ys) {
+  const int pw = ke - kb;
   const int tid = threadIdx.x;
   const bool live = tid < nr;
   const int64_t row = row0 + tid;
-  stage_l11_block(c, kb, pw, 0, 0, two_s, La);
-  if (n2 > 0) stage_l11_block(c, kb, pw, RH, 0, two_s, Lm);  // Lm[k * RH + j] = L11(RH + j, k)
-  double x[RH];
+  bool first_done = false;  // column p0 was scaled and stored with the previous pass's last one
+  for (int p0 = 0; p0 < pw; p0 += RH) {
+    const int n = min(RH, pw - p0);
+    stage_replay_pass(c, kb, pw, p0, two_s, Lp, dq);
+    double x[RH];
 #pragma unroll
-  for (int j = 0; j < RH; ++j) x[j] = (live && j < n1) ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : 0.0;
-  __syncthreads();
-  reg_unit_solve_n(x, La, n1);
-  bool straddle = false;
-  double s_last = 0.0, s_first = 0.0;  // scaled columns RH-1, RH when a 2x2 pair straddles the passes
-  if (n2 > 0) {
-    // every source column of this row is read before any of its columns is written
-    for (int j = 0; j < n2; ++j) {
-      double y = live ? c.P[row + (int64_t)(kb + cperm[RH + j]) * c.h] : 0.0;
-#pragma unroll
-      for (int k = 0; k < RH; ++k) {
-        if ((k & 7) == 0) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight (see above)
-        y = y - x[k] * Lm[k * RH + j];
-      }
-      if (j == 0 && two_s[RH - 1]) {
-        // pair (RH-1, RH): both columns are final now
-        straddle = true;
-        const double d11 = c.d[kb + RH - 1], d21 = c.ds[kb + RH - 1], d22 = c.d[kb + RH];
-        const double detinv = 1.0 / (d11 * d22 - d21 * d21);
-        const double wk = x[RH - 1];
-        s_last = (d22 * wk - d21 * y) * detinv;
-        s_first = (d11 * y - d21 * wk) * detinv;
-      }
-      if (live) ys[(int64_t)j * NT + tid] = y;
+    for (int j = 0; j < RH; ++j) {
+      x[j] = 0.0;
+      if (live && j < n) x[j] = p0 == 0 ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : ys[(int64_t)(p0 + j - RH) * NT + tid];
     }
-  }
-  reg_scale_store(c, kb, 0, straddle ? RH - 1 : n1, x, two_s, live, row, false);
-  if (straddle && live) c.P[row + (int64_t)(kb + RH - 1) * c.h] = s_last;
-  if (n2 > 0) {
     __syncthreads();
-    stage_l11_block(c, kb, pw, RH, RH, two_s, La);
 #pragma unroll
-    for (int j = 0; j < RH; ++j) x[j] = (live && j < n2) ? ys[(int64_t)j * NT + tid] : 0.0;
+    for (int k = 0; k < RH - 1; ++k) {
+      if (k + 1 < n) {
+        const double xk = x[k];
+#pragma unroll
+        for (int j = k + 1; j < RH; ++j) x[j] = x[j] - xk * Lp[k * LPW + j];
+      }
+    }
+    // later columns: every source column of this row is read (first pass)
+    // before any of its columns is written
+    const bool straddle = p0 + RH < pw && two_s[p0 + RH - 1];
+    double s_last = 0.0, s_first = 0.0;
+    constexpr int JB = 8;  // later columns per batch: their loads are issued together
+    for (int jb = p0 + RH; jb < pw; jb += JB) {
+      double y[JB];
+#pragma unroll
+      for (int u = 0; u < JB; ++u) {
+        const int j = jb + u;
+        y[u] = 0.0;
+        if (live && j < pw) y[u] = p0 == 0 ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : ys[(int64_t)(j - RH) * NT + tid];
+      }
+#pragma unroll
+      for (int u = 0; u < JB; ++u) {
+        const int j = jb + u;
+        if (j < pw) {
+#pragma unroll
+          for (int k = 0; k < RH; ++k) y[u] = y[u] - x[k] * Lp[k * LPW + (j - p0)];
+        }
+      }
+      if (jb == p0 + RH && straddle) {
+        // pair (p0 + RH - 1, p0 + RH): both columns are final now
+        const double4 q = dq[RH - 1];
+        const double wk = x[RH - 1];
+        s_last = (q.z * wk - q.y * y[0]) * q.w;
+        s_first = (q.x * y[0] - q.y * wk) * q.w;
+      }
+      if (live) {
+#pragma unroll
+        for (int u = 0; u < JB; ++u)
+          if (jb + u < pw) ys[(int64_t)(jb + u - RH) * NT + tid] = y[u];
+      }
+    }
+    replay_pass_store(c, kb, p0, n, x, two_s, dq, live, row, first_done, straddle);
+    if (straddle && live) {
+      c.P[row + (int64_t)(kb + p0 + RH - 1) * c.h] = s_last;
+      c.P[row + (int64_t)(kb + p0 + RH) * c.h] = s_first;
+    }
+    first_done = straddle;
     __syncthreads();
-    reg_unit_solve_n(x, La, n2);
-    reg_scale_store(c, kb, RH, n2, x, two_s, live, row, straddle);
-    if (straddle && live) c.P[row + (int64_t)(kb + RH) * c.h] = s_first;
   }
-  __syncthreads();
 }
 
 template <class T>
@@ -1277,10 +1291,14 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
     panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
     bool done = false;
     if constexpr (std::is_same<T, double>::value) {
-      if (ke - kb <= RH) {
+      // LDS: the pass's L11 panel and D coefficients, then the running values
+      // of the columns past the first pass (panels up to 2 RH wide)
+      static_assert(sizeof(double) * (RH * LPW + 4 * RH + RH * NT) <= sizeof(sh.u.rp), "replay scratch");
+      if (ke - kb <= 2 * RH) {
+        double* Lp = reinterpret_cast<double*>(sh.u.rp);
         for (int64_t row0 = ke; row0 < c.h; row0 += NT)
-          replay_rows_reg(c, kb, ke, row0, (int)min<int64_t>(NT, c.h - row0), sh.u.rp, nullptr, sh.cperm_s,
-                          sh.two_s, nullptr);
+          replay_rows_reg(c, kb, ke, row0, (int)min<int64_t>(NT, c.h - row0), Lp,
+                          reinterpret_cast<double4*>(Lp + RH * LPW), sh.cperm_s, sh.two_s, Lp + RH * LPW + 4 * RH);
         done = true;
       }
     }
@@ -1384,9 +1402,9 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
   __syncthreads();
   if constexpr (std::is_same<T, double>::value) {
     static_assert(REPLAY_ROWS == NT, "one thread per deep row");
-    double* La = reinterpret_cast<double*>(smem_raw);
-    replay_rows_reg(c, kb, ke, row0, nr, La, La + RH * RH, cperm, two_s,
-                    scratch + (int64_t)blockIdx.x * RH * REPLAY_ROWS);
+    double* Lp = reinterpret_cast<double*>(smem_raw);
+    replay_rows_reg(c, kb, ke, row0, nr, Lp, reinterpret_cast<double4*>(Lp + RH * LPW), cperm, two_s,
+                    scratch + (int64_t)blockIdx.x * (NBMAX - RH) * REPLAY_ROWS);
   } else {
     T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
     T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
@@ -1397,7 +1415,7 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
 
 template <class T>
 constexpr size_t replay_smem() {
-  return std::is_same<T, double>::value ? sizeof(double) * 2 * RH * RH
+  return std::is_same<T, double>::value ? sizeof(double) * (RH * LPW + 4 * RH)
                                         : sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX;
 }
 

@@ -308,8 +308,9 @@ class DeviceFactor final : public DeviceFactorBase {
       c_cap = std::min<int64_t>(P.c_elems_max, (int64_t)(fr / 4 / esz));
       if (c_cap > 0) cbuf.alloc(esz * (size_t)c_cap);
     }
-    // register replay of the wide path: RH x REPLAY_ROWS doubles per unit of a step
-    if (FieldOf<T>::id == 0) rscratch.alloc(sizeof(double) * 64 * REPLAY_ROWS * std::max<int64_t>(P.max_replay_items, 1));
+    // register replay of the wide path: running values of the columns past
+    // its first 32, (128 - 32) x REPLAY_ROWS doubles per unit of a step
+    if (FieldOf<T>::id == 0) rscratch.alloc(sizeof(double) * 96 * REPLAY_ROWS * std::max<int64_t>(P.max_replay_items, 1));
     status.alloc(sizeof(int32_t) * 2);
     scale_d.alloc(sizeof(double) * std::max<int64_t>(n, 1));
     vals_d.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
