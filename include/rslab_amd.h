@@ -116,6 +116,13 @@ int rsl_factor(rsl_symbolic *sym, int32_t field, const double *values,
  * (LdltSymbolic reuse, src/numeric/sparse_solver.rs:457-488). */
 int rsl_refactor(rsl_solver *f, const double *values);
 
+/* Device-resident variant of rsl_refactor (real field): d_values in device
+ * memory, same layout. The equilibration and the ForceAccept floor are
+ * computed on the device and give the host path's bits, so the factor equals
+ * rsl_refactor's on the same values. Ordered after the work enqueued on
+ * `stream` (a hipStream_t; NULL is the null stream); returns when done. */
+int rsl_refactor_device(rsl_solver *f, const void *d_values, void *stream);
+
 /* LdltSolver::solve_many (src/numeric/sparse_solver.rs:302-327) with
  * solve_ldlt_many (src/dense/ldlt_generic.rs:564-614): b, x host buffers,
  * row-major n x nrhs. */
@@ -147,6 +154,13 @@ int rsl_factor_download(rsl_solver *f, int64_t *l_col_ptr, int32_t *l_row_idx,
  * info[4*c..4*c+4) = {ir_iters, gmres_iters, backward_error, certified}. */
 int rsl_refine(rsl_solver *f, const double *a_values, const double *b, int64_t nrhs,
                double target, double *x, double *info);
+
+/* Device-resident variant of rsl_refine (real field): d_a_values, d_b, d_x
+ * in device memory, info on the host. Takes the same steps as rsl_refine on
+ * the same inputs. Ordered after the work enqueued on `stream`; returns when
+ * done. */
+int rsl_refine_device(rsl_solver *f, const void *d_a_values, const void *d_b, int64_t nrhs,
+                      double target, void *d_x, double *info, void *stream);
 
 /* Per-kernel timing (no counterpart in rslab; a measurement aid). While
  * enabled, every launch of this factor's numeric factor, solve and refinement

@@ -59,6 +59,8 @@ SIGNATURES = {
     "rsl_factor_nnz": (C.c_int, [_P, _I64P]),
     "rsl_factor_download": (C.c_int, [_P, _I64P, _I32P, _DP, _DP, _DP, _U8P, _I32P]),
     "rsl_refine": (C.c_int, [_P, _DP, _DP, C.c_int64, C.c_double, _DP, _DP]),
+    "rsl_refactor_device": (C.c_int, [_P, _P, _P]),
+    "rsl_refine_device": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, _P, _DP, _P]),
     "rsl_free_factor": (None, [_P]),
     "rsl_free_symbolic": (None, [_P]),
     "rsl_last_error": (C.c_char_p, []),

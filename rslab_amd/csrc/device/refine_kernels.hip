@@ -150,6 +150,11 @@ __global__ void k_scal(int32_t n, double a, const T* __restrict__ x, T* __restri
   if (i < n) y[i] = F<T>::scale(x[i], a);
 }
 
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 // ------------------------------------------------------------------ launchers
 namespace {
 inline unsigned blocks_for(int64_t m) { return (unsigned)((m + RT - 1) / RT); }
@@ -227,6 +232,80 @@ void launch_scal(int32_t n, double a, const T* x, T* y, hipStream_t st) {
 }
 
 int refine_reduction_blocks() { return RED_BLOCKS; }
+
+// ---- device-resident inputs (real field): the host computations of the
+// same quantities, in an order that gives the same bits ----
+
+// one-pass equilibration (onepass_scale, src/numeric/sparse_solver.rs:399-418):
+// row maxima of |a| over the symmetric matrix; maxima are order-free, so the
+// atomics give the host's value.
+__global__ void k_scale_max(int64_t nnz, const int32_t* __restrict__ a_row, const int32_t* __restrict__ a_col,
+                            const double* __restrict__ v, unsigned long long* __restrict__ smax) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  const double m = fabs(v[q]);
+  if (!(m > 0.0)) return;  // zero cannot raise a maximum; NaN never compares greater
+  const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+  const int32_t i = a_row[q], j = a_col[q];
+  atomicMax(smax + i, b);
+  if (i != j) atomicMax(smax + j, b);
+}
+
+// inv_sqrt_scale_guarded (src/scaling/mod.rs:85-96)
+__global__ void k_scale_final(int32_t n, const unsigned long long* __restrict__ smax, double* __restrict__ scale) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = __longlong_as_double((long long)smax[i]);
+  double out = 1.0;
+  if (m > 0.0) {
+    const double cand = 1.0 / sqrt(m);
+    if (isfinite(cand) && cand > 0.0) out = cand;
+  }
+  scale[i] = out;
+}
+
+// max |a_ij| s_i s_j over the entries (the ForceAccept pivot floor's |A_hat|max)
+__global__ void k_scaled_amax(int64_t nnz, const int32_t* __restrict__ a_row, const int32_t* __restrict__ a_col,
+                              const double* __restrict__ v, const double* __restrict__ scale,
+                              unsigned long long* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  const double m = fabs(v[q]) * scale[a_row[q]] * scale[a_col[q]];
+  if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// ||A||_inf (sym_norm_inf, src/numeric/mixed.rs:358-371): each row's sum taken
+// in the full symmetric CSR's order, which is the host loop's order of
+// accumulation for that row; the maximum over rows is order-free.
+__global__ void k_row_abs_max(int32_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ src,
+                              const double* __restrict__ v, unsigned long long* __restrict__ out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int64_t q = rp[i]; q < rp[i + 1]; ++q) acc += fabs(v[src[q]]);
+  if (acc > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(acc));
+}
+
+void launch_device_scale(int32_t n, int64_t nnz, const int32_t* a_row, const int32_t* a_col, const double* v,
+                         unsigned long long* smax, double* scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(std::max<int64_t>(n, 1))), dim3(RT), 0, st, smax, (int64_t)n);
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_scale_max, dim3(blocks_for(nnz)), dim3(RT), 0, st, nnz, a_row, a_col, v, smax);
+  if (n > 0) hipLaunchKernelGGL(k_scale_final, dim3(blocks_for(n)), dim3(RT), 0, st, n, smax, scale);
+}
+
+void launch_scaled_amax(int64_t nnz, const int32_t* a_row, const int32_t* a_col, const double* v,
+                        const double* scale, unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(RT), 0, st, out, (int64_t)1);
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_scaled_amax, dim3(blocks_for(nnz)), dim3(RT), 0, st, nnz, a_row, a_col, v, scale, out);
+}
+
+void launch_row_abs_max(int32_t n, const int64_t* rp, const int64_t* src, const double* v, unsigned long long* out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(RT), 0, st, out, (int64_t)1);
+  if (n > 0) hipLaunchKernelGGL(k_row_abs_max, dim3(blocks_for(n)), dim3(RT), 0, st, n, rp, src, v, out);
+}
 
 #define RSL_INST(T)                                                                                     \
   template void launch_expand<T>(int64_t, const int64_t*, const T*, T*, hipStream_t);                   \

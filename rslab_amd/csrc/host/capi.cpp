@@ -275,6 +275,23 @@ int rsl_refactor(rsl_solver* f, const double* values) {
   });
 }
 
+int rsl_refactor_device(rsl_solver* f, const void* d_values, void* stream) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    f->have_csc = false;
+    f->f->refactor_device(d_values, static_cast<hipStream_t>(stream));
+  });
+}
+
+int rsl_refine_device(rsl_solver* f, const void* d_a_values, const void* d_b, int64_t nrhs, double target,
+                      void* d_x, double* info, void* stream) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->refine_device(d_a_values, d_b, nrhs, target, d_x, info, static_cast<hipStream_t>(stream));
+  });
+}
+
 int rsl_solve(rsl_solver* f, const double* b, int64_t nrhs, double* x) {
   return guard([&] {
     std::lock_guard<std::mutex> lk(f->mu);

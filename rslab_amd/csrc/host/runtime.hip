@@ -18,6 +18,11 @@ template <class T> void launch_fill(T*, int64_t, T, hipStream_t);
 template <class T> void launch_rows_e(const DevSym&, const DevFac<T>&, int64_t, hipStream_t);
 template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, int, hipStream_t, LaunchProfiler*);
+void launch_device_scale(int32_t, int64_t, const int32_t*, const int32_t*, const double*, unsigned long long*,
+                         double*, hipStream_t);
+void launch_scaled_amax(int64_t, const int32_t*, const int32_t*, const double*, const double*, unsigned long long*,
+                        hipStream_t);
+void launch_row_abs_max(int32_t, const int64_t*, const int64_t*, const double*, unsigned long long*, hipStream_t);
 void launch_cmod_two_phase(const DevSym&, const DevFac<double>&, const CItem*, int, const CTask*, const PTile*,
                            int64_t, void*, int64_t, double, int, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, int, hipStream_t, LaunchProfiler*);
@@ -271,11 +276,17 @@ template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
-      work_y, work_b, work_p, cbuf, rscratch;
+      work_y, work_b, work_p, cbuf, rscratch, dev_u64;
+  hipEvent_t ev_caller = nullptr;
+  bool ev_ready = false;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
   DevFac<T> fv{};
   std::vector<double> scale;
   bool configured = false;
+
+  ~DeviceFactor() override {
+    if (ev_ready) (void)hipEventDestroy(ev_caller);
+  }
 
   DeviceFactor(std::shared_ptr<DeviceSymbolic> s, const FactorSettings& fs) {
     sym = std::move(s);
@@ -332,14 +343,8 @@ class DeviceFactor final : public DeviceFactorBase {
 
   void refactor(const double* values) override {
     const Plan& P = sym->plan;
-    const DevSym& S = sym->view;
     hipStream_t st = sym->stream;
-    if (!configured) {
-      hip_check(configure_factor_kernels<T>(), "configure kernels");
-      configured = true;
-    }
     auto t0 = std::chrono::steady_clock::now();
-    valid = false;
     if (settings.scaling == 1)
       onepass_scale(P.n, FieldOf<T>::id, sym->h_colp.data(), sym->h_rowi.data(), values, scale);
     else
@@ -365,12 +370,76 @@ class DeviceFactor final : public DeviceFactorBase {
               "upload scale");
     hip_check(hipMemcpyAsync(vals_d.p, values, sizeof(T) * P.nnz, hipMemcpyHostToDevice, st),
               "upload values");
+    run_factor(vals_d.as<T>(), pol, t0);
+  }
+
+  // values already in device memory (real field): the equilibration and the
+  // pivot floor are computed on the device, to the host path's bits
+  void refactor_device(const void* d_values, hipStream_t caller) override {
+    if constexpr (FieldOf<T>::id != 0) {
+      (void)d_values;
+      (void)caller;
+      throw Error(INVALID_INPUT, "device-resident refactor supports the real field only");
+    } else {
+      const Plan& P = sym->plan;
+      hipStream_t st = sym->stream;
+      auto t0 = std::chrono::steady_clock::now();
+      wait_for(caller);
+      const double* v = static_cast<const double*>(d_values);
+      if (dev_u64.bytes < sizeof(unsigned long long) * (size_t)(P.n + 1))
+        dev_u64.alloc(sizeof(unsigned long long) * (size_t)(P.n + 1));
+      auto* u64 = dev_u64.as<unsigned long long>();
+      if (settings.scaling == 1)
+        launch_device_scale(P.n, P.nnz, sym->a_row.as<int32_t>(), sym->a_col.as<int32_t>(), v, u64,
+                            scale_d.as<double>(), st);
+      else
+        launch_fill<double>(scale_d.as<double>(), P.n, 1.0, st);
+      PivotPolicy pol{0, 0.0};
+      if (settings.pivot == 1) {
+        pol.have_floor = 1;
+        pol.floor = std::max(settings.abs_floor, 0.0);
+      } else if (settings.pivot == 2) {
+        unsigned long long bits = 0;
+        launch_scaled_amax(P.nnz, sym->a_row.as<int32_t>(), sym->a_col.as<int32_t>(), v, scale_d.as<double>(),
+                           u64 + P.n, st);
+        hip_check(hipMemcpyAsync(&bits, u64 + P.n, sizeof(bits), hipMemcpyDeviceToHost, st), "amax");
+        hip_check(hipStreamSynchronize(st), "amax");
+        double amax;
+        std::memcpy(&amax, &bits, sizeof(amax));
+        pol.have_floor = 1;
+        pol.floor = std::max(amax, 1.0) * 2.220446049250313e-16;
+      }
+      run_factor(v, pol, t0);
+    }
+  }
+
+  // the library stream waits for work the caller enqueued before the call
+  void wait_for(hipStream_t caller) {
+    if (!ev_ready) {
+      hip_check(hipEventCreateWithFlags(&ev_caller, hipEventDisableTiming), "hipEventCreate");
+      ev_ready = true;
+    }
+    hip_check(hipEventRecord(ev_caller, caller), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(sym->stream, ev_caller, 0), "hipStreamWaitEvent");
+  }
+
+  // numeric factor from device values (user lower CSC) with scale_d in place
+  void run_factor(const T* vals, const PivotPolicy& pol,
+                  std::chrono::steady_clock::time_point t0) {
+    const Plan& P = sym->plan;
+    const DevSym& S = sym->view;
+    hipStream_t st = sym->stream;
+    if (!configured) {
+      hip_check(configure_factor_kernels<T>(), "configure kernels");
+      configured = true;
+    }
+    valid = false;
     hip_check(hipMemsetAsync(counters.p, 0, counters.bytes, st), "memset");
     const int32_t st_init[2] = {0, INT32_MAX};
     hip_check(hipMemcpyAsync(status.p, st_init, sizeof(st_init), hipMemcpyHostToDevice, st), "status");
     LaunchProfiler* pf = prof_on ? &prof : nullptr;
     launch_fill<T>(fv.pool, P.panel_total, F<T>::zero(), st);
-    launch_assemble<T>(S, fv, vals_d.as<T>(), scale_d.as<double>(), st, pf);
+    launch_assemble<T>(S, fv, vals, scale_d.as<double>(), st, pf);
     const int nb = P.params.panel_nb < 8 ? 8 : P.params.panel_nb;
     const int prec = static_cast<int>(settings.precision);
     const int32_t* lists = sym->level_lists.as<int32_t>();
@@ -525,6 +594,8 @@ class DeviceFactor final : public DeviceFactorBase {
   }
 
   void refine(const double*, const double*, int64_t, double, double*, double*) override;
+  void refine_device(const void*, const void*, int64_t, double, void*, double*, hipStream_t) override;
+  void refine_core(const T* av, const T* b, int32_t k, double target, double anorm, T* xout, double* info);
 
   void supernode_inertia(std::vector<int32_t>& out) override {
     const Plan& P = sym->plan;
@@ -591,11 +662,55 @@ void DeviceFactor<T>::refine(const double* a_values, const double* b, int64_t nr
   double anorm = 0.0;
   for (double r : rows) anorm = std::max(anorm, r);
 
-  const size_t blk = sizeof(T) * std::max<int64_t>((int64_t)n * k, 1);
-  DBuf av, afull, db, dx, dbest, dr, ddx, norms, mask_d, mask2_d;
+  DBuf av, db, dxo;
   av.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
+  db.alloc(sizeof(T) * std::max<int64_t>((int64_t)n * k, 1));
+  dxo.alloc(sizeof(T) * std::max<int64_t>((int64_t)n * k, 1));
+  hip_check(hipMemcpyAsync(av.p, a_values, sizeof(T) * P.nnz, hipMemcpyHostToDevice, st), "upload A");
+  hip_check(hipMemcpyAsync(db.p, b, sizeof(T) * n * k, hipMemcpyHostToDevice, st), "upload b");
+  refine_core(av.as<T>(), db.as<T>(), k, target, anorm, dxo.as<T>(), info);
+  hip_check(hipMemcpyAsync(x, dxo.p, sizeof(T) * n * k, hipMemcpyDeviceToHost, st), "download x");
+  hip_check(hipStreamSynchronize(st), "refine");
+}
+
+// a_values, b and x in device memory (real field): ||A||_inf on the device in
+// the host loop's order of accumulation, so the ladder takes the same steps
+template <class T>
+void DeviceFactor<T>::refine_device(const void* d_a, const void* d_b, int64_t nrhs, double target, void* d_x,
+                                    double* info, hipStream_t caller) {
+  if constexpr (FieldOf<T>::id != 0) {
+    (void)d_a, (void)d_b, (void)nrhs, (void)target, (void)d_x, (void)info, (void)caller;
+    throw Error(INVALID_INPUT, "device-resident refinement supports the real field only");
+  } else {
+    const Plan& P = sym->plan;
+    hipStream_t st = sym->stream;
+    sym->ensure_full_csr();
+    wait_for(caller);
+    if (dev_u64.bytes < sizeof(unsigned long long) * (size_t)(P.n + 1))
+      dev_u64.alloc(sizeof(unsigned long long) * (size_t)(P.n + 1));
+    unsigned long long bits = 0;
+    launch_row_abs_max(P.n, sym->fa_rp.as<int64_t>(), sym->fa_src.as<int64_t>(), static_cast<const double*>(d_a),
+                       dev_u64.as<unsigned long long>(), st);
+    hip_check(hipMemcpyAsync(&bits, dev_u64.p, sizeof(bits), hipMemcpyDeviceToHost, st), "anorm");
+    hip_check(hipStreamSynchronize(st), "anorm");
+    double anorm;
+    std::memcpy(&anorm, &bits, sizeof(anorm));
+    refine_core(static_cast<const T*>(d_a), static_cast<const T*>(d_b), (int32_t)nrhs, target, anorm,
+                static_cast<T*>(d_x), info);
+    hip_check(hipStreamSynchronize(st), "refine");
+  }
+}
+
+template <class T>
+void DeviceFactor<T>::refine_core(const T* av, const T* dbp, int32_t k, double target, double anorm, T* xout,
+                                  double* info) {
+  const Plan& P = sym->plan;
+  hipStream_t st = sym->stream;
+  const int32_t n = P.n;
+  sym->ensure_full_csr();
+  const size_t blk = sizeof(T) * std::max<int64_t>((int64_t)n * k, 1);
+  DBuf afull, dx, dbest, dr, ddx, norms, mask_d, mask2_d;
   afull.alloc(sizeof(T) * std::max<int64_t>(sym->nnz_full, 1));
-  db.alloc(blk);
   dx.alloc(blk);
   dbest.alloc(blk);
   dr.alloc(blk);
@@ -603,18 +718,16 @@ void DeviceFactor<T>::refine(const double* a_values, const double* b, int64_t nr
   norms.alloc(sizeof(unsigned long long) * 3 * std::max(k, 1));
   mask_d.alloc(std::max(k, 1));
   mask2_d.alloc(std::max(k, 1));
-  hip_check(hipMemcpyAsync(av.p, a_values, sizeof(T) * P.nnz, hipMemcpyHostToDevice, st), "upload A");
-  hip_check(hipMemcpyAsync(db.p, b, sizeof(T) * n * k, hipMemcpyHostToDevice, st), "upload b");
-  launch_expand<T>(sym->nnz_full, sym->fa_src.as<int64_t>(), av.as<T>(), afull.as<T>(), st);
+  launch_expand<T>(sym->nnz_full, sym->fa_src.as<int64_t>(), av, afull.as<T>(), st);
   const int64_t* rp = sym->fa_rp.as<int64_t>();
   const int32_t* ci = sym->fa_ci.as<int32_t>();
   const T* A = afull.as<T>();
   auto* nrm = norms.as<unsigned long long>();
 
   // x0 = low-precision solve of b; |b|_inf per column
-  solve_device(db.p, k, dx.p, st);
+  solve_device(dbp, k, dx.p, st);
   hip_check(hipMemsetAsync(norms.p, 0, norms.bytes, st), "memset");
-  launch_colmax<T>(n, k, db.as<T>(), nrm + 2 * k, st);
+  launch_colmax<T>(n, k, dbp, nrm + 2 * k, st);
 
   std::vector<int> ir(k, 0), phase(k, 0);  // 0 iterating, 1 certified, 2 escalate
   std::vector<double> best_be(k, INFINITY), prev_rn(k, INFINITY), bn(k, 0.0);
@@ -622,7 +735,7 @@ void DeviceFactor<T>::refine(const double* a_values, const double* b, int64_t nr
   std::vector<uint8_t> keep(k), step(k);
   bool first = true;
   for (;;) {
-    launch_residual<T>(n, k, rp, ci, A, dx.as<T>(), db.as<T>(), dr.as<T>(), st, prof_on ? &prof : nullptr);
+    launch_residual<T>(n, k, rp, ci, A, dx.as<T>(), dbp, dr.as<T>(), st, prof_on ? &prof : nullptr);
     hip_check(hipMemsetAsync(norms.p, 0, sizeof(unsigned long long) * 2 * k, st), "memset");
     launch_colmax<T>(n, k, dr.as<T>(), nrm, st);
     launch_colmax<T>(n, k, dx.as<T>(), nrm + k, st);
@@ -669,11 +782,10 @@ void DeviceFactor<T>::refine(const double* a_values, const double* b, int64_t nr
   std::vector<int> gm_iters(k, 0);
   for (int c = 0; c < k; ++c) {
     if (phase[c] != 2) continue;
-    gmres_column(n, k, c, rp, ci, A, db.as<T>(), dbest.as<T>(), target, anorm, bn[c], best_be[c],
+    gmres_column(n, k, c, rp, ci, A, dbp, dbest.as<T>(), target, anorm, bn[c], best_be[c],
                  gm_iters[c], st);
   }
-  hip_check(hipMemcpyAsync(x, dbest.p, sizeof(T) * n * k, hipMemcpyDeviceToHost, st), "download x");
-  hip_check(hipStreamSynchronize(st), "refine");
+  hip_check(hipMemcpyAsync(xout, dbest.p, sizeof(T) * n * k, hipMemcpyDeviceToDevice, st), "copy x");
   for (int c = 0; c < k; ++c) {
     info[4 * c + 0] = ir[c];
     info[4 * c + 1] = gm_iters[c];

@@ -134,6 +134,11 @@ class DeviceFactorBase {
   // {ir_iters, gmres_iters, backward_error, certified}.
   virtual void refine(const double* a_values, const double* b, int64_t nrhs, double target,
                       double* x, double* info) = 0;
+  // device-resident variants (real field): inputs in device memory, ordered
+  // after the work already enqueued on `caller`; the calls return when done
+  virtual void refactor_device(const void* d_values, hipStream_t caller) = 0;
+  virtual void refine_device(const void* d_a_values, const void* d_b, int64_t nrhs, double target, void* d_x,
+                             double* info, hipStream_t caller) = 0;
   // per-supernode (positive, negative, zero) pivot counts of the last factor
   virtual void supernode_inertia(std::vector<int32_t>& out) = 0;
   FactorStats stats;

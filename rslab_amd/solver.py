@@ -453,9 +453,15 @@ class LdltSolver:
         _check(lib.rsl_solve(self._h, ptr(bv, C.c_double), k, ptr(xv, C.c_double)))
         return x
 
+    def refactor_device(self, d_values: int, stream: int = 0):
+        """Same-pattern refactorization from lower-CSC values already in
+        device memory (real field; d_values is their device address), ordered
+        after the work enqueued on `stream` (0 = the null stream)."""
+        _check(lib.rsl_refactor_device(self._h, C.c_void_p(d_values), C.c_void_p(stream) if stream else None))
+
     def solve_device(self, d_b: int, nrhs: int, d_x: int, stream: int = 0):
         """Device-resident solve: d_b, d_x are device addresses of row-major
-        n x nrhs arrays; enqueued on `stream` (0 = the library stream)."""
+        n x nrhs arrays; enqueued on `stream` (0 = the null stream)."""
         if nrhs <= 0:
             raise DimensionMismatch("nrhs must be positive")
         _check(lib.rsl_solve_device(self._h, C.c_void_p(d_b), int(nrhs), C.c_void_p(d_x),
@@ -569,6 +575,20 @@ class MixedLdltSolver:
         if single:
             return x.reshape(a.n), infos[0]
         return x, infos
+
+    def solve_to_device(self, d_a_values: int, d_b: int, nrhs: int, d_x: int, target_backward_error: float,
+                        stream: int = 0):
+        """solve_to with the matrix values, b and x in device memory (real
+        field; device addresses, b and x row-major n x nrhs). Returns the per-column
+        MixedInfo list."""
+        if nrhs <= 0:
+            raise DimensionMismatch("nrhs must be positive")
+        info = np.zeros(4 * nrhs, dtype=np.float64)
+        _check(lib.rsl_refine_device(self.lo._h, C.c_void_p(d_a_values), C.c_void_p(d_b), int(nrhs),
+                                     float(target_backward_error), C.c_void_p(d_x), ptr(info, C.c_double),
+                                     C.c_void_p(stream) if stream else None))
+        return [MixedInfo(int(info[4 * c]), int(info[4 * c + 1]), float(info[4 * c + 2]), bool(info[4 * c + 3] != 0))
+                for c in range(nrhs)]
 
 
 def amd_order(n, col_ptr, row_idx):

@@ -277,6 +277,70 @@ def test_solve_device_matches_host():
         hip.hipFree(dx)
 
 
+def _device_copy(hip, arr):
+    import ctypes as C
+    d = C.c_void_p()
+    assert hip.hipMalloc(C.byref(d), C.c_size_t(max(arr.nbytes, 8))) == 0
+    assert hip.hipMemcpy(d, arr.ctypes.data_as(C.c_void_p), C.c_size_t(arr.nbytes), 1) == 0
+    return d
+
+
+@pytest.mark.parametrize("pivot", ["fail", "force_accept"])
+def test_refactor_device_matches_host_refactor(pivot):
+    """rsl_refactor_device on values already in device memory gives the host
+    call's factor bit for bit: the device equilibration and ForceAccept floor
+    reproduce the host's (maxima are order-free, the guarded 1/sqrt is
+    correctly rounded)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    a, _ = matgen.config3(m=20, nrhs=1)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
+    if pivot == "force_accept":
+        st = st.with_pivot(R.ZeroPivotAction.ForceAccept())
+    sym = R.LdltSymbolic.analyze_with(a, st)
+    f = sym.factor(a, st)
+    ref = f.factors()
+    ref_in, ref_np = f.inertia(), f.n_perturbed()
+    # overwrite the factor with other values, then refactor from device values
+    f.refactor(a.with_values(a.values * 1.5 - 0.25))
+    vals = np.ascontiguousarray(a.values_f64())
+    dv = _device_copy(hip, vals)
+    try:
+        f.refactor_device(dv.value)
+    finally:
+        hip.hipFree(dv)
+    got = f.factors()
+    for field in ("l_col_ptr", "l_row_idx", "l_values", "d_diag", "d_subdiag", "two_by_two", "perm"):
+        assert np.array_equal(getattr(got, field), getattr(ref, field)), field
+    assert f.inertia() == ref_in and f.n_perturbed() == ref_np
+
+
+@pytest.mark.parametrize("prec", [R.Precision.FP64, R.Precision.FP32_SCHUR])
+def test_refine_device_matches_host_refine(prec):
+    """rsl_refine_device with A, b and x in device memory takes the host
+    call's steps and returns its x bit for bit (||A||_inf is summed on the
+    device in the host loop's order)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    a, B = matgen.config3(m=20, nrhs=3)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND).with_precision(prec)
+    f = R.LdltSymbolic.analyze_with(a, st).factor(a, st)
+    mixed = R.MixedLdltSolver(f, a)
+    X, infos = mixed.solve_to(a, B, 1e-14)
+    B = np.ascontiguousarray(B)
+    vals = np.ascontiguousarray(a.values_f64())
+    da, db, dx = _device_copy(hip, vals), _device_copy(hip, B), _device_copy(hip, np.zeros_like(B))
+    try:
+        dinfos = mixed.solve_to_device(da.value, db.value, 3, dx.value, 1e-14)
+        Xd = np.empty_like(B)
+        assert hip.hipMemcpy(Xd.ctypes.data_as(C.c_void_p), dx, C.c_size_t(B.nbytes), 2) == 0
+    finally:
+        for d in (da, db, dx):
+            hip.hipFree(d)
+    assert np.array_equal(Xd, X)
+    assert dinfos == infos and all(i.certified for i in infos)
+
+
 # ------------------------------------------------------------ config-shaped cases
 def test_config1_matches_oracle():
     """100x100 2D Laplacian, A = L - 2I, AMD, one RHS b = A x_true.
