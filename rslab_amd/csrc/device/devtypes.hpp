@@ -87,9 +87,10 @@ struct WideStep {
   int32_t max_kb = 0;  // largest panel start of the step (columns whose rows are interchanged)
 };
 
-// Solve work unit on a wide supernode's own block: diagonal block j0 of
-// supernode s; for an update unit r0 = first own row of a 64-row chunk, for a
-// diagonal unit (r0, x) = (count, first index) of its partial sums.
+// Solve work unit on a wide supernode's own block, for the 64-column block
+// step j0 of supernode s: a forward row unit has r0 = the first own row of a
+// 64-row chunk below the block, a backward push unit r0 = the first column of
+// an earlier block; a diagonal unit (which stores the solved block) r0 = -1.
 struct SItem {
   int32_t s, j0, r0, x;
 };

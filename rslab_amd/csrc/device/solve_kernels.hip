@@ -45,30 +45,6 @@ __global__ void k_scatter_sol(int32_t n, int32_t nrhs, const int32_t* __restrict
   x[(int64_t)u * nrhs + c] = F<T>::scale(y[idx], scale[u]);
 }
 
-template <class T>
-__device__ inline T shfl_t(T v, int src) {
-  if constexpr (sizeof(T) == sizeof(double)) {
-    return __shfl(v, src, 64);
-  } else {
-    cplx o;
-    o.re = __shfl(v.re, src, 64);
-    o.im = __shfl(v.im, src, 64);
-    return o;
-  }
-}
-
-template <class T>
-__device__ inline T shfl_xor_t(T v, int m) {
-  if constexpr (sizeof(T) == sizeof(double)) {
-    return __shfl_xor(v, m, 64);
-  } else {
-    cplx o;
-    o.re = __shfl_xor(v.re, m, 64);
-    o.im = __shfl_xor(v.im, m, 64);
-    return o;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Own-block and update building blocks. Every loop that reads global memory
 // issues its loads as one batch into registers before using them, so a
@@ -159,6 +135,7 @@ __device__ void fwd_block_solve(const T* Ld, int eo, int j0, int nbk, int32_t nr
     const bool two_rhs = c2 < nrhs;
     T y0 = lane < nbk ? y[row * nrhs + c] : F<T>::zero();
     T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c2] : F<T>::zero();
+#pragma unroll 8
     for (int p = 0; p + 1 < nbk; ++p) {
       const T l = Ld[p * TB + lane];
       const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
@@ -182,6 +159,7 @@ __device__ void bwd_block_solve(const T* Lt, int eo, int j0, int nbk, int32_t nr
     const bool two_rhs = c2 < nrhs;
     T y0 = lane < nbk ? y[row * nrhs + c] : F<T>::zero();
     T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c2] : F<T>::zero();
+#pragma unroll 8
     for (int p = nbk - 1; p > 0; --p) {
       const T l = Lt[p * (TB + 1) + lane];
       const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
@@ -358,45 +336,6 @@ __global__ __launch_bounds__(SNT) void k_fwd_tri(DevSym S, DevFac<T> Fc,
   }
 }
 
-// Wide supernodes, forward step: the diagonal block j0 of each (one workgroup each)
-template <class T>
-__global__ __launch_bounds__(SNT) void k_ftri_diag(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items,
-                                                   int32_t nrhs, T* __restrict__ y) {
-  __shared__ T Ld[TB * TB];
-  const SItem it = items[blockIdx.x];
-  const int s = it.s, j0 = it.j0;
-  const int w = S.sn_ncol[s], eo = S.e_off[s];
-  const int nbk = min(TB, w - j0);
-  stage_block_lower(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, j0, nbk, Ld);
-  fwd_block_solve(Ld, eo, j0, nbk, nrhs, y);
-}
-
-// ... then the 64-row chunks below it inside the own block
-template <class T>
-__global__ __launch_bounds__(SNT) void k_ftri_upd(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items,
-                                                  int32_t nrhs, T* __restrict__ y) {
-  __shared__ T ys[64 * RCH];
-  __shared__ T red[4 * 64 * RCH];
-  const SItem it = items[blockIdx.x];
-  const int s = it.s, j0 = it.j0, r0 = it.r0;
-  const int w = S.sn_ncol[s], eo = S.e_off[s];
-  const int rb = j0 + min(TB, w - j0), nr = min(64, w - r0);
-  const int zr = (r0 == rb && Fc.two[eo + rb - 1]) ? 0 : -1;
-  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
-    const int nc = min(RCH, nrhs - c0);
-    rows_times_y(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], r0, nr, j0, rb, eo, nrhs, c0, nc, y, zr, rb - 1,
-                 ys, red);
-    for (int t = threadIdx.x; t < nr * RCH; t += SNT) {
-      const int rr = t / RCH, k = t % RCH;
-      if (k < nc) {
-        const int64_t o = (int64_t)(eo + r0 + rr) * nrhs + c0 + k;
-        y[o] = y[o] - red[rr * RCH + k];
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // Forward, part 2: u_s = L21 y_own for 64-row chunks of the off-diagonal rows
 // (items are (supernode, row offset) pairs of one level).
 template <class T>
@@ -537,58 +476,229 @@ __global__ __launch_bounds__(SNT) void k_bwd_tri(DevSym S, DevFac<T> Fc,
   }
 }
 
-// Wide supernodes, backward step j0: partial sums over one 64-row chunk of the
-// solved own rows below block j0, part[p][c] = sum_i L[i][j0+p] x_i,c.
+// ---- wide supernodes: one launch per 64-column block step ----
+// Every unit of a step solves the step's diagonal block itself (its y rows
+// are read only within the launch) into LDS; the diagonal unit stores it in z,
+// and the other units apply it: forward to 64 own rows below the block,
+// backward (push form) to an earlier own block. z is copied back to y once
+// the level's steps are done.
+
+// sb[p * RCH + k] = block j0 of y solved forward, right-hand sides c0 + k
 template <class T>
-__global__ __launch_bounds__(SNT) void k_btri_part(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items,
-                                                   int32_t nrhs, const T* __restrict__ y, T* __restrict__ part) {
-  __shared__ T Ls[TB * (TB + 1)];
-  __shared__ T xs[TB * RCH];
-  __shared__ T out[TB * RCH];
+__device__ void fwd_block_solve_to(const T* Ld, const T* __restrict__ y, int eo, int j0, int nbk, int32_t nrhs,
+                                   int c0, int nc, T* sb) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t row = eo + j0 + lane;
+  for (int k = wv; k < nc; k += 2 * NWV) {
+    const int k2 = k + NWV;
+    const bool two_rhs = k2 < nc;
+    T y0 = lane < nbk ? y[row * nrhs + c0 + k] : F<T>::zero();
+    T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c0 + k2] : F<T>::zero();
+#pragma unroll 8
+    for (int p = 0; p + 1 < nbk; ++p) {
+      const T l = Ld[p * TB + lane];
+      const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
+      y0 = y0 - l * a0;
+      y1 = y1 - l * a1;
+    }
+    sb[lane * RCH + k] = y0;
+    if (two_rhs) sb[lane * RCH + k2] = y1;
+  }
+}
+
+// backward (L^T of the block, Lt from stage_block_upper) into sb
+template <class T>
+__device__ void bwd_block_solve_to(const T* Lt, const T* __restrict__ y, int eo, int j0, int nbk, int32_t nrhs,
+                                   int c0, int nc, T* sb) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t row = eo + j0 + lane;
+  for (int k = wv; k < nc; k += 2 * NWV) {
+    const int k2 = k + NWV;
+    const bool two_rhs = k2 < nc;
+    T y0 = lane < nbk ? y[row * nrhs + c0 + k] : F<T>::zero();
+    T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c0 + k2] : F<T>::zero();
+#pragma unroll 8
+    for (int p = nbk - 1; p > 0; --p) {
+      const T l = Lt[p * (TB + 1) + lane];
+      const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
+      y0 = y0 - l * a0;
+      y1 = y1 - l * a1;
+    }
+    sb[lane * RCH + k] = y0;
+    if (two_rhs) sb[lane * RCH + k2] = y1;
+  }
+}
+
+// red[r][k] = sum_{p < nbk} L[row0 + r][j0 + p] * sb[p][k], the same
+// arithmetic as rows_times_y over columns [j0, j0 + nbk) (32-column batches,
+// thread (r, q) on columns q + 4m, fixed-order sum over q)
+template <class T>
+__device__ void rows_times_blk(const T* P, int64_t h, int64_t row0, int nr, int j0, int nbk, const T* sb, int zr,
+                               int zc, T* red) {
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  constexpr int CB = 32;
+  T acc[RCH];
+#pragma unroll
+  for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
+  for (int pb = 0; pb < nbk; pb += CB) {
+    T l[CB / NWV];
+#pragma unroll
+    for (int m = 0; m < CB / NWV; ++m) {
+      const int p = pb + q + NWV * m;
+      l[m] = (r < nr && p < nbk) ? P[(row0 + r) + (int64_t)(j0 + p) * h] : F<T>::zero();
+      if (r == zr && j0 + p == zc) l[m] = F<T>::zero();
+    }
+#pragma unroll
+    for (int m = 0; m < CB / NWV; ++m) {
+      const int p = pb + q + NWV * m;
+      if (p < nbk) {
+        const T* yp = sb + p * RCH;
+#pragma unroll
+        for (int k = 0; k < RCH; ++k) acc[k] = acc[k] + l[m] * yp[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RCH; ++k) red[(q * 64 + r) * RCH + k] = acc[k];
+  __syncthreads();
+  for (int t = threadIdx.x; t < 64 * RCH; t += SNT) {
+    T v = red[t];
+    v = v + red[64 * RCH + t];
+    v = v + red[2 * 64 * RCH + t];
+    v = v + red[3 * 64 * RCH + t];
+    red[t] = v;
+  }
+  __syncthreads();
+}
+
+// out[p][k] = sum_{i < nr} L[r0 + i][i0 + p] * sb[i][k] (64 columns of block
+// i0 against the solved rows r0..r0+nr), as chunk_t_times_x with x from LDS
+template <class T>
+__device__ void blk_t_times_sb(const T* P, int64_t h, const uint8_t* two, int i0, int r0, int nr, const T* sb,
+                               T* Ls, T* out) {
+  constexpr int PER = TB * TB / SNT;
+  T v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * SNT, i = t % TB, p = t / TB;
+    v[u] = i < nr ? P[(int64_t)(r0 + i) + (int64_t)(i0 + p) * h] : F<T>::zero();
+  }
+  // a 2x2 coupling entry (i0+64, i0+63) when the blocks are adjacent
+  const bool zc = r0 == i0 + TB && two[i0 + TB - 1];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * SNT, i = t % TB, p = t / TB;
+    Ls[i * (TB + 1) + p] = (zc && i == 0 && p == TB - 1) ? F<T>::zero() : v[u];
+  }
+  __syncthreads();
+  const int p = threadIdx.x & 63, kq = threadIdx.x >> 6;
+  T a0 = F<T>::zero(), a1 = F<T>::zero();
+#pragma unroll 8
+  for (int i = 0; i < TB; ++i) {
+    const T l = Ls[i * (TB + 1) + p];
+    a0 = a0 + l * sb[i * RCH + kq];
+    a1 = a1 + l * sb[i * RCH + kq + NWV];
+  }
+  out[p * RCH + kq] = a0;
+  out[p * RCH + kq + NWV] = a1;
+  __syncthreads();
+}
+
+// forward step: units [0, nu) are row units (s, j0, r0), the rest diagonal units
+template <class T>
+__global__ __launch_bounds__(SNT) void k_ftri_step(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items, int nu,
+                                                   int32_t nrhs, T* __restrict__ y, T* __restrict__ z) {
+  __shared__ T Ld[TB * TB];
+  __shared__ T sb[TB * RCH];
+  __shared__ T red[4 * 64 * RCH];
   const SItem it = items[blockIdx.x];
+  const bool diag = (int)blockIdx.x >= nu;
   const int s = it.s, j0 = it.j0;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
-  const int nbk = min(TB, w - j0), r0 = it.r0, nr = min(64, w - r0);
-  T* dst = part + (int64_t)blockIdx.x * TB * nrhs;
+  const int64_t h = S.sn_nrow[s];
+  const T* P = Fc.pool + S.panel_off[s];
+  const uint8_t* two = Fc.two + eo;
+  const int nbk = min(TB, w - j0), rb = j0 + nbk;
+  stage_block_lower(P, h, two, j0, nbk, Ld);
   for (int c0 = 0; c0 < nrhs; c0 += RCH) {
     const int nc = min(RCH, nrhs - c0);
-    chunk_t_times_x(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, eo, j0, nbk, r0, nr, nrhs, c0,
-                    nc, y, Ls, xs, out);
-    for (int t = threadIdx.x; t < TB * RCH; t += SNT) {
-      const int p = t / RCH, k = t % RCH;
-      if (k < nc) dst[(int64_t)p * nrhs + c0 + k] = out[t];
+    for (int t = threadIdx.x; t < TB * RCH; t += SNT) sb[t] = F<T>::zero();
+    __syncthreads();
+    fwd_block_solve_to(Ld, y, eo, j0, nbk, nrhs, c0, nc, sb);
+    __syncthreads();
+    if (diag) {
+      for (int t = threadIdx.x; t < nbk * RCH; t += SNT) {
+        const int p = t / RCH, k = t % RCH;
+        if (k < nc) z[(int64_t)(eo + j0 + p) * nrhs + c0 + k] = sb[t];
+      }
+    } else {
+      const int r0 = it.r0, nr = min(64, w - r0);
+      const int zr = (r0 == rb && two[rb - 1]) ? 0 : -1;
+      rows_times_blk(P, h, r0, nr, j0, nbk, sb, zr, rb - 1, red);
+      for (int t = threadIdx.x; t < nr * RCH; t += SNT) {
+        const int rr = t / RCH, k = t % RCH;
+        if (k < nc) {
+          const int64_t o = (int64_t)(eo + r0 + rr) * nrhs + c0 + k;
+          y[o] = y[o] - red[rr * RCH + k];
+        }
+      }
     }
     __syncthreads();
   }
 }
 
-// ... then each wide supernode's block j0 takes its partial sums (fixed
-// order) and is back-substituted.
+// backward step (last block first): units [0, nd) are diagonal units, the
+// rest push units (s, j0, i0)
 template <class T>
-__global__ __launch_bounds__(SNT) void k_btri_diag(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items,
-                                                   int32_t nrhs, T* __restrict__ y, const T* __restrict__ part) {
+__global__ __launch_bounds__(SNT) void k_btri_step(DevSym S, DevFac<T> Fc, const SItem* __restrict__ items, int nd,
+                                                   int32_t nrhs, T* __restrict__ y, T* __restrict__ z) {
   __shared__ T Lt[TB * (TB + 1)];
+  __shared__ T sb[TB * RCH];
+  __shared__ T out[TB * RCH];
   const SItem it = items[blockIdx.x];
+  const bool diag = (int)blockIdx.x < nd;
   const int s = it.s, j0 = it.j0;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
+  const int64_t h = S.sn_nrow[s];
+  const T* P = Fc.pool + S.panel_off[s];
+  const uint8_t* two = Fc.two + eo;
   const int nbk = min(TB, w - j0);
-  for (int t = threadIdx.x; t < nbk * nrhs; t += SNT) {
-    const int p = t / nrhs, c = t % nrhs;
-    T acc = F<T>::zero();
-    int q = 0;
-    for (; q + 4 <= it.r0; q += 4) {
-      T v[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) v[m] = part[((int64_t)(it.x + q + m) * TB + p) * nrhs + c];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) acc = acc + v[m];
+  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+    const int nc = min(RCH, nrhs - c0);
+    for (int t = threadIdx.x; t < TB * RCH; t += SNT) sb[t] = F<T>::zero();
+    stage_block_upper(P, h, two, j0, nbk, Lt);
+    bwd_block_solve_to(Lt, y, eo, j0, nbk, nrhs, c0, nc, sb);
+    __syncthreads();
+    if (diag) {
+      for (int t = threadIdx.x; t < nbk * RCH; t += SNT) {
+        const int p = t / RCH, k = t % RCH;
+        if (k < nc) z[(int64_t)(eo + j0 + p) * nrhs + c0 + k] = sb[t];
+      }
+      __syncthreads();
+    } else {
+      const int i0 = it.r0;
+      blk_t_times_sb(P, h, two, i0, j0, nbk, sb, Lt, out);
+      for (int t = threadIdx.x; t < TB * RCH; t += SNT) {
+        const int p = t / RCH, k = t % RCH;
+        if (k < nc) {
+          const int64_t o = (int64_t)(eo + i0 + p) * nrhs + c0 + k;
+          y[o] = y[o] - out[t];
+        }
+      }
+      __syncthreads();
     }
-    for (; q < it.r0; ++q) acc = acc + part[((int64_t)(it.x + q) * TB + p) * nrhs + c];
-    const int64_t o = (int64_t)(eo + j0 + p) * nrhs + c;
-    y[o] = y[o] - acc;
   }
-  stage_block_upper(Fc.pool + S.panel_off[s], (int64_t)S.sn_nrow[s], Fc.two + eo, j0, nbk, Lt);
-  bwd_block_solve(Lt, eo, j0, nbk, nrhs, y);
+}
+
+// y := z on the own blocks of a level's wide supernodes
+template <class T>
+__global__ __launch_bounds__(SNT) void k_zcopy(DevSym S, const SItem* __restrict__ items, int32_t nrhs,
+                                               const T* __restrict__ z, T* __restrict__ y) {
+  const SItem it = items[blockIdx.x];
+  const int w = S.sn_ncol[it.s], eo = S.e_off[it.s];
+  const int nbk = min(TB, w - it.j0);
+  const int64_t base = (int64_t)(eo + it.j0) * nrhs, n = (int64_t)nbk * nrhs;
+  for (int64_t t = threadIdx.x; t < n; t += SNT) y[base + t] = z[base + t];
 }
 
 template <class T>
@@ -642,28 +752,27 @@ void launch_fwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, i
   ProfScope ps(pf, LK_FWD_TRI, 0.0, bytes, st);
   hipLaunchKernelGGL(k_fwd_tri<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y);
 }
-// one 64-column step of the wide supernodes: diagonal blocks, then the rows below
+// one 64-column block step of the wide supernodes (one launch)
 template <class T>
 void launch_ftri_step(const DevSym& S, const DevFac<T>& Fc, const SItem* items, const SolveStep& stp,
-                      int32_t nrhs, T* y, hipStream_t st, LaunchProfiler* pf) {
+                      int32_t nrhs, T* y, T* z, hipStream_t st, LaunchProfiler* pf) {
+  if (stp.d_end <= stp.u_beg) return;
   ProfScope ps(pf, LK_FWD_TRI, 0.0, 0.0, st);
-  if (stp.d_end > stp.d_beg)
-    hipLaunchKernelGGL(k_ftri_diag<T>, dim3((unsigned)(stp.d_end - stp.d_beg)), dim3(SNT), 0, st, S, Fc,
-                       items + stp.d_beg, nrhs, y);
-  if (stp.u_end > stp.u_beg)
-    hipLaunchKernelGGL(k_ftri_upd<T>, dim3((unsigned)(stp.u_end - stp.u_beg)), dim3(SNT), 0, st, S, Fc,
-                       items + stp.u_beg, nrhs, y);
+  hipLaunchKernelGGL(k_ftri_step<T>, dim3((unsigned)(stp.d_end - stp.u_beg)), dim3(SNT), 0, st, S, Fc,
+                     items + stp.u_beg, (int)(stp.u_end - stp.u_beg), nrhs, y, z);
 }
 template <class T>
 void launch_btri_step(const DevSym& S, const DevFac<T>& Fc, const SItem* items, const SolveStep& stp,
-                      int32_t nrhs, T* y, T* part, hipStream_t st, LaunchProfiler* pf) {
+                      int32_t nrhs, T* y, T* z, hipStream_t st, LaunchProfiler* pf) {
+  if (stp.u_end <= stp.d_beg) return;
   ProfScope ps(pf, LK_BWD_TRI, 0.0, 0.0, st);
-  if (stp.u_end > stp.u_beg)
-    hipLaunchKernelGGL(k_btri_part<T>, dim3((unsigned)(stp.u_end - stp.u_beg)), dim3(SNT), 0, st, S, Fc,
-                       items + stp.u_beg, nrhs, y, part);
-  if (stp.d_end > stp.d_beg)
-    hipLaunchKernelGGL(k_btri_diag<T>, dim3((unsigned)(stp.d_end - stp.d_beg)), dim3(SNT), 0, st, S, Fc,
-                       items + stp.d_beg, nrhs, y, part);
+  hipLaunchKernelGGL(k_btri_step<T>, dim3((unsigned)(stp.u_end - stp.d_beg)), dim3(SNT), 0, st, S, Fc,
+                     items + stp.d_beg, (int)(stp.d_end - stp.d_beg), nrhs, y, z);
+}
+template <class T>
+void launch_zcopy(const DevSym& S, const SItem* items, int count, int32_t nrhs, const T* z, T* y, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_zcopy<T>, dim3((unsigned)count), dim3(SNT), 0, st, S, items, nrhs, z, y);
 }
 template <class T>
 void launch_fwd_update(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count,
@@ -706,9 +815,10 @@ void launch_bwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, i
   template void launch_fwd_tri<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
                                   T*, double, hipStream_t, LaunchProfiler*);                   \
   template void launch_ftri_step<T>(const DevSym&, const DevFac<T>&, const SItem*, const SolveStep&, \
-                                    int32_t, T*, hipStream_t, LaunchProfiler*);                \
+                                    int32_t, T*, T*, hipStream_t, LaunchProfiler*);            \
   template void launch_btri_step<T>(const DevSym&, const DevFac<T>&, const SItem*, const SolveStep&, \
                                     int32_t, T*, T*, hipStream_t, LaunchProfiler*);            \
+  template void launch_zcopy<T>(const DevSym&, const SItem*, int, int32_t, const T*, T*, hipStream_t); \
   template void launch_fwd_update<T>(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, \
                                      const T*, T*, double, hipStream_t, LaunchProfiler*);      \
   template void launch_dsolve<T>(int32_t, int32_t, const T*, const T*, const uint8_t*, T*,      \

@@ -252,7 +252,8 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   P.bwd_items.clear();
   P.gat_items.clear();
   P.sitems.clear();
-  P.max_step_upd = 0;
+  P.bitems.clear();
+  P.zitems.clear();
   for (LevelPlan& L : P.levels) {
     L.fwd_beg = (int64_t)P.fwd_items.size();
     L.bwd_beg = (int64_t)P.bwd_items.size();
@@ -280,24 +281,39 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     for (int32_t s : L.swide) maxb = std::max(maxb, (P.sn_ncol[s] + 63) / 64);
     for (int32_t b = 0; b < maxb; ++b) {
       SolveStep st;
+      // forward: row units (s, j0, r0) for the 64-row chunks below block b,
+      // then one diagonal unit (s, j0) per supernode, all in one launch
       st.u_beg = (int64_t)P.sitems.size();
-      std::vector<std::pair<int32_t, int32_t>> span;  // per wide supernode: (first unit, count)
       for (int32_t s : L.swide) {
         const int32_t j0 = b * 64, w = P.sn_ncol[s];
-        const int32_t first = (int32_t)((int64_t)P.sitems.size() - st.u_beg);
         if (j0 < w)
           for (int32_t r0 = j0 + 64; r0 < w; r0 += 64) P.sitems.push_back({s, j0, r0, 0});
-        span.push_back({first, (int32_t)((int64_t)P.sitems.size() - st.u_beg) - first});
       }
       st.u_end = st.d_beg = (int64_t)P.sitems.size();
-      for (size_t q = 0; q < L.swide.size(); ++q) {
-        const int32_t s = L.swide[q], j0 = b * 64;
-        if (j0 < P.sn_ncol[s]) P.sitems.push_back({s, j0, span[q].second, span[q].first});
-      }
+      for (int32_t s : L.swide)
+        if (b * 64 < P.sn_ncol[s]) P.sitems.push_back({s, b * 64, -1, 0});
       st.d_end = (int64_t)P.sitems.size();
-      P.max_step_upd = std::max(P.max_step_upd, st.u_end - st.u_beg);
       L.ssteps.push_back(st);
     }
+    // backward: block b (last first) is solved by every unit of its step; the
+    // diagonal unit stores it, unit (s, j0, i0) pushes it into block i0 < j0
+    L.bsteps.clear();
+    for (int32_t b = maxb - 1; b >= 0; --b) {
+      SolveStep st;
+      st.d_beg = (int64_t)P.bitems.size();
+      for (int32_t s : L.swide)
+        if (b * 64 < P.sn_ncol[s]) P.bitems.push_back({s, b * 64, -1, 0});
+      st.d_end = st.u_beg = (int64_t)P.bitems.size();
+      for (int32_t s : L.swide)
+        if (b * 64 < P.sn_ncol[s])
+          for (int32_t i0 = 0; i0 < b * 64; i0 += 64) P.bitems.push_back({s, b * 64, i0, 0});
+      st.u_end = (int64_t)P.bitems.size();
+      L.bsteps.push_back(st);
+    }
+    L.z_beg = (int64_t)P.zitems.size();
+    for (int32_t s : L.swide)
+      for (int32_t j0 = 0; j0 < P.sn_ncol[s]; j0 += 64) P.zitems.push_back({s, j0, 0, 0});
+    L.z_end = (int64_t)P.zitems.size();
   }
 
   // A scatter map: user lower entry (i >= j) -> slot of P^T A P in its panel

@@ -24,7 +24,9 @@ struct LevelPlan {
   int64_t bwd_beg = 0, bwd_end = 0;    // solve: (supernode, column chunk) L21^T x items
   int64_t gat_beg = 0, gat_end = 0;    // solve: (supernode, own-row chunk) gather items
   std::vector<int32_t> snarrow, swide; // solve: own block in one workgroup / in block steps
-  std::vector<SolveStep> ssteps;       // solve: block steps of the wide ones
+  std::vector<SolveStep> ssteps;       // solve: forward block steps of the wide ones
+  std::vector<SolveStep> bsteps;       // solve: backward block steps (last block first)
+  int64_t z_beg = 0, z_end = 0;        // solve: every own block of the wide ones (copy-back units)
   // algorithmic work of the level (real field; 2 flops per multiply-add)
   double cmod_flops = 0;   // sum over cmod tasks of 2 * nck * (lower-trapezoid pairs)
   double small_flops = 0;  // cdiv of the small supernodes: sum_k (h-k)^2 - (h-w)^2
@@ -61,7 +63,8 @@ struct Plan {
   std::vector<int64_t> gat_ptr, gat_idx;  // per permuted column, ascending updater
   std::vector<Int2> fwd_items, bwd_items, gat_items;
   std::vector<SItem> sitems;
-  int64_t max_step_upd = 0;     // largest number of update units in one solve step
+  std::vector<SItem> bitems;    // backward step units: (s, j0, i0) pushes of block j0 into block i0
+  std::vector<SItem> zitems;    // (s, j0) own blocks of the wide supernodes
   int64_t panel_total = 0;      // elements in the panel pool
   // user lower-CSC entries -> panel slot
   int64_t nnz = 0;

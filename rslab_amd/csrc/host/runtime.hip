@@ -32,8 +32,9 @@ template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, cons
 template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_fgather(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, const T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_fwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
-template <class T> void launch_ftri_step(const DevSym&, const DevFac<T>&, const SItem*, const SolveStep&, int32_t, T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_ftri_step(const DevSym&, const DevFac<T>&, const SItem*, const SolveStep&, int32_t, T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_btri_step(const DevSym&, const DevFac<T>&, const SItem*, const SolveStep&, int32_t, T*, T*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_zcopy(const DevSym&, const SItem*, int, int32_t, const T*, T*, hipStream_t);
 template <class T> void launch_fwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, const T*, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, const uint8_t*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
@@ -198,6 +199,8 @@ void DeviceSymbolic::upload() {
   witems.upload(P.witems, stream);
   gat_items.upload(P.gat_items, stream);
   sitems.upload(P.sitems, stream);
+  bitems.upload(P.bitems, stream);
+  zitems.upload(P.zitems, stream);
   nb_eff.upload(P.nb_eff, stream);
   hip_check(hipStreamSynchronize(stream), "plan upload");
   view.n = P.n;
@@ -276,7 +279,7 @@ template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
-      work_y, work_b, work_p, cbuf, rscratch, dev_u64;
+      work_y, work_b, work_z, cbuf, rscratch, dev_u64;
   hipEvent_t ev_caller = nullptr;
   bool ev_ready = false;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
@@ -495,10 +498,11 @@ class DeviceFactor final : public DeviceFactorBase {
     const int2* bi = sym->bwd_items.as<int2>();
     LaunchProfiler* pf = prof_on ? &prof : nullptr;
     const double ts = sizeof(T);
-    const size_t needp = sizeof(T) * std::max<int64_t>(P.max_step_upd * 64 * nrhs, 1);
-    if (work_p.bytes < needp) work_p.alloc(needp);
-    T* part = work_p.as<T>();
+    if (work_z.bytes < need) work_z.alloc(need);
+    T* z = work_z.as<T>();  // solved own blocks of the wide supernodes, copied back per level
     const SItem* si = sym->sitems.as<SItem>();
+    const SItem* bi2 = sym->bitems.as<SItem>();
+    const SItem* zi = sym->zitems.as<SItem>();
     const int2* gi = sym->gat_items.as<int2>();
     launch_gather_rhs<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(),
                          static_cast<const T*>(d_b), y, st, pf);
@@ -508,7 +512,8 @@ class DeviceFactor final : public DeviceFactorBase {
       launch_fgather<T>(S, fv, gi + lv.gat_beg, (int)(lv.gat_end - lv.gat_beg), (int32_t)nrhs, y, u, st, pf);
       launch_fwd_tri<T>(S, fv, lists + sym->level_snarrow_off[L], (int)lv.snarrow.size(), (int32_t)nrhs, y,
                         ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
-      for (const SolveStep& stp : lv.ssteps) launch_ftri_step<T>(S, fv, si, stp, (int32_t)nrhs, y, st, pf);
+      for (const SolveStep& stp : lv.ssteps) launch_ftri_step<T>(S, fv, si, stp, (int32_t)nrhs, y, z, st, pf);
+      launch_zcopy<T>(S, zi + lv.z_beg, (int)(lv.z_end - lv.z_beg), (int32_t)nrhs, z, y, st);
       launch_fwd_update<T>(S, fv, fi + lv.fwd_beg, (int)(lv.fwd_end - lv.fwd_beg), (int32_t)nrhs,
                            y, u, ts * lv.off_entries, st, pf);
     }
@@ -519,8 +524,8 @@ class DeviceFactor final : public DeviceFactorBase {
                            ts * lv.off_entries, st, pf);
       launch_bwd_tri<T>(S, fv, lists + sym->level_snarrow_off[L], (int)lv.snarrow.size(), (int32_t)nrhs, y,
                         ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
-      for (size_t q = lv.ssteps.size(); q-- > 0;)
-        launch_btri_step<T>(S, fv, si, lv.ssteps[q], (int32_t)nrhs, y, part, st, pf);
+      for (const SolveStep& stp : lv.bsteps) launch_btri_step<T>(S, fv, bi2, stp, (int32_t)nrhs, y, z, st, pf);
+      launch_zcopy<T>(S, zi + lv.z_beg, (int)(lv.z_end - lv.z_beg), (int32_t)nrhs, z, y, st);
     }
     launch_scatter_sol<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(), y,
                           static_cast<T*>(d_x), st, pf);
