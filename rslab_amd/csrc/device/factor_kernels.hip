@@ -536,6 +536,27 @@ __device__ inline int packed_col(int e, int pw) {
   return j;
 }
 
+// Column k0 of the trailing block is the next step's pivot column: its new
+// entries fall to threads 1..pw-k0-1 in the first pass (thread t holds row
+// k0 + t), which leave its squared-magnitude maximum and first row reaching it
+// per wave (waves 0 and 1: pw <= 128) in cm_s / ci_s, the values the next
+// step's column scan would find.
+template <class T>
+__device__ inline void next_column_candidates(int k0, int pw, double m, double* cm_s, int* ci_s) {
+  const int tid = threadIdx.x, wv = tid >> 6;
+  if (wv < 2) {
+    const int i0 = k0 + 64 * wv;
+    int idx = (m > 0.0) ? tid + k0 : i0 - 1;
+    double mm = m > 0.0 ? m : 0.0;
+    wave_argmax_dpp(mm, idx, i0);
+    if ((tid & 63) == 0) {
+      cm_s[wv] = mm;
+      ci_s[wv] = idx;
+    }
+  }
+  (void)pw;
+}
+
 // Trailing update of one Bunch-Kaufman step on the packed block, element-
 // parallel: the columns k0..pw-1 form one contiguous run of the packed
 // triangle, swept in storage order (each thread every NTG-th entry, B of them
@@ -545,9 +566,10 @@ __device__ inline int packed_col(int e, int pw) {
 //   2x2:  x  = x - w1[i] * l1[j] - w2[i] * l2[j]
 template <int NTG, bool TWO, class T>
 __device__ inline void packed_trailing_update(T* Ab, int pw, int k0, const T* w1s, const T* w2s, const T* l1s,
-                                              const T* l2s) {
+                                              const T* l2s, double* cm_s, int* ci_s) {
   constexpr int B = 4;
   const int base = pk_off(k0, k0, pw), tot = pw * (pw + 1) / 2;
+  double cand = 0.0;
   for (int e0 = base + (int)threadIdx.x; e0 < tot; e0 += NTG * B) {
     int ii[B], jj[B];
 #pragma unroll
@@ -577,23 +599,30 @@ __device__ inline void packed_trailing_update(T* Ab, int pw, int k0, const T* w1
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       if (ii[u] >= 0) {
-        if constexpr (TWO)
-          Ab[e0 + NTG * u] = x[u] - a[u] * m[u] - b[u] * m2[u];
-        else if (!F<T>::is_zero(m[u]))
-          Ab[e0 + NTG * u] = x[u] - a[u] * m[u];
+        T v = x[u];
+        if constexpr (TWO) {
+          v = x[u] - a[u] * m[u] - b[u] * m2[u];
+          Ab[e0 + NTG * u] = v;
+        } else if (!F<T>::is_zero(m[u])) {
+          v = x[u] - a[u] * m[u];
+          Ab[e0 + NTG * u] = v;
+        }
+        if (u == 0 && jj[0] == k0 && ii[0] > k0) cand = F<T>::magsq(v);
       }
     }
   }
+  next_column_candidates<T>(k0, pw, cand, cm_s, ci_s);
 }
 
 // The 1x1 step's update reading column k in place: entry (i, j) of the
 // trailing block takes w = A(i, k) and the multiplier A(j, k) * dinv (the
 // same product the staged form stores), skipped when that multiplier is zero.
 template <int NTG, class T>
-__device__ inline void packed_trailing_update_1x1(T* Ab, int pw, int k, T dinv) {
+__device__ inline void packed_trailing_update_1x1(T* Ab, int pw, int k, T dinv, double* cm_s, int* ci_s) {
   constexpr int B = 4;
   const int k0 = k + 1, base = pk_off(k0, k0, pw), tot = pw * (pw + 1) / 2;
   const T* colk = Ab + pk_off(k, k, pw) - k;  // colk[i] = A(i, k)
+  double cand = 0.0;  // |new A(k0 + tid, k0)|^2 for this thread's candidate row
   for (int e0 = base + (int)threadIdx.x; e0 < tot; e0 += NTG * B) {
     int ii[B], jj[B];
 #pragma unroll
@@ -620,10 +649,16 @@ __device__ inline void packed_trailing_update_1x1(T* Ab, int pw, int k, T dinv) 
     for (int u = 0; u < B; ++u) {
       if (ii[u] >= 0) {
         const T mu = m[u] * dinv;
-        if (!F<T>::is_zero(mu)) Ab[e0 + NTG * u] = x[u] - a[u] * mu;
+        T v = x[u];
+        if (!F<T>::is_zero(mu)) {
+          v = x[u] - a[u] * mu;
+          Ab[e0 + NTG * u] = v;
+        }
+        if (u == 0 && jj[0] == k0 && ii[0] > k0) cand = F<T>::magsq(v);
       }
     }
   }
+  next_column_candidates<T>(k0, pw, cand, cm_s, ci_s);
 }
 
 // Bunch-Kaufman on the diagonal block [kb,ke) held in LDS (packed lower).
@@ -643,21 +678,36 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
     two_s[q] = 0;
     lperm_s[q] = c.lperm[kb + q];
   }
+  __shared__ double cand_m[2];
+  __shared__ int cand_i[2];
   __syncthreads();
   int k = 0;
+  bool have_cand = false;  // the previous step's update left this column's max
   while (k < pw) {
     // ---- pivot search (every wave computes the same answer) ----
     const double absakk = F<T>::mag(A_(k, k));
     double cm = 0.0;
     int imax = k;
-    for (int i = k + 1 + lane; i < pw; i += 64) {
-      double m = F<T>::magsq(A_(i, k));
-      if (m > cm) {
-        cm = m;
-        imax = i;
+    if (have_cand) {
+      // wave 0 holds the rows below wave 1's: it wins ties
+      const double c0 = cand_m[0], c1 = cand_m[1];
+      if (c0 >= c1 && c0 > 0.0) {
+        cm = c0;
+        imax = cand_i[0];
+      } else if (c1 > 0.0) {
+        cm = c1;
+        imax = cand_i[1];
       }
+    } else {
+      for (int i = k + 1 + lane; i < pw; i += 64) {
+        double m = F<T>::magsq(A_(i, k));
+        if (m > cm) {
+          cm = m;
+          imax = i;
+        }
+      }
+      wave_argmax_dpp(cm, imax, k + 1);  // pw <= 128: at most two candidate rows per lane
     }
-    wave_argmax_dpp(cm, imax, k + 1);  // pw <= 128: at most two candidate rows per lane
     const double colmax = sqrt(cm);
     // decision below reads row imax; every wave must finish reading before
     // any wave starts the interchange / update of this step
@@ -740,7 +790,8 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       // which is replaced by its multipliers once every thread is past the
       // update (the next step reads column k again only after its first
       // barrier, and scaling a column commutes with its interchanges)
-      packed_trailing_update_1x1<NTG>(Ab, pw, k, dinv);
+      packed_trailing_update_1x1<NTG>(Ab, pw, k, dinv, cand_m, cand_i);
+      have_cand = true;
       __syncthreads();
       for (int i = k + 1 + tid; i < pw; i += NTG) A_(i, k) = A_(i, k) * dinv;
       k += 1;
@@ -780,7 +831,8 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         w2s[i] = wik1;
       }
       __syncthreads();
-      packed_trailing_update<NTG, true>(Ab, pw, k + 2, w1s, w2s, l1s, l2s);
+      packed_trailing_update<NTG, true>(Ab, pw, k + 2, w1s, w2s, l1s, l2s, cand_m, cand_i);
+      have_cand = true;
       for (int i = k + 2 + tid; i < pw; i += NTG) {
         A_(i, k) = l1s[i];
         A_(i, k + 1) = l2s[i];
