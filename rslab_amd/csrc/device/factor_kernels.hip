@@ -792,8 +792,11 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       // barrier, and scaling a column commutes with its interchanges)
       packed_trailing_update_1x1<NTG>(Ab, pw, k, dinv, cand_m, cand_i);
       have_cand = true;
+      // column k keeps its unscaled entries until the block is written back
+      // (panel_store multiplies them by w1s[k] = 1/d_k then; the later
+      // interchanges only move entries within finished columns)
+      if (tid == 0) w1s[k] = dinv;
       __syncthreads();
-      for (int i = k + 1 + tid; i < pw; i += NTG) A_(i, k) = A_(i, k) * dinv;
       k += 1;
     } else {
       T d11 = A_(k, k), d21 = A_(k + 1, k), d22 = A_(k + 1, k + 1);
@@ -876,12 +879,19 @@ __device__ void panel_load(const SnCtx<T>& c, int kb, int ke, T* Ab) {
 // row / column that sat at position cperm[q] before this panel's steps).
 template <int NTG, bool SWAP_COLS, class T>
 __device__ void panel_store(const SnCtx<T>& c, int kb, int ke, T* Ab, const int32_t* deep_s,
-                            const int32_t* two_s, const int32_t* lperm_s, int32_t* cperm_s, int* any_s) {
+                            const int32_t* two_s, const int32_t* lperm_s, int32_t* cperm_s, int* any_s,
+                            const T* colscale) {
   const int pw = ke - kb;
+  // 1x1 columns below the diagonal hold A(i, k); their multipliers are
+  // A(i, k) * (1/d_k) (colscale[k]); 2x2 columns hold their multipliers
 #pragma unroll 4
   for (int e = threadIdx.x; e < pw * pw; e += NTG) {
     const int i = e % pw, j = e / pw;
-    if (i >= j) c.P[(kb + i) + (int64_t)(kb + j) * c.h] = Ab[pk_off(i, j, pw)];
+    if (i >= j) {
+      T v = Ab[pk_off(i, j, pw)];
+      if (i > j && !two_s[j] && !(j > 0 && two_s[j - 1])) v = v * colscale[j];
+      c.P[(kb + i) + (int64_t)(kb + j) * c.h] = v;
+    }
   }
   for (int q = threadIdx.x; q < pw; q += NTG) c.lperm[kb + q] = lperm_s[q];
   if (threadIdx.x == 0) {
@@ -1340,7 +1350,7 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
       flag_failure(Fc, s);
       return;
     }
-    panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
+    panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap, sh.w1s);
     bool done = false;
     if constexpr (std::is_same<T, double>::value) {
       // LDS: the pass's L11 panel and D coefficients, then the running values
@@ -1391,7 +1401,7 @@ __global__ __launch_bounds__(GT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const
     flag_failure(Fc, s);
     return;
   }
-  panel_store<GT, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap);
+  panel_store<GT, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap, sh.w1s);
   if (threadIdx.x == 0 && sh.npert) atomicAdd(&Fc.counters[3], (unsigned long long)sh.npert);
 }
 
