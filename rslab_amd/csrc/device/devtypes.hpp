@@ -13,6 +13,7 @@ struct DevSym {
   const int32_t* sn_nrow;    // panel height h = |rs[s]|
   const int32_t* e_off;      // first elimination position of s
   const int64_t* panel_off;  // panel offset in the value pool (column-major h x w)
+  const int64_t* g_off;      // offset of L21 D in the G pool ((h - w) x w, column-major)
   const int64_t* rs_off;     // offset of rs[s] in rs_idx
   const int32_t* rs_idx;     // row structures (global permuted indices)
   const int32_t* upd_ptr;    // update lists (ns+1)

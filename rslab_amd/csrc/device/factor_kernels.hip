@@ -407,6 +407,114 @@ __global__ __launch_bounds__(NT) void k_cmod_prod(DevSym S, DevFac<double> Fc, c
   }
 }
 
+// G = L21 D for every supernode of a finished level (real field), kept so the
+// cmod products read their second operand with plain loads: block (y, x) of
+// the grid covers 16 columns of the y-th supernode of the list.
+__global__ __launch_bounds__(NT) void k_make_g(DevSym S, DevFac<double> Fc, const int32_t* __restrict__ list,
+                                               double* __restrict__ gpool) {
+  const int s = list[blockIdx.y];
+  const int w = S.sn_ncol[s], eo = S.e_off[s], c0 = blockIdx.x * 16;
+  const int64_t h = S.sn_nrow[s], m = h - w;
+  if (c0 >= w || m <= 0) return;
+  const double* P = Fc.pool + S.panel_off[s];
+  double* G = gpool + S.g_off[s];
+  const int nc = min(16, w - c0);
+  // D of columns c0-1 .. c0+16 (a 2x2 pair may straddle the chunk's edges)
+  __shared__ double dd[18], dsd[18];
+  __shared__ int tw[18];
+  if (threadIdx.x < 18) {
+    const int c = c0 - 1 + (int)threadIdx.x;
+    const bool in = c >= 0 && c < w;
+    dd[threadIdx.x] = in ? Fc.d[eo + c] : 0.0;
+    dsd[threadIdx.x] = in ? Fc.ds[eo + c] : 0.0;
+    tw[threadIdx.x] = in ? Fc.two[eo + c] : 0;
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < m; i += NT) {
+    double v[18];
+#pragma unroll
+    for (int t = 0; t < 18; ++t) {
+      const int c = c0 - 1 + t;
+      v[t] = (c >= 0 && c < w) ? P[(w + i) + (int64_t)c * h] : 0.0;
+    }
+    // the expressions of g_entry_d
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (q < nc) {
+        const int t = q + 1;
+        double g;
+        if (tw[t]) g = dd[t] * v[t] + dsd[t] * v[t + 1];
+        else if (c0 + q > 0 && tw[t - 1]) g = dsd[t - 1] * v[t - 1] + dd[t] * v[t];
+        else g = v[t] * dd[t];
+        G[i + (int64_t)(c0 + q) * m] = g;
+      }
+    }
+  }
+}
+
+// FP64 product tiles straight from HBM/L2 into the matrix cores' operand
+// registers (no LDS staging, no barriers): each wave owns a 32x32
+// quadrant, and for every 4-column step lane l loads A[l%16 (+16)][k + l/16]
+// and G[...] in the MFMA operand layout. The next group of GK steps is loaded
+// while the current one multiplies. Same products, same order as k_cmod_prod.
+template <int PR>
+__global__ __launch_bounds__(NT) void k_cmod_prod_reg(DevSym S, DevFac<double> Fc, const double* __restrict__ gpool,
+                                                      const PTile* __restrict__ pt, int64_t npt,
+                                                      void* __restrict__ cbuf) {
+  static_assert(PR == TP_F64, "reduced-precision products use the LDS-staged kernel");
+  typedef typename TileCfg<PR>::S Sx;
+  constexpr int GK = 2;  // 4-column steps per register group
+  const PTile p = pt[xcd_contiguous(blockIdx.x, npt)];
+  const int k = p.k, nck = S.sn_ncol[k];
+  const int64_t hk = S.sn_nrow[k], ldg = hk - nck;
+  const int ma = min(TM, p.m - p.mt * TM), mb = min(TN, p.n - p.nt * TN);
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32, kq = l >> 4;
+  const int ra = wr + (l & 15), rb = wc + (l & 15);
+  const double* pa = Fc.pool + S.panel_off[k] + nck + p.p0 + p.mt * TM + ra;  // A(r, c) = pa[c * hk]
+  const double* pg = gpool + S.g_off[k] + p.p0 + p.nt * TN + rb;              // G(r, c) = pg[c * ldg]
+  const bool a0 = ra < ma, a1 = ra + 16 < ma, b0 = rb < mb, b1 = rb + 16 < mb;
+  TileAcc<PR> acc;
+  tile_zero(acc);
+  Sx A[2][GK][2], B[2][GK][2];
+  auto load = [&](int buf, int c0) {
+#pragma unroll
+    for (int st = 0; st < GK; ++st) {
+      const int c = c0 + 4 * st + kq;
+      const bool ok = c < nck;
+      const int64_t oa = (int64_t)c * hk, og = (int64_t)c * ldg;
+      A[buf][st][0] = (Sx)((ok && a0) ? pa[oa] : 0.0);
+      A[buf][st][1] = (Sx)((ok && a1) ? pa[oa + 16] : 0.0);
+      B[buf][st][0] = (Sx)((ok && b0) ? pg[og] : 0.0);
+      B[buf][st][1] = (Sx)((ok && b1) ? pg[og + 16] : 0.0);
+    }
+  };
+  auto mma = [&](int buf) {
+#pragma unroll
+    for (int st = 0; st < GK; ++st)
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          acc.v[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[buf][st][bi], B[buf][st][bj], acc.v[bi][bj], 0, 0, 0);
+        }
+  };
+  constexpr int CG = 4 * GK;  // columns per group
+  load(0, 0);
+  for (int c0 = 0; c0 < nck; c0 += 2 * CG) {
+    if (c0 + CG < nck) load(1, c0 + CG);
+    mma(0);
+    if (c0 + CG >= nck) break;
+    if (c0 + 2 * CG < nck) load(0, c0 + 2 * CG);
+    mma(1);
+  }
+  const int64_t ldc = p.n, r0 = (int64_t)p.mt * TM, cc0 = (int64_t)p.nt * TN;
+  double* Cp = static_cast<double*>(cbuf) + p.c_off;
+  tile_for_each<PR>(acc, [&](int i, int j, double v) {
+    if (i < ma && j < mb) Cp[(r0 + i) * ldc + cc0 + j] = v;
+  });
+}
+
 template <class CT>
 __global__ __launch_bounds__(NT) void k_cmod_sum(DevSym S, DevFac<double> Fc, const CItem* __restrict__ items,
                                                  const CTask* __restrict__ tasks, const CT* __restrict__ cbuf) {
@@ -1560,15 +1668,31 @@ void launch_cmod(const DevSym& S, const DevFac<T>& Fc, const CItem* items, int n
   }
 }
 
+void launch_make_g(const DevSym& S, const DevFac<double>& Fc, const int32_t* list, int count, int max_w,
+                   double* gpool, hipStream_t st) {
+  if (count <= 0 || max_w <= 0) return;
+  hipLaunchKernelGGL(k_make_g, dim3((unsigned)((max_w + 15) / 16), (unsigned)count), dim3(NT), 0, st, S, Fc, list,
+                     gpool);
+}
+
 void launch_cmod_two_phase(const DevSym& S, const DevFac<double>& Fc, const CItem* items, int nitems,
                            const CTask* tasks, const PTile* ptiles, int64_t nptiles, void* cbuf, int64_t c_elems,
-                           double flops, int prec, hipStream_t st, LaunchProfiler* pf) {
+                           const double* gpool, double flops, int prec, hipStream_t st, LaunchProfiler* pf) {
   if (nitems <= 0 || nptiles <= 0) return;
   const size_t esz = prec == 0 ? sizeof(double) : sizeof(float);
   {
     ProfScope ps(pf, LK_CMOD, flops, 0.0, st);
-    RSL_PREC_SWITCH(double, prec, hipLaunchKernelGGL((k_cmod_prod<P_>), dim3((unsigned)nptiles), dim3(NT), 0, st, S,
-                                                     Fc, ptiles, nptiles, cbuf));
+    // FP64 from registers; the reduced-precision products measured faster
+    // through the LDS-staged kernel
+    if (prec == TP_F64)
+      hipLaunchKernelGGL((k_cmod_prod_reg<TP_F64>), dim3((unsigned)nptiles), dim3(NT), 0, st, S, Fc, gpool, ptiles,
+                         nptiles, cbuf);
+    else if (prec == TP_F32)
+      hipLaunchKernelGGL((k_cmod_prod<TP_F32>), dim3((unsigned)nptiles), dim3(NT), 0, st, S, Fc, ptiles, nptiles,
+                         cbuf);
+    else
+      hipLaunchKernelGGL((k_cmod_prod<TP_BF16>), dim3((unsigned)nptiles), dim3(NT), 0, st, S, Fc, ptiles, nptiles,
+                         cbuf);
   }
   ProfScope ps(pf, LK_CMOD_SUM, 0.0, (double)c_elems * esz, st);
   if (prec == 0)

@@ -24,6 +24,7 @@ enum LaunchKind : int {
   LK_SOLVE_SCATTER,
   LK_RESIDUAL,
   LK_CMOD_SUM,
+  LK_MAKE_G,
   LK_COUNT
 };
 

@@ -88,6 +88,12 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     }
   }
   P.panel_total = poff;
+  P.g_off.assign(ns, 0);
+  P.g_total = 0;
+  for (int32_t s = 0; s < ns; ++s) {
+    P.g_off[s] = P.g_total;
+    P.g_total += (int64_t)(P.sn_nrow[s] - P.sn_ncol[s]) * P.sn_ncol[s];
+  }
 
   // cmod tiles and tasks
   P.items.clear();

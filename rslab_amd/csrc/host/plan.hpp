@@ -47,6 +47,8 @@ struct Plan {
   int32_t n = 0, ns = 0;
   std::vector<int32_t> sn_first, sn_ncol, sn_nrow, e_off, nb_eff;
   std::vector<int64_t> panel_off, rs_off;
+  std::vector<int64_t> g_off;   // offset of supernode s's L21 D block in the G pool ((h - w) x w)
+  int64_t g_total = 0;
   std::vector<int32_t> rs_idx;
   std::vector<int32_t> upd_ptr, upd_idx;
   std::vector<int32_t> level;   // level of each supernode (leaves 0)

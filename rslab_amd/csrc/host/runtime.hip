@@ -24,7 +24,8 @@ void launch_scaled_amax(int64_t, const int32_t*, const int32_t*, const double*, 
                         hipStream_t);
 void launch_row_abs_max(int32_t, const int64_t*, const int64_t*, const double*, unsigned long long*, hipStream_t);
 void launch_cmod_two_phase(const DevSym&, const DevFac<double>&, const CItem*, int, const CTask*, const PTile*,
-                           int64_t, void*, int64_t, double, int, hipStream_t, LaunchProfiler*);
+                           int64_t, void*, int64_t, const double*, double, int, hipStream_t, LaunchProfiler*);
+void launch_make_g(const DevSym&, const DevFac<double>&, const int32_t*, int, int, double*, hipStream_t);
 template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, int, hipStream_t, LaunchProfiler*);
 template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, int, double*, hipStream_t, LaunchProfiler*);
 template <class T> hipError_t configure_factor_kernels();
@@ -56,7 +57,7 @@ const char* launch_kind_name(int k) {
   static const char* names[LK_COUNT] = {"assemble",     "cmod",        "cdiv_small", "panel_getf2",
                                         "panel_replay", "panel_schur", "emit",       "solve_gather",
                                         "fwd_tri",      "fwd_update",  "dsolve",     "bwd_update",
-                                        "bwd_tri",      "solve_scatter", "residual",   "cmod_sum"};
+                                        "bwd_tri",      "solve_scatter", "residual",   "cmod_sum",   "make_g"};
   return (k >= 0 && k < LK_COUNT) ? names[k] : "?";
 }
 
@@ -164,6 +165,7 @@ void DeviceSymbolic::upload() {
   sn_nrow.upload(P.sn_nrow, stream);
   e_off.upload(P.e_off, stream);
   panel_off.upload(P.panel_off, stream);
+  g_off.upload(P.g_off, stream);
   rs_off.upload(P.rs_off, stream);
   rs_idx.upload(P.rs_idx, stream);
   upd_ptr.upload(P.upd_ptr, stream);
@@ -207,6 +209,7 @@ void DeviceSymbolic::upload() {
   view.ns = P.ns;
   view.sn_first = sn_first.as<int32_t>();
   view.sn_ncol = sn_ncol.as<int32_t>();
+  view.g_off = g_off.as<int64_t>();
   view.sn_nrow = sn_nrow.as<int32_t>();
   view.e_off = e_off.as<int32_t>();
   view.panel_off = panel_off.as<int64_t>();
@@ -279,7 +282,7 @@ template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
-      work_y, work_b, work_z, cbuf, rscratch, dev_u64;
+      work_y, work_b, work_z, cbuf, gpool, rscratch, dev_u64;
   hipEvent_t ev_caller = nullptr;
   bool ev_ready = false;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
@@ -320,7 +323,10 @@ class DeviceFactor final : public DeviceFactorBase {
       size_t fr = 0, tot = 0;
       hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
       c_cap = std::min<int64_t>(P.c_elems_max, (int64_t)(fr / 4 / esz));
-      if (c_cap > 0) cbuf.alloc(esz * (size_t)c_cap);
+      if (c_cap > 0) {
+        cbuf.alloc(esz * (size_t)c_cap);
+        gpool.alloc(sizeof(double) * (size_t)std::max<int64_t>(P.g_total, 1));  // L21 D of every supernode
+      }
     }
     // register replay of the wide path: running values of the columns past
     // its first 32, (128 - 32) x REPLAY_ROWS doubles per unit of a step
@@ -453,7 +459,7 @@ class DeviceFactor final : public DeviceFactorBase {
         if (lv.c_elems <= c_cap) {
           launch_cmod_two_phase(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
                                 sym->tasks.as<CTask>(), sym->ptiles.as<PTile>() + lv.ptile_beg,
-                                lv.ptile_end - lv.ptile_beg, cbuf.p, lv.c_elems, lv.cmod_flops, prec, st, pf);
+                                lv.ptile_end - lv.ptile_beg, cbuf.p, lv.c_elems, gpool.as<double>(), lv.cmod_flops, prec, st, pf);
           done = true;
         }
       }
@@ -464,6 +470,14 @@ class DeviceFactor final : public DeviceFactorBase {
                            lv.small_flops, prec, st, pf);
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
                            lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, rscratch.as<double>(), st, pf);
+      if constexpr (std::is_same<T, double>::value) {
+        if (c_cap > 0 && L + 1 < P.levels.size()) {
+          int maxw = 0;
+          for (int32_t sn : lv.all) maxw = std::max(maxw, P.sn_ncol[sn]);
+          ProfScope ps(pf, LK_MAKE_G, 0.0, 0.0, st);
+          launch_make_g(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), maxw, gpool.as<double>(), st);
+        }
+      }
     }
     launch_rows_e<T>(S, fv, P.rs_off[P.ns], st);
     hip_check(hipGetLastError(), "factor launch");
