@@ -1289,22 +1289,23 @@ constexpr int RH = 32;
 // elsewhere; dq[kk] = the pass's D coefficients: {1/d} for a 1x1 column,
 // {d11, d21, d22, 1/det} for the first column of a 2x2 pair.
 constexpr int LPW = NBMAX;
+template <int NTH>
 __device__ void stage_replay_pass(const SnCtx<double>& c, int kb, int pw, int p0, const int32_t* two_s, double* Lp,
                                   double4* dq) {
-  constexpr int SB = 8;
+  constexpr int SB = 2048 / NTH;  // loads in flight per thread
   const double* L = c.P + kb + (int64_t)kb * c.h;
   const int nj = pw - p0, n = RH * LPW;
-  for (int e0 = threadIdx.x; e0 < n; e0 += NT * SB) {
+  for (int e0 = threadIdx.x; e0 < n; e0 += NTH * SB) {
     double v[SB];
 #pragma unroll
     for (int u = 0; u < SB; ++u) {
-      const int e = e0 + u * NT, jj = e % LPW, k = e / LPW, gj = p0 + jj, gk = p0 + k;
+      const int e = e0 + u * NTH, jj = e % LPW, k = e / LPW, gj = p0 + jj, gk = p0 + k;
       const bool in = e < n && jj < nj && gk < pw && gj > gk && !(gj == gk + 1 && two_s[gk]);
       v[u] = in ? L[gj + (int64_t)gk * c.h] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < SB; ++u)
-      if (e0 + u * NT < n) Lp[e0 + u * NT] = v[u];
+      if (e0 + u * NTH < n) Lp[e0 + u * NTH] = v[u];
   }
   if (threadIdx.x < RH) {
     const int g = p0 + threadIdx.x;
@@ -1349,9 +1350,11 @@ __device__ __forceinline__ void replay_pass_store(const SnCtx<double>& c, int kb
   }
 }
 
-// Rows [row0, row0 + nr), nr <= NT. Lp: RH * LPW doubles of LDS, dq: RH
-// entries of LDS; ys: (pw - RH) * NT doubles (scratch for the columns past the
-// first pass; ys[(j - RH) * NT + t] for column j of thread t's row).
+// Rows [row0, row0 + nr), nr <= NTH (the workgroup size). Lp: RH * LPW
+// doubles of LDS, dq: RH entries of LDS; ys: (pw - RH) * NTH doubles (scratch
+// for the columns past the first pass; ys[(j - RH) * NTH + t] for column j of
+// thread t's row).
+template <int NTH>
 __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, int ke, int64_t row0, int nr,
                                                 double* Lp, double4* dq, const int32_t* cperm, const int32_t* two_s,
                                                 double* ys) {
@@ -1362,12 +1365,12 @@ __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, 
   bool first_done = false;  // column p0 was scaled and stored with the previous pass's last one
   for (int p0 = 0; p0 < pw; p0 += RH) {
     const int n = min(RH, pw - p0);
-    stage_replay_pass(c, kb, pw, p0, two_s, Lp, dq);
+    stage_replay_pass<NTH>(c, kb, pw, p0, two_s, Lp, dq);
     double x[RH];
 #pragma unroll
     for (int j = 0; j < RH; ++j) {
       x[j] = 0.0;
-      if (live && j < n) x[j] = p0 == 0 ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : ys[(int64_t)(p0 + j - RH) * NT + tid];
+      if (live && j < n) x[j] = p0 == 0 ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : ys[(int64_t)(p0 + j - RH) * NTH + tid];
     }
     __syncthreads();
 #pragma unroll
@@ -1389,7 +1392,7 @@ __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, 
       for (int u = 0; u < JB; ++u) {
         const int j = jb + u;
         y[u] = 0.0;
-        if (live && j < pw) y[u] = p0 == 0 ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : ys[(int64_t)(j - RH) * NT + tid];
+        if (live && j < pw) y[u] = p0 == 0 ? c.P[row + (int64_t)(kb + cperm[j]) * c.h] : ys[(int64_t)(j - RH) * NTH + tid];
       }
 #pragma unroll
       for (int u = 0; u < JB; ++u) {
@@ -1409,7 +1412,7 @@ __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, 
       if (live) {
 #pragma unroll
         for (int u = 0; u < JB; ++u)
-          if (jb + u < pw) ys[(int64_t)(jb + u - RH) * NT + tid] = y[u];
+          if (jb + u < pw) ys[(int64_t)(jb + u - RH) * NTH + tid] = y[u];
       }
     }
     replay_pass_store(c, kb, p0, n, x, two_s, dq, live, row, first_done, straddle);
@@ -1418,6 +1421,130 @@ __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, 
       c.P[row + (int64_t)(kb + p0 + RH) * c.h] = s_first;
     }
     first_done = straddle;
+    __syncthreads();
+  }
+}
+
+// Deep-row replay of the wide path on matrix cores: one unit is RU = 64 rows
+// and four waves. Per pass of RH columns, wave 0 (one row per lane) runs the
+// pass's triangle in ascending k as replay_rows_reg does, scales and stores
+// the pass's columns; the pass's updates of every later column (the bulk of
+// the flops) are one product  Y -= X_pass * L11(later, pass)^T  on
+// v_mfma_f64_16x16x4f64, with Y held in registers across passes (wave w owns
+// rows 16w..16w+15 of every 16-column tile, in the MFMA result layout). The
+// later-column sums therefore run in the matrix core's order, not term by
+// term; every other operation is the register replay's.
+constexpr int RU = 64;
+constexpr int RQLD = NBMAX + 4;  // Lq row stride (doubles)
+constexpr int RXLD = RU + 4;     // Xq row stride
+struct ReplayMfmaLds {
+  double Lq[RH * RQLD];  // Lq[k * RQLD + j] = L11(j, p0 + k) for j > p0 + k (no 2x2 coupling), else 0
+  double Xq[RH * RXLD];  // the pass's columns: Xq[kk * RXLD + r] (wave 0 writes -x after the triangle)
+  double4 dq[RH];
+};
+
+__device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t row0, int nr,
+                                 ReplayMfmaLds& sh, const int32_t* cperm, const int32_t* two_s) {
+  constexpr int NCT = NBMAX / 16;
+  const int pw = ke - kb, tid = threadIdx.x, l = tid & 63, w = tid >> 6, cl = l & 15;
+  const int rb = 16 * w + (l >> 4);  // unit row of result register r: rb + 4 r
+  const double* L = c.P + kb + (int64_t)kb * c.h;
+  mf_d4 y[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int col = ct * 16 + cl;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rb + 4 * r;
+      y[ct][r] = (col < pw && row < nr) ? c.P[row0 + row + (int64_t)(kb + cperm[col]) * c.h] : 0.0;
+    }
+  }
+  double xprev = 0.0;  // wave 0: unscaled last column of the previous pass
+  double4 qprev = make_double4(1.0, 0.0, 0.0, 1.0);
+  bool first_done = false;
+  const bool live = l < nr;
+  const int64_t row = row0 + l;
+  for (int p0 = 0; p0 < pw; p0 += RH) {
+    const int n = min(RH, pw - p0), pc = p0 / 16;  // first column tile of the pass
+    // stage the pass's L11 columns (rows p0.. of NBMAX, zero outside the strict lower part)
+    {
+      constexpr int SB = 8;
+      const int nj = NBMAX - p0, tot = RH * nj;
+      for (int e0 = tid; e0 < tot; e0 += NT * SB) {
+        double v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int e = e0 + u * NT, k = e / nj, j = p0 + e % nj, gk = p0 + k;
+          const bool in = e < tot && j < pw && gk < pw && j > gk && !(j == gk + 1 && two_s[gk]);
+          v[u] = in ? L[j + (int64_t)gk * c.h] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int e = e0 + u * NT;
+          if (e < tot) sh.Lq[(e / nj) * RQLD + p0 + e % nj] = v[u];
+        }
+      }
+    }
+    if (tid < RH) {
+      const int g = p0 + tid;
+      double4 q = make_double4(1.0, 0.0, 0.0, 1.0);
+      if (g < pw) {
+        if (two_s[g]) {
+          const double d11 = c.d[kb + g], d21 = c.ds[kb + g], d22 = c.d[kb + g + 1];
+          q = make_double4(d11, d21, d22, 1.0 / (d11 * d22 - d21 * d21));
+        } else if (!(g > 0 && two_s[g - 1])) {
+          q.x = 1.0 / c.d[kb + g];
+        }
+      }
+      sh.dq[tid] = q;
+    }
+    // the pass's two column tiles, from their owners' registers
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct)
+      if (ct == pc || ct == pc + 1) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sh.Xq[((ct - pc) * 16 + cl) * RXLD + rb + 4 * r] = y[ct][r];
+      }
+    __syncthreads();
+    const bool straddle = p0 + RH < pw && two_s[p0 + RH - 1];
+    if (w == 0) {
+      double x[RH];
+#pragma unroll
+      for (int kk = 0; kk < RH; ++kk) x[kk] = sh.Xq[kk * RXLD + l];
+      if (first_done && live) {
+        // pair (p0 - 1, p0), both final: scaled and stored together
+        const double wk = xprev, wk1 = x[0];
+        c.P[row + (int64_t)(kb + p0 - 1) * c.h] = (qprev.z * wk - qprev.y * wk1) * qprev.w;
+        c.P[row + (int64_t)(kb + p0) * c.h] = (qprev.x * wk1 - qprev.y * wk) * qprev.w;
+      }
+#pragma unroll
+      for (int k = 0; k < RH - 1; ++k) {
+        if (k + 1 < n) {
+          const double xk = x[k];
+#pragma unroll
+          for (int j = k + 1; j < RH; ++j) x[j] = x[j] - xk * sh.Lq[k * RQLD + p0 + j];
+        }
+      }
+      xprev = x[RH - 1];
+      qprev = sh.dq[RH - 1];
+#pragma unroll
+      for (int kk = 0; kk < RH; ++kk) sh.Xq[kk * RXLD + l] = -x[kk];
+      replay_pass_store(c, kb, p0, n, x, two_s, sh.dq, live, row, first_done, straddle);
+    }
+    first_done = straddle;
+    __syncthreads();
+    if (p0 + RH < pw) {
+#pragma unroll
+      for (int kk = 0; kk < RH; kk += 4) {
+        const double a = sh.Xq[(kk + (l >> 4)) * RXLD + 16 * w + cl];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          if (ct >= pc + 2 && ct * 16 < pw) {
+            const double b = sh.Lq[(kk + (l >> 4)) * RQLD + ct * 16 + cl];
+            y[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, y[ct], 0, 0, 0);
+          }
+      }
+    }
     __syncthreads();
   }
 }
@@ -1467,7 +1594,7 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
       if (ke - kb <= 2 * RH) {
         double* Lp = reinterpret_cast<double*>(sh.u.rp);
         for (int64_t row0 = ke; row0 < c.h; row0 += NT)
-          replay_rows_reg(c, kb, ke, row0, (int)min<int64_t>(NT, c.h - row0), Lp,
+          replay_rows_reg<NT>(c, kb, ke, row0, (int)min<int64_t>(NT, c.h - row0), Lp,
                           reinterpret_cast<double4*>(Lp + RH * LPW), sh.cperm_s, sh.two_s, Lp + RH * LPW + 4 * RH);
         done = true;
       }
@@ -1565,16 +1692,15 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
   const int ke = min(kb + c.nb, c.w), pw = ke - kb;
   const int64_t row0 = ke + (int64_t)it.a;
   const int nr = (int)min<int64_t>(REPLAY_ROWS, c.h - row0);
-  for (int j = threadIdx.x; j < pw; j += NT) {
+  for (int j = threadIdx.x; j < pw; j += blockDim.x) {
     cperm[j] = c.deep[NBMAX + j];
     two_s[j] = c.two[kb + j];
   }
   __syncthreads();
   if constexpr (std::is_same<T, double>::value) {
-    static_assert(REPLAY_ROWS == NT, "one thread per deep row");
-    double* Lp = reinterpret_cast<double*>(smem_raw);
-    replay_rows_reg(c, kb, ke, row0, nr, Lp, reinterpret_cast<double4*>(Lp + RH * LPW), cperm, two_s,
-                    scratch + (int64_t)blockIdx.x * (NBMAX - RH) * REPLAY_ROWS);
+    static_assert(REPLAY_ROWS == RU && RH == 32, "one MFMA unit per work item, two column tiles per pass");
+    replay_unit_mfma(c, kb, ke, row0, nr, *reinterpret_cast<ReplayMfmaLds*>(smem_raw), cperm, two_s);
+    (void)scratch;
   } else {
     T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
     T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
@@ -1585,8 +1711,7 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
 
 template <class T>
 constexpr size_t replay_smem() {
-  return std::is_same<T, double>::value ? sizeof(double) * (RH * LPW + 4 * RH)
-                                        : sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX;
+  return std::is_same<T, double>::value ? sizeof(ReplayMfmaLds) : sizeof(T) * (RR + replay_chunk<T>() + 1) * NBMAX;
 }
 
 template <class T, int PR>
