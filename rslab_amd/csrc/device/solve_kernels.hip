@@ -483,126 +483,17 @@ __global__ __launch_bounds__(SNT) void k_bwd_tri(DevSym S, DevFac<T> Fc,
 // backward (push form) to an earlier own block. z is copied back to y once
 // the level's steps are done.
 
-// sb[p * RCH + k] = block j0 of y solved forward, right-hand sides c0 + k
-template <class T>
-__device__ void fwd_block_solve_to(const T* Ld, const T* __restrict__ y, int eo, int j0, int nbk, int32_t nrhs,
-                                   int c0, int nc, T* sb) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t row = eo + j0 + lane;
-  for (int k = wv; k < nc; k += 2 * NWV) {
-    const int k2 = k + NWV;
-    const bool two_rhs = k2 < nc;
-    T y0 = lane < nbk ? y[row * nrhs + c0 + k] : F<T>::zero();
-    T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c0 + k2] : F<T>::zero();
-#pragma unroll 8
-    for (int p = 0; p + 1 < nbk; ++p) {
-      const T l = Ld[p * TB + lane];
-      const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
-      y0 = y0 - l * a0;
-      y1 = y1 - l * a1;
-    }
-    sb[lane * RCH + k] = y0;
-    if (two_rhs) sb[lane * RCH + k2] = y1;
-  }
-}
-
-// backward (L^T of the block, Lt from stage_block_upper) into sb
-template <class T>
-__device__ void bwd_block_solve_to(const T* Lt, const T* __restrict__ y, int eo, int j0, int nbk, int32_t nrhs,
-                                   int c0, int nc, T* sb) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t row = eo + j0 + lane;
-  for (int k = wv; k < nc; k += 2 * NWV) {
-    const int k2 = k + NWV;
-    const bool two_rhs = k2 < nc;
-    T y0 = lane < nbk ? y[row * nrhs + c0 + k] : F<T>::zero();
-    T y1 = (two_rhs && lane < nbk) ? y[row * nrhs + c0 + k2] : F<T>::zero();
-#pragma unroll 8
-    for (int p = nbk - 1; p > 0; --p) {
-      const T l = Lt[p * (TB + 1) + lane];
-      const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
-      y0 = y0 - l * a0;
-      y1 = y1 - l * a1;
-    }
-    sb[lane * RCH + k] = y0;
-    if (two_rhs) sb[lane * RCH + k2] = y1;
-  }
-}
-
-// red[r][k] = sum_{p < nbk} L[row0 + r][j0 + p] * sb[p][k], the same
-// arithmetic as rows_times_y over columns [j0, j0 + nbk) (32-column batches,
-// thread (r, q) on columns q + 4m, fixed-order sum over q)
-template <class T>
-__device__ void rows_times_blk(const T* P, int64_t h, int64_t row0, int nr, int j0, int nbk, const T* sb, int zr,
-                               int zc, T* red) {
-  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
-  constexpr int CB = 32;
-  T acc[RCH];
-#pragma unroll
-  for (int k = 0; k < RCH; ++k) acc[k] = F<T>::zero();
-  for (int pb = 0; pb < nbk; pb += CB) {
-    T l[CB / NWV];
-#pragma unroll
-    for (int m = 0; m < CB / NWV; ++m) {
-      const int p = pb + q + NWV * m;
-      l[m] = (r < nr && p < nbk) ? P[(row0 + r) + (int64_t)(j0 + p) * h] : F<T>::zero();
-      if (r == zr && j0 + p == zc) l[m] = F<T>::zero();
-    }
-#pragma unroll
-    for (int m = 0; m < CB / NWV; ++m) {
-      const int p = pb + q + NWV * m;
-      if (p < nbk) {
-        const T* yp = sb + p * RCH;
-#pragma unroll
-        for (int k = 0; k < RCH; ++k) acc[k] = acc[k] + l[m] * yp[k];
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < RCH; ++k) red[(q * 64 + r) * RCH + k] = acc[k];
-  __syncthreads();
-  for (int t = threadIdx.x; t < 64 * RCH; t += SNT) {
-    T v = red[t];
-    v = v + red[64 * RCH + t];
-    v = v + red[2 * 64 * RCH + t];
-    v = v + red[3 * 64 * RCH + t];
-    red[t] = v;
-  }
-  __syncthreads();
-}
-
-// out[p][k] = sum_{i < nr} L[r0 + i][i0 + p] * sb[i][k] (64 columns of block
-// i0 against the solved rows r0..r0+nr), as chunk_t_times_x with x from LDS
-template <class T>
-__device__ void blk_t_times_sb(const T* P, int64_t h, const uint8_t* two, int i0, int r0, int nr, const T* sb,
-                               T* Ls, T* out) {
-  constexpr int PER = TB * TB / SNT;
-  T v[PER];
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int t = threadIdx.x + u * SNT, i = t % TB, p = t / TB;
-    v[u] = i < nr ? P[(int64_t)(r0 + i) + (int64_t)(i0 + p) * h] : F<T>::zero();
-  }
-  // a 2x2 coupling entry (i0+64, i0+63) when the blocks are adjacent
-  const bool zc = r0 == i0 + TB && two[i0 + TB - 1];
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int t = threadIdx.x + u * SNT, i = t % TB, p = t / TB;
-    Ls[i * (TB + 1) + p] = (zc && i == 0 && p == TB - 1) ? F<T>::zero() : v[u];
-  }
-  __syncthreads();
-  const int p = threadIdx.x & 63, kq = threadIdx.x >> 6;
-  T a0 = F<T>::zero(), a1 = F<T>::zero();
-#pragma unroll 8
-  for (int i = 0; i < TB; ++i) {
-    const T l = Ls[i * (TB + 1) + p];
-    a0 = a0 + l * sb[i * RCH + kq];
-    a1 = a1 + l * sb[i * RCH + kq + NWV];
-  }
-  out[p * RCH + kq] = a0;
-  out[p * RCH + kq + NWV] = a1;
-  __syncthreads();
-}
+// A step issues every global load of a right-hand-side chunk (the diagonal
+// block, its y entries, the unit's 64 x 64 L operand and the y entries it
+// updates) before using any of them, so one memory latency covers the step
+// instead of one per phase. Arithmetic per unit: the block solve is
+// fwd/bwd_block_solve's (lane i holds y_i, one pair of right-hand sides per
+// wave); a forward row unit sums L[r][j0 + p] * y_p over 32-column batches
+// (thread (r, q) on columns q + 4m, then a fixed-order sum over q), as
+// rows_times_y; a backward push unit sums L[j0 + i][i0 + p] * x_i over the
+// block's 64 rows in order, as chunk_t_times_x.
+static_assert(RCH == 2 * NWV, "one pair of right-hand sides per wave and chunk");
+constexpr int TPER = TB * TB / SNT;  // 64 x 64 block entries per thread
 
 // forward step: units [0, nu) are row units (s, j0, r0), the rest diagonal units
 template <class T>
@@ -619,28 +510,88 @@ __global__ __launch_bounds__(SNT) void k_ftri_step(DevSym S, DevFac<T> Fc, const
   const T* P = Fc.pool + S.panel_off[s];
   const uint8_t* two = Fc.two + eo;
   const int nbk = min(TB, w - j0), rb = j0 + nbk;
-  stage_block_lower(P, h, two, j0, nbk, Ld);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // the block's strict lower part (stage_block_lower's entries)
+  T vst[TPER];
+#pragma unroll
+  for (int u = 0; u < TPER; ++u) {
+    const int t = tid + u * SNT, i = t % TB, p = t / TB;
+    const bool in = p < nbk && i < nbk && i > p && !(i == p + 1 && two[j0 + p]);
+    vst[u] = in ? P[(int64_t)(j0 + i) + (int64_t)(j0 + p) * h] : F<T>::zero();
+  }
+  // row unit: L[r0 + r][j0 + p] for r = lane, p = pb + wv + NWV m (rows_times_blk's operand)
+  const int r0 = diag ? 0 : it.r0, nr = diag ? 0 : min(64, w - r0);
+  const int zr = (!diag && r0 == rb && two[rb - 1]) ? 0 : -1;
+  T lr[TPER];
+#pragma unroll
+  for (int u = 0; u < TPER; ++u) {
+    const int p = (u / 8) * 32 + wv + NWV * (u % 8);
+    lr[u] = (lane < nr && p < nbk) ? P[(int64_t)(r0 + lane) + (int64_t)(j0 + p) * h] : F<T>::zero();
+    if (lane == zr && j0 + p == rb - 1) lr[u] = F<T>::zero();
+  }
   for (int c0 = 0; c0 < nrhs; c0 += RCH) {
     const int nc = min(RCH, nrhs - c0);
-    for (int t = threadIdx.x; t < TB * RCH; t += SNT) sb[t] = F<T>::zero();
+    const int k = wv, k2 = wv + NWV;
+    const int64_t brow = eo + j0 + lane;
+    T y0 = (k < nc && lane < nbk) ? y[brow * nrhs + c0 + k] : F<T>::zero();
+    T y1 = (k2 < nc && lane < nbk) ? y[brow * nrhs + c0 + k2] : F<T>::zero();
+    T yr[RCH * 64 / SNT];
+#pragma unroll
+    for (int u = 0; u < RCH * 64 / SNT; ++u) {
+      const int t = tid + u * SNT, rr = t / RCH, kk = t % RCH;
+      yr[u] = (rr < nr && kk < nc) ? y[(int64_t)(eo + r0 + rr) * nrhs + c0 + kk] : F<T>::zero();
+    }
+    if (c0 == 0) {
+#pragma unroll
+      for (int u = 0; u < TPER; ++u) Ld[tid + u * SNT] = vst[u];
+    }
+    for (int t = tid; t < TB * RCH; t += SNT) sb[t] = F<T>::zero();
     __syncthreads();
-    fwd_block_solve_to(Ld, y, eo, j0, nbk, nrhs, c0, nc, sb);
+    if (k < nc) {
+#pragma unroll 8
+      for (int p = 0; p + 1 < nbk; ++p) {
+        const T l = Ld[p * TB + lane];
+        const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
+        y0 = y0 - l * a0;
+        y1 = y1 - l * a1;
+      }
+      sb[lane * RCH + k] = y0;
+      if (k2 < nc) sb[lane * RCH + k2] = y1;
+    }
     __syncthreads();
     if (diag) {
-      for (int t = threadIdx.x; t < nbk * RCH; t += SNT) {
-        const int p = t / RCH, k = t % RCH;
-        if (k < nc) z[(int64_t)(eo + j0 + p) * nrhs + c0 + k] = sb[t];
+      for (int t = tid; t < nbk * RCH; t += SNT) {
+        const int p = t / RCH, kk = t % RCH;
+        if (kk < nc) z[(int64_t)(eo + j0 + p) * nrhs + c0 + kk] = sb[t];
       }
     } else {
-      const int r0 = it.r0, nr = min(64, w - r0);
-      const int zr = (r0 == rb && two[rb - 1]) ? 0 : -1;
-      rows_times_blk(P, h, r0, nr, j0, nbk, sb, zr, rb - 1, red);
-      for (int t = threadIdx.x; t < nr * RCH; t += SNT) {
-        const int rr = t / RCH, k = t % RCH;
-        if (k < nc) {
-          const int64_t o = (int64_t)(eo + r0 + rr) * nrhs + c0 + k;
-          y[o] = y[o] - red[rr * RCH + k];
+      T acc[RCH];
+#pragma unroll
+      for (int kk = 0; kk < RCH; ++kk) acc[kk] = F<T>::zero();
+#pragma unroll
+      for (int u = 0; u < TPER; ++u) {
+        const int pb = (u / 8) * 32, p = pb + wv + NWV * (u % 8);
+        if (pb < nbk && p < nbk) {
+          const T* yp = sb + p * RCH;
+#pragma unroll
+          for (int kk = 0; kk < RCH; ++kk) acc[kk] = acc[kk] + lr[u] * yp[kk];
         }
+      }
+#pragma unroll
+      for (int kk = 0; kk < RCH; ++kk) red[(wv * 64 + lane) * RCH + kk] = acc[kk];
+      __syncthreads();
+      for (int t = tid; t < 64 * RCH; t += SNT) {
+        T v = red[t];
+        v = v + red[64 * RCH + t];
+        v = v + red[2 * 64 * RCH + t];
+        v = v + red[3 * 64 * RCH + t];
+        red[t] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < RCH * 64 / SNT; ++u) {
+        const int t = tid + u * SNT, rr = t / RCH, kk = t % RCH;
+        if (rr < nr && kk < nc) y[(int64_t)(eo + r0 + rr) * nrhs + c0 + kk] = yr[u] - red[rr * RCH + kk];
       }
     }
     __syncthreads();
@@ -663,27 +614,84 @@ __global__ __launch_bounds__(SNT) void k_btri_step(DevSym S, DevFac<T> Fc, const
   const T* P = Fc.pool + S.panel_off[s];
   const uint8_t* two = Fc.two + eo;
   const int nbk = min(TB, w - j0);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // the block's transposed strict lower part (stage_block_upper's entries)
+  T vst[TPER];
+#pragma unroll
+  for (int u = 0; u < TPER; ++u) {
+    const int t = tid + u * SNT, p = t % TB, i = t / TB;
+    const bool in = p < nbk && i < nbk && p > i && !(p == i + 1 && two[j0 + i]);
+    vst[u] = in ? P[(int64_t)(j0 + p) + (int64_t)(j0 + i) * h] : F<T>::zero();
+  }
+  // push unit: L[j0 + i][i0 + p] (blk_t_times_sb's operand, rows j0.. of block i0's columns)
+  const int i0 = diag ? 0 : it.r0;
+  const bool zc = !diag && j0 == i0 + TB && two[i0 + TB - 1];
+  T vp[TPER];
+#pragma unroll
+  for (int u = 0; u < TPER; ++u) {
+    const int t = tid + u * SNT, i = t % TB, p = t / TB;
+    vp[u] = (!diag && i < nbk) ? P[(int64_t)(j0 + i) + (int64_t)(i0 + p) * h] : F<T>::zero();
+    if (zc && i == 0 && p == TB - 1) vp[u] = F<T>::zero();
+  }
   for (int c0 = 0; c0 < nrhs; c0 += RCH) {
     const int nc = min(RCH, nrhs - c0);
-    for (int t = threadIdx.x; t < TB * RCH; t += SNT) sb[t] = F<T>::zero();
-    stage_block_upper(P, h, two, j0, nbk, Lt);
-    bwd_block_solve_to(Lt, y, eo, j0, nbk, nrhs, c0, nc, sb);
+    const int k = wv, k2 = wv + NWV;
+    const int64_t brow = eo + j0 + lane;
+    T y0 = (k < nc && lane < nbk) ? y[brow * nrhs + c0 + k] : F<T>::zero();
+    T y1 = (k2 < nc && lane < nbk) ? y[brow * nrhs + c0 + k2] : F<T>::zero();
+    T yt[RCH * TB / SNT];
+#pragma unroll
+    for (int u = 0; u < RCH * TB / SNT; ++u) {
+      const int t = tid + u * SNT, p = t / RCH, kk = t % RCH;
+      yt[u] = (!diag && kk < nc) ? y[(int64_t)(eo + i0 + p) * nrhs + c0 + kk] : F<T>::zero();
+    }
+#pragma unroll
+    for (int u = 0; u < TPER; ++u) {
+      const int t = tid + u * SNT, p = t % TB, i = t / TB;
+      Lt[p * (TB + 1) + i] = vst[u];
+    }
+    for (int t = tid; t < TB * RCH; t += SNT) sb[t] = F<T>::zero();
+    __syncthreads();
+    if (k < nc) {
+#pragma unroll 8
+      for (int p = nbk - 1; p > 0; --p) {
+        const T l = Lt[p * (TB + 1) + lane];
+        const T a0 = lane_bcast(y0, p), a1 = lane_bcast(y1, p);
+        y0 = y0 - l * a0;
+        y1 = y1 - l * a1;
+      }
+      sb[lane * RCH + k] = y0;
+      if (k2 < nc) sb[lane * RCH + k2] = y1;
+    }
     __syncthreads();
     if (diag) {
-      for (int t = threadIdx.x; t < nbk * RCH; t += SNT) {
-        const int p = t / RCH, k = t % RCH;
-        if (k < nc) z[(int64_t)(eo + j0 + p) * nrhs + c0 + k] = sb[t];
+      for (int t = tid; t < nbk * RCH; t += SNT) {
+        const int p = t / RCH, kk = t % RCH;
+        if (kk < nc) z[(int64_t)(eo + j0 + p) * nrhs + c0 + kk] = sb[t];
       }
       __syncthreads();
     } else {
-      const int i0 = it.r0;
-      blk_t_times_sb(P, h, two, i0, j0, nbk, sb, Lt, out);
-      for (int t = threadIdx.x; t < TB * RCH; t += SNT) {
-        const int p = t / RCH, k = t % RCH;
-        if (k < nc) {
-          const int64_t o = (int64_t)(eo + i0 + p) * nrhs + c0 + k;
-          y[o] = y[o] - out[t];
-        }
+#pragma unroll
+      for (int u = 0; u < TPER; ++u) {
+        const int t = tid + u * SNT, i = t % TB, p = t / TB;
+        Lt[i * (TB + 1) + p] = vp[u];
+      }
+      __syncthreads();
+      const int p = lane, kq = wv;
+      T a0 = F<T>::zero(), a1 = F<T>::zero();
+#pragma unroll 8
+      for (int i = 0; i < TB; ++i) {
+        const T l = Lt[i * (TB + 1) + p];
+        a0 = a0 + l * sb[i * RCH + kq];
+        a1 = a1 + l * sb[i * RCH + kq + NWV];
+      }
+      out[p * RCH + kq] = a0;
+      out[p * RCH + kq + NWV] = a1;
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < RCH * TB / SNT; ++u) {
+        const int t = tid + u * SNT, pp = t / RCH, kk = t % RCH;
+        if (kk < nc) y[(int64_t)(eo + i0 + pp) * nrhs + c0 + kk] = yt[u] - out[t];
       }
       __syncthreads();
     }
