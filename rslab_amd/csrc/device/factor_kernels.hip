@@ -769,6 +769,66 @@ __device__ inline void packed_trailing_update_1x1(T* Ab, int pw, int k, T dinv, 
   next_column_candidates<T>(k0, pw, cand, cm_s, ci_s);
 }
 
+// Static ownership of the packed block for the 1x1 trailing update: the
+// strict lower part is cut into row strips (row i, columns 4 jb .. 4 jb + 3,
+// i >= 4 jb), ordered by jb then i and dealt to the threads cyclically, so the
+// strips still live at step k (row and some column past k) stay spread over
+// all threads. A thread computes its strips' packed offsets once per panel
+// and per step needs no index arithmetic beyond a few adds.
+template <int NTG>
+struct PanelStrips {
+  // at most NBMAX^2 / 8 + NBMAX / 2 strips (pw = NBMAX)
+  static constexpr int MAXS = (NBMAX * NBMAX / 8 + NBMAX / 2 + NTG - 1) / NTG;
+  int i[MAXS], j0[MAXS], o[MAXS];  // row, first column, packed offset of (i, j0); i = -1: none
+  __device__ void init(int pw) {
+    // strips of column block jb: rows 4 jb .. pw - 1; S(jb) strips before it
+    const int ns = (pw + 3) / 4 * pw - 2 * ((pw + 3) / 4) * ((pw + 3) / 4 - 1);
+#pragma unroll
+    for (int q = 0; q < MAXS; ++q) {
+      const int sidx = threadIdx.x + q * NTG;
+      i[q] = -1;
+      j0[q] = 0;
+      o[q] = 0;
+      if (sidx < ns) {
+        // largest jb with S(jb) = jb pw - 2 jb (jb - 1) <= sidx
+        const float b = (float)(pw + 2);
+        int jb = (int)((b - sqrtf(b * b - 8.0f * (float)sidx)) * 0.25f);
+        if (jb * pw - 2 * jb * (jb - 1) > sidx) --jb;
+        if ((jb + 1) * pw - 2 * (jb + 1) * jb <= sidx) ++jb;
+        const int row = 4 * jb + (sidx - (jb * pw - 2 * jb * (jb - 1)));
+        i[q] = row;
+        j0[q] = 4 * jb;
+        o[q] = pk_off(row, 4 * jb, pw);
+      }
+    }
+  }
+};
+
+// The 1x1 step's update (packed_trailing_update_1x1's arithmetic: entry (i,
+// j) -= A(i, k) * (A(j, k) * dinv), skipped for a zero multiplier) over the
+// thread's strips.
+template <int NTG, class T>
+__device__ inline void strips_update_1x1(T* Ab, int pw, int k, T dinv, const PanelStrips<NTG>& st) {
+  const T* colk = Ab + pk_off(k, k, pw) - k;  // colk[i] = A(i, k)
+#pragma unroll
+  for (int q = 0; q < PanelStrips<NTG>::MAXS; ++q) {
+    const int i = st.i[q], j0 = st.j0[q];
+    if (i > k && j0 + 3 > k) {
+      const T a = colk[i];
+      int o = st.o[q];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int j = j0 + jj;
+        if (j > k && j <= i) {
+          const T mu = colk[j] * dinv;
+          if (!F<T>::is_zero(mu)) Ab[o] = Ab[o] - a * mu;
+        }
+        o += pw - j - 1;
+      }
+    }
+  }
+}
+
 // Bunch-Kaufman on the diagonal block [kb,ke) held in LDS (packed lower).
 // Candidates, interchanges and updates bounded to the panel rows, as
 // ll_cdiv_emit's getf2 (multifrontal_ldlt.rs:2846-3016). Returns 1 when the
@@ -788,6 +848,8 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
   }
   __shared__ double cand_m[2];
   __shared__ int cand_i[2];
+  PanelStrips<NTG> strips;
+  strips.init(pw);
   __syncthreads();
   int k = 0;
   bool have_cand = false;  // the previous step's update left this column's max
@@ -898,8 +960,8 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       // which is replaced by its multipliers once every thread is past the
       // update (the next step reads column k again only after its first
       // barrier, and scaling a column commutes with its interchanges)
-      packed_trailing_update_1x1<NTG>(Ab, pw, k, dinv, cand_m, cand_i);
-      have_cand = true;
+      strips_update_1x1<NTG>(Ab, pw, k, dinv, strips);
+      have_cand = false;
       // column k keeps its unscaled entries until the block is written back
       // (panel_store multiplies them by w1s[k] = 1/d_k then; the later
       // interchanges only move entries within finished columns)
@@ -1620,6 +1682,7 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
 template <class T>
 __global__ __launch_bounds__(GT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
                                                    PivotPolicy pol) {
+  constexpr int G = GT;
   extern __shared__ unsigned char smem_raw[];
   CdivShared<T>& sh = *reinterpret_cast<CdivShared<T>*>(smem_raw);
   const WItem it = items[blockIdx.x];
@@ -1627,16 +1690,16 @@ __global__ __launch_bounds__(GT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const
   SnCtx<T> c = make_ctx(S, Fc, s, S.nb_eff[s]);
   if (Fc.status[0]) return;  // the factorization has already failed
   if (threadIdx.x == 0) sh.npert = 0;
-  if (kb == 0) cdiv_init<GT>(c);
+  if (kb == 0) cdiv_init<G>(c);
   __syncthreads();
   const int ke = min(kb + c.nb, c.w);
-  panel_load<GT>(c, kb, ke, sh.u.Ab);
-  int st = panel_getf2<GT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert);
+  panel_load<G>(c, kb, ke, sh.u.Ab);
+  int st = panel_getf2<G>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert);
   if (st) {
     flag_failure(Fc, s);
     return;
   }
-  panel_store<GT, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap, sh.w1s);
+  panel_store<G, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap, sh.w1s);
   if (threadIdx.x == 0 && sh.npert) atomicAdd(&Fc.counters[3], (unsigned long long)sh.npert);
 }
 
