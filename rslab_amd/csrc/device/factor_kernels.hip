@@ -774,22 +774,23 @@ __device__ inline void packed_trailing_update_1x1(T* Ab, int pw, int k, T dinv, 
 // i >= 4 jb), ordered by jb then i and dealt to the threads cyclically, so the
 // strips still live at step k (row and some column past k) stay spread over
 // all threads. A thread computes its strips' packed offsets once per panel
-// and per step needs no index arithmetic beyond a few adds.
-template <int NTG>
+// and per step needs no index arithmetic beyond a few adds. Threads below T0
+// own no strips.
+template <int NTG, int T0>
 struct PanelStrips {
   // at most NBMAX^2 / 8 + NBMAX / 2 strips (pw = NBMAX)
-  static constexpr int MAXS = (NBMAX * NBMAX / 8 + NBMAX / 2 + NTG - 1) / NTG;
+  static constexpr int MAXS = (NBMAX * NBMAX / 8 + NBMAX / 2 + (NTG - T0) - 1) / (NTG - T0);
   int i[MAXS], j0[MAXS], o[MAXS];  // row, first column, packed offset of (i, j0); i = -1: none
   __device__ void init(int pw) {
     // strips of column block jb: rows 4 jb .. pw - 1; S(jb) strips before it
     const int ns = (pw + 3) / 4 * pw - 2 * ((pw + 3) / 4) * ((pw + 3) / 4 - 1);
 #pragma unroll
     for (int q = 0; q < MAXS; ++q) {
-      const int sidx = threadIdx.x + q * NTG;
+      const int sidx = ((int)threadIdx.x - T0) + q * (NTG - T0);
       i[q] = -1;
       j0[q] = 0;
       o[q] = 0;
-      if (sidx < ns) {
+      if ((int)threadIdx.x >= T0 && sidx < ns) {
         // largest jb with S(jb) = jb pw - 2 jb (jb - 1) <= sidx
         const float b = (float)(pw + 2);
         int jb = (int)((b - sqrtf(b * b - 8.0f * (float)sidx)) * 0.25f);
@@ -806,26 +807,67 @@ struct PanelStrips {
 
 // The 1x1 step's update (packed_trailing_update_1x1's arithmetic: entry (i,
 // j) -= A(i, k) * (A(j, k) * dinv), skipped for a zero multiplier) over the
-// thread's strips.
-template <int NTG, class T>
-__device__ inline void strips_update_1x1(T* Ab, int pw, int k, T dinv, const PanelStrips<NTG>& st) {
+// thread's strips, columns jmin.. (k + 1, or k + 2 when column k + 1 is
+// updated separately).
+template <int NTG, int T0, class T>
+__device__ inline void strips_update_1x1(T* Ab, int pw, int k, int jmin, T dinv, const PanelStrips<NTG, T0>& st) {
   const T* colk = Ab + pk_off(k, k, pw) - k;  // colk[i] = A(i, k)
 #pragma unroll
-  for (int q = 0; q < PanelStrips<NTG>::MAXS; ++q) {
+  for (int q = 0; q < PanelStrips<NTG, T0>::MAXS; ++q) {
     const int i = st.i[q], j0 = st.j0[q];
-    if (i > k && j0 + 3 > k) {
+    if (i >= jmin && j0 + 3 >= jmin) {
       const T a = colk[i];
       int o = st.o[q];
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
         const int j = j0 + jj;
-        if (j > k && j <= i) {
+        if (j >= jmin && j <= i) {
           const T mu = colk[j] * dinv;
           if (!F<T>::is_zero(mu)) Ab[o] = Ab[o] - a * mu;
         }
         o += pw - j - 1;
       }
     }
+  }
+}
+
+// Wave 0's share of a pipelined 1x1 step (FP64): column k0 = k + 1 brought up
+// to date with the same arithmetic (every lane forms the diagonal entry, lane
+// 0 stores it; lane l takes rows k0 + 1 + l + 64 q), then the next step's
+// pivot search on it — the column scan, its argmax, sqrt and 1 / A(k0, k0) —
+// published in dec (absakk, colmax, akk, 1/akk) and deci (imax) for every wave
+// to read after the step's closing barrier.
+__device__ inline void pivot_column_step(double* Ab, int pw, int k, double dinv, double* dec, int* deci) {
+  const int lane = threadIdx.x & 63, k0 = k + 1;
+  const double* colk = Ab + pk_off(k, k, pw) - k;
+  const double mu = colk[k0] * dinv;
+  const bool upd = !F<double>::is_zero(mu);
+  const int od = pk_off(k0, k0, pw);
+  double akk = Ab[od];
+  if (upd) akk = akk - colk[k0] * mu;
+  double cm = 0.0;
+  int imax = k0;
+  for (int i = k0 + 1 + lane; i < pw; i += 64) {
+    const int o = od + (i - k0);
+    double v = Ab[o];
+    if (upd) {
+      v = v - colk[i] * mu;
+      Ab[o] = v;
+    }
+    const double m = F<double>::magsq(v);
+    if (m > cm) {
+      cm = m;
+      imax = i;
+    }
+  }
+  wave_argmax_dpp(cm, imax, k0 + 1);
+  if (lane == 0) {
+    if (upd) Ab[od] = akk;
+    dec[0] = F<double>::mag(akk);
+    dec[1] = sqrt(cm);
+    dec[2] = akk;
+    dec[3] = F<double>::recip(akk);
+    deci[0] = imax;
   }
 }
 
@@ -848,16 +890,36 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
   }
   __shared__ double cand_m[2];
   __shared__ int cand_i[2];
-  PanelStrips<NTG> strips;
+  // Pipelined 1x1 steps (FP64, 512 threads): wave 0 updates the next pivot
+  // column and runs the next search while waves 1.. update the strips; the
+  // search result is double-buffered by step parity.
+  constexpr bool PIPE = NTG >= 512 && std::is_same<T, double>::value;
+  constexpr int T0 = PIPE ? 64 : 0;
+  __shared__ double dec[2][4];
+  __shared__ int deci[2];
+  PanelStrips<NTG, T0> strips;
   strips.init(pw);
   __syncthreads();
-  int k = 0;
+  int k = 0, par = 0;
   bool have_cand = false;  // the previous step's update left this column's max
+  bool have_dec = false;   // the previous step published this step's search
   while (k < pw) {
     // ---- pivot search (every wave computes the same answer) ----
-    const double absakk = F<T>::mag(A_(k, k));
-    double cm = 0.0;
+    const bool pre = PIPE && have_dec;
+    T akk = F<T>::zero(), dinv_pre = F<T>::zero();
+    double absakk, colmax;
     int imax = k;
+    if (pre) {
+      if constexpr (PIPE) {
+        absakk = dec[par][0];
+        colmax = dec[par][1];
+        akk = dec[par][2];
+        dinv_pre = dec[par][3];
+        imax = deci[par];
+      }
+    } else {
+    absakk = F<T>::mag(A_(k, k));
+    double cm = 0.0;
     if (have_cand) {
       // wave 0 holds the rows below wave 1's: it wins ties
       const double c0 = cand_m[0], c1 = cand_m[1];
@@ -878,17 +940,22 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       }
       wave_argmax_dpp(cm, imax, k + 1);  // pw <= 128: at most two candidate rows per lane
     }
-    const double colmax = sqrt(cm);
+    colmax = sqrt(cm);
+    }
     // decision below reads row imax; every wave must finish reading before
-    // any wave starts the interchange / update of this step
+    // any wave starts the interchange / update of this step (a pipelined
+    // plain 1x1 step reads nothing the update writes)
     int kstep, kp;
+    bool plain = false;
     if (fmax(absakk, colmax) == 0.0) {
       if (!pol.have_floor) return 1;
       kstep = 1;
       kp = k;
+      plain = true;
     } else if (absakk >= alpha * colmax) {
       kstep = 1;
       kp = k;
+      plain = true;
     } else {
       double rm = 0.0;
       for (int j = k + lane; j < imax; j += 64) rm = fmax(rm, F<T>::magsq(A_(imax, j)));
@@ -906,7 +973,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         kp = imax;
       }
     }
-    __syncthreads();
+    if (!(pre && plain)) __syncthreads();
     // ---- symmetric interchange of p < q inside the block ----
     const int p = (kstep == 1) ? k : k + 1;
     if (kp != p) {
@@ -942,10 +1009,14 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       __syncthreads();
     }
     if (kstep == 1) {
-      T dk = A_(k, k);
+      // a pipelined step without interchange takes A(k, k) as published
+      const bool use_pre = pre && kp == p;
+      T dk = use_pre ? akk : A_(k, k);
+      bool fresh = !use_pre;
       if (pol.have_floor) {
         if (F<T>::mag(dk) < pol.floor) {
           dk = perturb_pivot(dk, pol.floor);
+          fresh = true;
           if (tid == 0) {
             A_(k, k) = dk;
             (*npert)++;
@@ -955,12 +1026,21 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
         return 1;
       }
       if (tid == 0) c.d[kb + k] = dk;
-      const T dinv = F<T>::recip(dk);
+      const T dinv = fresh ? F<T>::recip(dk) : dinv_pre;
       // in-block rank-1 update (rows of this panel) straight from column k,
       // which is replaced by its multipliers once every thread is past the
       // update (the next step reads column k again only after its first
       // barrier, and scaling a column commutes with its interchanges)
-      strips_update_1x1<NTG>(Ab, pw, k, dinv, strips);
+      if constexpr (PIPE) {
+        if (tid < 64) {
+          if (k + 1 < pw) pivot_column_step(Ab, pw, k, dinv, dec[par ^ 1], &deci[par ^ 1]);
+        } else {
+          strips_update_1x1<NTG, T0>(Ab, pw, k, k + 2, dinv, strips);
+        }
+        have_dec = k + 1 < pw;
+      } else {
+        strips_update_1x1<NTG, T0>(Ab, pw, k, k + 1, dinv, strips);
+      }
       have_cand = false;
       // column k keeps its unscaled entries until the block is written back
       // (panel_store multiplies them by w1s[k] = 1/d_k then; the later
@@ -968,6 +1048,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       if (tid == 0) w1s[k] = dinv;
       __syncthreads();
       k += 1;
+      par ^= 1;
     } else {
       T d11 = A_(k, k), d21 = A_(k + 1, k), d22 = A_(k + 1, k + 1);
       T det = d11 * d22 - d21 * d21;
@@ -1006,6 +1087,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
       __syncthreads();
       packed_trailing_update<NTG, true>(Ab, pw, k + 2, w1s, w2s, l1s, l2s, cand_m, cand_i);
       have_cand = true;
+      have_dec = false;
       for (int i = k + 2 + tid; i < pw; i += NTG) {
         A_(i, k) = l1s[i];
         A_(i, k + 1) = l2s[i];
