@@ -72,7 +72,7 @@ rsl::Ordering ordering_of(const rsl_settings& st) {
 rsl::PlanParams plan_params(const rsl_settings& st) {
   rsl::PlanParams pp;
   pp.panel_nb = st.panel_nb < 8 ? 8 : st.panel_nb;
-  pp.wide_flops = st.wide_flops > 0 ? st.wide_flops : 4.0e6;
+  pp.wide_flops = st.wide_flops > 0 ? st.wide_flops : 2.5e5;
   return pp;
 }
 }  // namespace
@@ -92,7 +92,7 @@ void rsl_settings_default(rsl_settings* s) {
   s->scaling = 1;
   s->precision = RSL_PREC_FP64;
   s->nd_seed = 1;
-  s->wide_flops = 4.0e6;
+  s->wide_flops = 2.5e5;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }

@@ -10,7 +10,7 @@ namespace rsl {
 
 struct PlanParams {
   int panel_nb = 64;            // SolverSettings::panel_nb default (gemm_tuning.rs:27)
-  double wide_flops = 4.0e6;    // supernodes with h*w*w above this get multi-workgroup cdiv
+  double wide_flops = 2.5e5;    // supernodes with h*w*w above this get multi-workgroup cdiv
 };
 
 struct LevelPlan {

@@ -206,7 +206,7 @@ class SolverSettings:
     scaling: str = "OnePassInfNorm"
     precision: Precision = Precision.FP64
     nd_seed: int = 1
-    wide_flops: float = 4.0e6
+    wide_flops: float = 2.5e5
 
     # builders (multifrontal_ldlt.rs:616-779)
     def with_ordering(self, o):
