@@ -298,7 +298,7 @@ __global__ __launch_bounds__(SNT) void k_fgather(DevSym S, DevFac<T> Fc, const i
   }
 }
 
-// Forward own-block solve of a narrow supernode (w <= 128): one workgroup.
+// Forward own-block solve of a narrow supernode (w <= 32): one workgroup.
 template <class T>
 __global__ __launch_bounds__(SNT) void k_fwd_tri(DevSym S, DevFac<T> Fc,
                                                  const int32_t* __restrict__ list, int32_t nrhs,
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
   }
 }
 
-// Backward own-block solve of a narrow supernode (w <= 128): one workgroup
+// Backward own-block solve of a narrow supernode (w <= 32): one workgroup
 // back-substitutes L11^T x = y from the last 64-block up.
 template <class T>
 __global__ __launch_bounds__(SNT) void k_bwd_tri(DevSym S, DevFac<T> Fc,

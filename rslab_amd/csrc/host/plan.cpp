@@ -11,7 +11,7 @@ namespace rsl {
 
 namespace {
 constexpr int TILE = 64;
-constexpr int SOLVE_WIDE = 128;  // own blocks wider than this are solved in 64-column steps
+constexpr int SOLVE_WIDE = 32;  // own blocks wider than this are solved in 64-column steps
 }
 
 void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanParams& prm) {
