@@ -362,14 +362,16 @@ __global__ __launch_bounds__(SNT) void k_fwd_update(DevSym S, DevFac<T> Fc,
 }
 
 // Backward, part 1: y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]] for 16-column
-// chunks (items (supernode, column offset)). Lanes run over rows, 256 rows
-// per pass (their x rows gathered into LDS), each wave owns four columns;
-// the lane partial sums are reduced once at the end.
+// chunks (items (supernode, column offset)). Lanes run over rows, 512 rows
+// per pass (their x rows gathered into LDS; each pass costs a dependent
+// gather, so few large passes), each wave owns four columns; the lane
+// partial sums (lane l: rows = l mod 64, ascending) are reduced once at the
+// end.
 template <class T>
 __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
                                                     const int2* __restrict__ items, int32_t nrhs,
                                                     T* __restrict__ y) {
-  constexpr int RB = 256, CW = 4;  // rows per pass, columns per wave
+  constexpr int RB = 512, CW = 4;  // rows per pass, columns per wave
   __shared__ T xs[RB * RCH];
   const int2 it = items[blockIdx.x];
   const int s = it.x, p0 = it.y;
@@ -388,20 +390,23 @@ __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
       for (int k = 0; k < RCH; ++k) acc[a][k] = F<T>::zero();
     for (int64_t ib = w; ib < h; ib += RB) {
       const int nr = (int)min<int64_t>(RB, h - ib);
-      // x rows of the pass: thread t gathers row t's RCH values
+      // x rows of the pass: thread t gathers rows t + SNT q (RCH values each)
       {
-        const int t = threadIdx.x;
-        T xv[RCH];
-        if (t < nr) {
-          const int64_t e = re[ib + t];
+        int64_t e[RB / SNT];
 #pragma unroll
-          for (int k = 0; k < RCH; ++k) xv[k] = k < nc ? y[e * nrhs + c0 + k] : F<T>::zero();
-        } else {
-#pragma unroll
-          for (int k = 0; k < RCH; ++k) xv[k] = F<T>::zero();
+        for (int q = 0; q < RB / SNT; ++q) {
+          const int t = threadIdx.x + q * SNT;
+          e[q] = t < nr ? (int64_t)re[ib + t] : -1;
         }
+        T xv[RB / SNT][RCH];
 #pragma unroll
-        for (int k = 0; k < RCH; ++k) xs[k * RB + t] = xv[k];  // [k][row]: lanes read consecutive words
+        for (int q = 0; q < RB / SNT; ++q)
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) xv[q][k] = (e[q] >= 0 && k < nc) ? y[e[q] * nrhs + c0 + k] : F<T>::zero();
+#pragma unroll
+        for (int q = 0; q < RB / SNT; ++q)
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) xs[k * RB + threadIdx.x + q * SNT] = xv[q][k];  // [k][row]
       }
       T l[RB / 64][CW];
 #pragma unroll
