@@ -8,6 +8,7 @@ from .solver import (  # noqa: F401
     DimensionMismatch,
     Inertia,
     InvalidInput,
+    IoError,
     LdltFactors,
     LdltSolver,
     LdltSymbolic,
@@ -30,6 +31,7 @@ __all__ = [
     "DimensionMismatch",
     "Inertia",
     "InvalidInput",
+    "IoError",
     "LdltFactors",
     "LdltSolver",
     "LdltSymbolic",

@@ -51,6 +51,10 @@ class OutOfMemory(RslabError):
     pass
 
 
+class IoError(RslabError):
+    """RslabError::IoError (error.rs:16): unreadable or malformed input files."""
+
+
 _ERR = {
     _lib.RSL_NUMERICALLY_RANK_DEFICIENT: NumericallyRankDeficient,
     _lib.RSL_INVALID_INPUT: InvalidInput,
