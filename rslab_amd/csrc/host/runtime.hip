@@ -283,6 +283,14 @@ class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
       work_y, work_b, work_z, cbuf, gpool, rscratch, dev_u64;
+  // refinement workspace, kept across calls (grown on demand; calls on one
+  // handle are serialized by the C ABI)
+  DBuf rf_afull, rf_dx, rf_dbest, rf_dr, rf_ddx, rf_norms, rf_mask, rf_mask2;
+  DBuf hs_db, hs_dx, hr_av, hr_db, hr_dxo;                                   // host-buffer calls
+  DBuf gm_bv, gm_xv, gm_rv, gm_wv, gm_V, gm_Z, gm_part, gm_hdev, gm_nrm;  // FGMRES
+  static void grow(DBuf& b, size_t nb) {
+    if (b.bytes < nb) b.alloc(nb);
+  }
   hipEvent_t ev_caller = nullptr;
   bool ev_ready = false;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
@@ -550,9 +558,9 @@ class DeviceFactor final : public DeviceFactorBase {
     const Plan& P = sym->plan;
     hipStream_t st = sym->stream;
     const size_t bytes = sizeof(T) * P.n * nrhs;
-    DBuf db, dx;
-    db.alloc(std::max<size_t>(bytes, 1));
-    dx.alloc(std::max<size_t>(bytes, 1));
+    grow(hs_db, std::max<size_t>(bytes, 1));
+    grow(hs_dx, std::max<size_t>(bytes, 1));
+    DBuf &db = hs_db, &dx = hs_dx;
     hip_check(hipMemcpyAsync(db.p, b, bytes, hipMemcpyHostToDevice, st), "upload rhs");
     solve_device(db.p, nrhs, dx.p, st);
     hip_check(hipMemcpyAsync(x, dx.p, bytes, hipMemcpyDeviceToHost, st), "download solution");
@@ -681,10 +689,10 @@ void DeviceFactor<T>::refine(const double* a_values, const double* b, int64_t nr
   double anorm = 0.0;
   for (double r : rows) anorm = std::max(anorm, r);
 
-  DBuf av, db, dxo;
-  av.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
-  db.alloc(sizeof(T) * std::max<int64_t>((int64_t)n * k, 1));
-  dxo.alloc(sizeof(T) * std::max<int64_t>((int64_t)n * k, 1));
+  grow(hr_av, sizeof(T) * std::max<int64_t>(P.nnz, 1));
+  grow(hr_db, sizeof(T) * std::max<int64_t>((int64_t)n * k, 1));
+  grow(hr_dxo, sizeof(T) * std::max<int64_t>((int64_t)n * k, 1));
+  DBuf &av = hr_av, &db = hr_db, &dxo = hr_dxo;
   hip_check(hipMemcpyAsync(av.p, a_values, sizeof(T) * P.nnz, hipMemcpyHostToDevice, st), "upload A");
   hip_check(hipMemcpyAsync(db.p, b, sizeof(T) * n * k, hipMemcpyHostToDevice, st), "upload b");
   refine_core(av.as<T>(), db.as<T>(), k, target, anorm, dxo.as<T>(), info);
@@ -728,15 +736,16 @@ void DeviceFactor<T>::refine_core(const T* av, const T* dbp, int32_t k, double t
   const int32_t n = P.n;
   sym->ensure_full_csr();
   const size_t blk = sizeof(T) * std::max<int64_t>((int64_t)n * k, 1);
-  DBuf afull, dx, dbest, dr, ddx, norms, mask_d, mask2_d;
-  afull.alloc(sizeof(T) * std::max<int64_t>(sym->nnz_full, 1));
-  dx.alloc(blk);
-  dbest.alloc(blk);
-  dr.alloc(blk);
-  ddx.alloc(blk);
-  norms.alloc(sizeof(unsigned long long) * 3 * std::max(k, 1));
-  mask_d.alloc(std::max(k, 1));
-  mask2_d.alloc(std::max(k, 1));
+  grow(rf_afull, sizeof(T) * std::max<int64_t>(sym->nnz_full, 1));
+  grow(rf_dx, blk);
+  grow(rf_dbest, blk);
+  grow(rf_dr, blk);
+  grow(rf_ddx, blk);
+  grow(rf_norms, sizeof(unsigned long long) * 3 * std::max(k, 1));
+  grow(rf_mask, std::max(k, 1));
+  grow(rf_mask2, std::max(k, 1));
+  DBuf &afull = rf_afull, &dx = rf_dx, &dbest = rf_dbest, &dr = rf_dr, &ddx = rf_ddx, &norms = rf_norms,
+       &mask_d = rf_mask, &mask2_d = rf_mask2;
   launch_expand<T>(sym->nnz_full, sym->fa_src.as<int64_t>(), av, afull.as<T>(), st);
   const int64_t* rp = sym->fa_rp.as<int64_t>();
   const int32_t* ci = sym->fa_ci.as<int32_t>();
@@ -745,7 +754,7 @@ void DeviceFactor<T>::refine_core(const T* av, const T* dbp, int32_t k, double t
 
   // x0 = low-precision solve of b; |b|_inf per column
   solve_device(dbp, k, dx.p, st);
-  hip_check(hipMemsetAsync(norms.p, 0, norms.bytes, st), "memset");
+  hip_check(hipMemsetAsync(norms.p, 0, sizeof(unsigned long long) * 3 * k, st), "memset");
   launch_colmax<T>(n, k, dbp, nrm + 2 * k, st);
 
   std::vector<int> ir(k, 0), phase(k, 0);  // 0 iterating, 1 certified, 2 escalate
@@ -825,16 +834,17 @@ void DeviceFactor<T>::gmres_column(int32_t n, int32_t k, int32_t c, const int64_
   const int m = GMRES_RESTART;
   const double eta = 0.70710678118654752440;
   const size_t vb = sizeof(T) * std::max(n, 1);
-  DBuf bv, xv, rv, wv, V, Z, part, hdev, nrm;
-  bv.alloc(vb);
-  xv.alloc(vb);
-  rv.alloc(vb);
-  wv.alloc(vb);
-  V.alloc(vb * (m + 1));
-  Z.alloc(vb * m);
-  part.alloc(sizeof(T) * refine_reduction_blocks());
-  hdev.alloc(sizeof(T) * (2 * (m + 1) + 2));
-  nrm.alloc(sizeof(unsigned long long) * 2);
+  grow(gm_bv, vb);
+  grow(gm_xv, vb);
+  grow(gm_rv, vb);
+  grow(gm_wv, vb);
+  grow(gm_V, vb * (m + 1));
+  grow(gm_Z, vb * m);
+  grow(gm_part, sizeof(T) * refine_reduction_blocks());
+  grow(gm_hdev, sizeof(T) * (2 * (m + 1) + 2));
+  grow(gm_nrm, sizeof(unsigned long long) * 2);
+  DBuf &bv = gm_bv, &xv = gm_xv, &rv = gm_rv, &wv = gm_wv, &V = gm_V, &Z = gm_Z, &part = gm_part, &hdev = gm_hdev,
+       &nrm = gm_nrm;
   T* b1 = bv.as<T>();
   T* x1 = xv.as<T>();
   T* r1 = rv.as<T>();
