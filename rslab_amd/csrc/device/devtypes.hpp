@@ -79,9 +79,10 @@ struct WItem {
 // One panel step of a level's wide supernodes, run as three batched launches
 // (getf2 on every supernode that has a p-th panel, then the deep-row replay,
 // then the deferred Schur tiles). Ranges index the WItem array.
-// Deep rows per replay work unit of the wide path: one wavefront, one row
-// per lane (the unit's time is bound by its per-row dependence chains, so
-// small units spread a step's rows over more CUs).
+// Deep rows per replay work unit of the wide path: four waves, one 16-row
+// MFMA tile each, with one row per lane of wave 0 in the in-pass triangle
+// (the unit's time is bound by its per-row dependence chains, so small units
+// spread a step's rows over more CUs).
 constexpr int REPLAY_ROWS = 64;
 
 struct WideStep {

@@ -1580,6 +1580,44 @@ __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, 
 // term; every other operation is the register replay's.
 constexpr int RU = 64;
 constexpr int RQLD = NBMAX + 4;  // Lq row stride (doubles)
+
+// D scaling and store of one pass's columns for replay_unit_mfma, without
+// branches on LDS data: the pass's 2x2 flags arrive as a wave-uniform bit mask
+// (bit kk: column p0 + kk opens a 2x2 pivot) and every D coefficient is read
+// unconditionally, so the 32 columns' LDS reads overlap. Same arithmetic as
+// replay_pass_store.
+__device__ __forceinline__ void replay_pass_scale_store(const SnCtx<double>& c, int kb, int p0, int n,
+                                                        double (&x)[RH], uint32_t opens, const double4* dq,
+                                                        bool live, int64_t row, bool first_done,
+                                                        bool last_deferred) {
+  static_assert(RH == 32, "one mask bit per column of a pass");
+  const uint32_t in_pass = n >= RH ? 0xffffffffu : ((1u << n) - 1u);
+  const uint32_t closes = (opens << 1) | (first_done ? 1u : 0u);  // second column of a pair
+  const uint32_t last = last_deferred ? (1u << (n - 1)) : 0u;      // pair finished by the next pass
+  const uint32_t act = in_pass & ~closes & ~last;                  // 1x1 columns and pair openers
+#pragma unroll
+  for (int kk = 0; kk < RH; ++kk) {
+    if (kk % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // coefficient reads in chunks of 8 columns
+    const double4 q = dq[kk];
+    const bool a = (act >> kk) & 1u;
+    if (kk + 1 < RH) {
+      const bool pr = a && ((opens >> kk) & 1u);
+      const double wk = x[kk], wk1 = x[kk + 1];
+      const double p1 = (q.z * wk - q.y * wk1) * q.w;
+      const double p2 = (q.x * wk1 - q.y * wk) * q.w;
+      x[kk] = a ? (pr ? p1 : wk * q.x) : wk;
+      x[kk + 1] = pr ? p2 : wk1;
+    } else {
+      x[kk] = a ? x[kk] * q.x : x[kk];
+    }
+  }
+  const uint32_t stored = in_pass & ~(first_done ? 1u : 0u) & ~last;
+  if (live) {
+#pragma unroll
+    for (int kk = 0; kk < RH; ++kk)
+      if ((stored >> kk) & 1u) c.P[row + (int64_t)(kb + p0 + kk) * c.h] = x[kk];
+  }
+}
 constexpr int RXLD = RU + 4;     // Xq row stride
 struct ReplayMfmaLds {
   double Lq[RH * RQLD];  // Lq[k * RQLD + j] = L11(j, p0 + k) for j > p0 + k (no 2x2 coupling), else 0
@@ -1590,19 +1628,77 @@ struct ReplayMfmaLds {
 __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t row0, int nr,
                                  ReplayMfmaLds& sh, const int32_t* cperm, const int32_t* two_s) {
   constexpr int NCT = NBMAX / 16;
+  constexpr int SB = 8;           // staging loads in flight per thread (first pass)
+  constexpr int NPF = NT - 64;    // waves 1-3 prefetch the next pass's staging
+  constexpr int PF = 16;          // prefetched staging entries per thread
+  static_assert(RH * (NBMAX - RH) <= PF * NPF, "a later pass's staging fits the prefetch registers");
   const int pw = ke - kb, tid = threadIdx.x, l = tid & 63, w = tid >> 6, cl = l & 15;
   const int rb = 16 * w + (l >> 4);  // unit row of result register r: rb + 4 r
   const double* L = c.P + kb + (int64_t)kb * c.h;
+  // Staged columns of pass p0: p0 .. max(pw rounded up to a tile, p0 + RH) - 1
+  // (the triangle reads RH of them, the products whole 16-column tiles).
+  auto stage_cols = [&](int p0) { return min(NBMAX, max((pw + 15) & ~15, p0 + RH)) - p0; };
+  // bit kk: column p0 + kk opens a 2x2 pivot (wave-uniform)
+  auto pass_opens = [&](int p0) {
+    return (uint32_t)__ballot(l < RH && p0 + l < pw && two_s[min(p0 + l, NBMAX - 1)] != 0);
+  };
+  // staging entry e of pass p0: L11(j, gk), zero outside the strict lower part and on 2x2 couplings
+  auto lq_value = [&](int p0, int nj, uint32_t opens, int e) {
+    const int k = e / nj, j = p0 + e % nj, gk = p0 + k;
+    const bool in = j < pw && gk < pw && j > gk && !(j == gk + 1 && ((opens >> (k & 31)) & 1u));
+    return in ? L[j + (int64_t)gk * c.h] : 0.0;
+  };
+  // D coefficients of column g: 1/d (1x1), (d11, d21, d22, 1/det) for a pair's first column
+  auto dq_value = [&](int g) {
+    double4 q = make_double4(1.0, 0.0, 0.0, 1.0);
+    if (g < pw) {
+      if (two_s[g]) {
+        const double d11 = c.d[kb + g], d21 = c.ds[kb + g], d22 = c.d[kb + g + 1];
+        q = make_double4(d11, d21, d22, 1.0 / (d11 * d22 - d21 * d21));
+      } else if (!(g > 0 && two_s[g - 1])) {
+        q.x = 1.0 / c.d[kb + g];
+      }
+    }
+    return q;
+  };
   mf_d4 y[NCT];
+  {
+    // interchange partners first, so the loads of the unit's rows go out together
+    int cp[NCT];
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
-    const int col = ct * 16 + cl;
+    for (int ct = 0; ct < NCT; ++ct) cp[ct] = cperm[min(ct * 16 + cl, pw - 1)];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = rb + 4 * r;
-      y[ct][r] = (col < pw && row < nr) ? c.P[row0 + row + (int64_t)(kb + cperm[col]) * c.h] : 0.0;
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int col = ct * 16 + cl;
+      const double* src = c.P + row0 + (int64_t)(kb + cp[ct]) * c.h;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + 4 * r;
+        y[ct][r] = (col < pw && row < nr) ? src[row] : 0.0;
+      }
     }
   }
+  // first pass: staging and D coefficients by every thread
+  {
+    const uint32_t opens = pass_opens(0);
+    const int nj = stage_cols(0), tot = RH * nj;
+    for (int e0 = tid; e0 < tot; e0 += NT * SB) {
+      double v[SB];
+#pragma unroll
+      for (int u = 0; u < SB; ++u) {
+        const int e = e0 + u * NT;
+        v[u] = e < tot ? lq_value(0, nj, opens, e) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < SB; ++u) {
+        const int e = e0 + u * NT;
+        if (e < tot) sh.Lq[(e / nj) * RQLD + e % nj] = v[u];
+      }
+    }
+    if (tid < RH) sh.dq[tid] = dq_value(tid);
+  }
+  double pre[PF];
+  double4 dqn = make_double4(1.0, 0.0, 0.0, 1.0);
   double xprev = 0.0;  // wave 0: unscaled last column of the previous pass
   double4 qprev = make_double4(1.0, 0.0, 0.0, 1.0);
   bool first_done = false;
@@ -1610,37 +1706,16 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
   const int64_t row = row0 + l;
   for (int p0 = 0; p0 < pw; p0 += RH) {
     const int n = min(RH, pw - p0), pc = p0 / 16;  // first column tile of the pass
-    // stage the pass's L11 columns (rows p0.. of NBMAX, zero outside the strict lower part)
-    {
-      constexpr int SB = 8;
-      const int nj = NBMAX - p0, tot = RH * nj;
-      for (int e0 = tid; e0 < tot; e0 += NT * SB) {
-        double v[SB];
+    const uint32_t opens = pass_opens(p0);
+    if (p0 > 0 && w > 0) {
+      // the staging waves 1-3 loaded during the previous pass's triangle
+      const int nj = stage_cols(p0), tot = RH * nj;
 #pragma unroll
-        for (int u = 0; u < SB; ++u) {
-          const int e = e0 + u * NT, k = e / nj, j = p0 + e % nj, gk = p0 + k;
-          const bool in = e < tot && j < pw && gk < pw && j > gk && !(j == gk + 1 && two_s[gk]);
-          v[u] = in ? L[j + (int64_t)gk * c.h] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < SB; ++u) {
-          const int e = e0 + u * NT;
-          if (e < tot) sh.Lq[(e / nj) * RQLD + p0 + e % nj] = v[u];
-        }
+      for (int u = 0; u < PF; ++u) {
+        const int e = (tid - 64) + u * NPF;
+        if (e < tot) sh.Lq[(e / nj) * RQLD + p0 + e % nj] = pre[u];
       }
-    }
-    if (tid < RH) {
-      const int g = p0 + tid;
-      double4 q = make_double4(1.0, 0.0, 0.0, 1.0);
-      if (g < pw) {
-        if (two_s[g]) {
-          const double d11 = c.d[kb + g], d21 = c.ds[kb + g], d22 = c.d[kb + g + 1];
-          q = make_double4(d11, d21, d22, 1.0 / (d11 * d22 - d21 * d21));
-        } else if (!(g > 0 && two_s[g - 1])) {
-          q.x = 1.0 / c.d[kb + g];
-        }
-      }
-      sh.dq[tid] = q;
+      if (tid - 64 < RH) sh.dq[tid - 64] = dqn;
     }
     // the pass's two column tiles, from their owners' registers
 #pragma unroll
@@ -1661,19 +1736,44 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
         c.P[row + (int64_t)(kb + p0 - 1) * c.h] = (qprev.z * wk - qprev.y * wk1) * qprev.w;
         c.P[row + (int64_t)(kb + p0) * c.h] = (qprev.x * wk1 - qprev.y * wk) * qprev.w;
       }
+      // straight-line: Lq is zero in rows and columns at or past n, and so are
+      // the columns of x past n, so the terms of those columns add nothing.
+      // Row k + 1 of Lq is read while row k's terms are applied; the
+      // scheduling barriers keep the reads from being hoisted any further
+      // (register pressure).
+      const double* lq = sh.Lq + p0;
+      double cur[RH], nxt[RH];
+#pragma unroll
+      for (int j = 1; j < RH; ++j) cur[j] = lq[j];
 #pragma unroll
       for (int k = 0; k < RH - 1; ++k) {
-        if (k + 1 < n) {
-          const double xk = x[k];
+        if (k + 1 < RH - 1) {
 #pragma unroll
-          for (int j = k + 1; j < RH; ++j) x[j] = x[j] - xk * sh.Lq[k * RQLD + p0 + j];
+          for (int j = k + 2; j < RH; ++j) nxt[j] = lq[(k + 1) * RQLD + j];
         }
+        __builtin_amdgcn_sched_barrier(0);
+        const double xk = x[k];
+#pragma unroll
+        for (int j = k + 1; j < RH; ++j) x[j] = x[j] - xk * cur[j];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = k + 2; j < RH; ++j) cur[j] = nxt[j];
       }
       xprev = x[RH - 1];
       qprev = sh.dq[RH - 1];
 #pragma unroll
       for (int kk = 0; kk < RH; ++kk) sh.Xq[kk * RXLD + l] = -x[kk];
-      replay_pass_store(c, kb, p0, n, x, two_s, sh.dq, live, row, first_done, straddle);
+      replay_pass_scale_store(c, kb, p0, n, x, opens, sh.dq, live, row, first_done, straddle);
+    } else if (p0 + RH < pw) {
+      // waves 1-3: load the next pass's staging and D coefficients meanwhile
+      const int q0 = p0 + RH, nj = stage_cols(q0), tot = RH * nj;
+      const uint32_t on = pass_opens(q0);
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int e = (tid - 64) + u * NPF;
+        pre[u] = e < tot ? lq_value(q0, nj, on, e) : 0.0;
+      }
+      if (tid - 64 < RH) dqn = dq_value(q0 + tid - 64);
     }
     first_done = straddle;
     __syncthreads();
@@ -1825,8 +1925,7 @@ __global__ __launch_bounds__(NT) void k_wide_swapcols(DevSym S, DevFac<T> Fc, co
 }
 
 template <class T>
-__global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items,
-                                                   double* __restrict__ scratch) {
+__global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, const WItem* __restrict__ items) {
   extern __shared__ unsigned char smem_raw[];
   __shared__ int32_t cperm[NBMAX];
   __shared__ int32_t two_s[NBMAX];
@@ -1845,7 +1944,6 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
   if constexpr (std::is_same<T, double>::value) {
     static_assert(REPLAY_ROWS == RU && RH == 32, "one MFMA unit per work item, two column tiles per pass");
     replay_unit_mfma(c, kb, ke, row0, nr, *reinterpret_cast<ReplayMfmaLds*>(smem_raw), cperm, two_s);
-    (void)scratch;
   } else {
     T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
     T* Lc = X + RR * NBMAX;                 // [RCW + 1][NBMAX]: L11 columns of the current chunk
@@ -1984,8 +2082,7 @@ void launch_cdiv_small(const DevSym& S, const DevFac<T>& Fc, const int32_t* list
 
 template <class T>
 void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems, const WideStep* steps,
-                       int nsteps, const int32_t* wide_list, int nwide, PivotPolicy pol, int prec, double* rscratch,
-                       hipStream_t st, LaunchProfiler* pf) {
+                       int nsteps, const int32_t* wide_list, int nwide, PivotPolicy pol, int prec, hipStream_t st, LaunchProfiler* pf) {
   if (nwide <= 0) return;
   const double fm = sizeof(T) == 16 ? 4.0 : 1.0;
   const size_t replay_lds = replay_smem<T>();
@@ -2002,7 +2099,7 @@ void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems
     if (w.r_end > w.r_beg) {
       ProfScope ps(pf, LK_PANEL_REPLAY, fm * w.replay_flops, 0.0, st);
       hipLaunchKernelGGL(k_wide_replay<T>, dim3((unsigned)(w.r_end - w.r_beg)), dim3(NT), replay_lds, st, S,
-                         Fc, witems + w.r_beg, rscratch);
+                         Fc, witems + w.r_beg);
     }
     if (w.t_end > w.t_beg) {
       ProfScope ps(pf, LK_PANEL_SCHUR, fm * w.schur_flops, 0.0, st);
@@ -2039,8 +2136,8 @@ hipError_t configure_factor_kernels() {
   template void launch_cdiv_small<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int, \
                                      PivotPolicy, double, int, hipStream_t, LaunchProfiler*);  \
   template void launch_wide_level<T>(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, \
-                                     int, const int32_t*, int, PivotPolicy, int, double*,        \
-                                     hipStream_t, LaunchProfiler*);                              \
+                                     int, const int32_t*, int, PivotPolicy, int, hipStream_t,    \
+                                     LaunchProfiler*);                              \
   template hipError_t configure_factor_kernels<T>();                                           \
   template size_t cdiv_smem<T>();
 

@@ -188,7 +188,6 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
 
   // batched wide path: per level, one step per panel index
   P.witems.clear();
-  P.max_replay_items = 0;
   for (LevelPlan& L : P.levels) {
     L.wsteps.clear();
     int maxp = 0;
@@ -216,7 +215,6 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
         ws.replay_flops += (double)mt * pw * (pw - 1);
       }
       ws.r_end = ws.t_beg = (int64_t)P.witems.size();
-      P.max_replay_items = std::max(P.max_replay_items, ws.r_end - ws.r_beg);
       for (int32_t s : L.wide) {
         const int32_t kb = p * P.nb_eff[s];
         if (kb >= P.sn_ncol[s]) continue;

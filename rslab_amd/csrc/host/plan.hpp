@@ -58,7 +58,6 @@ struct Plan {
   std::vector<PTile> ptiles;
   int64_t c_elems_max = 0;      // largest c_elems over the levels
   std::vector<WItem> witems;    // batched wide-path work units
-  int64_t max_replay_items = 0; // most replay units in one wide step
   // solve plan
   std::vector<int64_t> u_off;   // scratch rows of supernode s (h - w of them)
   int64_t u_total = 0;

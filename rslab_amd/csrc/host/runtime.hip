@@ -27,7 +27,7 @@ void launch_cmod_two_phase(const DevSym&, const DevFac<double>&, const CItem*, i
                            int64_t, void*, int64_t, const double*, double, int, hipStream_t, LaunchProfiler*);
 void launch_make_g(const DevSym&, const DevFac<double>&, const int32_t*, int, int, double*, hipStream_t);
 template <class T> void launch_cdiv_small(const DevSym&, const DevFac<T>&, const int32_t*, int, int, PivotPolicy, double, int, hipStream_t, LaunchProfiler*);
-template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, int, double*, hipStream_t, LaunchProfiler*);
+template <class T> void launch_wide_level(const DevSym&, const DevFac<T>&, const WItem*, const WideStep*, int, const int32_t*, int, PivotPolicy, int, hipStream_t, LaunchProfiler*);
 template <class T> hipError_t configure_factor_kernels();
 template <class T> void launch_gather_rhs(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_scatter_sol(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t, LaunchProfiler*);
@@ -282,7 +282,7 @@ template <class T>
 class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
-      work_y, work_b, work_z, cbuf, gpool, rscratch, dev_u64;
+      work_y, work_b, work_z, cbuf, gpool, dev_u64;
   // refinement workspace, kept across calls (grown on demand; calls on one
   // handle are serialized by the C ABI)
   DBuf rf_afull, rf_dx, rf_dbest, rf_dr, rf_ddx, rf_norms, rf_mask, rf_mask2;
@@ -336,9 +336,6 @@ class DeviceFactor final : public DeviceFactorBase {
         gpool.alloc(sizeof(double) * (size_t)std::max<int64_t>(P.g_total, 1));  // L21 D of every supernode
       }
     }
-    // register replay of the wide path: running values of the columns past
-    // its first 32, (128 - 32) x REPLAY_ROWS doubles per unit of a step
-    if (FieldOf<T>::id == 0) rscratch.alloc(sizeof(double) * 96 * REPLAY_ROWS * std::max<int64_t>(P.max_replay_items, 1));
     status.alloc(sizeof(int32_t) * 2);
     scale_d.alloc(sizeof(double) * std::max<int64_t>(n, 1));
     vals_d.alloc(sizeof(T) * std::max<int64_t>(P.nnz, 1));
@@ -477,7 +474,7 @@ class DeviceFactor final : public DeviceFactorBase {
       launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
                            lv.small_flops, prec, st, pf);
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
-                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, rscratch.as<double>(), st, pf);
+                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, st, pf);
       if constexpr (std::is_same<T, double>::value) {
         if (c_cap > 0 && L + 1 < P.levels.size()) {
           int maxw = 0;
