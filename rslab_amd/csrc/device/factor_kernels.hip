@@ -1580,45 +1580,50 @@ __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, 
 // term; every other operation is the register replay's.
 constexpr int RU = 64;
 constexpr int RQLD = NBMAX + 4;  // Lq row stride (doubles)
+constexpr int RXLD = RU + 4;     // Xq row stride
 
-// D scaling and store of one pass's columns for replay_unit_mfma, without
-// branches on LDS data: the pass's 2x2 flags arrive as a wave-uniform bit mask
-// (bit kk: column p0 + kk opens a 2x2 pivot) and every D coefficient is read
-// unconditionally, so the 32 columns' LDS reads overlap. Same arithmetic as
-// replay_pass_store.
-__device__ __forceinline__ void replay_pass_scale_store(const SnCtx<double>& c, int kb, int p0, int n,
-                                                        double (&x)[RH], uint32_t opens, const double4* dq,
-                                                        bool live, int64_t row, bool first_done,
-                                                        bool last_deferred) {
-  static_assert(RH == 32, "one mask bit per column of a pass");
-  const uint32_t in_pass = n >= RH ? 0xffffffffu : ((1u << n) - 1u);
-  const uint32_t closes = (opens << 1) | (first_done ? 1u : 0u);  // second column of a pair
-  const uint32_t last = last_deferred ? (1u << (n - 1)) : 0u;      // pair finished by the next pass
-  const uint32_t act = in_pass & ~closes & ~last;                  // 1x1 columns and pair openers
+// D scaling and store of 8 of a pass's columns (k0 .. k0+7) for
+// replay_unit_mfma, one row per lane, from the pass's unscaled values held
+// negated in LDS (xq: this row's entry of column 0, column stride RXLD). act:
+// columns scaled here (1x1 pivots and 2x2 openers; not a pair's second
+// column, nor an opener whose partner is in the next pass); pairs: the 2x2
+// openers among them. A pair opened at k0+7 is finished and stored here, so
+// a chunk skips a first column closed from the previous chunk. Same
+// arithmetic as replay_pass_store.
+__device__ __forceinline__ void replay_chunk_scale_store(const SnCtx<double>& c, int kb, int p0, int k0,
+                                                         const double* xq, uint32_t act, uint32_t pairs,
+                                                         const double4* dq, bool live, int64_t row) {
+  constexpr int CW = 8;
+  static_assert(RH == 32 && RH % CW == 0, "one mask bit per column of a pass");
+  double v[CW + 1];
 #pragma unroll
-  for (int kk = 0; kk < RH; ++kk) {
-    if (kk % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // coefficient reads in chunks of 8 columns
+  for (int i = 0; i <= CW; ++i) v[i] = (k0 + i < RH) ? -xq[(k0 + i) * RXLD] : 0.0;
+#pragma unroll
+  for (int i = 0; i < CW; ++i) {
+    const int kk = k0 + i;
     const double4 q = dq[kk];
-    const bool a = (act >> kk) & 1u;
-    if (kk + 1 < RH) {
-      const bool pr = a && ((opens >> kk) & 1u);
-      const double wk = x[kk], wk1 = x[kk + 1];
-      const double p1 = (q.z * wk - q.y * wk1) * q.w;
-      const double p2 = (q.x * wk1 - q.y * wk) * q.w;
-      x[kk] = a ? (pr ? p1 : wk * q.x) : wk;
-      x[kk + 1] = pr ? p2 : wk1;
-    } else {
-      x[kk] = a ? x[kk] * q.x : x[kk];
+    const bool a = (act >> kk) & 1u, pr = (pairs >> kk) & 1u;
+    const double wk = v[i], wk1 = v[i + 1];
+    const double p1 = (q.z * wk - q.y * wk1) * q.w;
+    const double p2 = (q.x * wk1 - q.y * wk) * q.w;
+    v[i] = a ? (pr ? p1 : wk * q.x) : wk;
+    v[i + 1] = pr ? p2 : wk1;
+  }
+  if (live) {
+    const uint32_t computed = act | (pairs << 1);
+#pragma unroll
+    for (int i = 0; i <= CW; ++i) {
+      const int kk = k0 + i;
+      bool own;
+      if (i < CW)
+        own = ((computed >> kk) & 1u) && !(i == 0 && kk > 0 && ((pairs >> (kk - 1)) & 1u));
+      else
+        own = kk < RH && ((pairs >> (kk - 1)) & 1u);
+      if (own) c.P[row + (int64_t)(kb + p0 + kk) * c.h] = v[i];
     }
   }
-  const uint32_t stored = in_pass & ~(first_done ? 1u : 0u) & ~last;
-  if (live) {
-#pragma unroll
-    for (int kk = 0; kk < RH; ++kk)
-      if ((stored >> kk) & 1u) c.P[row + (int64_t)(kb + p0 + kk) * c.h] = x[kk];
-  }
 }
-constexpr int RXLD = RU + 4;     // Xq row stride
+
 struct ReplayMfmaLds {
   double Lq[RH * RQLD];  // Lq[k * RQLD + j] = L11(j, p0 + k) for j > p0 + k (no 2x2 coupling), else 0
   double Xq[RH * RXLD];  // the pass's columns: Xq[kk * RXLD + r] (wave 0 writes -x after the triangle)
@@ -1707,6 +1712,12 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
   for (int p0 = 0; p0 < pw; p0 += RH) {
     const int n = min(RH, pw - p0), pc = p0 / 16;  // first column tile of the pass
     const uint32_t opens = pass_opens(p0);
+    const bool straddle = p0 + RH < pw && two_s[p0 + RH - 1];
+    // columns scaled in this pass: not a pair's second column (closed from
+    // this pass or, at column 0, from the previous one), not an opener whose
+    // partner is in the next pass
+    const uint32_t in_pass = n >= RH ? 0xffffffffu : ((1u << n) - 1u);
+    const uint32_t act = in_pass & ~((opens << 1) | (first_done ? 1u : 0u)) & ~(straddle ? (1u << (n - 1)) : 0u);
     if (p0 > 0 && w > 0) {
       // the staging waves 1-3 loaded during the previous pass's triangle
       const int nj = stage_cols(p0), tot = RH * nj;
@@ -1725,7 +1736,6 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
         for (int r = 0; r < 4; ++r) sh.Xq[((ct - pc) * 16 + cl) * RXLD + rb + 4 * r] = y[ct][r];
       }
     __syncthreads();
-    const bool straddle = p0 + RH < pw && two_s[p0 + RH - 1];
     if (w == 0) {
       double x[RH];
 #pragma unroll
@@ -1738,32 +1748,38 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
       }
       // straight-line: Lq is zero in rows and columns at or past n, and so are
       // the columns of x past n, so the terms of those columns add nothing.
-      // Row k + 1 of Lq is read while row k's terms are applied; the
-      // scheduling barriers keep the reads from being hoisted any further
-      // (register pressure).
+      // The coefficients stream through a ring of TW registers, each read TW
+      // terms before its use; the scheduling barriers keep the reads from
+      // being hoisted further (register pressure).
+      // Terms run k ascending, j = k+1 .. RH-1 within k; (pk, pj) is the term
+      // whose coefficient is read next (all indices fold to constants).
+      constexpr int TW = 24;
       const double* lq = sh.Lq + p0;
-      double cur[RH], nxt[RH];
+      double ring[TW];
+      int pk = 0, pj = 1, t = 0;
 #pragma unroll
-      for (int j = 1; j < RH; ++j) cur[j] = lq[j];
+      for (int u = 0; u < TW; ++u) {
+        ring[u] = lq[pk * RQLD + pj];
+        if (++pj == RH) { ++pk; pj = pk + 1; }
+      }
 #pragma unroll
       for (int k = 0; k < RH - 1; ++k) {
-        if (k + 1 < RH - 1) {
 #pragma unroll
-          for (int j = k + 2; j < RH; ++j) nxt[j] = lq[(k + 1) * RQLD + j];
+        for (int j = k + 1; j < RH; ++j) {
+          const double v = ring[t % TW];
+          if (pk < RH - 1) {
+            ring[t % TW] = lq[pk * RQLD + pj];
+            if (++pj == RH) { ++pk; pj = pk + 1; }
+          }
+          x[j] = x[j] - x[k] * v;
+          if (t % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+          ++t;
         }
-        __builtin_amdgcn_sched_barrier(0);
-        const double xk = x[k];
-#pragma unroll
-        for (int j = k + 1; j < RH; ++j) x[j] = x[j] - xk * cur[j];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = k + 2; j < RH; ++j) cur[j] = nxt[j];
       }
       xprev = x[RH - 1];
       qprev = sh.dq[RH - 1];
 #pragma unroll
       for (int kk = 0; kk < RH; ++kk) sh.Xq[kk * RXLD + l] = -x[kk];
-      replay_pass_scale_store(c, kb, p0, n, x, opens, sh.dq, live, row, first_done, straddle);
     } else if (p0 + RH < pw) {
       // waves 1-3: load the next pass's staging and D coefficients meanwhile
       const int q0 = p0 + RH, nj = stage_cols(q0), tot = RH * nj;
@@ -1789,6 +1805,8 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
           }
       }
     }
+    // the pass's D scaling and stores, 8 columns per wave
+    replay_chunk_scale_store(c, kb, p0, 8 * w, sh.Xq + l, act, act & opens, sh.dq, live, row);
     __syncthreads();
   }
 }
