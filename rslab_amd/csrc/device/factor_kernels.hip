@@ -1570,34 +1570,41 @@ __device__ __forceinline__ void replay_rows_reg(const SnCtx<double>& c, int kb, 
 }
 
 // Deep-row replay of the wide path on matrix cores: one unit is RU = 64 rows
-// and four waves. Per pass of RH columns, wave 0 (one row per lane) runs the
-// pass's triangle in ascending k as replay_rows_reg does, scales and stores
-// the pass's columns; the pass's updates of every later column (the bulk of
-// the flops) are one product  Y -= X_pass * L11(later, pass)^T  on
-// v_mfma_f64_16x16x4f64, with Y held in registers across passes (wave w owns
-// rows 16w..16w+15 of every 16-column tile, in the MFMA result layout). The
-// later-column sums therefore run in the matrix core's order, not term by
-// term; every other operation is the register replay's.
+// and four waves. Per pass of RPW = 16 columns (one column tile), wave 0 (one
+// row per lane) runs the pass's triangle in ascending k as replay_rows_reg
+// does, while waves 1-3 load the next pass's L11 columns and D entries; the
+// pass's updates of every later column (the bulk of the flops) are one
+// product  Y -= X_pass * L11(later, pass)^T  on v_mfma_f64_16x16x4f64, with Y
+// held in registers across passes (wave w owns rows 16w..16w+15 of every
+// 16-column tile, in the MFMA result layout); then every wave scales and
+// stores 4 of the pass's columns. The L11 staging and D coefficients
+// alternate between two LDS buffers by pass. The later-column sums run in
+// the matrix core's order, not term by term; every other operation is the
+// register replay's.
 constexpr int RU = 64;
 constexpr int RQLD = NBMAX + 4;  // Lq row stride (doubles)
 constexpr int RXLD = RU + 4;     // Xq row stride
 
-// D scaling and store of 8 of a pass's columns (k0 .. k0+7) for
+// Columns per pass of the matrix-core replay, and per wave in its D scaling.
+constexpr int RPW = 16;
+constexpr int RCW = RPW / 4;
+
+// D scaling and store of RCW of a pass's columns (k0 .. k0+RCW-1) for
 // replay_unit_mfma, one row per lane, from the pass's unscaled values held
 // negated in LDS (xq: this row's entry of column 0, column stride RXLD). act:
 // columns scaled here (1x1 pivots and 2x2 openers; not a pair's second
 // column, nor an opener whose partner is in the next pass); pairs: the 2x2
-// openers among them. A pair opened at k0+7 is finished and stored here, so
+// openers among them. A pair opened at k0+RCW-1 is finished and stored here, so
 // a chunk skips a first column closed from the previous chunk. Same
 // arithmetic as replay_pass_store.
 __device__ __forceinline__ void replay_chunk_scale_store(const SnCtx<double>& c, int kb, int p0, int k0,
                                                          const double* xq, uint32_t act, uint32_t pairs,
                                                          const double4* dq, bool live, int64_t row) {
-  constexpr int CW = 8;
-  static_assert(RH == 32 && RH % CW == 0, "one mask bit per column of a pass");
+  constexpr int CW = RCW;
+  static_assert(RPW <= 32 && RPW % 16 == 0, "one mask bit per column of a pass, whole column tiles");
   double v[CW + 1];
 #pragma unroll
-  for (int i = 0; i <= CW; ++i) v[i] = (k0 + i < RH) ? -xq[(k0 + i) * RXLD] : 0.0;
+  for (int i = 0; i <= CW; ++i) v[i] = (k0 + i < RPW) ? -xq[(k0 + i) * RXLD] : 0.0;
 #pragma unroll
   for (int i = 0; i < CW; ++i) {
     const int kk = k0 + i;
@@ -1618,16 +1625,18 @@ __device__ __forceinline__ void replay_chunk_scale_store(const SnCtx<double>& c,
       if (i < CW)
         own = ((computed >> kk) & 1u) && !(i == 0 && kk > 0 && ((pairs >> (kk - 1)) & 1u));
       else
-        own = kk < RH && ((pairs >> (kk - 1)) & 1u);
+        own = kk < RPW && ((pairs >> (kk - 1)) & 1u);
       if (own) c.P[row + (int64_t)(kb + p0 + kk) * c.h] = v[i];
     }
   }
 }
 
 struct ReplayMfmaLds {
-  double Lq[RH * RQLD];  // Lq[k * RQLD + j] = L11(j, p0 + k) for j > p0 + k (no 2x2 coupling), else 0
-  double Xq[RH * RXLD];  // the pass's columns: Xq[kk * RXLD + r] (wave 0 writes -x after the triangle)
-  double4 dq[RH];
+  // two buffers, alternating by pass: Lq[b][k * RQLD + j] = L11(j, p0 + k)
+  // for j > p0 + k (no 2x2 coupling), else 0
+  double Lq[2][RPW * RQLD];
+  double Xq[RPW * RXLD];  // the pass's columns: Xq[kk * RXLD + r] (wave 0 writes -x after the triangle)
+  double4 dq[2][RPW];  // D coefficients of the pass's columns, alternating like Lq
 };
 
 __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t row0, int nr,
@@ -1635,17 +1644,17 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
   constexpr int NCT = NBMAX / 16;
   constexpr int SB = 8;           // staging loads in flight per thread (first pass)
   constexpr int NPF = NT - 64;    // waves 1-3 prefetch the next pass's staging
-  constexpr int PF = 16;          // prefetched staging entries per thread
-  static_assert(RH * (NBMAX - RH) <= PF * NPF, "a later pass's staging fits the prefetch registers");
+  constexpr int PF = (RPW * (NBMAX - RPW) + NPF - 1) / NPF;  // prefetched staging entries per thread
+  static_assert(RPW * (NBMAX - RPW) <= PF * NPF, "a later pass's staging fits the prefetch registers");
   const int pw = ke - kb, tid = threadIdx.x, l = tid & 63, w = tid >> 6, cl = l & 15;
   const int rb = 16 * w + (l >> 4);  // unit row of result register r: rb + 4 r
   const double* L = c.P + kb + (int64_t)kb * c.h;
-  // Staged columns of pass p0: p0 .. max(pw rounded up to a tile, p0 + RH) - 1
-  // (the triangle reads RH of them, the products whole 16-column tiles).
-  auto stage_cols = [&](int p0) { return min(NBMAX, max((pw + 15) & ~15, p0 + RH)) - p0; };
+  // Staged columns of pass p0: p0 .. max(pw rounded up to a tile, p0 + RPW) - 1
+  // (the triangle reads RPW of them, the products whole 16-column tiles).
+  auto stage_cols = [&](int p0) { return min(NBMAX, max((pw + 15) & ~15, p0 + RPW)) - p0; };
   // bit kk: column p0 + kk opens a 2x2 pivot (wave-uniform)
   auto pass_opens = [&](int p0) {
-    return (uint32_t)__ballot(l < RH && p0 + l < pw && two_s[min(p0 + l, NBMAX - 1)] != 0);
+    return (uint32_t)__ballot(l < RPW && p0 + l < pw && two_s[min(p0 + l, NBMAX - 1)] != 0);
   };
   // staging entry e of pass p0: L11(j, gk), zero outside the strict lower part and on 2x2 couplings
   auto lq_value = [&](int p0, int nj, uint32_t opens, int e) {
@@ -1653,19 +1662,42 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
     const bool in = j < pw && gk < pw && j > gk && !(j == gk + 1 && ((opens >> (k & 31)) & 1u));
     return in ? L[j + (int64_t)gk * c.h] : 0.0;
   };
-  // D coefficients of column g: 1/d (1x1), (d11, d21, d22, 1/det) for a pair's first column
-  auto dq_value = [&](int g) {
-    double4 q = make_double4(1.0, 0.0, 0.0, 1.0);
+  // D coefficients of column g: 1/d (1x1), (d11, d21, d22, 1/det) for a
+  // pair's first column. dq_load reads the D entries (w: 1 for a 1x1 pivot,
+  // 2 for a pair's first column, 0 otherwise), dq_make forms the coefficients,
+  // so a prefetch does not wait for the loads.
+  auto dq_load = [&](int g) {
+    double4 r = make_double4(1.0, 0.0, 1.0, 0.0);
     if (g < pw) {
       if (two_s[g]) {
-        const double d11 = c.d[kb + g], d21 = c.ds[kb + g], d22 = c.d[kb + g + 1];
-        q = make_double4(d11, d21, d22, 1.0 / (d11 * d22 - d21 * d21));
+        r = make_double4(c.d[kb + g], c.ds[kb + g], c.d[kb + g + 1], 2.0);
       } else if (!(g > 0 && two_s[g - 1])) {
-        q.x = 1.0 / c.d[kb + g];
+        r.x = c.d[kb + g];
+        r.w = 1.0;
       }
     }
+    return r;
+  };
+  auto dq_make = [](const double4& r) {
+    double4 q = make_double4(1.0, 0.0, 0.0, 1.0);
+    if (r.w == 2.0)
+      q = make_double4(r.x, r.y, r.z, 1.0 / (r.x * r.z - r.y * r.y));
+    else if (r.w == 1.0)
+      q.x = 1.0 / r.x;
     return q;
   };
+  // first pass: staging loads (issued ahead of the unit's row loads, so the
+  // staging writes wait for them alone), then the rows, then the writes and
+  // the D coefficients
+  static_assert(RPW * NBMAX <= NT * SB, "the first pass's staging is one batch");
+  const uint32_t opens0 = pass_opens(0);
+  const int nj0 = stage_cols(0), tot0 = RPW * nj0;
+  double v0[SB];
+#pragma unroll
+  for (int u = 0; u < SB; ++u) {
+    const int e = tid + u * NT;
+    v0[u] = e < tot0 ? lq_value(0, nj0, opens0, e) : 0.0;
+  }
   mf_d4 y[NCT];
   {
     // interchange partners first, so the loads of the unit's rows go out together
@@ -1683,63 +1715,46 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
       }
     }
   }
-  // first pass: staging and D coefficients by every thread
-  {
-    const uint32_t opens = pass_opens(0);
-    const int nj = stage_cols(0), tot = RH * nj;
-    for (int e0 = tid; e0 < tot; e0 += NT * SB) {
-      double v[SB];
 #pragma unroll
-      for (int u = 0; u < SB; ++u) {
-        const int e = e0 + u * NT;
-        v[u] = e < tot ? lq_value(0, nj, opens, e) : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < SB; ++u) {
-        const int e = e0 + u * NT;
-        if (e < tot) sh.Lq[(e / nj) * RQLD + e % nj] = v[u];
-      }
-    }
-    if (tid < RH) sh.dq[tid] = dq_value(tid);
+  for (int u = 0; u < SB; ++u) {
+    const int e = tid + u * NT;
+    if (e < tot0) sh.Lq[0][(e / nj0) * RQLD + e % nj0] = v0[u];
   }
+  if (tid < RPW) sh.dq[0][tid] = dq_make(dq_load(tid));
+  // the unit's rows have arrived before the pass loop (else the wait at the
+  // loop head, for them, would also wait out every pass's stores)
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+  // waves 1-3 load the next pass's staging and D entries into registers
+  // during the triangle
   double pre[PF];
-  double4 dqn = make_double4(1.0, 0.0, 0.0, 1.0);
+  double4 dqn = make_double4(1.0, 0.0, 1.0, 0.0);
   double xprev = 0.0;  // wave 0: unscaled last column of the previous pass
   double4 qprev = make_double4(1.0, 0.0, 0.0, 1.0);
   bool first_done = false;
   const bool live = l < nr;
   const int64_t row = row0 + l;
-  for (int p0 = 0; p0 < pw; p0 += RH) {
-    const int n = min(RH, pw - p0), pc = p0 / 16;  // first column tile of the pass
+  for (int p0 = 0; p0 < pw; p0 += RPW) {
+    const int n = min(RPW, pw - p0), pc = p0 / 16;  // first column tile of the pass
     const uint32_t opens = pass_opens(p0);
-    const bool straddle = p0 + RH < pw && two_s[p0 + RH - 1];
+    const bool straddle = p0 + RPW < pw && two_s[p0 + RPW - 1];
     // columns scaled in this pass: not a pair's second column (closed from
     // this pass or, at column 0, from the previous one), not an opener whose
     // partner is in the next pass
-    const uint32_t in_pass = n >= RH ? 0xffffffffu : ((1u << n) - 1u);
+    const uint32_t in_pass = n >= 32 ? 0xffffffffu : ((1u << n) - 1u);
     const uint32_t act = in_pass & ~((opens << 1) | (first_done ? 1u : 0u)) & ~(straddle ? (1u << (n - 1)) : 0u);
-    if (p0 > 0 && w > 0) {
-      // the staging waves 1-3 loaded during the previous pass's triangle
-      const int nj = stage_cols(p0), tot = RH * nj;
-#pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        const int e = (tid - 64) + u * NPF;
-        if (e < tot) sh.Lq[(e / nj) * RQLD + p0 + e % nj] = pre[u];
-      }
-      if (tid - 64 < RH) sh.dq[tid - 64] = dqn;
-    }
-    // the pass's two column tiles, from their owners' registers
+    const int cb = (p0 / RPW) & 1;  // this pass's Lq / dq buffer
+    // the pass's column tiles, from their owners' registers
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct)
-      if (ct == pc || ct == pc + 1) {
+      if (ct >= pc && ct < pc + RPW / 16) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) sh.Xq[((ct - pc) * 16 + cl) * RXLD + rb + 4 * r] = y[ct][r];
       }
     __syncthreads();
     if (w == 0) {
-      double x[RH];
+      double x[RPW];
 #pragma unroll
-      for (int kk = 0; kk < RH; ++kk) x[kk] = sh.Xq[kk * RXLD + l];
+      for (int kk = 0; kk < RPW; ++kk) x[kk] = sh.Xq[kk * RXLD + l];
       if (first_done && live) {
         // pair (p0 - 1, p0), both final: scaled and stored together
         const double wk = xprev, wk1 = x[0];
@@ -1751,62 +1766,72 @@ __device__ void replay_unit_mfma(const SnCtx<double>& c, int kb, int ke, int64_t
       // The coefficients stream through a ring of TW registers, each read TW
       // terms before its use; the scheduling barriers keep the reads from
       // being hoisted further (register pressure).
-      // Terms run k ascending, j = k+1 .. RH-1 within k; (pk, pj) is the term
+      // Terms run k ascending, j = k+1 .. RPW-1 within k; (pk, pj) is the term
       // whose coefficient is read next (all indices fold to constants).
       constexpr int TW = 24;
-      const double* lq = sh.Lq + p0;
+      const double* lq = sh.Lq[cb] + p0;
       double ring[TW];
       int pk = 0, pj = 1, t = 0;
 #pragma unroll
       for (int u = 0; u < TW; ++u) {
         ring[u] = lq[pk * RQLD + pj];
-        if (++pj == RH) { ++pk; pj = pk + 1; }
+        if (++pj == RPW) { ++pk; pj = pk + 1; }
       }
 #pragma unroll
-      for (int k = 0; k < RH - 1; ++k) {
+      for (int k = 0; k < RPW - 1; ++k) {
 #pragma unroll
-        for (int j = k + 1; j < RH; ++j) {
+        for (int j = k + 1; j < RPW; ++j) {
           const double v = ring[t % TW];
-          if (pk < RH - 1) {
+          if (pk < RPW - 1) {
             ring[t % TW] = lq[pk * RQLD + pj];
-            if (++pj == RH) { ++pk; pj = pk + 1; }
+            if (++pj == RPW) { ++pk; pj = pk + 1; }
           }
           x[j] = x[j] - x[k] * v;
           if (t % 8 == 7) __builtin_amdgcn_sched_barrier(0);
           ++t;
         }
       }
-      xprev = x[RH - 1];
-      qprev = sh.dq[RH - 1];
+      xprev = x[RPW - 1];
+      qprev = sh.dq[cb][RPW - 1];
 #pragma unroll
-      for (int kk = 0; kk < RH; ++kk) sh.Xq[kk * RXLD + l] = -x[kk];
-    } else if (p0 + RH < pw) {
-      // waves 1-3: load the next pass's staging and D coefficients meanwhile
-      const int q0 = p0 + RH, nj = stage_cols(q0), tot = RH * nj;
+      for (int kk = 0; kk < RPW; ++kk) sh.Xq[kk * RXLD + l] = -x[kk];
+    } else if (p0 + RPW < pw) {
+      const int q0 = p0 + RPW, nj = stage_cols(q0), tot = RPW * nj;
       const uint32_t on = pass_opens(q0);
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int e = (tid - 64) + u * NPF;
         pre[u] = e < tot ? lq_value(q0, nj, on, e) : 0.0;
       }
-      if (tid - 64 < RH) dqn = dq_value(q0 + tid - 64);
+      if (tid - 64 < RPW) dqn = dq_load(q0 + tid - 64);
     }
     first_done = straddle;
     __syncthreads();
-    if (p0 + RH < pw) {
+    if (p0 + RPW < pw) {
 #pragma unroll
-      for (int kk = 0; kk < RH; kk += 4) {
+      for (int kk = 0; kk < RPW; kk += 4) {
         const double a = sh.Xq[(kk + (l >> 4)) * RXLD + 16 * w + cl];
 #pragma unroll
         for (int ct = 0; ct < NCT; ++ct)
-          if (ct >= pc + 2 && ct * 16 < pw) {
-            const double b = sh.Lq[(kk + (l >> 4)) * RQLD + ct * 16 + cl];
+          if (ct >= pc + RPW / 16 && ct * 16 < pw) {
+            const double b = sh.Lq[cb][(kk + (l >> 4)) * RQLD + ct * 16 + cl];
             y[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, y[ct], 0, 0, 0);
           }
       }
+      if (w > 0) {
+        // the next pass's staging and D coefficients into the other buffer
+        // (last read in the previous pass) before this pass's stores
+        const int q0 = p0 + RPW, nj = stage_cols(q0), tot = RPW * nj;
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int e = (tid - 64) + u * NPF;
+          if (e < tot) sh.Lq[cb ^ 1][(e / nj) * RQLD + q0 + e % nj] = pre[u];
+        }
+        if (tid - 64 < RPW) sh.dq[cb ^ 1][tid - 64] = dq_make(dqn);
+      }
     }
-    // the pass's D scaling and stores, 8 columns per wave
-    replay_chunk_scale_store(c, kb, p0, 8 * w, sh.Xq + l, act, act & opens, sh.dq, live, row);
+    // the pass's D scaling and stores, RCW columns per wave
+    replay_chunk_scale_store(c, kb, p0, RCW * w, sh.Xq + l, act, act & opens, sh.dq[cb], live, row);
     __syncthreads();
   }
 }
@@ -1960,7 +1985,7 @@ __global__ __launch_bounds__(NT) void k_wide_replay(DevSym S, DevFac<T> Fc, cons
   }
   __syncthreads();
   if constexpr (std::is_same<T, double>::value) {
-    static_assert(REPLAY_ROWS == RU && RH == 32, "one MFMA unit per work item, two column tiles per pass");
+    static_assert(REPLAY_ROWS == RU, "one MFMA unit per work item");
     replay_unit_mfma(c, kb, ke, row0, nr, *reinterpret_cast<ReplayMfmaLds*>(smem_raw), cperm, two_s);
   } else {
     T* X = reinterpret_cast<T*>(smem_raw);  // [pw][RR] column-major
