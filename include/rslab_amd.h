@@ -64,6 +64,13 @@ typedef struct {
   uint64_t nd_seed;       /* rslab-metis seed, default 1 */
   double wide_flops;      /* h*w*w above which a supernode's cdiv uses the
                              multi-workgroup panel path (tuning knob) */
+  double drop_tol;        /* SolverSettings::drop_tol (multifrontal_ldlt.rs:640-643):
+                             < 0 = None (default); tau >= 0 drops stored L
+                             entries below tau * (largest in their column) */
+  int64_t cmod_cap;       /* cmod product-buffer cap in elements; < 0 = automatic
+                             (a quarter of the free device memory); levels that
+                             need more take the gathered-tile cmod, 0 forces it
+                             for every level (real field) */
 } rsl_settings;
 
 typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */
@@ -91,11 +98,13 @@ int rsl_factor_inertia_batch(rsl_solver *f, int32_t count, int64_t *inertia);
 
 /* Symbolic sizes: out[0]=n out[1]=nsuper out[2]=sum of column counts
  * out[3]=panel entries out[4]=levels out[5]=children entries
- * out[6]=full permuted pattern nnz out[7]=stored L entries (padded) and
+ * out[6]=full permuted pattern nnz out[7]=stored L entries (padded)
+ * out[8]=cmod product-buffer elements of the largest level
+ * out[9]=solve scratch rows (off-diagonal panel rows) and
  * work figures: flops[0]=factor flops sum (h-k)^2, flops[1]=geom flops
  * sum h^2 w (MultifrontalSymbolic / estimate_memory,
  * src/numeric/sparse_solver.rs:556-672). */
-int rsl_symbolic_info(const rsl_symbolic *sym, int64_t out[8], double flops[2]);
+int rsl_symbolic_info(const rsl_symbolic *sym, int64_t out[10], double flops[2]);
 
 /* SymbolicFactorization fields (src/symbolic/mod.rs:333-405): permutation
  * (new->old), inverse, supernode first column / width / panel height,
@@ -129,9 +138,12 @@ int rsl_refactor_device(rsl_solver *f, const void *d_values, void *stream);
 int rsl_solve(rsl_solver *f, const double *b, int64_t nrhs, double *x);
 
 /* Device-resident variant of rsl_solve: d_b, d_x device pointers (row-major
- * n x nrhs), enqueued on `stream` (a hipStream_t; NULL is the null stream) and
- * asynchronous with respect to the host. Calls on one factor handle are
- * serialized by the library (they share its device workspace). */
+ * n x nrhs), asynchronous with respect to the host. The solve starts after
+ * the work enqueued on `stream` (a hipStream_t; NULL is the null stream)
+ * before the call, and work enqueued on `stream` after the call starts after
+ * the solve. It runs on the handle's own stream, so every later call on the
+ * same handle (refactor, refine, solve, on any stream) is ordered after it and
+ * cannot overwrite the factor or the shared workspace it reads. */
 int rsl_solve_device(rsl_solver *f, const void *d_b, int64_t nrhs, void *d_x,
                      void *stream);
 

@@ -12,7 +12,35 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Reduced-precision Schur products (real field only): 0 = FP64, 1 = FP32,
+ * 2 = BF16 inputs. The cmod products L_k G^T and the deferred Schur products
+ * L21 (L D)^T then take both operands rounded to FP32 (to BF16 by way of FP32,
+ * round to nearest even) and sum in FP32; the sum is subtracted from the
+ * FP64 panel. Pivoting, D, the 2x2 inverses and the replay stay FP64. */
+static int ora_prec = 0;
+
+static inline float ora_lowp(double v) {
+  float f = (float)v;
+  if (ora_prec == 2) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+  }
+  return f;
+}
+
+float ora_round_lowp(double v, int prec) {
+  int keep = ora_prec;
+  ora_prec = prec;
+  float f = ora_lowp(v);
+  ora_prec = keep;
+  return f;
+}
+
 /* ---- real field ---- */
+#define LOWP 1
 #define T double
 #define SFX f64
 #define MAG(x) fabs(x)
@@ -36,8 +64,10 @@
 #undef ZERO
 #undef ONE
 #undef FMADD
+#undef LOWP
 
 /* ---- complex-symmetric field ---- */
+#define LOWP 0
 typedef double _Complex c64;
 static inline c64 c64_recip(c64 z) {
   double ns = creal(z) * creal(z) + cimag(z) * cimag(z);
@@ -58,10 +88,11 @@ static inline c64 c64_recip(c64 z) {
 
 ora_factors *ora_factor(const ora_sym *s, int cplx, const int64_t *a_colp,
                         const int32_t *a_rowi, const double *a_val, int panel_nb,
-                        int pivot, double abs_floor, double drop_tol) {
+                        int pivot, double abs_floor, double drop_tol, int prec) {
   ora_factors *f = calloc(1, sizeof(ora_factors));
   f->n = s->n;
   f->cplx = cplx;
+  ora_prec = cplx ? 0 : prec;
   if (cplx)
     factor_c64(s, a_colp, a_rowi, (const c64 *)a_val, panel_nb, pivot, abs_floor, drop_tol, f);
   else

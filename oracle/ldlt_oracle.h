@@ -55,10 +55,15 @@ typedef struct {
 
 /* pivot: 0 = Fail, 1 = PerturbToEps{abs_floor}, 2 = ForceAccept.
  * a_val holds the (already equilibrated) values of the user-order lower CSC.
- * drop_tol < 0 means None. */
+ * drop_tol < 0 means None. prec (real field): 0 = FP64; 1 = FP32, 2 = BF16
+ * Schur products -- the cmod and deferred-Schur products take operands
+ * rounded to that format and sum in FP32, the sum is subtracted in FP64
+ * (emulation of the device's reduced-precision paths; not in rslab). */
 ora_factors *ora_factor(const ora_sym *s, int cplx, const int64_t *a_colp,
                         const int32_t *a_rowi, const double *a_val, int panel_nb,
-                        int pivot, double abs_floor, double drop_tol);
+                        int pivot, double abs_floor, double drop_tol, int prec);
+/* v rounded as the reduced-precision products round their operands. */
+float ora_round_lowp(double v, int prec);
 void ora_free(ora_factors *f);
 /* solve_ldlt_many: b, x row-major n x nrhs (complex interleaved). 0 = ok. */
 int ora_solve_many(const ora_factors *f, const double *b, int32_t nrhs, double *x);

@@ -223,6 +223,7 @@ static int FN(factor)(const ora_sym *s, const int64_t *a_colp, const int32_t *a_
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   int status = 0;
   T *gbuf = NULL, *ubuf = NULL;
+  float *fbuf = NULL; /* reduced-precision product sums */
   int64_t gcap = 0, ucap = 0;
 
   for (int64_t sn = 0; sn < ns && status == 0; ++sn) {
@@ -250,10 +251,28 @@ static int FN(factor)(const ora_sym *s, const int64_t *a_colp, const int32_t *a_
       const FN(node) *K = &nodes[kk];
       int64_t mrows = nok - p0;
       if (gcap < npk * nck) { free(gbuf); gcap = npk * nck; gbuf = malloc(sizeof(T) * gcap); }
-      if (ucap < mrows) { free(ubuf); ucap = mrows; ubuf = malloc(sizeof(T) * ucap); }
+      if (ucap < mrows) {
+        free(ubuf); free(fbuf);
+        ucap = mrows; ubuf = malloc(sizeof(T) * ucap); fbuf = malloc(sizeof(float) * ucap);
+      }
       FN(apply_d)(K->panel, nrk, nck + p0, npk, nck, K->d, K->dsub, K->two, gbuf);
       for (int64_t c = 0; c < npk; ++c) {
         int64_t tcol = ok[p0 + c] - first;
+        if (LOWP && ora_prec) {
+          /* reduced-precision product: rounded operands, FP32 sums, FP64 subtract */
+          for (int64_t r = c; r < mrows; ++r) fbuf[r] = 0.0f;
+          for (int64_t ck = 0; ck < nck; ++ck) {
+            float g = ora_lowp(REAL(gbuf[c + ck * npk]));
+            const T *lc = K->panel + nck + p0 + ck * nrk;
+            for (int64_t r = c; r < mrows; ++r) fbuf[r] = fbuf[r] + ora_lowp(REAL(lc[r])) * g;
+          }
+          T *dstl = P + tcol * nrow;
+          for (int64_t r = c; r < mrows; ++r) {
+            int64_t li = gloc[ok[p0 + r]];
+            dstl[li] = dstl[li] - (double)fbuf[r];
+          }
+          continue;
+        }
         /* U[r, c] = sum_ck L_k[p0 + r, ck] * G[c, ck] for r >= c */
         for (int64_t r = c; r < mrows; ++r) ubuf[r] = ZERO;
         for (int64_t ck = 0; ck < nck; ++ck) {
@@ -381,6 +400,20 @@ static int FN(factor)(const ora_sym *s, const int64_t *a_colp, const int32_t *a_
         T *g = malloc(sizeof(T) * mt * pw);
         /* G = L21 * D with L21 = P[ke.., kb..ke) (the panel is contiguous) */
         FN(apply_d)(P + kb * nrow, nrow, ke, mt, pw, d + kb, dsub + kb, two + kb, g);
+        if (LOWP && ora_prec) {
+          /* reduced-precision product L21 (L D)^T: rounded operands, FP32
+           * sums over the panel's columns, FP64 subtract */
+          for (int64_t cc = 0; cc < cw; ++cc) {
+            T *dst = P + (ke + cc) * nrow + ke;
+            for (int64_t rr = cc; rr < mt; ++rr) {
+              float acc = 0.0f;
+              for (int64_t q = 0; q < pw; ++q)
+                acc = acc + ora_lowp(REAL(P[(ke + rr) + (kb + q) * nrow])) * ora_lowp(REAL(g[q * mt + cc]));
+              dst[rr] = dst[rr] - (double)acc;
+            }
+          }
+          cw = 0; /* done: skip the FP64 loop below */
+        }
         for (int64_t cc = 0; cc < cw; ++cc) {
           T *dst = P + (ke + cc) * nrow + ke;
           for (int64_t q = 0; q < pw; ++q) {
@@ -486,7 +519,7 @@ static int FN(factor)(const ora_sym *s, const int64_t *a_colp, const int32_t *a_
     free(nodes[sn].two); free(nodes[sn].lperm); free(rs[sn]);
   }
   free(nodes); free(rs); free(rsn); free(tbuf); free(mark); free(col2s); free(ucnt);
-  free(uidx); free(upos); free(e_off); free(e_of_g); free(gloc); free(gbuf); free(ubuf);
+  free(uidx); free(upos); free(e_off); free(e_of_g); free(gloc); free(gbuf); free(ubuf); free(fbuf);
   free(pcnt); free(prow); free(pval); free(pos);
   return status;
 }

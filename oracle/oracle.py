@@ -56,7 +56,17 @@ class OraFactors(C.Structure):
 _lib = _load()
 _lib.ora_factor.restype = C.POINTER(OraFactors)
 _lib.ora_factor.argtypes = [C.POINTER(OraSym), C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
-                            C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_double]
+                            C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
+_lib.ora_round_lowp.restype = C.c_float
+_lib.ora_round_lowp.argtypes = [C.c_double, C.c_int]
+
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+
+
+def round_lowp(v: float, prec: int) -> float:
+    """v rounded as the reduced-precision products round their operands
+    (prec 1: to FP32; 2: to BF16 by way of FP32, round to nearest even)."""
+    return float(_lib.ora_round_lowp(float(v), int(prec)))
 _lib.ora_free.argtypes = [C.POINTER(OraFactors)]
 _lib.ora_solve_many.restype = C.c_int
 _lib.ora_solve_many.argtypes = [C.POINTER(OraFactors), C.POINTER(C.c_double), C.c_int32,
@@ -176,8 +186,11 @@ def sym_norm_inf(a) -> float:
                                        _p(v, C.c_double)))
 
 
-def factor(sym, a_scaled, panel_nb=64, pivot=0, abs_floor=0.0, drop_tol=-1.0) -> OracleFactors:
-    """factor_left_looking on an equilibrated matrix given the exported symbolic."""
+def factor(sym, a_scaled, panel_nb=64, pivot=0, abs_floor=0.0, drop_tol=-1.0, prec=0) -> OracleFactors:
+    """factor_left_looking on an equilibrated matrix given the exported symbolic
+    (prec: 0 FP64, 1 FP32 / 2 BF16 Schur-product emulation, real field)."""
+    if drop_tol is None:
+        drop_tol = -1.0
     keep = dict(
         perm=np.ascontiguousarray(sym.perm, dtype=np.int32),
         perm_inv=np.ascontiguousarray(sym.perm_inv, dtype=np.int32),
@@ -196,7 +209,7 @@ def factor(sym, a_scaled, panel_nb=64, pivot=0, abs_floor=0.0, drop_tol=-1.0) ->
     ri = np.ascontiguousarray(a_scaled.row_idx, dtype=np.int32)
     cpt = np.ascontiguousarray(a_scaled.col_ptr, dtype=np.int64)
     h = _lib.ora_factor(C.byref(s), cp, _p(cpt, C.c_int64), _p(ri, C.c_int32), _p(v, C.c_double),
-                        panel_nb, pivot, abs_floor, drop_tol)
+                        panel_nb, pivot, abs_floor, drop_tol, int(prec))
     f = h.contents
     if f.status != 0:
         _lib.ora_free(h)

@@ -36,6 +36,8 @@ class Settings(C.Structure):
         ("precision", C.c_int32),
         ("nd_seed", C.c_uint64),
         ("wide_flops", C.c_double),
+        ("drop_tol", C.c_double),
+        ("cmod_cap", C.c_int64),
     ]
 
 

@@ -41,7 +41,7 @@ struct DevFac {
   int32_t* rs_e;      // per panel row (rs_idx order): its elimination position
   int32_t* perm_e;    // elimination position -> user index
   int32_t* deep;      // per-panel interchange partners (scratch, nb per supernode)
-  unsigned long long* counters;  // [0] pos [1] neg [2] zero [3] perturbed
+  unsigned long long* counters;  // [0] pos [1] neg [2] zero [3] perturbed [4] first failure (s << 32 | column)
   int32_t* sn_inertia;          // per supernode: pos, neg, zero
   int32_t* status;    // [0] 0 ok / 1 rank deficient, [1] failing supernode
 };

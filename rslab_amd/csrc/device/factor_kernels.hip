@@ -123,6 +123,21 @@ __global__ void k_rows_e(DevSym S, DevFac<T> Fc, int64_t nrows) {
   if (q < nrows) Fc.rs_e[q] = Fc.e_of_g[S.rs_idx[q]];
 }
 
+// Factor status at the start of a numeric factor: inertia and perturbation
+// counters cleared, no failure recorded (a kernel, not a memset, so the
+// launch sequence can be captured and replayed as a graph).
+__global__ void k_factor_init(unsigned long long* __restrict__ counters, int32_t* __restrict__ status) {
+  const int t = threadIdx.x;
+  if (t < 4) counters[t] = 0ull;
+  if (t == 4) counters[4] = ~0ull;
+  if (t == 5) status[0] = 0;
+  if (t == 6) status[1] = INT32_MAX;
+}
+
+void launch_factor_init(unsigned long long* counters, int32_t* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_factor_init, dim3(1), dim3(64), 0, st, counters, status);
+}
+
 template <class T>
 __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -948,7 +963,7 @@ __device__ int panel_getf2(const SnCtx<T>& c, int kb, int ke, T* Ab, T* l1s, T* 
     int kstep, kp;
     bool plain = false;
     if (fmax(absakk, colmax) == 0.0) {
-      if (!pol.have_floor) return 1;
+      if (!pol.have_floor) return 1 + k;  // nonzero: the failing column, plus one
       kstep = 1;
       kp = k;
       plain = true;
@@ -1290,11 +1305,14 @@ __device__ void cdiv_emit(const DevSym& S, const DevFac<T>& Fc, const SnCtx<T>& 
   }
 }
 
+// A zero pivot under the Fail policy at column col of supernode s: the
+// first failure in (supernode, column) order is kept in counters[4].
 template <class T>
-__device__ void flag_failure(const DevFac<T>& Fc, int s) {
+__device__ void flag_failure(const DevFac<T>& Fc, int s, int col) {
   if (threadIdx.x == 0) {
     atomicMax(&Fc.status[0], 1);
     atomicMin(&Fc.status[1], s);
+    atomicMin(&Fc.counters[4], ((unsigned long long)(unsigned)s << 32) | (unsigned)col);
   }
 }
 
@@ -1869,7 +1887,7 @@ __global__ __launch_bounds__(NT) void k_cdiv_small(DevSym S, DevFac<T> Fc,
     panel_load<NT>(c, kb, ke, sh.u.Ab);
     int st = panel_getf2<NT>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert);
     if (st) {
-      flag_failure(Fc, s);
+      flag_failure(Fc, s, kb + st - 1);
       return;
     }
     panel_store<NT, true>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap, sh.w1s);
@@ -1921,7 +1939,7 @@ __global__ __launch_bounds__(GT) void k_wide_getf2(DevSym S, DevFac<T> Fc, const
   panel_load<G>(c, kb, ke, sh.u.Ab);
   int st = panel_getf2<G>(c, kb, ke, sh.u.Ab, sh.l1s, sh.l2s, sh.w1s, sh.w2s, sh.deep_s, sh.two_s, sh.lperm_s, pol, &sh.npert);
   if (st) {
-    flag_failure(Fc, s);
+    flag_failure(Fc, s, kb + st - 1);
     return;
   }
   panel_store<G, false>(c, kb, ke, sh.u.Ab, sh.deep_s, sh.two_s, sh.lperm_s, sh.cperm_s, &sh.any_swap, sh.w1s);
@@ -2154,6 +2172,35 @@ void launch_wide_level(const DevSym& S, const DevFac<T>& Fc, const WItem* witems
   hipLaunchKernelGGL(k_wide_emit<T>, dim3((unsigned)nwide), dim3(NT), 0, st, S, Fc, wide_list);
 }
 
+// Drop tolerance at emit (ldlt_emit_and_free, multifrontal_ldlt.rs:2344-2366):
+// in every stored column, the entries below the unit diagonal (past the 2x2
+// coupling row) whose magnitude is not >= tau * (largest such magnitude) are
+// zeroed; the downloaded L then omits them as the reference's emit does, and
+// the solve reads zeros there. One workgroup per supernode, one wave per
+// column (columns are contiguous, leading dimension h).
+template <class T>
+__global__ __launch_bounds__(256) void k_drop(DevSym S, DevFac<T> Fc, double tau) {
+  const int s = blockIdx.x;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], eo = S.e_off[s];
+  T* P = Fc.pool + S.panel_off[s];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int p = wv; p < w; p += 4) {  // wave-uniform: every lane reaches the reduction
+    const int64_t i0 = Fc.two[eo + p] ? p + 2 : p + 1;
+    T* col = P + (int64_t)p * h;
+    double m = 0.0;
+    for (int64_t i = i0 + lane; i < h; i += 64) m = fmax(m, F<T>::mag(col[i]));
+    const double thr = tau * wave_max_dpp(m);
+    for (int64_t i = i0 + lane; i < h; i += 64)
+      if (!(F<T>::mag(col[i]) >= thr)) col[i] = F<T>::zero();
+  }
+}
+
+template <class T>
+void launch_drop(const DevSym& S, const DevFac<T>& Fc, double tau, hipStream_t st) {
+  if (S.ns > 0) hipLaunchKernelGGL(k_drop<T>, dim3((unsigned)S.ns), dim3(256), 0, st, S, Fc, tau);
+}
+
 template <class T>
 hipError_t configure_factor_kernels() {
   hipError_t e = hipSuccess;
@@ -2182,6 +2229,7 @@ hipError_t configure_factor_kernels() {
                                      int, const int32_t*, int, PivotPolicy, int, hipStream_t,    \
                                      LaunchProfiler*);                              \
   template hipError_t configure_factor_kernels<T>();                                           \
+  template void launch_drop<T>(const DevSym&, const DevFac<T>&, double, hipStream_t);          \
   template size_t cdiv_smem<T>();
 
 RSL_INST(double)

@@ -1,5 +1,6 @@
 // C ABI over the host runtime (declarations and reference citations in
 // include/rslab_amd.h).
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <memory>
@@ -50,6 +51,9 @@ rsl::FactorSettings factor_settings(const rsl_settings& s) {
   if (s.precision < RSL_PREC_FP64 || s.precision > RSL_PREC_BF16_SCHUR)
     throw rsl::Error(RSL_INVALID_INPUT, "unknown precision");
   fs.precision = static_cast<rsl::Precision>(s.precision);
+  if (std::isnan(s.drop_tol)) throw rsl::Error(RSL_INVALID_INPUT, "drop_tol must not be NaN");
+  fs.drop_tol = s.drop_tol;
+  fs.cmod_cap = s.cmod_cap;
   return fs;
 }
 }  // namespace
@@ -93,6 +97,8 @@ void rsl_settings_default(rsl_settings* s) {
   s->precision = RSL_PREC_FP64;
   s->nd_seed = 1;
   s->wide_flops = 2.5e5;
+  s->drop_tol = -1.0;
+  s->cmod_cap = -1;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }
@@ -101,6 +107,15 @@ int rsl_order(int32_t method, int64_t n, const int64_t* col_ptr, const int64_t* 
               uint64_t seed, int32_t* perm, double* stats) {
   return guard([&] {
     if (n < 0 || n >= INT32_MAX) throw rsl::Error(RSL_INVALID_INPUT, "dimension out of range");
+    if (!col_ptr || !perm || (n > 0 && col_ptr[n] > 0 && !row_idx))
+      throw rsl::Error(RSL_INVALID_INPUT, "null pattern or output pointer");
+    // the orderings index their work arrays by these entries: reject a
+    // malformed pattern here instead of writing out of bounds there
+    if (col_ptr[0] != 0) throw rsl::Error(RSL_INVALID_INPUT, "col_ptr[0] must be 0");
+    for (int64_t j = 0; j < n; ++j)
+      if (col_ptr[j + 1] < col_ptr[j]) throw rsl::Error(RSL_INVALID_INPUT, "col_ptr is not non-decreasing");
+    for (int64_t k = 0; k < col_ptr[n]; ++k)
+      if (row_idx[k] < 0 || row_idx[k] >= n) throw rsl::Error(RSL_INVALID_INPUT, "row index out of bounds");
     rsl::Pattern p;
     p.n = (int32_t)n;
     p.colp.assign(col_ptr, col_ptr + n + 1);
@@ -203,7 +218,7 @@ int rsl_analyze_batch(int32_t count, const int64_t* n, const int64_t* const* col
   });
 }
 
-int rsl_symbolic_info(const rsl_symbolic* sym, int64_t out[8], double flops[2]) {
+int rsl_symbolic_info(const rsl_symbolic* sym, int64_t out[10], double flops[2]) {
   return guard([&] {
     const rsl::Plan& P = sym->ds->plan;
     int64_t nch = 0;
@@ -217,6 +232,8 @@ int rsl_symbolic_info(const rsl_symbolic* sym, int64_t out[8], double flops[2]) 
       out[5] = nch;
       out[6] = (int64_t)P.sym.permuted.rowi.size();
       out[7] = P.nnzl_panel;
+      out[8] = P.c_elems_max;
+      out[9] = P.u_total;
     }
     if (flops) {
       flops[0] = P.factor_flops;
@@ -304,7 +321,7 @@ int rsl_solve_device(rsl_solver* f, const void* d_b, int64_t nrhs, void* d_x, vo
   return guard([&] {
     std::lock_guard<std::mutex> lk(f->mu);
     if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
-    f->f->solve_device(d_b, nrhs, d_x, static_cast<hipStream_t>(stream));
+    f->f->solve_async(d_b, nrhs, d_x, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -15,6 +15,8 @@ namespace rsl {
 
 // kernel launchers (factor_kernels.hip, solve_kernels.hip, refine_kernels.hip)
 template <class T> void launch_fill(T*, int64_t, T, hipStream_t);
+void launch_factor_init(unsigned long long*, int32_t*, hipStream_t);
+template <class T> void launch_drop(const DevSym&, const DevFac<T>&, double, hipStream_t);
 template <class T> void launch_rows_e(const DevSym&, const DevFac<T>&, int64_t, hipStream_t);
 template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, int, hipStream_t, LaunchProfiler*);
@@ -291,7 +293,7 @@ class DeviceFactor final : public DeviceFactorBase {
   static void grow(DBuf& b, size_t nb) {
     if (b.bytes < nb) b.alloc(nb);
   }
-  hipEvent_t ev_caller = nullptr;
+  hipEvent_t ev_caller = nullptr, ev_done = nullptr;
   bool ev_ready = false;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
   DevFac<T> fv{};
@@ -299,7 +301,10 @@ class DeviceFactor final : public DeviceFactorBase {
   bool configured = false;
 
   ~DeviceFactor() override {
-    if (ev_ready) (void)hipEventDestroy(ev_caller);
+    if (ev_ready) {
+      (void)hipEventDestroy(ev_caller);
+      (void)hipEventDestroy(ev_done);
+    }
   }
 
   DeviceFactor(std::shared_ptr<DeviceSymbolic> s, const FactorSettings& fs) {
@@ -318,7 +323,7 @@ class DeviceFactor final : public DeviceFactorBase {
     e_of_g.alloc(sizeof(int32_t) * std::max<int64_t>(n, 1));
     perm_e.alloc(sizeof(int32_t) * std::max<int64_t>(n, 1));
     deep.alloc(sizeof(int32_t) * 2 * 128 * std::max<int64_t>(P.ns, 1));
-    counters.alloc(sizeof(unsigned long long) * 4);
+    counters.alloc(sizeof(unsigned long long) * 8);
     sn_inert.alloc(sizeof(int32_t) * 3 * std::max<int64_t>(P.ns, 1));
     rows_e.alloc(sizeof(int32_t) * std::max<int64_t>(P.rs_off[P.ns], 1));
     // index maps start at 0 so no later read can leave the vectors
@@ -331,6 +336,7 @@ class DeviceFactor final : public DeviceFactorBase {
       size_t fr = 0, tot = 0;
       hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
       c_cap = std::min<int64_t>(P.c_elems_max, (int64_t)(fr / 4 / esz));
+      if (fs.cmod_cap >= 0) c_cap = std::min<int64_t>(c_cap, fs.cmod_cap);
       if (c_cap > 0) {
         cbuf.alloc(esz * (size_t)c_cap);
         gpool.alloc(sizeof(double) * (size_t)std::max<int64_t>(P.g_total, 1));  // L21 D of every supernode
@@ -427,14 +433,35 @@ class DeviceFactor final : public DeviceFactorBase {
     }
   }
 
+  void make_events() {
+    if (ev_ready) return;
+    hip_check(hipEventCreateWithFlags(&ev_caller, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming), "hipEventCreate");
+    ev_ready = true;
+  }
+
   // the library stream waits for work the caller enqueued before the call
   void wait_for(hipStream_t caller) {
-    if (!ev_ready) {
-      hip_check(hipEventCreateWithFlags(&ev_caller, hipEventDisableTiming), "hipEventCreate");
-      ev_ready = true;
-    }
+    make_events();
     hip_check(hipEventRecord(ev_caller, caller), "hipEventRecord");
     hip_check(hipStreamWaitEvent(sym->stream, ev_caller, 0), "hipStreamWaitEvent");
+  }
+
+  // the caller's stream waits for the work this call enqueued on the library stream
+  void signal_to(hipStream_t caller) {
+    hip_check(hipEventRecord(ev_done, sym->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(caller, ev_done, 0), "hipStreamWaitEvent");
+  }
+
+  // Asynchronous solve for a caller's stream: runs on the library stream after
+  // the caller's earlier work, and the caller's stream waits for it. Every
+  // later call on this handle (refactor, refine, solve) also runs on the
+  // library stream, so it cannot overwrite the factor or the workspace while
+  // this solve still reads them.
+  void solve_async(const void* d_b, int64_t nrhs, void* d_x, hipStream_t caller) override {
+    wait_for(caller);
+    solve_device(d_b, nrhs, d_x, sym->stream);
+    signal_to(caller);
   }
 
   // numeric factor from device values (user lower CSC) with scale_d in place
@@ -448,9 +475,7 @@ class DeviceFactor final : public DeviceFactorBase {
       configured = true;
     }
     valid = false;
-    hip_check(hipMemsetAsync(counters.p, 0, counters.bytes, st), "memset");
-    const int32_t st_init[2] = {0, INT32_MAX};
-    hip_check(hipMemcpyAsync(status.p, st_init, sizeof(st_init), hipMemcpyHostToDevice, st), "status");
+    launch_factor_init(counters.as<unsigned long long>(), status.as<int32_t>(), st);
     LaunchProfiler* pf = prof_on ? &prof : nullptr;
     launch_fill<T>(fv.pool, P.panel_total, F<T>::zero(), st);
     launch_assemble<T>(S, fv, vals, scale_d.as<double>(), st, pf);
@@ -485,8 +510,11 @@ class DeviceFactor final : public DeviceFactorBase {
       }
     }
     launch_rows_e<T>(S, fv, P.rs_off[P.ns], st);
+    // drop tolerance at emit (ldlt_emit_and_free, multifrontal_ldlt.rs:2358-2366):
+    // after the last consumer of every panel, so it changes the stored L only
+    if (settings.drop_tol >= 0.0) launch_drop<T>(S, fv, settings.drop_tol, st);
     hip_check(hipGetLastError(), "factor launch");
-    unsigned long long cnt[4];
+    unsigned long long cnt[5];
     int32_t stat[2];
     hip_check(hipMemcpyAsync(cnt, counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st), "counters");
     hip_check(hipMemcpyAsync(stat, status.p, sizeof(stat), hipMemcpyDeviceToHost, st), "status");
@@ -494,7 +522,16 @@ class DeviceFactor final : public DeviceFactorBase {
     stats.t_factor_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stat[0] == 2) throw Error(DEVICE_ERROR, "device: inconsistent pivot map in the factor");
-    if (stat[0] != 0) throw Error(NUMERICALLY_RANK_DEFICIENT, "numerically rank deficient");
+    if (stat[0] != 0) {
+      // the first zero pivot in (supernode, column) order
+      const int32_t s = (int32_t)(cnt[4] >> 32), col = (int32_t)(cnt[4] & 0xffffffffu);
+      std::string msg = "numerically rank deficient";
+      if (cnt[4] != ~0ull && s >= 0 && s < P.ns)
+        msg += ": zero pivot at column " + std::to_string(col) + " of supernode " + std::to_string(s) + " (" +
+               std::to_string(P.sn_ncol[s]) + " columns, elimination index " + std::to_string(P.e_off[s] + col) +
+               ")";
+      throw Error(NUMERICALLY_RANK_DEFICIENT, msg);
+    }
     stats.inertia[0] = (int64_t)cnt[0];
     stats.inertia[1] = (int64_t)cnt[1];
     stats.inertia[2] = (int64_t)cnt[2];

@@ -59,6 +59,8 @@ struct FactorSettings {
   double abs_floor = 0.0;
   int scaling = 1;  // 1 OnePassInfNorm, 0 Identity
   Precision precision = Precision::FP64;
+  double drop_tol = -1.0;  // < 0: None
+  int64_t cmod_cap = -1;   // product-buffer element cap, < 0 automatic
 };
 
 // Device-resident plan of one analysed pattern.
@@ -125,6 +127,8 @@ class DeviceFactorBase {
   virtual void refactor(const double* values) = 0;
   virtual void solve_host(const double* b, int64_t nrhs, double* x) = 0;
   virtual void solve_device(const void* d_b, int64_t nrhs, void* d_x, hipStream_t st) = 0;
+  // solve_device on the library stream, ordered after and before `caller`'s work
+  virtual void solve_async(const void* d_b, int64_t nrhs, void* d_x, hipStream_t caller) = 0;
   virtual void download(std::vector<int64_t>& colp, std::vector<int32_t>& rowi,
                         std::vector<double>& lval, std::vector<double>& d,
                         std::vector<double>& ds, std::vector<uint8_t>& two,

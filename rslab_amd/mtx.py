@@ -5,7 +5,8 @@ symmetric` (`read_mtx_complex` / `parse_mtx_complex`: `complex symmetric`,
 data lines `i j re im`), with the reference parser's rules (mtx.rs:182-360):
 the banner compared token by token, case-insensitively; comment and blank
 lines skipped before the size line `rows cols nnz` (square only); after it,
-blank lines skipped and every other line must be `i j value`; 1-based
+blank lines skipped and every other line must be `i j value`; the entry
+count must equal the declared nnz; 1-based
 indices checked against n; non-finite values rejected; upper-triangle
 entries folded to the lower triangle. `MtxMatrix.to_csc` is
 `CscMatrix::from_triplets` (duplicates summed, rows sorted per column).
@@ -53,7 +54,7 @@ def _parse(contents: str, source: str, field: str, nvt: int) -> MtxMatrix:
     if len(parts) != 3:
         raise IoError(f"{source}: line {k + 1}: expected 'rows cols nnz', got '{lines[k].strip()}'")
     try:
-        nr, nc, _nnz = (int(p) for p in parts)
+        nr, nc, nnz = (int(p) for p in parts)
     except ValueError:
         raise IoError(f"{source}: line {k + 1}: invalid size line '{lines[k].strip()}'") from None
     if nr != nc or nr < 0:
@@ -85,6 +86,9 @@ def _parse(contents: str, source: str, field: str, nvt: int) -> MtxMatrix:
         rows.append(i0)
         cols.append(j0)
         vals.append(v)
+    if len(rows) != nnz:
+        # the size line's nnz is the number of entries that follow (mtx.rs:374-381)
+        raise IoError(f"{source}: declared nnz {nnz} does not match the {len(rows)} entries in the file")
     dt = np.complex128 if nvt == 2 else np.float64
     return MtxMatrix(n, np.array(rows, dtype=np.int64), np.array(cols, dtype=np.int64), np.array(vals, dtype=dt))
 

@@ -211,8 +211,20 @@ class SolverSettings:
     precision: Precision = Precision.FP64
     nd_seed: int = 1
     wide_flops: float = 2.5e5
+    drop_tol: Optional[float] = None
+    # cmod product-buffer cap in elements (None: automatic); levels needing
+    # more take the gathered-tile cmod, 0 forces it everywhere
+    cmod_cap: Optional[int] = None
 
     # builders (multifrontal_ldlt.rs:616-779)
+    def with_drop_tol(self, tau: float):
+        """Incomplete-factor threshold dropping (with_drop_tol, multifrontal_ldlt.rs:640-643):
+        stored L entries below tau times the largest in their column are dropped."""
+        return dataclasses.replace(self, drop_tol=float(tau))
+
+    def with_cmod_cap(self, elems: Optional[int]):
+        return dataclasses.replace(self, cmod_cap=None if elems is None else int(elems))
+
     def with_ordering(self, o):
         return dataclasses.replace(self, ordering=OrderingMethod(o))
 
@@ -257,6 +269,10 @@ class SolverSettings:
         s.precision = int(self.precision)
         s.nd_seed = int(self.nd_seed)
         s.wide_flops = float(self.wide_flops)
+        if self.drop_tol is not None and not (self.drop_tol >= 0.0):
+            raise InvalidInput(f"drop_tol must be a non-negative number, got {self.drop_tol!r}")
+        s.drop_tol = -1.0 if self.drop_tol is None else float(self.drop_tol)
+        s.cmod_cap = -1 if self.cmod_cap is None else int(self.cmod_cap)
         return s
 
 
@@ -365,13 +381,17 @@ class LdltSymbolic:
         return self._n
 
     def info(self) -> dict:
-        out = np.zeros(8, dtype=np.int64)
+        out = np.zeros(10, dtype=np.int64)
         fl = np.zeros(2, dtype=np.float64)
         _check(lib.rsl_symbolic_info(self._h, ptr(out, C.c_int64), ptr(fl, C.c_double)))
         return dict(n=int(out[0]), nsuper=int(out[1]), colcount_nnz=int(out[2]),
                     panel_entries=int(out[3]), levels=int(out[4]), children=int(out[5]),
-                    pattern_nnz=int(out[6]), nnzl_padded=int(out[7]),
-                    factor_flops=float(fl[0]), geom_flops=float(fl[1]))
+                    pattern_nnz=int(out[6]), nnzl_padded=int(out[7]), cmod_elems_max=int(out[8]),
+                    solve_scratch_rows=int(out[9]), factor_flops=float(fl[0]), geom_flops=float(fl[1]))
+
+    def cmod_elems_max(self) -> int:
+        """Product-buffer elements the largest level's two-phase cmod needs."""
+        return self.info()["cmod_elems_max"]
 
     def export(self) -> SymbolicExport:
         inf = self.info()

@@ -114,3 +114,22 @@ def test_rng_matches_reference_xorshift():
         want.append(-2.0 + 5.0 * ((v >> 11) / float(1 << 53)))
     got = matgen.rng_uniform(5, -2.0, 3.0, 4)
     assert np.array_equal(got, np.array(want))
+
+
+@pytest.mark.parametrize("case", ["oob", "negative", "colptr", "colptr0"])
+def test_order_rejects_malformed_pattern(case):
+    """rsl_order validates the pattern before the orderings index their work
+    arrays by it (an out-of-range row, a negative row, decreasing or offset
+    column pointers are InvalidInput, not out-of-bounds writes)."""
+    cp, ri = [0, 2, 3, 4], [0, 1, 1, 2]
+    if case == "oob":
+        ri = [0, 7, 1, 2]
+    elif case == "negative":
+        ri = [0, -1, 1, 2]
+    elif case == "colptr":
+        cp = [0, 3, 2, 4]
+    else:
+        cp = [1, 2, 3, 4]
+    for order in (R.amd_order, R.nd_order):
+        with pytest.raises(R.InvalidInput):
+            order(3, np.array(cp), np.array(ri))

@@ -239,3 +239,55 @@ def test_pivot_sequence_stability_decides_the_parity_bar():
         a4 = perturbed(a3, seed)
         g1 = O.factor(e3, O.scaled(a4, O.onepass_scale(a4)))
         assert np.array_equal(g0.perm, g1.perm) and np.array_equal(g0.two_by_two, g1.two_by_two)
+
+
+def _random_sym(n, seed, cplx, density=0.05):
+    """Random symmetric (complex-symmetric when cplx) lower CSC with a full diagonal."""
+    rng = np.random.default_rng(seed)
+    M = np.where(rng.random((n, n)) < density, rng.standard_normal((n, n)), 0.0)
+    if cplx:
+        M = M + 1j * np.where(M != 0, rng.standard_normal((n, n)), 0.0)
+    M = np.tril(M, -1)
+    M = M + M.T  # plain transpose: complex-symmetric, never Hermitian
+    M[np.arange(n), np.arange(n)] = rng.standard_normal(n) + (1j * rng.standard_normal(n) if cplx else 0)
+    return dense_lower_csc(M), M
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_symv_and_norm_inf_equal_dense_products(cplx):
+    """CscMatrix::symv (csc.rs:294-309) applies the stored lower triangle as
+    the full symmetric matrix, and sym_norm_inf (mixed.rs:358-371) is its
+    largest absolute row sum: both equal numpy on the dense matrix. Every
+    backward-error bar of the device tests goes through these two."""
+    a, M = _random_sym(157, 3 + cplx, cplx)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(a.n) + (1j * rng.standard_normal(a.n) if cplx else 0)
+    y = O.symv(a, x)
+    ref = M @ x
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+    nrm = O.sym_norm_inf(a)
+    assert nrm == pytest.approx(np.abs(M).sum(1).max(), rel=1e-14)
+    # the Python mirror (used by MixedLdltSolver) agrees with both
+    assert np.abs(a.symv(x) - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert a.norm_inf() == pytest.approx(nrm, rel=1e-14)
+    # planted mistakes the comparison must catch: the transposed half dropped
+    # (lower-triangle product), a conjugated transpose (Hermitian reading),
+    # and the row sum of the stored triangle only
+    cols = np.repeat(np.arange(a.n), np.diff(a.col_ptr))
+    Ml = np.zeros_like(M)
+    Ml[a.row_idx, cols] = a.values
+    assert np.abs(Ml @ x - ref).max() > 1e-3 * np.abs(ref).max()
+    assert np.abs(np.abs(Ml).sum(1).max() - nrm) > 1e-6 * nrm or np.abs(np.abs(Ml).sum(0).max() - nrm) > 1e-6 * nrm
+    if cplx:
+        Mh = np.tril(M) + np.tril(M, -1).conj().T
+        assert np.abs(Mh @ x - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+def test_backward_error_known_answer():
+    """Normwise backward error |b - A x|inf / (|A|inf |x|inf + |b|inf)
+    (mixed.rs:151-155) on a hand-computed 2x2 case: A = [[2, 1], [1, -3]],
+    x = (1, 1), b = (3, -1): r = (0, 1), |A|inf = 4, so 1 / (4 + 3)."""
+    a = R.CscMatrix.from_triplets(2, [0, 1, 1], [0, 0, 1], [2.0, 1.0, -3.0])
+    be = O.backward_error(a, np.array([1.0, 1.0]), np.array([3.0, -1.0]))
+    assert be[0] == pytest.approx(1.0 / 7.0, rel=1e-15)
+    assert O.sym_norm_inf(a) == 4.0

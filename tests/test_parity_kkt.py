@@ -115,6 +115,8 @@ def test_reader_rules():
         "%%MatrixMarket matrix coordinate real symmetric\n2 2 1\n1 1\n",     # missing value
         "%%MatrixMarket matrix coordinate real symmetric\n2 2 1\n% late\n1 1 1\n",  # comment after size
         "%%MatrixMarket matrix coordinate real symmetric\n% only comments\n",
+        "%%MatrixMarket matrix coordinate real symmetric\n2 2 3\n1 1 1\n2 2 1\n",  # fewer than declared
+        "%%MatrixMarket matrix coordinate real symmetric\n2 2 1\n1 1 1\n2 2 1\n",  # more than declared
     ]
     for t in bad:
         with pytest.raises(R.IoError):
