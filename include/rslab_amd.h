@@ -14,6 +14,7 @@
 #ifndef RSLAB_AMD_H
 #define RSLAB_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -71,6 +72,9 @@ typedef struct {
                              (a quarter of the free device memory); levels that
                              need more take the gathered-tile cmod, 0 forces it
                              for every level (real field) */
+  int32_t graphs;         /* 1 (default): a same-pattern refactor captures the
+                             factor's launch sequence once as a HIP graph and
+                             replays it; 0: every factor launches eagerly */
 } rsl_settings;
 
 typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */
@@ -83,6 +87,24 @@ void rsl_settings_default(rsl_settings *s);
  * lower-triangle CSC, rows sorted (CscMatrix, src/sparse/csc.rs:15-20). */
 int rsl_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
                 const rsl_settings *s, rsl_symbolic **out);
+
+/* LdltSymbolic from the caller's own symbolic analysis, the input that
+ * factor_numeric takes (&MultifrontalSymbolic, src/numeric/
+ * multifrontal_ldlt.rs:1851-1855, whose SymbolicFactorization fields are
+ * src/symbolic/mod.rs:333-405): the lower-CSC pattern of A, the permutation
+ * perm (new -> old), and the supernode partition of the permuted columns in
+ * postorder -- supernode s holds columns sn_first[s] .. sn_first[s] +
+ * sn_ncol[s] - 1, consecutive from 0 -- with its tree (sn_parent[s] a later
+ * supernode, or -1 for a root). Row structures and update lists are derived as
+ * the reference derives them (multifrontal_ldlt.rs:2121-2152, 3386-3401).
+ * RSL_INVALID_INPUT for a perm that is not a permutation, a partition that
+ * does not tile 0..n-1, a parent that is not a later supernode, or a
+ * supernode with rows in a supernode outside its ancestors. The ordering and
+ * amalgamation fields of s are unused; its numeric fields apply. */
+int rsl_plan_create(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
+                    const int32_t *perm, int32_t nsuper, const int32_t *sn_first,
+                    const int32_t *sn_ncol, const int32_t *sn_parent,
+                    const rsl_settings *s, rsl_symbolic **out);
 
 /* Batch of independent systems (no counterpart in rslab, which factors one
  * matrix per call; for callers with many small systems of uneven size):
@@ -160,6 +182,18 @@ int rsl_factor_download(rsl_solver *f, int64_t *l_col_ptr, int32_t *l_row_idx,
                         double *l_values, double *d_diag, double *d_subdiag,
                         uint8_t *two_by_two, int32_t *perm);
 
+/* Factor once, solve elsewhere (rslab keeps LdltFactors in memory only and
+ * has no file format; this one is the library's own). rsl_factor_save writes
+ * the handle's factor -- the scale, panels, D, pivot maps and inertia, which
+ * is all a solve or a refinement reads -- with a digest of the symbolic plan.
+ * rsl_factor_load reads it into a new handle for a symbolic of the same plan
+ * (the same pattern and analysis, e.g. rsl_analyze with the same settings in
+ * another process) without a numeric factorization; a file written for
+ * another plan or field is RSL_INVALID_INPUT. The loaded handle solves,
+ * refines and refactors like the original. */
+int rsl_factor_save(rsl_solver *f, const char *path);
+int rsl_factor_load(rsl_symbolic *sym, const char *path, rsl_solver **out);
+
 /* MixedLdltSolver::solve_to (src/numeric/mixed.rs:267-285) around
  * refine_ladder (:140-221): IR with FP64 residuals against the original
  * matrix values a_values, escalating to FGMRES-IR. b, x row-major n x nrhs;
@@ -173,6 +207,30 @@ int rsl_refine(rsl_solver *f, const double *a_values, const double *b, int64_t n
  * done. */
 int rsl_refine_device(rsl_solver *f, const void *d_a_values, const void *d_b, int64_t nrhs,
                       double target, void *d_x, double *info, void *stream);
+
+/* Krylov solvers with this factor as preconditioner (Preconditioner for
+ * LdltSolver, src/numeric/iterative.rs:178-191: apply_block is the
+ * multi-RHS device solve) and A as operator. a_values are A's values in the
+ * analysed pattern (the operator may differ in value from the factored
+ * matrix: a factor of a nearby matrix preconditions A); b, x (and x0) host
+ * buffers, row-major n x nrhs (interleaved for complex); precond = 1 applies
+ * the factor, 0 the identity (NoPreconditioner, :167-176). The right-hand
+ * sides advance in lockstep, each following the single-vector recurrence on
+ * its own, at most 256 per call. info[4*c..4*c+4) = {iterations, final
+ * relative residual |b - A x|_2 / |b|_2, converged, stop reason (0
+ * converged, 1 iteration budget, 2 breakdown; StopReason :335-348)}.
+ *   rsl_gmres_block  gmres_block / gmres (:1881, :822-1021): FGMRES(restart),
+ *                    MGS with DGKS re-orthogonalization per column, x0 may
+ *                    be NULL (zero);
+ *   rsl_cocg         cocg (:402-489), x0 = 0;
+ *   rsl_cocr         cocr (:495-583), x0 = 0. */
+int rsl_gmres_block(rsl_solver *f, const double *a_values, const double *b, int64_t nrhs, double tol,
+                    int32_t max_iter, int32_t restart, int32_t precond, const double *x0, double *x,
+                    double *info);
+int rsl_cocg(rsl_solver *f, const double *a_values, const double *b, int64_t nrhs, double tol,
+             int32_t max_iter, int32_t precond, double *x, double *info);
+int rsl_cocr(rsl_solver *f, const double *a_values, const double *b, int64_t nrhs, double tol,
+             int32_t max_iter, int32_t precond, double *x, double *info);
 
 /* Per-kernel timing (no counterpart in rslab; a measurement aid). While
  * enabled, every launch of this factor's numeric factor, solve and refinement

@@ -9,6 +9,9 @@
 
 #include <complex.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -38,6 +41,30 @@ float ora_round_lowp(double v, int prec) {
   ora_prec = keep;
   return f;
 }
+
+/* Threads of the factor (OpenMP): all available unless ora_set_threads
+ * names a count. Every entry is computed in the same order whatever the
+ * count, so the factor is the same bit for bit. */
+static int ora_nthreads = 0;
+
+static inline int ora_threads(void) {
+#ifdef _OPENMP
+  return ora_nthreads > 0 ? ora_nthreads : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static inline int ora_thread(void) {
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
+
+void ora_set_threads(int n) { ora_nthreads = n > 0 ? n : 0; }
+int ora_get_threads(void) { return ora_threads(); }
 
 /* ---- real field ---- */
 #define LOWP 1

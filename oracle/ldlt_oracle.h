@@ -64,6 +64,11 @@ ora_factors *ora_factor(const ora_sym *s, int cplx, const int64_t *a_colp,
                         int pivot, double abs_floor, double drop_tol, int prec);
 /* v rounded as the reduced-precision products round their operands. */
 float ora_round_lowp(double v, int prec);
+/* Threads for ora_factor (OpenMP; 0 = all available). The supernodes of one
+ * tree level run concurrently, or, on narrow levels, the column loops inside
+ * one supernode; the result does not depend on the count. */
+void ora_set_threads(int n);
+int ora_get_threads(void);
 void ora_free(ora_factors *f);
 /* solve_ldlt_many: b, x row-major n x nrhs (complex interleaved). 0 = ok. */
 int ora_solve_many(const ora_factors *f, const double *b, int32_t nrhs, double *x);

@@ -99,7 +99,334 @@ typedef struct {
   int64_t *lperm;
 } FN(node);
 
-/* Sequential left-looking factor; returns 0 or 1 (rank deficient). */
+/* Shared state of one factorization: the symbolic, the permuted matrix, the
+ * row structures and update lists, and the output maps (each supernode writes
+ * only its own entries of them). */
+typedef struct {
+  const ora_sym *s;
+  int64_t n, ns, panel_nb;
+  int have_floor;
+  double floor;
+  const int64_t *pcnt, *prow;
+  const T *pval;
+  int64_t **rs;
+  const int64_t *rsn, *ucnt, *uidx, *e_off;
+  int64_t *e_of_g;
+  T *d_out, *ds_out;
+  uint8_t *two_out;
+  int32_t *perm_out;
+  FN(node) *nodes;
+  int inner; /* run the loops inside a supernode on all threads */
+} FN(ctx);
+
+/* Per-thread scratch. gloc maps a global row to its local panel row while a
+ * supernode is being factored and is all -1 between supernodes. */
+typedef struct {
+  int64_t *gloc;
+  T *gbuf, *ubuf;
+  float *fbuf;
+  int64_t gcap, ucap;
+  int64_t cnt[4]; /* positive, negative, zero, perturbed */
+} FN(work);
+
+/* cmod (ll_factor_node, multifrontal_ldlt.rs:2452-2728) for one target
+ * column: U[r, c] = sum_ck L_k[p0 + r, ck] * G[c, ck] for r >= c, subtracted
+ * from the panel. ub / fb are mrows of scratch. */
+static void FN(cmod_col)(T *P, int64_t nrow, int64_t first, const FN(node) *K, int64_t nck, int64_t nrk,
+                         const int64_t *ok, int64_t p0, int64_t mrows, int64_t npk, const T *gbuf,
+                         const int64_t *gloc, int64_t c, T *ub, float *fb) {
+  int64_t tcol = ok[p0 + c] - first;
+  T *dst = P + tcol * nrow;
+  if (LOWP && ora_prec) {
+    /* reduced-precision product: rounded operands, FP32 sums, FP64 subtract */
+    for (int64_t r = c; r < mrows; ++r) fb[r] = 0.0f;
+    for (int64_t ck = 0; ck < nck; ++ck) {
+      float g = ora_lowp(REAL(gbuf[c + ck * npk]));
+      const T *lc = K->panel + nck + p0 + ck * nrk;
+      for (int64_t r = c; r < mrows; ++r) fb[r] = fb[r] + ora_lowp(REAL(lc[r])) * g;
+    }
+    for (int64_t r = c; r < mrows; ++r) {
+      int64_t li = gloc[ok[p0 + r]];
+      dst[li] = dst[li] - (double)fb[r];
+    }
+    return;
+  }
+  for (int64_t r = c; r < mrows; ++r) ub[r] = ZERO;
+  for (int64_t ck = 0; ck < nck; ++ck) {
+    T g = gbuf[c + ck * npk];
+    const T *lc = K->panel + nck + p0 + ck * nrk;
+    for (int64_t r = c; r < mrows; ++r) ub[r] = ub[r] + lc[r] * g;
+  }
+  for (int64_t r = c; r < mrows; ++r) {
+    int64_t li = gloc[ok[p0 + r]];
+    dst[li] = dst[li] - ub[r];
+  }
+}
+
+/* deferred Schur (multifrontal_ldlt.rs:3085-3167) for target column cc of
+ * the trailing block: P[ke + rr, ke + cc] -= sum_q L21[rr, q] G[cc, q], rr >= cc,
+ * with g = L21 D (mt x pw, column-major). */
+static void FN(schur_col)(T *P, int64_t nrow, int64_t kb, int64_t ke, int64_t pw, int64_t mt, const T *g,
+                          int64_t cc) {
+  T *dst = P + (ke + cc) * nrow + ke;
+  if (LOWP && ora_prec) {
+    /* reduced-precision product L21 (L D)^T: rounded operands, FP32 sums
+     * over the panel's columns, FP64 subtract */
+    for (int64_t rr = cc; rr < mt; ++rr) {
+      float acc = 0.0f;
+      for (int64_t q = 0; q < pw; ++q)
+        acc = acc + ora_lowp(REAL(P[(ke + rr) + (kb + q) * nrow])) * ora_lowp(REAL(g[q * mt + cc]));
+      dst[rr] = dst[rr] - (double)acc;
+    }
+    return;
+  }
+  for (int64_t q = 0; q < pw; ++q) {
+    T r = P[(ke + cc) + (kb + q) * nrow];
+    const T *gq = g + q * mt;
+    for (int64_t rr = cc; rr < mt; ++rr) dst[rr] = dst[rr] - gq[rr] * r;
+  }
+}
+
+/* ll_factor_node + ll_cdiv_emit for supernode sn (its updaters are finished).
+ * Returns 0 or 1 (rank deficient). */
+static int FN(factor_node)(const FN(ctx) *X, FN(work) *W, int64_t sn) {
+  const ora_sym *s = X->s;
+  const int64_t panel_nb = X->panel_nb;
+  const int have_floor = X->have_floor;
+  const double floor = X->floor;
+  const int64_t *pcnt = X->pcnt, *prow = X->prow;
+  const T *pval = X->pval;
+  int64_t **rs = X->rs;
+  const int64_t *rsn = X->rsn, *ucnt = X->ucnt, *uidx = X->uidx, *e_off = X->e_off;
+  int64_t *e_of_g = X->e_of_g;
+  T *d_out = X->d_out, *ds_out = X->ds_out;
+  uint8_t *two_out = X->two_out;
+  int32_t *perm_out = X->perm_out;
+  FN(node) *nodes = X->nodes;
+  int64_t *gloc = W->gloc;
+  T *gbuf = W->gbuf, *ubuf = W->ubuf;
+  float *fbuf = W->fbuf;
+  int64_t gcap = W->gcap, ucap = W->ucap;
+  const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+  int status = 0;
+  int64_t ipos = 0, ineg = 0, izero = 0, nperturbed = 0;
+  const int64_t first = s->sn_first[sn], ncol = s->sn_ncol[sn], nrow = rsn[sn];
+  T *P = calloc((size_t)(nrow * ncol > 0 ? nrow * ncol : 1), sizeof(T));
+  for (int64_t li = 0; li < nrow; ++li) gloc[rs[sn][li]] = li;
+  /* assembly */
+  for (int64_t p = 0; p < ncol; ++p) {
+    int64_t c = first + p;
+    for (int64_t k = pcnt[c]; k < pcnt[c + 1]; ++k) {
+      int64_t li = gloc[prow[k]];
+      P[li + p * nrow] = P[li + p * nrow] + pval[k];
+    }
+  }
+  /* cmod: every descendant updater, ascending */
+  for (int64_t u = ucnt[sn]; u < ucnt[sn + 1]; ++u) {
+    int64_t kk = uidx[u], nck = s->sn_ncol[kk], nrk = rsn[kk];
+    const int64_t *ok = rs[kk] + nck;
+    int64_t nok = nrk - nck, p0 = 0, p1;
+    while (p0 < nok && ok[p0] < first) ++p0;
+    p1 = p0;
+    while (p1 < nok && ok[p1] < first + ncol) ++p1;
+    int64_t npk = p1 - p0;
+    if (npk == 0) continue;
+    const FN(node) *K = &nodes[kk];
+    int64_t mrows = nok - p0;
+    if (gcap < npk * nck) { free(gbuf); gcap = npk * nck; gbuf = malloc(sizeof(T) * gcap); }
+    if (ucap < mrows) {
+      free(ubuf); free(fbuf);
+      ucap = mrows; ubuf = malloc(sizeof(T) * ucap); fbuf = malloc(sizeof(float) * ucap);
+    }
+    FN(apply_d)(K->panel, nrk, nck + p0, npk, nck, K->d, K->dsub, K->two, gbuf);
+    /* each target column c is written by its own product column only */
+    if (X->inner && npk > 1) {
+#pragma omp parallel num_threads(ora_threads())
+      {
+        T *ub = malloc(sizeof(T) * mrows);
+        float *fb = malloc(sizeof(float) * mrows);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t c = 0; c < npk; ++c)
+          FN(cmod_col)(P, nrow, first, K, nck, nrk, ok, p0, mrows, npk, gbuf, gloc, c, ub, fb);
+        free(ub);
+        free(fb);
+      }
+    } else {
+      for (int64_t c = 0; c < npk; ++c)
+        FN(cmod_col)(P, nrow, first, K, nck, nrk, ok, p0, mrows, npk, gbuf, gloc, c, ubuf, fbuf);
+    }
+  }
+  /* cdiv */
+  int64_t nb = panel_nb;
+  if (ncol >= 512 && nb < 128) nb = 128;
+  T *d = calloc(ncol ? ncol : 1, sizeof(T));
+  T *dsub = calloc(ncol ? ncol : 1, sizeof(T));
+  uint8_t *two = calloc(ncol ? ncol : 1, 1);
+  int64_t *lperm = malloc(sizeof(int64_t) * (nrow ? nrow : 1));
+  for (int64_t i = 0; i < nrow; ++i) lperm[i] = i;
+  T *l1 = calloc(nrow ? nrow : 1, sizeof(T)), *l2 = calloc(nrow ? nrow : 1, sizeof(T));
+  int64_t *deep = malloc(sizeof(int64_t) * nb);
+  T *snap = calloc(nb * nb, sizeof(T));
+  for (int64_t kb = 0; kb < ncol && status == 0;) {
+    int64_t ke = kb + nb < ncol ? kb + nb : ncol;
+    for (int64_t q = 0; q < nb; ++q) deep[q] = -1;
+    int64_t k = kb;
+    while (k < ke) {
+      double absakk = MAG(P[k + k * nrow]);
+      double colmax_sq = 0.0;
+      int64_t imax = k;
+      for (int64_t i = k + 1; i < ke; ++i) {
+        double m = MAGSQ(P[k * nrow + i]);
+        if (m > colmax_sq) { colmax_sq = m; imax = i; }
+      }
+      double colmax = sqrt(colmax_sq);
+      int kstep;
+      int64_t kp;
+      if ((absakk > colmax ? absakk : colmax) == 0.0) {
+        if (!have_floor) { status = 1; break; }
+        kstep = 1; kp = k;
+      } else if (absakk >= alpha * colmax) {
+        kstep = 1; kp = k;
+      } else {
+        double rowmax_sq = 0.0;
+        for (int64_t j = k; j < imax; ++j) {
+          double m = MAGSQ(P[j * nrow + imax]);
+          if (m > rowmax_sq) rowmax_sq = m;
+        }
+        for (int64_t i = imax + 1; i < ke; ++i) {
+          double m = MAGSQ(P[imax * nrow + i]);
+          if (m > rowmax_sq) rowmax_sq = m;
+        }
+        double rowmax = sqrt(rowmax_sq);
+        if (absakk >= alpha * colmax * (colmax / rowmax)) { kstep = 1; kp = k; }
+        else if (MAG(P[imax * nrow + imax]) >= alpha * rowmax) { kstep = 1; kp = imax; }
+        else { kstep = 2; kp = imax; }
+      }
+      if (kstep == 1) {
+        if (kp != k) {
+          FN(swap_sym)(P, nrow, k, kp, ke);
+          int64_t t = lperm[k]; lperm[k] = lperm[kp]; lperm[kp] = t;
+          deep[k - kb] = kp;
+        }
+        T dk = P[k + k * nrow];
+        if (have_floor) {
+          if (MAG(dk) < floor) { dk = FN(perturb)(dk, floor); P[k + k * nrow] = dk; nperturbed++; }
+        } else if (IS_ZERO(dk)) { status = 1; break; }
+        d[k] = dk;
+        T dinv = RECIP(dk);
+        for (int64_t j = k + 1; j < ke; ++j) {
+          T m = P[k * nrow + j] * dinv;
+          snap[(k - kb) * nb + (j - kb)] = m;
+          if (!IS_ZERO(m))
+            for (int64_t i = j; i < ke; ++i) P[j * nrow + i] = P[j * nrow + i] - P[k * nrow + i] * m;
+        }
+        for (int64_t i = k + 1; i < ke; ++i) P[k * nrow + i] = P[k * nrow + i] * dinv;
+        k += 1;
+      } else {
+        if (kp != k + 1) {
+          FN(swap_sym)(P, nrow, k + 1, kp, ke);
+          int64_t t = lperm[k + 1]; lperm[k + 1] = lperm[kp]; lperm[kp] = t;
+          deep[k - kb] = kp;
+        }
+        T d11 = P[k + k * nrow], d21 = P[k * nrow + k + 1], d22 = P[(k + 1) + (k + 1) * nrow];
+        T det = d11 * d22 - d21 * d21;
+        double scale = MAG(d11);
+        if (MAG(d22) > scale) scale = MAG(d22);
+        if (MAG(d21) > scale) scale = MAG(d21);
+        double growth_floor = 1e-14 * scale * scale;
+        if (have_floor) {
+          double fl = floor * floor > growth_floor ? floor * floor : growth_floor;
+          if (MAG(det) < fl) {
+            double lift = floor > scale * sqrt(1e-14) ? floor : scale * sqrt(1e-14);
+            d11 = d11 + FROM_REAL(lift);
+            d22 = d22 + FROM_REAL(lift);
+            det = d11 * d22 - d21 * d21;
+            if (MAG(det) < fl) det = det + FROM_REAL(fl);
+            nperturbed++;
+          }
+        } else if (MAG(det) <= growth_floor) { status = 1; break; }
+        T detinv = RECIP(det);
+        d[k] = d11; dsub[k] = d21; d[k + 1] = d22; two[k] = 1;
+        for (int64_t i = k + 2; i < ke; ++i) {
+          T wik = P[k * nrow + i], wik1 = P[(k + 1) * nrow + i];
+          l1[i] = (d22 * wik - d21 * wik1) * detinv;
+          l2[i] = (d11 * wik1 - d21 * wik) * detinv;
+          snap[(k - kb) * nb + (i - kb)] = l1[i];
+          snap[(k + 1 - kb) * nb + (i - kb)] = l2[i];
+        }
+        for (int64_t j = k + 2; j < ke; ++j)
+          for (int64_t i = j; i < ke; ++i)
+            P[j * nrow + i] = P[j * nrow + i] - P[k * nrow + i] * l1[j] - P[(k + 1) * nrow + i] * l2[j];
+        for (int64_t i = k + 2; i < ke; ++i) { P[k * nrow + i] = l1[i]; P[(k + 1) * nrow + i] = l2[i]; }
+        k += 2;
+      }
+    }
+    if (status) break;
+    /* deep-row replay: every row on its own, so row blocks run concurrently */
+    if (nrow > ke) {
+      if (X->inner && nrow - ke > 256) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(ora_threads())
+        for (int64_t r0 = ke; r0 < nrow; r0 += 128)
+          FN(apply_trailing)(P, nrow, kb, ke, d, dsub, two, deep, snap, nb, r0, r0 + 128 < nrow ? r0 + 128 : nrow);
+      } else {
+        FN(apply_trailing)(P, nrow, kb, ke, d, dsub, two, deep, snap, nb, ke, nrow);
+      }
+    }
+    /* deferred Schur: P[ke.., ke..ncol] -= L21 D R^T (lower part); a column
+     * cc reads only the panel's columns and g, and writes only column ke + cc */
+    int64_t pw = ke - kb, cw = ncol - ke, mt = nrow - ke;
+    if (pw > 0 && cw > 0 && mt > 0) {
+      T *g = malloc(sizeof(T) * mt * pw);
+      /* G = L21 * D with L21 = P[ke.., kb..ke) (the panel is contiguous) */
+      FN(apply_d)(P + kb * nrow, nrow, ke, mt, pw, d + kb, dsub + kb, two + kb, g);
+      if (X->inner && cw > 1) {
+#pragma omp parallel for schedule(dynamic, 2) num_threads(ora_threads())
+        for (int64_t cc = 0; cc < cw; ++cc) FN(schur_col)(P, nrow, kb, ke, pw, mt, g, cc);
+      } else {
+        for (int64_t cc = 0; cc < cw; ++cc) FN(schur_col)(P, nrow, kb, ke, pw, mt, g, cc);
+      }
+      free(g);
+    }
+    kb = ke;
+  }
+  for (int64_t li = 0; li < nrow; ++li) gloc[rs[sn][li]] = -1;
+  free(l1); free(l2); free(deep); free(snap);
+  if (status) { free(P); free(d); free(dsub); free(two); free(lperm); goto out; }
+  /* emit maps + inertia */
+  int64_t eoff = e_off[sn];
+  for (int64_t pp = 0; pp < ncol;) {
+    int64_t g = rs[sn][lperm[pp]], e = eoff + pp;
+    e_of_g[g] = e;
+    perm_out[e] = s->perm[g];
+    d_out[e] = d[pp];
+    if (two[pp]) {
+      int64_t g2 = rs[sn][lperm[pp + 1]];
+      e_of_g[g2] = e + 1;
+      perm_out[e + 1] = s->perm[g2];
+      d_out[e + 1] = d[pp + 1];
+      ds_out[e] = dsub[pp];
+      two_out[e] = 1;
+      double det_r = REAL(d[pp] * d[pp + 1] - dsub[pp] * dsub[pp]);
+      double tr_r = REAL(d[pp] + d[pp + 1]);
+      if (det_r < 0.0) { ipos++; ineg++; }
+      else if (det_r > 0.0) { if (tr_r >= 0.0) ipos += 2; else ineg += 2; }
+      else { izero++; if (tr_r >= 0.0) ipos++; else ineg++; }
+      pp += 2;
+    } else {
+      double r = REAL(d[pp]);
+      if (r > 0.0) ipos++; else if (r < 0.0) ineg++; else izero++;
+      pp += 1;
+    }
+  }
+  nodes[sn].panel = P; nodes[sn].d = d; nodes[sn].dsub = dsub;
+  nodes[sn].two = two; nodes[sn].lperm = lperm;
+out:
+  W->gbuf = gbuf; W->ubuf = ubuf; W->fbuf = fbuf; W->gcap = gcap; W->ucap = ucap;
+  W->cnt[0] += ipos; W->cnt[1] += ineg; W->cnt[2] += izero; W->cnt[3] += nperturbed;
+  return status;
+}
+
+/* Left-looking factor; returns 0 or 1 (rank deficient). */
 static int FN(factor)(const ora_sym *s, const int64_t *a_colp, const int32_t *a_rowi,
                       const T *a_val, int panel_nb, int pivot, double abs_floor,
                       double drop_tol, ora_factors *out) {
@@ -218,246 +545,57 @@ static int FN(factor)(const ora_sym *s, const int64_t *a_colp, const int32_t *a_
   int64_t ipos = 0, ineg = 0, izero = 0, nperturbed = 0;
 
   FN(node) *nodes = calloc(ns ? ns : 1, sizeof(FN(node)));
-  int64_t *gloc = malloc(sizeof(int64_t) * (n ? n : 1));
-  for (int64_t i = 0; i < n; ++i) gloc[i] = -1;
-  const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   int status = 0;
-  T *gbuf = NULL, *ubuf = NULL;
-  float *fbuf = NULL; /* reduced-precision product sums */
-  int64_t gcap = 0, ucap = 0;
 
-  for (int64_t sn = 0; sn < ns && status == 0; ++sn) {
-    const int64_t first = s->sn_first[sn], ncol = s->sn_ncol[sn], nrow = rsn[sn];
-    T *P = calloc((size_t)(nrow * ncol > 0 ? nrow * ncol : 1), sizeof(T));
-    for (int64_t li = 0; li < nrow; ++li) gloc[rs[sn][li]] = li;
-    /* assembly */
-    for (int64_t p = 0; p < ncol; ++p) {
-      int64_t c = first + p;
-      for (int64_t k = pcnt[c]; k < pcnt[c + 1]; ++k) {
-        int64_t li = gloc[prow[k]];
-        P[li + p * nrow] = P[li + p * nrow] + pval[k];
-      }
-    }
-    /* cmod: every descendant updater, ascending */
-    for (int64_t u = ucnt[sn]; u < ucnt[sn + 1]; ++u) {
-      int64_t kk = uidx[u], nck = s->sn_ncol[kk], nrk = rsn[kk];
-      const int64_t *ok = rs[kk] + nck;
-      int64_t nok = nrk - nck, p0 = 0, p1;
-      while (p0 < nok && ok[p0] < first) ++p0;
-      p1 = p0;
-      while (p1 < nok && ok[p1] < first + ncol) ++p1;
-      int64_t npk = p1 - p0;
-      if (npk == 0) continue;
-      const FN(node) *K = &nodes[kk];
-      int64_t mrows = nok - p0;
-      if (gcap < npk * nck) { free(gbuf); gcap = npk * nck; gbuf = malloc(sizeof(T) * gcap); }
-      if (ucap < mrows) {
-        free(ubuf); free(fbuf);
-        ucap = mrows; ubuf = malloc(sizeof(T) * ucap); fbuf = malloc(sizeof(float) * ucap);
-      }
-      FN(apply_d)(K->panel, nrk, nck + p0, npk, nck, K->d, K->dsub, K->two, gbuf);
-      for (int64_t c = 0; c < npk; ++c) {
-        int64_t tcol = ok[p0 + c] - first;
-        if (LOWP && ora_prec) {
-          /* reduced-precision product: rounded operands, FP32 sums, FP64 subtract */
-          for (int64_t r = c; r < mrows; ++r) fbuf[r] = 0.0f;
-          for (int64_t ck = 0; ck < nck; ++ck) {
-            float g = ora_lowp(REAL(gbuf[c + ck * npk]));
-            const T *lc = K->panel + nck + p0 + ck * nrk;
-            for (int64_t r = c; r < mrows; ++r) fbuf[r] = fbuf[r] + ora_lowp(REAL(lc[r])) * g;
-          }
-          T *dstl = P + tcol * nrow;
-          for (int64_t r = c; r < mrows; ++r) {
-            int64_t li = gloc[ok[p0 + r]];
-            dstl[li] = dstl[li] - (double)fbuf[r];
-          }
-          continue;
-        }
-        /* U[r, c] = sum_ck L_k[p0 + r, ck] * G[c, ck] for r >= c */
-        for (int64_t r = c; r < mrows; ++r) ubuf[r] = ZERO;
-        for (int64_t ck = 0; ck < nck; ++ck) {
-          T g = gbuf[c + ck * npk];
-          const T *lc = K->panel + nck + p0 + ck * nrk;
-          for (int64_t r = c; r < mrows; ++r) ubuf[r] = ubuf[r] + lc[r] * g;
-        }
-        T *dst = P + tcol * nrow;
-        for (int64_t r = c; r < mrows; ++r) {
-          int64_t li = gloc[ok[p0 + r]];
-          dst[li] = dst[li] - ubuf[r];
-        }
-      }
-    }
-    /* cdiv */
-    int64_t nb = panel_nb;
-    if (ncol >= 512 && nb < 128) nb = 128;
-    T *d = calloc(ncol ? ncol : 1, sizeof(T));
-    T *dsub = calloc(ncol ? ncol : 1, sizeof(T));
-    uint8_t *two = calloc(ncol ? ncol : 1, 1);
-    int64_t *lperm = malloc(sizeof(int64_t) * (nrow ? nrow : 1));
-    for (int64_t i = 0; i < nrow; ++i) lperm[i] = i;
-    T *l1 = calloc(nrow ? nrow : 1, sizeof(T)), *l2 = calloc(nrow ? nrow : 1, sizeof(T));
-    int64_t *deep = malloc(sizeof(int64_t) * nb);
-    T *snap = calloc(nb * nb, sizeof(T));
-    for (int64_t kb = 0; kb < ncol && status == 0;) {
-      int64_t ke = kb + nb < ncol ? kb + nb : ncol;
-      for (int64_t q = 0; q < nb; ++q) deep[q] = -1;
-      int64_t k = kb;
-      while (k < ke) {
-        double absakk = MAG(P[k + k * nrow]);
-        double colmax_sq = 0.0;
-        int64_t imax = k;
-        for (int64_t i = k + 1; i < ke; ++i) {
-          double m = MAGSQ(P[k * nrow + i]);
-          if (m > colmax_sq) { colmax_sq = m; imax = i; }
-        }
-        double colmax = sqrt(colmax_sq);
-        int kstep;
-        int64_t kp;
-        if ((absakk > colmax ? absakk : colmax) == 0.0) {
-          if (!have_floor) { status = 1; break; }
-          kstep = 1; kp = k;
-        } else if (absakk >= alpha * colmax) {
-          kstep = 1; kp = k;
-        } else {
-          double rowmax_sq = 0.0;
-          for (int64_t j = k; j < imax; ++j) {
-            double m = MAGSQ(P[j * nrow + imax]);
-            if (m > rowmax_sq) rowmax_sq = m;
-          }
-          for (int64_t i = imax + 1; i < ke; ++i) {
-            double m = MAGSQ(P[imax * nrow + i]);
-            if (m > rowmax_sq) rowmax_sq = m;
-          }
-          double rowmax = sqrt(rowmax_sq);
-          if (absakk >= alpha * colmax * (colmax / rowmax)) { kstep = 1; kp = k; }
-          else if (MAG(P[imax * nrow + imax]) >= alpha * rowmax) { kstep = 1; kp = imax; }
-          else { kstep = 2; kp = imax; }
-        }
-        if (kstep == 1) {
-          if (kp != k) {
-            FN(swap_sym)(P, nrow, k, kp, ke);
-            int64_t t = lperm[k]; lperm[k] = lperm[kp]; lperm[kp] = t;
-            deep[k - kb] = kp;
-          }
-          T dk = P[k + k * nrow];
-          if (have_floor) {
-            if (MAG(dk) < floor) { dk = FN(perturb)(dk, floor); P[k + k * nrow] = dk; nperturbed++; }
-          } else if (IS_ZERO(dk)) { status = 1; break; }
-          d[k] = dk;
-          T dinv = RECIP(dk);
-          for (int64_t j = k + 1; j < ke; ++j) {
-            T m = P[k * nrow + j] * dinv;
-            snap[(k - kb) * nb + (j - kb)] = m;
-            if (!IS_ZERO(m))
-              for (int64_t i = j; i < ke; ++i) P[j * nrow + i] = P[j * nrow + i] - P[k * nrow + i] * m;
-          }
-          for (int64_t i = k + 1; i < ke; ++i) P[k * nrow + i] = P[k * nrow + i] * dinv;
-          k += 1;
-        } else {
-          if (kp != k + 1) {
-            FN(swap_sym)(P, nrow, k + 1, kp, ke);
-            int64_t t = lperm[k + 1]; lperm[k + 1] = lperm[kp]; lperm[kp] = t;
-            deep[k - kb] = kp;
-          }
-          T d11 = P[k + k * nrow], d21 = P[k * nrow + k + 1], d22 = P[(k + 1) + (k + 1) * nrow];
-          T det = d11 * d22 - d21 * d21;
-          double scale = MAG(d11);
-          if (MAG(d22) > scale) scale = MAG(d22);
-          if (MAG(d21) > scale) scale = MAG(d21);
-          double growth_floor = 1e-14 * scale * scale;
-          if (have_floor) {
-            double fl = floor * floor > growth_floor ? floor * floor : growth_floor;
-            if (MAG(det) < fl) {
-              double lift = floor > scale * sqrt(1e-14) ? floor : scale * sqrt(1e-14);
-              d11 = d11 + FROM_REAL(lift);
-              d22 = d22 + FROM_REAL(lift);
-              det = d11 * d22 - d21 * d21;
-              if (MAG(det) < fl) det = det + FROM_REAL(fl);
-              nperturbed++;
-            }
-          } else if (MAG(det) <= growth_floor) { status = 1; break; }
-          T detinv = RECIP(det);
-          d[k] = d11; dsub[k] = d21; d[k + 1] = d22; two[k] = 1;
-          for (int64_t i = k + 2; i < ke; ++i) {
-            T wik = P[k * nrow + i], wik1 = P[(k + 1) * nrow + i];
-            l1[i] = (d22 * wik - d21 * wik1) * detinv;
-            l2[i] = (d11 * wik1 - d21 * wik) * detinv;
-            snap[(k - kb) * nb + (i - kb)] = l1[i];
-            snap[(k + 1 - kb) * nb + (i - kb)] = l2[i];
-          }
-          for (int64_t j = k + 2; j < ke; ++j)
-            for (int64_t i = j; i < ke; ++i)
-              P[j * nrow + i] = P[j * nrow + i] - P[k * nrow + i] * l1[j] - P[(k + 1) * nrow + i] * l2[j];
-          for (int64_t i = k + 2; i < ke; ++i) { P[k * nrow + i] = l1[i]; P[(k + 1) * nrow + i] = l2[i]; }
-          k += 2;
-        }
-      }
-      if (status) break;
-      if (nrow > ke) FN(apply_trailing)(P, nrow, kb, ke, d, dsub, two, deep, snap, nb, ke, nrow);
-      /* deferred Schur: P[ke.., ke..ncol] -= L21 D R^T (lower part) */
-      int64_t pw = ke - kb, cw = ncol - ke, mt = nrow - ke;
-      if (pw > 0 && cw > 0 && mt > 0) {
-        T *g = malloc(sizeof(T) * mt * pw);
-        /* G = L21 * D with L21 = P[ke.., kb..ke) (the panel is contiguous) */
-        FN(apply_d)(P + kb * nrow, nrow, ke, mt, pw, d + kb, dsub + kb, two + kb, g);
-        if (LOWP && ora_prec) {
-          /* reduced-precision product L21 (L D)^T: rounded operands, FP32
-           * sums over the panel's columns, FP64 subtract */
-          for (int64_t cc = 0; cc < cw; ++cc) {
-            T *dst = P + (ke + cc) * nrow + ke;
-            for (int64_t rr = cc; rr < mt; ++rr) {
-              float acc = 0.0f;
-              for (int64_t q = 0; q < pw; ++q)
-                acc = acc + ora_lowp(REAL(P[(ke + rr) + (kb + q) * nrow])) * ora_lowp(REAL(g[q * mt + cc]));
-              dst[rr] = dst[rr] - (double)acc;
-            }
-          }
-          cw = 0; /* done: skip the FP64 loop below */
-        }
-        for (int64_t cc = 0; cc < cw; ++cc) {
-          T *dst = P + (ke + cc) * nrow + ke;
-          for (int64_t q = 0; q < pw; ++q) {
-            T r = P[(ke + cc) + (kb + q) * nrow];
-            const T *gq = g + q * mt;
-            for (int64_t rr = cc; rr < mt; ++rr) dst[rr] = dst[rr] - gq[rr] * r;
-          }
-        }
-        free(g);
-      }
-      kb = ke;
-    }
-    for (int64_t li = 0; li < nrow; ++li) gloc[rs[sn][li]] = -1;
-    free(l1); free(l2); free(deep); free(snap);
-    if (status) { free(P); free(d); free(dsub); free(two); free(lperm); break; }
-    /* emit maps + inertia */
-    int64_t eoff = e_off[sn];
-    for (int64_t pp = 0; pp < ncol;) {
-      int64_t g = rs[sn][lperm[pp]], e = eoff + pp;
-      e_of_g[g] = e;
-      perm_out[e] = s->perm[g];
-      d_out[e] = d[pp];
-      if (two[pp]) {
-        int64_t g2 = rs[sn][lperm[pp + 1]];
-        e_of_g[g2] = e + 1;
-        perm_out[e + 1] = s->perm[g2];
-        d_out[e + 1] = d[pp + 1];
-        ds_out[e] = dsub[pp];
-        two_out[e] = 1;
-        double det_r = REAL(d[pp] * d[pp + 1] - dsub[pp] * dsub[pp]);
-        double tr_r = REAL(d[pp] + d[pp + 1]);
-        if (det_r < 0.0) { ipos++; ineg++; }
-        else if (det_r > 0.0) { if (tr_r >= 0.0) ipos += 2; else ineg += 2; }
-        else { izero++; if (tr_r >= 0.0) ipos++; else ineg++; }
-        pp += 2;
-      } else {
-        double r = REAL(d[pp]);
-        if (r > 0.0) ipos++; else if (r < 0.0) ineg++; else izero++;
-        pp += 1;
-      }
-    }
-    nodes[sn].panel = P; nodes[sn].d = d; nodes[sn].dsub = dsub;
-    nodes[sn].two = two; nodes[sn].lperm = lperm;
+  /* Levels of the supernode tree (leaves 0): a supernode's updaters are its
+   * descendants, so the supernodes of one level are independent. Wide levels
+   * run their supernodes on all threads; narrow ones run them one at a time
+   * with the loops inside each supernode on all threads. Every entry is
+   * computed by the same operations in the same order as in the sequential
+   * postorder, so the factor is the same bit for bit. */
+  int64_t *lev = calloc(ns ? ns : 1, sizeof(int64_t)), nlev = 0;
+  for (int64_t sn = 0; sn < ns; ++sn) {
+    int64_t l = 0;
+    for (int64_t c = s->ch_ptr[sn]; c < s->ch_ptr[sn + 1]; ++c)
+      if (lev[s->ch_idx[c]] + 1 > l) l = lev[s->ch_idx[c]] + 1;
+    lev[sn] = l;
+    if (l + 1 > nlev) nlev = l + 1;
   }
+  int64_t *lptr = calloc(nlev + 2, sizeof(int64_t)), *lord = malloc(sizeof(int64_t) * (ns ? ns : 1));
+  for (int64_t sn = 0; sn < ns; ++sn) lptr[lev[sn] + 2]++;
+  for (int64_t l = 0; l < nlev; ++l) lptr[l + 2] += lptr[l + 1];
+  for (int64_t sn = 0; sn < ns; ++sn) lord[lptr[lev[sn] + 1]++] = sn;
+  /* small matrices stay on one thread: starting parallel regions costs more
+   * than their work */
+  const int nthr = n >= 20000 ? ora_threads() : 1;
+  FN(work) *works = calloc(nthr, sizeof(FN(work)));
+  for (int t = 0; t < nthr; ++t) {
+    works[t].gloc = malloc(sizeof(int64_t) * (n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) works[t].gloc[i] = -1;
+  }
+  FN(ctx) X = {s, n, ns, panel_nb, have_floor, floor, pcnt, prow, pval, rs, rsn, ucnt, uidx, e_off,
+               e_of_g, d_out, ds_out, two_out, perm_out, nodes, 0};
+  for (int64_t l = 0; l < nlev && status == 0; ++l) {
+    const int64_t b = lptr[l], e = lptr[l + 1];
+    int lstat = 0;
+    if (nthr > 1 && e - b >= nthr) {
+      X.inner = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthr) reduction(max : lstat)
+      for (int64_t q = b; q < e; ++q) {
+        int st = FN(factor_node)(&X, &works[ora_thread()], lord[q]);
+        if (st > lstat) lstat = st;
+      }
+    } else {
+      X.inner = nthr > 1;
+      for (int64_t q = b; q < e && lstat == 0; ++q) lstat = FN(factor_node)(&X, &works[0], lord[q]);
+    }
+    status = lstat;
+  }
+  for (int t = 0; t < nthr; ++t) {
+    ipos += works[t].cnt[0]; ineg += works[t].cnt[1]; izero += works[t].cnt[2]; nperturbed += works[t].cnt[3];
+    free(works[t].gloc); free(works[t].gbuf); free(works[t].ubuf); free(works[t].fbuf);
+  }
+  free(works); free(lev); free(lptr); free(lord);
 
   if (status == 0) {
     /* compact L per supernode, concatenated in supernode order */
@@ -519,7 +657,7 @@ static int FN(factor)(const ora_sym *s, const int64_t *a_colp, const int32_t *a_
     free(nodes[sn].two); free(nodes[sn].lperm); free(rs[sn]);
   }
   free(nodes); free(rs); free(rsn); free(tbuf); free(mark); free(col2s); free(ucnt);
-  free(uidx); free(upos); free(e_off); free(e_of_g); free(gloc); free(gbuf); free(ubuf); free(fbuf);
+  free(uidx); free(upos); free(e_off); free(e_of_g);
   free(pcnt); free(prow); free(pval); free(pos);
   return status;
 }

@@ -26,6 +26,9 @@ def build():
 
 
 def _load():
+    # idle OpenMP threads sleep instead of spinning, so the oracle does not
+    # starve other work on a shared host (its results do not depend on it)
+    os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
     if not os.path.exists(LIB):
         build()
     lib = C.CDLL(LIB)
@@ -60,7 +63,20 @@ _lib.ora_factor.argtypes = [C.POINTER(OraSym), C.c_int, C.POINTER(C.c_int64), C.
 _lib.ora_round_lowp.restype = C.c_float
 _lib.ora_round_lowp.argtypes = [C.c_double, C.c_int]
 
+_lib.ora_set_threads.argtypes = [C.c_int]
+_lib.ora_get_threads.restype = C.c_int
+
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+
+
+def set_threads(n: int):
+    """Threads of the oracle's factor (0 = all available); the factor does
+    not depend on the count."""
+    _lib.ora_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(_lib.ora_get_threads())
 
 
 def round_lowp(v: float, prec: int) -> float:
@@ -259,6 +275,146 @@ def backward_error(a, x, b, anorm=None):
         den = max(anorm * np.abs(X[:, c]).max() + np.abs(B[:, c]).max(), np.finfo(float).tiny)
         out.append(float(np.abs(r).max() / den))
     return np.array(out)
+
+
+# ------------------------------------------------------------ refinement ladder
+CERT_FACTOR, IR_MAX, IR_CONTRACT, GMRES_RESTART, GMRES_MAX = 64.0, 8, 0.5, 30, 120  # mixed.rs:97-105
+_TINY = np.finfo(float).tiny
+
+
+def _nrm2(v):
+    return float(np.sqrt(np.vdot(v, v).real))
+
+
+def gmres(op, b, pre, tol, max_iter=GMRES_MAX, restart=GMRES_RESTART, x0=None):
+    """Flexible restarted GMRES (iterative.rs:822-1021): right preconditioner
+    `pre`, modified Gram-Schmidt with DGKS re-orthogonalization (eta = 1/sqrt 2),
+    Givens rotations (c, s) = (f, g)/sqrt(|f|^2+|g|^2) applied as
+    [[conj c, conj s], [-s, c]], early exit of a cycle on the least-squares
+    estimate, the true residual relative to ||b||_2 tested at the top of each
+    cycle, the breakdown-guarded back substitution (well_conditioned_dim_flat,
+    :646-661) and x += Z y. Returns (x, iterations, converged)."""
+    eta = 1.0 / np.sqrt(2.0)
+    n = b.shape[0]
+    m = max(restart, 1)
+    dt = np.result_type(b, np.float64)
+    x = np.zeros(n, dtype=dt) if x0 is None else np.array(x0, dtype=dt)
+    bnorm = _nrm2(b)
+    if bnorm == 0.0:
+        return np.zeros(n, dtype=dt), 0, True
+    total = 0
+    while True:
+        r = b - op(x)
+        beta = _nrm2(r)
+        final = beta / bnorm
+        if final <= tol or total >= max_iter:
+            break
+        V = np.zeros((m + 1, n), dtype=dt)
+        Z = np.zeros((m, n), dtype=dt)
+        h = np.zeros((m + 1, m), dtype=dt)
+        cs = np.zeros(m, dtype=dt)
+        sn = np.zeros(m, dtype=dt)
+        g = np.zeros(m + 1, dtype=dt)
+        V[0] = r / beta
+        g[0] = beta
+        jdim = 0
+        for j in range(m):
+            if total >= max_iter:
+                break
+            Z[j] = pre(V[j])
+            w = op(Z[j])
+            w0 = _nrm2(w)
+            for i in range(j + 1):
+                hij = np.vdot(V[i], w)
+                h[i, j] = hij
+                w = w - hij * V[i]
+            hn = _nrm2(w)
+            if hn < eta * w0:
+                for i in range(j + 1):
+                    s_ = np.vdot(V[i], w)
+                    h[i, j] += s_
+                    w = w - s_ * V[i]
+                hn = _nrm2(w)
+            h[j + 1, j] = hn
+            V[j + 1] = w / hn if hn > 0.0 else 0.0
+            for i in range(j):
+                t = np.conj(cs[i]) * h[i, j] + np.conj(sn[i]) * h[i + 1, j]
+                h[i + 1, j] = -sn[i] * h[i, j] + cs[i] * h[i + 1, j]
+                h[i, j] = t
+            f_, g_ = h[j, j], h[j + 1, j]
+            if g_ == 0:
+                c_, s2 = 1.0, 0.0
+            elif f_ == 0:
+                c_, s2 = 0.0, 1.0
+            else:
+                rr = np.sqrt(abs(f_) ** 2 + abs(g_) ** 2)
+                c_, s2 = f_ / rr, g_ / rr
+            cs[j], sn[j] = c_, s2
+            h[j, j] = np.conj(c_) * f_ + np.conj(s2) * g_
+            h[j + 1, j] = 0.0
+            g_next = -s2 * g[j]
+            g[j] = np.conj(c_) * g[j]
+            g[j + 1] = g_next
+            total += 1
+            jdim = j + 1
+            if abs(g[j + 1]) / bnorm <= tol:
+                break
+        hmax = max((abs(h[i, i]) for i in range(jdim)), default=0.0)
+        jd = jdim
+        for i in range(jdim):
+            if abs(h[i, i]) <= np.finfo(float).eps * hmax:
+                jd = i
+                break
+        y = np.zeros(m, dtype=dt)
+        for i in range(jd - 1, -1, -1):
+            s_ = g[i]
+            for k in range(i + 1, jd):
+                s_ = s_ - h[i, k] * y[k]
+            y[i] = s_ / h[i, i]
+        for i in range(jd):
+            x = x + Z[i] * y[i]
+    return x, total, final <= tol
+
+
+def refine_ladder(a, fo, s, b, target, anorm=None):
+    """refine_ladder (mixed.rs:140-221) for one right-hand side: x0 = the
+    factor's solve of b; plain IR with the FP64 residual against the original
+    A while the backward error |r|inf / (|A|inf |x|inf + |b|inf) is above
+    `target`, at most IR_MAX steps and each shrinking |r|inf by IR_CONTRACT;
+    then FGMRES warm-started at the best iterate, preconditioned by the same
+    factor; the better of the two kept. `fo` factors the equilibrated matrix
+    with scale `s` (LdltSolver::solve applies it). Returns (x, ir_iters,
+    gmres_iters, backward_error, certified)."""
+    anorm = sym_norm_inf(a) if anorm is None else anorm
+    b = np.asarray(b)
+    pre = lambda r: fo.solve_many(r * s) * s  # noqa: E731
+    op = lambda v: symv(a, v)  # noqa: E731
+    bn = float(np.abs(b).max()) if b.size else 0.0
+
+    def berr(x, r):
+        return float(np.abs(r).max()) / max(anorm * float(np.abs(x).max()) + bn, _TINY)
+
+    x = pre(b)
+    ir = 0
+    best_x, best_be, prev_rn = x.copy(), np.inf, np.inf
+    while True:
+        r = b - op(x)
+        be = berr(x, r)
+        if be < best_be:
+            best_be, best_x = be, x.copy()
+        if be <= target:
+            return best_x, ir, 0, best_be, True
+        rn = float(np.abs(r).max())
+        if ir >= IR_MAX or (ir > 0 and rn > prev_rn * IR_CONTRACT):
+            break
+        prev_rn = rn
+        x = x + pre(r)
+        ir += 1
+    xg, its, _ = gmres(op, b, pre, target, GMRES_MAX, GMRES_RESTART, best_x)
+    be2 = berr(xg, b - op(xg))
+    if be2 < best_be:
+        return xg, ir, its, be2, be2 <= target
+    return best_x, ir, its, best_be, best_be <= target
 
 
 # ------------------------------------------------------------ symbolic pins

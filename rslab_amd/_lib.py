@@ -38,6 +38,7 @@ class Settings(C.Structure):
         ("wide_flops", C.c_double),
         ("drop_tol", C.c_double),
         ("cmod_cap", C.c_int64),
+        ("graphs", C.c_int32),
     ]
 
 
@@ -52,6 +53,8 @@ SIGNATURES = {
     "rsl_settings_default": (None, [C.POINTER(Settings)]),
     "rsl_analyze": (C.c_int, [C.c_int64, _I64P, _I64P, C.POINTER(Settings), C.POINTER(_P)]),
     "rsl_symbolic_info": (C.c_int, [_P, _I64P, _DP]),
+    "rsl_plan_create": (C.c_int, [C.c_int64, _I64P, _I64P, _I32P, C.c_int32, _I32P, _I32P, _I32P,
+                                  C.POINTER(Settings), C.POINTER(_P)]),
     "rsl_symbolic_export": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _I64P, _I32P]),
     "rsl_factor": (C.c_int, [_P, C.c_int32, _DP, C.POINTER(Settings), C.POINTER(_P)]),
     "rsl_refactor": (C.c_int, [_P, _DP]),
@@ -72,6 +75,12 @@ SIGNATURES = {
     "rsl_analyze_batch": (C.c_int, [C.c_int32, _I64P, C.POINTER(_I64P), C.POINTER(_I64P), C.POINTER(Settings),
                                     C.POINTER(_P)]),
     "rsl_factor_inertia_batch": (C.c_int, [_P, C.c_int32, _I64P]),
+    "rsl_gmres_block": (C.c_int, [_P, _DP, _DP, C.c_int64, C.c_double, C.c_int32, C.c_int32, C.c_int32, _DP,
+                                  _DP, _DP]),
+    "rsl_cocg": (C.c_int, [_P, _DP, _DP, C.c_int64, C.c_double, C.c_int32, C.c_int32, _DP, _DP]),
+    "rsl_cocr": (C.c_int, [_P, _DP, _DP, C.c_int64, C.c_double, C.c_int32, C.c_int32, _DP, _DP]),
+    "rsl_factor_save": (C.c_int, [_P, C.c_char_p]),
+    "rsl_factor_load": (C.c_int, [_P, C.c_char_p, C.POINTER(_P)]),
     "rsl_profile_read": (C.c_int, [_P, C.c_int32, _DP]),
     "rsl_profile_kind_name": (C.c_char_p, [C.c_int32]),
 }

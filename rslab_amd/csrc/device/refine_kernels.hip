@@ -16,14 +16,6 @@
 namespace rsl {
 
 constexpr int RT = 256;      // threads per block
-constexpr int RED_BLOCKS = 512;  // partials per reduction
-
-template <class T>
-__device__ inline T conj_t(T v);
-template <>
-__device__ inline double conj_t<double>(double v) { return v; }
-template <>
-__device__ inline cplx conj_t<cplx>(cplx v) { return {v.re, -v.im}; }
 
 // full-CSR value q := user lower value src[q]
 template <class T>
@@ -84,70 +76,12 @@ __global__ void k_copy_cols(int32_t n, int32_t k, const uint8_t* __restrict__ ma
   if (mask[idx % k]) dst[idx] = src[idx];
 }
 
-// column c of a row-major n x k block <-> contiguous vector
+// column cb of the row-major n x kb block B := column ca of the n x ka block A
 template <class T>
-__global__ void k_get_col(int32_t n, int32_t k, int32_t c, const T* __restrict__ blk, T* __restrict__ v) {
+__global__ void k_copy_col(int32_t n, int32_t ka, int32_t ca, const T* __restrict__ A, int32_t kb, int32_t cb,
+                           T* __restrict__ B) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = blk[i * k + c];
-}
-template <class T>
-__global__ void k_set_col(int32_t n, int32_t k, int32_t c, T* __restrict__ blk, const T* __restrict__ v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) blk[i * k + c] = v[i];
-}
-
-// partial[b] = sum over block b's slice of conj(x) * y (x == y gives |x|^2)
-template <class T>
-__global__ __launch_bounds__(RT) void k_dotc_partial(int32_t n, const T* __restrict__ x,
-                                                     const T* __restrict__ y, T* __restrict__ partial) {
-  T acc = F<T>::zero();
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
-    acc = acc + conj_t(x[i]) * y[i];
-  __shared__ T sm[RT];
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = RT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
-}
-
-// out[0] = sum of partial[0..m) in a fixed tree order
-template <class T>
-__global__ __launch_bounds__(RT) void k_sum_partials(int32_t m, const T* __restrict__ partial,
-                                                     T* __restrict__ out) {
-  T acc = F<T>::zero();
-  for (int i = threadIdx.x; i < m; i += RT) acc = acc + partial[i];
-  __shared__ T sm[RT];
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = RT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = sm[0];
-}
-
-// w -= h[0] * v with h read from device memory (Gram-Schmidt step, no host trip)
-template <class T>
-__global__ void k_axpy_dev(int32_t n, const T* __restrict__ h, const T* __restrict__ v, T* __restrict__ w) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) w[i] = w[i] - h[0] * v[i];
-}
-
-// y = a * x + y with a host scalar
-template <class T>
-__global__ void k_axpy(int32_t n, T a, const T* __restrict__ x, T* __restrict__ y) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = y[i] + a * x[i];
-}
-
-// y = a * x with a host real scalar
-template <class T>
-__global__ void k_scal(int32_t n, double a, const T* __restrict__ x, T* __restrict__ y) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = F<T>::scale(x[i], a);
+  if (i < n) B[i * kb + cb] = A[i * ka + ca];
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n) {
@@ -195,43 +129,10 @@ void launch_copy_cols(int32_t n, int32_t k, const uint8_t* mask, T* dst, const T
 }
 
 template <class T>
-void launch_get_col(int32_t n, int32_t k, int32_t c, const T* blk, T* v, hipStream_t st) {
+void launch_copy_col(int32_t n, int32_t ka, int32_t ca, const T* A, int32_t kb, int32_t cb, T* B, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_get_col<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, k, c, blk, v);
+  hipLaunchKernelGGL(k_copy_col<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, ka, ca, A, kb, cb, B);
 }
-
-template <class T>
-void launch_set_col(int32_t n, int32_t k, int32_t c, T* blk, const T* v, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_set_col<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, k, c, blk, v);
-}
-
-// out[0] = conj(x) . y ; partial must hold RED_BLOCKS entries
-template <class T>
-void launch_dotc(int32_t n, const T* x, const T* y, T* partial, T* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_dotc_partial<T>, dim3(RED_BLOCKS), dim3(RT), 0, st, n, x, y, partial);
-  hipLaunchKernelGGL(k_sum_partials<T>, dim3(1), dim3(RT), 0, st, (int32_t)RED_BLOCKS, partial, out);
-}
-
-template <class T>
-void launch_axpy_dev(int32_t n, const T* h, const T* v, T* w, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_axpy_dev<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, h, v, w);
-}
-
-template <class T>
-void launch_axpy(int32_t n, T a, const T* x, T* y, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_axpy<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, a, x, y);
-}
-
-template <class T>
-void launch_scal(int32_t n, double a, const T* x, T* y, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_scal<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, a, x, y);
-}
-
-int refine_reduction_blocks() { return RED_BLOCKS; }
 
 // ---- device-resident inputs (real field): the host computations of the
 // same quantities, in an order that gives the same bits ----
@@ -314,12 +215,7 @@ void launch_row_abs_max(int32_t n, const int64_t* rp, const int64_t* src, const 
   template void launch_colmax<T>(int32_t, int32_t, const T*, unsigned long long*, hipStream_t);         \
   template void launch_update_cols<T>(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);     \
   template void launch_copy_cols<T>(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);       \
-  template void launch_get_col<T>(int32_t, int32_t, int32_t, const T*, T*, hipStream_t);                \
-  template void launch_set_col<T>(int32_t, int32_t, int32_t, T*, const T*, hipStream_t);                \
-  template void launch_dotc<T>(int32_t, const T*, const T*, T*, T*, hipStream_t);                       \
-  template void launch_axpy_dev<T>(int32_t, const T*, const T*, T*, hipStream_t);                       \
-  template void launch_axpy<T>(int32_t, T, const T*, T*, hipStream_t);                                  \
-  template void launch_scal<T>(int32_t, double, const T*, T*, hipStream_t);
+  template void launch_copy_col<T>(int32_t, int32_t, int32_t, const T*, int32_t, int32_t, T*, hipStream_t);
 RSL_INST(double)
 RSL_INST(cplx)
 #undef RSL_INST

@@ -54,6 +54,7 @@ rsl::FactorSettings factor_settings(const rsl_settings& s) {
   if (std::isnan(s.drop_tol)) throw rsl::Error(RSL_INVALID_INPUT, "drop_tol must not be NaN");
   fs.drop_tol = s.drop_tol;
   fs.cmod_cap = s.cmod_cap;
+  fs.graphs = s.graphs != 0;
   return fs;
 }
 }  // namespace
@@ -99,6 +100,7 @@ void rsl_settings_default(rsl_settings* s) {
   s->wide_flops = 2.5e5;
   s->drop_tol = -1.0;
   s->cmod_cap = -1;
+  s->graphs = 1;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }
@@ -171,6 +173,44 @@ int rsl_analyze(int64_t n, const int64_t* col_ptr, const int64_t* row_idx, const
   });
 }
 
+
+int rsl_plan_create(int64_t n, const int64_t* col_ptr, const int64_t* row_idx, const int32_t* perm,
+                    int32_t nsuper, const int32_t* sn_first, const int32_t* sn_ncol, const int32_t* sn_parent,
+                    const rsl_settings* s, rsl_symbolic** out) {
+  *out = nullptr;
+  return guard([&] {
+    rsl_settings st;
+    if (s) st = *s;
+    else rsl_settings_default(&st);
+    if (n < 0 || n >= INT32_MAX) throw rsl::Error(RSL_INVALID_INPUT, "dimension out of range");
+    if (nsuper < 0 || (n > 0 && (!col_ptr || !perm || !sn_first || !sn_ncol || !sn_parent)))
+      throw rsl::Error(RSL_INVALID_INPUT, "null symbolic array");
+    if (st.panel_nb > 128) throw rsl::Error(RSL_INVALID_INPUT, "panel_nb above 128 is not supported");
+    const int64_t nnz = col_ptr[n];
+    std::vector<int32_t> ri(nnz);
+    for (int64_t k = 0; k < nnz; ++k) {
+      if (row_idx[k] < 0 || row_idx[k] >= n) throw rsl::Error(RSL_INVALID_INPUT, "row index out of bounds");
+      ri[k] = (int32_t)row_idx[k];
+    }
+    std::string v = rsl::validate_lower_csc((int32_t)n, col_ptr, ri.data(), nnz);
+    if (!v.empty()) throw rsl::Error(RSL_INVALID_INPUT, v);
+    auto ds = std::make_shared<rsl::DeviceSymbolic>();
+    try {
+      ds->plan.sym = rsl::from_partition((int32_t)n, col_ptr, ri.data(), std::vector<int32_t>(perm, perm + n),
+                                         std::vector<int32_t>(sn_first, sn_first + nsuper),
+                                         std::vector<int32_t>(sn_ncol, sn_ncol + nsuper),
+                                         std::vector<int32_t>(sn_parent, sn_parent + nsuper));
+    } catch (const std::runtime_error& e) {
+      throw rsl::Error(RSL_INVALID_INPUT, std::string("plan_create: ") + e.what());
+    }
+    rsl::build_plan(ds->plan, col_ptr, ri.data(), plan_params(st));
+    ds->h_colp.assign(col_ptr, col_ptr + n + 1);
+    ds->h_rowi = std::move(ri);
+    auto* h = new rsl_symbolic;
+    h->ds = std::move(ds);
+    *out = h;
+  });
+}
 
 int rsl_analyze_batch(int32_t count, const int64_t* n, const int64_t* const* col_ptr,
                       const int64_t* const* row_idx, const rsl_settings* s, rsl_symbolic** out) {
@@ -396,6 +436,61 @@ int rsl_refine(rsl_solver* f, const double* a_values, const double* b, int64_t n
     std::lock_guard<std::mutex> lk(f->mu);
     if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
     f->f->refine(a_values, b, nrhs, target, x, info);
+  });
+}
+
+int rsl_gmres_block(rsl_solver* f, const double* a_values, const double* b, int64_t nrhs, double tol,
+                    int32_t max_iter, int32_t restart, int32_t precond, const double* x0, double* x,
+                    double* info) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->krylov(0, a_values, b, nrhs, tol, max_iter, restart, precond, x0, x, info);
+  });
+}
+
+int rsl_cocg(rsl_solver* f, const double* a_values, const double* b, int64_t nrhs, double tol, int32_t max_iter,
+             int32_t precond, double* x, double* info) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->krylov(1, a_values, b, nrhs, tol, max_iter, 1, precond, nullptr, x, info);
+  });
+}
+
+int rsl_cocr(rsl_solver* f, const double* a_values, const double* b, int64_t nrhs, double tol, int32_t max_iter,
+             int32_t precond, double* x, double* info) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->krylov(2, a_values, b, nrhs, tol, max_iter, 1, precond, nullptr, x, info);
+  });
+}
+
+int rsl_factor_save(rsl_solver* f, const char* path) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (!path) throw rsl::Error(RSL_INVALID_INPUT, "null path");
+    std::FILE* fh = std::fopen(path, "wb");
+    if (!fh) throw rsl::Error(RSL_INVALID_INPUT, std::string("factor file: cannot create ") + path);
+    try {
+      f->f->save(fh);
+    } catch (...) {
+      std::fclose(fh);
+      throw;
+    }
+    if (std::fclose(fh) != 0) throw rsl::Error(RSL_INVALID_INPUT, "factor file: write failed");
+  });
+}
+
+int rsl_factor_load(rsl_symbolic* sym, const char* path, rsl_solver** out) {
+  *out = nullptr;
+  return guard([&] {
+    if (!path) throw rsl::Error(RSL_INVALID_INPUT, "null path");
+    auto f = rsl::load_factor(sym->ds, path);
+    auto* h = new rsl_solver;
+    h->f = std::move(f);
+    *out = h;
   });
 }
 

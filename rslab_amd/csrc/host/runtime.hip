@@ -10,6 +10,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+
+#include "../../../include/rslab_amd.h"
+#include "krylov.hpp"
 
 namespace rsl {
 
@@ -47,13 +51,7 @@ template <class T> void launch_residual(int32_t, int32_t, const int64_t*, const 
 template <class T> void launch_colmax(int32_t, int32_t, const T*, unsigned long long*, hipStream_t);
 template <class T> void launch_update_cols(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);
 template <class T> void launch_copy_cols(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);
-template <class T> void launch_get_col(int32_t, int32_t, int32_t, const T*, T*, hipStream_t);
-template <class T> void launch_set_col(int32_t, int32_t, int32_t, T*, const T*, hipStream_t);
-template <class T> void launch_dotc(int32_t, const T*, const T*, T*, T*, hipStream_t);
-template <class T> void launch_axpy_dev(int32_t, const T*, const T*, T*, hipStream_t);
-template <class T> void launch_axpy(int32_t, T, const T*, T*, hipStream_t);
-template <class T> void launch_scal(int32_t, double, const T*, T*, hipStream_t);
-int refine_reduction_blocks();
+template <class T> void launch_copy_col(int32_t, int32_t, int32_t, const T*, int32_t, int32_t, T*, hipStream_t);
 
 const char* launch_kind_name(int k) {
   static const char* names[LK_COUNT] = {"assemble",     "cmod",        "cdiv_small", "panel_getf2",
@@ -289,7 +287,76 @@ class DeviceFactor final : public DeviceFactorBase {
   // handle are serialized by the C ABI)
   DBuf rf_afull, rf_dx, rf_dbest, rf_dr, rf_ddx, rf_norms, rf_mask, rf_mask2;
   DBuf hs_db, hs_dx, hr_av, hr_db, hr_dxo;                                   // host-buffer calls
-  DBuf gm_bv, gm_xv, gm_rv, gm_wv, gm_V, gm_Z, gm_part, gm_hdev, gm_nrm;  // FGMRES
+  DBuf gm_bv, gm_xv, gm_rv, gm_nrm;  // GMRES-IR: the escalated columns' b, x, r and norms
+  KrylovEngine<T> kry;
+  const T* kry_A = nullptr;  // operator values (full symmetric CSR) of the current Krylov call
+  bool kry_pre = true;       // precondition with this factor (else the identity)
+
+  // The Krylov engine's operator is A (full symmetric CSR values on the
+  // device, the plan's pattern), its preconditioner this factor's solve.
+  void bind_krylov(const T* A, bool pre) {
+    sym->ensure_full_csr();
+    kry_A = A;
+    kry_pre = pre;
+    kry.n = sym->plan.n;
+    kry.st = sym->stream;
+    const int64_t* rp = sym->fa_rp.as<int64_t>();
+    const int32_t* ci = sym->fa_ci.as<int32_t>();
+    kry.apply = [this, rp, ci](const T* x, T* y, int32_t k) {
+      launch_residual<T>(sym->plan.n, k, rp, ci, kry_A, x, nullptr, y, sym->stream, prof_on ? &prof : nullptr);
+    };
+    kry.resid = [this, rp, ci](const T* x, const T* b, T* r, int32_t k) {
+      launch_residual<T>(sym->plan.n, k, rp, ci, kry_A, x, b, r, sym->stream, prof_on ? &prof : nullptr);
+    };
+    kry.precond = [this](const T* r, T* z, int32_t k) {
+      if (kry_pre)
+        solve_device(r, k, z, sym->stream);
+      else
+        hip_check(hipMemcpyAsync(z, r, sizeof(T) * (size_t)sym->plan.n * k, hipMemcpyDeviceToDevice, sym->stream),
+                  "identity preconditioner");
+    };
+  }
+
+  // Krylov solvers over host buffers (method 0: FGMRES / block GMRES,
+  // 1: COCG, 2: COCR); info[4c..4c+4) = {iterations, final relative residual,
+  // converged, stop reason}
+  void krylov(int method, const double* a_values, const double* b, int64_t nrhs, double tol, int max_iter,
+              int restart, int precond, const double* x0, double* x, double* info) override {
+    const Plan& P = sym->plan;
+    hipStream_t st = sym->stream;
+    const int32_t n = P.n, k = (int32_t)nrhs;
+    if (precond && !valid) throw Error(INVALID_INPUT, "the last factorization of this handle failed; refactor first");
+    if (k > 256) throw Error(INVALID_INPUT, "Krylov solvers take at most 256 right-hand sides per call");
+    if (max_iter < 0 || (method == 0 && restart < 1)) throw Error(INVALID_INPUT, "iteration limits must be positive");
+    sym->ensure_full_csr();
+    const size_t bytes = sizeof(T) * (size_t)n * k;
+    grow(hr_av, sizeof(T) * std::max<int64_t>(P.nnz, 1));
+    grow(rf_afull, sizeof(T) * std::max<int64_t>(sym->nnz_full, 1));
+    grow(hr_db, std::max<size_t>(bytes, 1));
+    grow(hr_dxo, std::max<size_t>(bytes, 1));
+    hip_check(hipMemcpyAsync(hr_av.p, a_values, sizeof(T) * P.nnz, hipMemcpyHostToDevice, st), "upload A");
+    launch_expand<T>(sym->nnz_full, sym->fa_src.as<int64_t>(), hr_av.as<T>(), rf_afull.as<T>(), st);
+    hip_check(hipMemcpyAsync(hr_db.p, b, bytes, hipMemcpyHostToDevice, st), "upload b");
+    if (x0) hip_check(hipMemcpyAsync(hr_dxo.p, x0, bytes, hipMemcpyHostToDevice, st), "upload x0");
+    bind_krylov(rf_afull.as<T>(), precond != 0);
+    std::vector<KrylovCol> res;
+    if (method == 0)
+      kry.fgmres(hr_db.as<T>(), hr_dxo.as<T>(), k, tol, max_iter, restart, x0 != nullptr, res);
+    else if (method == 1)
+      kry.cocg(hr_db.as<T>(), hr_dxo.as<T>(), k, tol, max_iter, res);
+    else if (method == 2)
+      kry.cocr(hr_db.as<T>(), hr_dxo.as<T>(), k, tol, max_iter, res);
+    else
+      throw Error(INVALID_INPUT, "unknown Krylov method");
+    hip_check(hipMemcpyAsync(x, hr_dxo.p, bytes, hipMemcpyDeviceToHost, st), "download x");
+    hip_check(hipStreamSynchronize(st), "krylov");
+    for (int c = 0; c < k; ++c) {
+      info[4 * c + 0] = res[c].iters;
+      info[4 * c + 1] = res[c].final_res;
+      info[4 * c + 2] = res[c].converged ? 1.0 : 0.0;
+      info[4 * c + 3] = res[c].stop;
+    }
+  }
   static void grow(DBuf& b, size_t nb) {
     if (b.bytes < nb) b.alloc(nb);
   }
@@ -301,6 +368,7 @@ class DeviceFactor final : public DeviceFactorBase {
   bool configured = false;
 
   ~DeviceFactor() override {
+    drop_graph();
     if (ev_ready) {
       (void)hipEventDestroy(ev_caller);
       (void)hipEventDestroy(ev_done);
@@ -357,6 +425,7 @@ class DeviceFactor final : public DeviceFactorBase {
     fv.sn_inertia = sn_inert.as<int32_t>();
     fv.rs_e = rows_e.as<int32_t>();
     fv.status = status.as<int32_t>();
+    make_events();
   }
 
   int field() const override { return FieldOf<T>::id; }
@@ -429,7 +498,13 @@ class DeviceFactor final : public DeviceFactorBase {
         pol.have_floor = 1;
         pol.floor = std::max(amax, 1.0) * 2.220446049250313e-16;
       }
-      run_factor(v, pol, t0);
+      // the replayed graph reads the handle's own values buffer
+      const double* src = v;
+      if (settings.graphs && !prof_on) {
+        hip_check(hipMemcpyAsync(vals_d.p, v, sizeof(double) * P.nnz, hipMemcpyDeviceToDevice, st), "copy values");
+        src = vals_d.as<double>();
+      }
+      run_factor(src, pol, t0);
     }
   }
 
@@ -465,18 +540,97 @@ class DeviceFactor final : public DeviceFactorBase {
   }
 
   // numeric factor from device values (user lower CSC) with scale_d in place
+  // A same-pattern refactor replays the factor's launch sequence as one HIP
+  // graph: captured on the first refactor (the plan, the storage and every
+  // grid are fixed per handle), replayed while the values buffer and the
+  // pivot policy are the ones it was captured with. The sequence holds no
+  // memset or host copy (the status is reset by a kernel). Eager when
+  // profiling (each launch is bracketed by events) or when graphs are off.
+  hipGraphExec_t fgraph = nullptr;
+  const T* fgraph_vals = nullptr;
+  PivotPolicy fgraph_pol{0, 0.0};
+  bool graph_broken = false;  // capture failed once: stay eager
+  int64_t factors_done = 0;
+
+  void drop_graph() {
+    if (fgraph) (void)hipGraphExecDestroy(fgraph);
+    fgraph = nullptr;
+  }
+
+  void launch_factor(const T* vals, const PivotPolicy& pol, hipStream_t st) {
+    const bool want = settings.graphs && !prof_on && !graph_broken && factors_done > 0;
+    if (!want) {
+      enqueue_factor(vals, pol, st, prof_on ? &prof : nullptr);
+      return;
+    }
+    if (!fgraph || fgraph_vals != vals || fgraph_pol.have_floor != pol.have_floor || fgraph_pol.floor != pol.floor) {
+      drop_graph();
+      hipGraph_t g = nullptr;
+      hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        enqueue_factor(vals, pol, st, nullptr);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (e == hipSuccess && g) e = hipGraphInstantiate(&fgraph, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (e != hipSuccess || !fgraph) {
+        (void)hipGetLastError();
+        fgraph = nullptr;
+        graph_broken = true;
+        enqueue_factor(vals, pol, st, nullptr);
+        return;
+      }
+      fgraph_vals = vals;
+      fgraph_pol = pol;
+    }
+    hip_check(hipGraphLaunch(fgraph, st), "hipGraphLaunch");
+  }
+
   void run_factor(const T* vals, const PivotPolicy& pol,
                   std::chrono::steady_clock::time_point t0) {
     const Plan& P = sym->plan;
-    const DevSym& S = sym->view;
     hipStream_t st = sym->stream;
     if (!configured) {
       hip_check(configure_factor_kernels<T>(), "configure kernels");
       configured = true;
     }
     valid = false;
+    launch_factor(vals, pol, st);
+    ++factors_done;
+    unsigned long long cnt[5];
+    int32_t stat[2];
+    hip_check(hipMemcpyAsync(cnt, counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st), "counters");
+    hip_check(hipMemcpyAsync(stat, status.p, sizeof(stat), hipMemcpyDeviceToHost, st), "status");
+    hip_check(hipStreamSynchronize(st), "factor");
+    stats.t_factor_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stat[0] == 2) throw Error(DEVICE_ERROR, "device: inconsistent pivot map in the factor");
+    if (stat[0] != 0) {
+      // the first zero pivot in (supernode, column) order
+      const int32_t s = (int32_t)(cnt[4] >> 32), col = (int32_t)(cnt[4] & 0xffffffffu);
+      std::string msg = "numerically rank deficient";
+      if (cnt[4] != ~0ull && s >= 0 && s < P.ns)
+        msg += ": zero pivot at column " + std::to_string(col) + " of supernode " + std::to_string(s) + " (" +
+               std::to_string(P.sn_ncol[s]) + " columns, elimination index " + std::to_string(P.e_off[s] + col) +
+               ")";
+      throw Error(NUMERICALLY_RANK_DEFICIENT, msg);
+    }
+    stats.inertia[0] = (int64_t)cnt[0];
+    stats.inertia[1] = (int64_t)cnt[1];
+    stats.inertia[2] = (int64_t)cnt[2];
+    stats.n_perturbed = (int64_t)cnt[3];
+    valid = true;
+  }
+
+  // the numeric factor's launch sequence on `st` (no host synchronization)
+  void enqueue_factor(const T* vals, const PivotPolicy& pol, hipStream_t st, LaunchProfiler* pf) {
+    const Plan& P = sym->plan;
+    const DevSym& S = sym->view;
     launch_factor_init(counters.as<unsigned long long>(), status.as<int32_t>(), st);
-    LaunchProfiler* pf = prof_on ? &prof : nullptr;
     launch_fill<T>(fv.pool, P.panel_total, F<T>::zero(), st);
     launch_assemble<T>(S, fv, vals, scale_d.as<double>(), st, pf);
     const int nb = P.params.panel_nb < 8 ? 8 : P.params.panel_nb;
@@ -514,48 +668,29 @@ class DeviceFactor final : public DeviceFactorBase {
     // after the last consumer of every panel, so it changes the stored L only
     if (settings.drop_tol >= 0.0) launch_drop<T>(S, fv, settings.drop_tol, st);
     hip_check(hipGetLastError(), "factor launch");
-    unsigned long long cnt[5];
-    int32_t stat[2];
-    hip_check(hipMemcpyAsync(cnt, counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st), "counters");
-    hip_check(hipMemcpyAsync(stat, status.p, sizeof(stat), hipMemcpyDeviceToHost, st), "status");
-    hip_check(hipStreamSynchronize(st), "factor");
-    stats.t_factor_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (stat[0] == 2) throw Error(DEVICE_ERROR, "device: inconsistent pivot map in the factor");
-    if (stat[0] != 0) {
-      // the first zero pivot in (supernode, column) order
-      const int32_t s = (int32_t)(cnt[4] >> 32), col = (int32_t)(cnt[4] & 0xffffffffu);
-      std::string msg = "numerically rank deficient";
-      if (cnt[4] != ~0ull && s >= 0 && s < P.ns)
-        msg += ": zero pivot at column " + std::to_string(col) + " of supernode " + std::to_string(s) + " (" +
-               std::to_string(P.sn_ncol[s]) + " columns, elimination index " + std::to_string(P.e_off[s] + col) +
-               ")";
-      throw Error(NUMERICALLY_RANK_DEFICIENT, msg);
-    }
-    stats.inertia[0] = (int64_t)cnt[0];
-    stats.inertia[1] = (int64_t)cnt[1];
-    stats.inertia[2] = (int64_t)cnt[2];
-    stats.n_perturbed = (int64_t)cnt[3];
-    valid = true;
   }
 
   // y (device, e-order) := solve in place; b/x device user-order buffers
   void solve_device(const void* d_b, int64_t nrhs, void* d_x, hipStream_t st) override {
     if (!valid) throw Error(INVALID_INPUT, "the last factorization of this handle failed; refactor first");
     const Plan& P = sym->plan;
-    const DevSym& S = sym->view;
     const size_t need = sizeof(T) * std::max<int64_t>(P.n * nrhs, 1);
     if (work_y.bytes < need) work_y.alloc(need);
     const size_t needu = sizeof(T) * std::max<int64_t>(P.u_total * nrhs, 1);
     if (work_b.bytes < needu) work_b.alloc(needu);
-    T* y = work_y.as<T>();
-    T* u = work_b.as<T>();
+    if (work_z.bytes < need) work_z.alloc(need);
+    solve_core(d_b, nrhs, d_x, st, work_y.as<T>(), work_b.as<T>(), work_z.as<T>(), prof_on ? &prof : nullptr);
+  }
+
+  // the solve's launch sequence on `st` with caller-provided scratch:
+  // y, z hold n x nrhs, u holds u_total x nrhs elements
+  void solve_core(const void* d_b, int64_t nrhs, void* d_x, hipStream_t st, T* y, T* u, T* z,
+                  LaunchProfiler* pf) {
+    const Plan& P = sym->plan;
+    const DevSym& S = sym->view;
     const int2* fi = sym->fwd_items.as<int2>();
     const int2* bi = sym->bwd_items.as<int2>();
-    LaunchProfiler* pf = prof_on ? &prof : nullptr;
     const double ts = sizeof(T);
-    if (work_z.bytes < need) work_z.alloc(need);
-    T* z = work_z.as<T>();  // solved own blocks of the wide supernodes, copied back per level
     const SItem* si = sym->sitems.as<SItem>();
     const SItem* bi2 = sym->bitems.as<SItem>();
     const SItem* zi = sym->zitems.as<SItem>();
@@ -658,6 +793,96 @@ class DeviceFactor final : public DeviceFactorBase {
   void refine_device(const void*, const void*, int64_t, double, void*, double*, hipStream_t) override;
   void refine_core(const T* av, const T* b, int32_t k, double target, double anorm, T* xout, double* info);
 
+  // ---- factor file: everything a solve or a refinement reads ----
+  // Layout (little-endian): magic "RSLFACT1", u64 plan digest, i32 field,
+  // i32 precision, i32 pivot, i32 scaling, f64 abs_floor, f64 drop_tol,
+  // i64 inertia[3], i64 n_perturbed, then the arrays, each as u64 byte count
+  // and bytes: scale, panel pool, d, d_sub, 2x2 flags, local pivot order,
+  // position of each permuted index, user index of each position, position
+  // of each panel row, per-supernode inertia.
+  struct Arr {
+    DBuf* b;
+    size_t bytes;
+  };
+  std::vector<Arr> file_arrays() {
+    const Plan& P = sym->plan;
+    const size_t n = (size_t)P.n;
+    return {{&scale_d, sizeof(double) * n},
+            {&pool, sizeof(T) * (size_t)P.panel_total},
+            {&d, sizeof(T) * n},
+            {&ds, sizeof(T) * n},
+            {&two, n},
+            {&lperm, sizeof(int32_t) * n},
+            {&e_of_g, sizeof(int32_t) * n},
+            {&perm_e, sizeof(int32_t) * n},
+            {&rows_e, sizeof(int32_t) * (size_t)P.rs_off[P.ns]},
+            {&sn_inert, sizeof(int32_t) * 3 * (size_t)P.ns}};
+  }
+
+  void save(std::FILE* fh) override {
+    if (!valid) throw Error(INVALID_INPUT, "the last factorization of this handle failed; nothing to save");
+    hipStream_t st = sym->stream;
+    hip_check(hipStreamSynchronize(st), "save");
+    auto put = [&](const void* p, size_t nb) {
+      if (nb && std::fwrite(p, 1, nb, fh) != nb) throw Error(INVALID_INPUT, "factor file: write failed");
+    };
+    put("RSLFACT1", 8);
+    const uint64_t dg = sym->digest();
+    const int32_t hdr[4] = {FieldOf<T>::id, (int32_t)settings.precision, settings.pivot, settings.scaling};
+    put(&dg, 8);
+    put(hdr, sizeof(hdr));
+    put(&settings.abs_floor, 8);
+    put(&settings.drop_tol, 8);
+    put(stats.inertia, sizeof(stats.inertia));
+    put(&stats.n_perturbed, 8);
+    std::vector<unsigned char> host;
+    for (const Arr& a : file_arrays()) {
+      const uint64_t nb = a.bytes;
+      put(&nb, 8);
+      host.resize(a.bytes);
+      if (a.bytes)
+        hip_check(hipMemcpyAsync(host.data(), a.b->p, a.bytes, hipMemcpyDeviceToHost, st), "save download");
+      hip_check(hipStreamSynchronize(st), "save download");
+      put(host.data(), a.bytes);
+    }
+  }
+
+  void load(std::FILE* fh) override {
+    hipStream_t st = sym->stream;
+    auto get = [&](void* p, size_t nb) {
+      if (nb && std::fread(p, 1, nb, fh) != nb) throw Error(INVALID_INPUT, "factor file: truncated");
+    };
+    char magic[8];
+    get(magic, 8);
+    if (std::memcmp(magic, "RSLFACT1", 8) != 0) throw Error(INVALID_INPUT, "factor file: not a factor file");
+    uint64_t dg = 0;
+    int32_t hdr[4];
+    get(&dg, 8);
+    get(hdr, sizeof(hdr));
+    if (dg != sym->digest())
+      throw Error(INVALID_INPUT, "factor file: written for another symbolic analysis (pattern, ordering or partition)");
+    if (hdr[0] != FieldOf<T>::id) throw Error(INVALID_INPUT, "factor file: written for the other field");
+    settings.precision = static_cast<Precision>(hdr[1]);
+    settings.pivot = hdr[2];
+    settings.scaling = hdr[3];
+    get(&settings.abs_floor, 8);
+    get(&settings.drop_tol, 8);
+    get(stats.inertia, sizeof(stats.inertia));
+    get(&stats.n_perturbed, 8);
+    std::vector<unsigned char> host;
+    for (const Arr& a : file_arrays()) {
+      uint64_t nb = 0;
+      get(&nb, 8);
+      if (nb != a.bytes) throw Error(INVALID_INPUT, "factor file: array size does not match the plan");
+      host.resize(a.bytes);
+      get(host.data(), a.bytes);
+      if (a.bytes) hip_check(hipMemcpyAsync(a.b->p, host.data(), a.bytes, hipMemcpyHostToDevice, st), "load upload");
+      hip_check(hipStreamSynchronize(st), "load upload");
+    }
+    stats.t_factor_ms = 0.0;
+    valid = true;
+  }
+
   void supernode_inertia(std::vector<int32_t>& out) override {
     const Plan& P = sym->plan;
     out.assign(3 * (size_t)P.ns, 0);
@@ -665,11 +890,6 @@ class DeviceFactor final : public DeviceFactorBase {
                              sym->stream), "inertia download");
     hip_check(hipStreamSynchronize(sym->stream), "inertia download");
   }
-
- private:
-  void gmres_column(int32_t n, int32_t k, int32_t c, const int64_t* rp, const int32_t* ci, const T* A,
-                    const T* bblk, T* best, double tol, double anorm, double bninf, double& best_be,
-                    int& iters, hipStream_t st);
 };
 
 // ---------------------------------------------------------------- refinement
@@ -841,11 +1061,48 @@ void DeviceFactor<T>::refine_core(const T* av, const T* dbp, int32_t k, double t
     hip_check(hipStreamSynchronize(st), "refine step");
   }
 
+  // GMRES-IR (mixed.rs:195-219): FGMRES with this factor as preconditioner,
+  // warm-started at each escalated column's best iterate, the escalated
+  // columns advancing together; a column keeps the GMRES result when its
+  // backward error is smaller
   std::vector<int> gm_iters(k, 0);
-  for (int c = 0; c < k; ++c) {
-    if (phase[c] != 2) continue;
-    gmres_column(n, k, c, rp, ci, A, dbp, dbest.as<T>(), target, anorm, bn[c], best_be[c],
-                 gm_iters[c], st);
+  std::vector<int> esc;
+  for (int c = 0; c < k; ++c)
+    if (phase[c] == 2) esc.push_back(c);
+  if (!esc.empty()) {
+    const int32_t ke = (int32_t)esc.size();
+    const size_t eb = sizeof(T) * (size_t)n * ke;
+    grow(gm_bv, eb);
+    grow(gm_xv, eb);
+    grow(gm_rv, eb);
+    T *be_ = gm_bv.as<T>(), *xe = gm_xv.as<T>(), *re_ = gm_rv.as<T>();
+    for (int q = 0; q < ke; ++q) {
+      launch_copy_col<T>(n, k, esc[q], dbp, ke, q, be_, st);
+      launch_copy_col<T>(n, k, esc[q], dbest.as<T>(), ke, q, xe, st);
+    }
+    bind_krylov(A, true);
+    std::vector<KrylovCol> res;
+    kry.fgmres(be_, xe, ke, target, GMRES_MAX, GMRES_RESTART, true, res);
+    // backward error of each GMRES result
+    launch_residual<T>(n, ke, rp, ci, A, xe, be_, re_, st, prof_on ? &prof : nullptr);
+    grow(gm_nrm, sizeof(unsigned long long) * 2 * ke);
+    auto* nr = gm_nrm.as<unsigned long long>();
+    hip_check(hipMemsetAsync(nr, 0, sizeof(unsigned long long) * 2 * ke, st), "memset");
+    launch_colmax<T>(n, ke, re_, nr, st);
+    launch_colmax<T>(n, ke, xe, nr + ke, st);
+    std::vector<unsigned long long> hb(2 * ke);
+    hip_check(hipMemcpyAsync(hb.data(), nr, sizeof(unsigned long long) * 2 * ke, hipMemcpyDeviceToHost, st), "norms");
+    hip_check(hipStreamSynchronize(st), "gmres norms");
+    for (int q = 0; q < ke; ++q) {
+      const int c = esc[q];
+      gm_iters[c] = res[q].iters;
+      const double be2 = bits_to_double(hb[q]) /
+                         std::max(anorm * bits_to_double(hb[ke + q]) + bn[c], 2.2250738585072014e-308);
+      if (be2 < best_be[c]) {
+        best_be[c] = be2;
+        launch_copy_col<T>(n, ke, q, xe, k, c, dbest.as<T>(), st);
+      }
+    }
   }
   hip_check(hipMemcpyAsync(xout, dbest.p, sizeof(T) * n * k, hipMemcpyDeviceToDevice, st), "copy x");
   for (int c = 0; c < k; ++c) {
@@ -853,160 +1110,6 @@ void DeviceFactor<T>::refine_core(const T* av, const T* dbp, int32_t k, double t
     info[4 * c + 1] = gm_iters[c];
     info[4 * c + 2] = best_be[c];
     info[4 * c + 3] = best_be[c] <= target ? 1.0 : 0.0;
-  }
-}
-
-// gmres (src/numeric/iterative.rs:822-1021): flexible restarted GMRES(30),
-// at most 120 iterations, MGS with DGKS re-orthogonalization (eta = 1/sqrt 2),
-// right preconditioner = the device solve with this factor, warm start at
-// column c of `best`, stop on the true residual relative to ||b||_2. The
-// result replaces column c of `best` when its backward error is smaller.
-template <class T>
-void DeviceFactor<T>::gmres_column(int32_t n, int32_t k, int32_t c, const int64_t* rp, const int32_t* ci,
-                                   const T* A, const T* bblk, T* best, double tol, double anorm, double bninf,
-                                   double& best_be, int& iters, hipStream_t st) {
-  const int m = GMRES_RESTART;
-  const double eta = 0.70710678118654752440;
-  const size_t vb = sizeof(T) * std::max(n, 1);
-  grow(gm_bv, vb);
-  grow(gm_xv, vb);
-  grow(gm_rv, vb);
-  grow(gm_wv, vb);
-  grow(gm_V, vb * (m + 1));
-  grow(gm_Z, vb * m);
-  grow(gm_part, sizeof(T) * refine_reduction_blocks());
-  grow(gm_hdev, sizeof(T) * (2 * (m + 1) + 2));
-  grow(gm_nrm, sizeof(unsigned long long) * 2);
-  DBuf &bv = gm_bv, &xv = gm_xv, &rv = gm_rv, &wv = gm_wv, &V = gm_V, &Z = gm_Z, &part = gm_part, &hdev = gm_hdev,
-       &nrm = gm_nrm;
-  T* b1 = bv.as<T>();
-  T* x1 = xv.as<T>();
-  T* r1 = rv.as<T>();
-  T* w1 = wv.as<T>();
-  T* Vb = V.as<T>();
-  T* Zb = Z.as<T>();
-  T* hd = hdev.as<T>();
-  auto vcol = [&](int i) { return Vb + (size_t)i * n; };
-  auto zcol = [&](int i) { return Zb + (size_t)i * n; };
-  auto dot_to_host = [&](const T* a, const T* bb) {
-    launch_dotc<T>(n, a, bb, part.as<T>(), hd, st);
-    T out;
-    hip_check(hipMemcpyAsync(&out, hd, sizeof(T), hipMemcpyDeviceToHost, st), "dot");
-    hip_check(hipStreamSynchronize(st), "dot");
-    return out;
-  };
-  launch_get_col<T>(n, k, c, bblk, b1, st);
-  launch_get_col<T>(n, k, c, best, x1, st);
-  const double bnorm = std::sqrt(hreal(dot_to_host(b1, b1)));
-  iters = 0;
-  if (bnorm == 0.0) return;  // x = 0 is exact, and best already has berr 0 then
-  std::vector<T> h((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 1), hcol2(m + 1);
-  int total = 0;
-  for (;;) {
-    launch_residual<T>(n, 1, rp, ci, A, x1, b1, r1, st, prof_on ? &prof : nullptr);
-    const double beta = std::sqrt(hreal(dot_to_host(r1, r1)));
-    if (beta / bnorm <= tol || total >= GMRES_MAX) break;
-    launch_scal<T>(n, 1.0 / beta, r1, vcol(0), st);
-    std::fill(h.begin(), h.end(), F<T>::zero());
-    std::fill(cs.begin(), cs.end(), F<T>::zero());
-    std::fill(sn.begin(), sn.end(), F<T>::zero());
-    std::fill(g.begin(), g.end(), F<T>::zero());
-    g[0] = hfrom<T>(beta);
-    int jdim = 0;
-    for (int j = 0; j < m; ++j) {
-      if (total >= GMRES_MAX) break;
-      solve_device(vcol(j), 1, zcol(j), st);
-      launch_residual<T>(n, 1, rp, ci, A, zcol(j), nullptr, w1, st, prof_on ? &prof : nullptr);
-      launch_dotc<T>(n, w1, w1, part.as<T>(), hd + 2 * (m + 1), st);  // ||w0||^2
-      for (int i = 0; i <= j; ++i) {
-        launch_dotc<T>(n, vcol(i), w1, part.as<T>(), hd + i, st);
-        launch_axpy_dev<T>(n, hd + i, vcol(i), w1, st);
-      }
-      launch_dotc<T>(n, w1, w1, part.as<T>(), hd + 2 * (m + 1) + 1, st);
-      std::vector<T> tail(2);
-      hip_check(hipMemcpyAsync(hcol.data(), hd, sizeof(T) * (j + 1), hipMemcpyDeviceToHost, st), "h");
-      hip_check(hipMemcpyAsync(tail.data(), hd + 2 * (m + 1), sizeof(T) * 2, hipMemcpyDeviceToHost, st), "h");
-      hip_check(hipStreamSynchronize(st), "arnoldi");
-      const double wnorm0 = std::sqrt(hreal(tail[0]));
-      double hnrm = std::sqrt(hreal(tail[1]));
-      for (int i = 0; i <= j; ++i) h[(size_t)i * m + j] = hcol[i];
-      if (hnrm < eta * wnorm0) {
-        for (int i = 0; i <= j; ++i) {
-          launch_dotc<T>(n, vcol(i), w1, part.as<T>(), hd + (m + 1) + i, st);
-          launch_axpy_dev<T>(n, hd + (m + 1) + i, vcol(i), w1, st);
-        }
-        launch_dotc<T>(n, w1, w1, part.as<T>(), hd + 2 * (m + 1) + 1, st);
-        hip_check(hipMemcpyAsync(hcol2.data(), hd + (m + 1), sizeof(T) * (j + 1), hipMemcpyDeviceToHost, st),
-                  "h");
-        hip_check(hipMemcpyAsync(tail.data(), hd + 2 * (m + 1), sizeof(T) * 2, hipMemcpyDeviceToHost, st), "h");
-        hip_check(hipStreamSynchronize(st), "arnoldi");
-        for (int i = 0; i <= j; ++i) h[(size_t)i * m + j] = h[(size_t)i * m + j] + hcol2[i];
-        hnrm = std::sqrt(hreal(tail[1]));
-      }
-      h[(size_t)(j + 1) * m + j] = hfrom<T>(hnrm);
-      launch_scal<T>(n, hnrm > 0.0 ? 1.0 / hnrm : 0.0, w1, vcol(j + 1), st);
-      for (int i = 0; i < j; ++i) {
-        const T a = h[(size_t)i * m + j], bb = h[(size_t)(i + 1) * m + j];
-        const T temp = hconj(cs[i]) * a + hconj(sn[i]) * bb;
-        h[(size_t)(i + 1) * m + j] = F<T>::zero() - sn[i] * a + cs[i] * bb;
-        h[(size_t)i * m + j] = temp;
-      }
-      // givens (iterative.rs:599-609)
-      const T f = h[(size_t)j * m + j], gg = h[(size_t)(j + 1) * m + j];
-      T cj, sj;
-      if (hzero(gg)) {
-        cj = F<T>::one();
-        sj = F<T>::zero();
-      } else if (hzero(f)) {
-        cj = F<T>::zero();
-        sj = F<T>::one();
-      } else {
-        const double r = std::sqrt(F<T>::magsq(f) + F<T>::magsq(gg));
-        cj = F<T>::scale(f, 1.0 / r);
-        sj = F<T>::scale(gg, 1.0 / r);
-      }
-      cs[j] = cj;
-      sn[j] = sj;
-      h[(size_t)j * m + j] = hconj(cj) * f + hconj(sj) * gg;
-      h[(size_t)(j + 1) * m + j] = F<T>::zero();
-      const T g_next = F<T>::zero() - sj * g[j];
-      g[j] = hconj(cj) * g[j];
-      g[j + 1] = g_next;
-      ++total;
-      jdim = j + 1;
-      if (hmag(g[j + 1]) / bnorm <= tol) break;
-    }
-    // well_conditioned_dim_flat (iterative.rs:646-661), back substitution, x += Z y
-    double hmax = 0.0;
-    for (int i = 0; i < jdim; ++i) hmax = std::max(hmax, hmag(h[(size_t)i * m + i]));
-    int jd = jdim;
-    for (int i = 0; i < jdim; ++i)
-      if (hmag(h[(size_t)i * m + i]) <= 2.220446049250313e-16 * hmax) {
-        jd = i;
-        break;
-      }
-    for (int i = jd - 1; i >= 0; --i) {
-      T s = g[i];
-      for (int q = i + 1; q < jd; ++q) s = s - h[(size_t)i * m + q] * y[q];
-      y[i] = s * F<T>::recip(h[(size_t)i * m + i]);
-    }
-    for (int i = 0; i < jd; ++i) launch_axpy<T>(n, y[i], zcol(i), x1, st);
-  }
-  iters = total;
-  // backward error of the GMRES result; keep whichever iterate is better
-  launch_residual<T>(n, 1, rp, ci, A, x1, b1, r1, st, prof_on ? &prof : nullptr);
-  hip_check(hipMemsetAsync(nrm.p, 0, nrm.bytes, st), "memset");
-  launch_colmax<T>(n, 1, r1, nrm.as<unsigned long long>(), st);
-  launch_colmax<T>(n, 1, x1, nrm.as<unsigned long long>() + 1, st);
-  unsigned long long hb[2];
-  hip_check(hipMemcpyAsync(hb, nrm.p, sizeof(hb), hipMemcpyDeviceToHost, st), "norms");
-  hip_check(hipStreamSynchronize(st), "gmres");
-  const double be2 =
-      bits_to_double(hb[0]) / std::max(anorm * bits_to_double(hb[1]) + bninf, 2.2250738585072014e-308);
-  if (be2 < best_be) {
-    best_be = be2;
-    launch_set_col<T>(n, k, c, best, x1, st);
-    hip_check(hipStreamSynchronize(st), "gmres");
   }
 }
 
@@ -1020,6 +1123,56 @@ std::unique_ptr<DeviceFactorBase> make_factor(std::shared_ptr<DeviceSymbolic> sy
   else if (field == 1) f.reset(new DeviceFactor<cplx>(sym, fs));
   else throw Error(INVALID_INPUT, "unknown field");
   f->refactor(values);
+  return f;
+}
+
+uint64_t DeviceSymbolic::digest() const {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t nb) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < nb; ++i) {
+      h ^= c[i];
+      h *= 1099511628211ull;
+    }
+  };
+  const Plan& P = plan;
+  const int64_t dims[4] = {P.n, P.ns, P.panel_total, P.nnz};
+  mix(dims, sizeof(dims));
+  mix(h_colp.data(), sizeof(int64_t) * h_colp.size());
+  mix(h_rowi.data(), sizeof(int32_t) * h_rowi.size());
+  mix(P.sym.perm.data(), sizeof(int32_t) * P.sym.perm.size());
+  mix(P.sn_first.data(), sizeof(int32_t) * P.sn_first.size());
+  mix(P.sn_ncol.data(), sizeof(int32_t) * P.sn_ncol.size());
+  mix(P.rs_idx.data(), sizeof(int32_t) * P.rs_idx.size());
+  mix(P.panel_off.data(), sizeof(int64_t) * P.panel_off.size());
+  return h;
+}
+
+std::unique_ptr<DeviceFactorBase> load_factor(std::shared_ptr<DeviceSymbolic> sym, const char* path) {
+  std::FILE* fh = std::fopen(path, "rb");
+  if (!fh) throw Error(INVALID_INPUT, std::string("factor file: cannot open ") + path);
+  struct Closer {
+    std::FILE* f;
+    ~Closer() { std::fclose(f); }
+  } closer{fh};
+  char magic[8];
+  uint64_t dg = 0;
+  int32_t hdr[4];
+  if (std::fread(magic, 1, 8, fh) != 8 || std::fread(&dg, 1, 8, fh) != 8 || std::fread(hdr, 1, sizeof(hdr), fh) != sizeof(hdr))
+    throw Error(INVALID_INPUT, "factor file: truncated header");
+  if (std::memcmp(magic, "RSLFACT1", 8) != 0) throw Error(INVALID_INPUT, "factor file: not a factor file");
+  if (hdr[1] < 0 || hdr[1] > 2) throw Error(INVALID_INPUT, "factor file: unknown precision");
+  FactorSettings fs;
+  fs.precision = static_cast<Precision>(hdr[1]);
+  fs.pivot = hdr[2];
+  fs.scaling = hdr[3];
+  sym->upload();
+  std::unique_ptr<DeviceFactorBase> f;
+  if (hdr[0] == 0) f.reset(new DeviceFactor<double>(sym, fs));
+  else if (hdr[0] == 1) f.reset(new DeviceFactor<cplx>(sym, fs));
+  else throw Error(INVALID_INPUT, "factor file: unknown field");
+  std::rewind(fh);
+  f->load(fh);
   return f;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -61,6 +62,7 @@ struct FactorSettings {
   Precision precision = Precision::FP64;
   double drop_tol = -1.0;  // < 0: None
   int64_t cmod_cap = -1;   // product-buffer element cap, < 0 automatic
+  bool graphs = true;      // replay refactors as a captured HIP graph
 };
 
 // Device-resident plan of one analysed pattern.
@@ -86,6 +88,10 @@ struct DeviceSymbolic {
   ~DeviceSymbolic();
   void upload();
   void ensure_full_csr();
+  // FNV-1a digest of the plan's structure (dimension, pattern, permutation,
+  // partition, row structures, panel layout): two plans with equal digests
+  // lay out a factor identically
+  uint64_t digest() const;
 };
 
 // Per-launch event timing, attached to the launchers while enabled.
@@ -145,6 +151,15 @@ class DeviceFactorBase {
                              double* info, hipStream_t caller) = 0;
   // per-supernode (positive, negative, zero) pivot counts of the last factor
   virtual void supernode_inertia(std::vector<int32_t>& out) = 0;
+  // Krylov solvers with A (values in the plan's pattern, host) as operator
+  // and this factor (precond != 0) or the identity as preconditioner; b, x0,
+  // x host row-major n x nrhs; method 0 FGMRES, 1 COCG, 2 COCR; info per
+  // column {iterations, final relative residual, converged, stop reason}
+  virtual void krylov(int method, const double* a_values, const double* b, int64_t nrhs, double tol,
+                      int max_iter, int restart, int precond, const double* x0, double* x, double* info) = 0;
+  // factor file (a valid factor's device state, tied to the plan by a digest)
+  virtual void save(std::FILE* fh) = 0;
+  virtual void load(std::FILE* fh) = 0;
   FactorStats stats;
   FactorSettings settings;
   EventProfiler prof;
@@ -155,6 +170,9 @@ class DeviceFactorBase {
 
 std::unique_ptr<DeviceFactorBase> make_factor(std::shared_ptr<DeviceSymbolic> sym, int field,
                                               const double* values, const FactorSettings& fs);
+// A factor read from a file written by DeviceFactorBase::save for the same
+// plan (no numeric factorization runs).
+std::unique_ptr<DeviceFactorBase> load_factor(std::shared_ptr<DeviceSymbolic> sym, const char* path);
 
 // onepass_scale (src/numeric/sparse_solver.rs:399-418) on the host
 void onepass_scale(int32_t n, int field, const int64_t* colp, const int32_t* rowi,

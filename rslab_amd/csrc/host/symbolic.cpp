@@ -501,6 +501,70 @@ Symbolic compose_block_diagonal(const std::vector<Symbolic>& parts) {
   return S;
 }
 
+Symbolic from_partition(idx n, const off* colp, const idx* rowi, const std::vector<idx>& perm,
+                        const std::vector<idx>& sn_first, const std::vector<idx>& sn_ncol,
+                        const std::vector<idx>& sn_parent) {
+  Symbolic S;
+  S.n = n;
+  S.method = Ordering::Natural;  // the caller's ordering, applied through perm
+  if ((idx)perm.size() != n) throw std::runtime_error("perm must hold n entries");
+  S.perm = perm;
+  S.perm_inv.assign(n, -1);
+  for (idx k = 0; k < n; ++k) {
+    const idx o = perm[k];
+    if (o < 0 || o >= n || S.perm_inv[o] != -1) throw std::runtime_error("perm is not a permutation of 0..n-1");
+    S.perm_inv[o] = k;
+  }
+  const idx ns = (idx)sn_first.size();
+  if ((idx)sn_ncol.size() != ns || (idx)sn_parent.size() != ns)
+    throw std::runtime_error("supernode arrays must have nsuper entries");
+  if (n > 0 && ns == 0) throw std::runtime_error("no supernodes for a nonempty matrix");
+  idx next = 0;
+  for (idx s = 0; s < ns; ++s) {
+    if (sn_first[s] != next || sn_ncol[s] < 1)
+      throw std::runtime_error("supernode " + std::to_string(s) + " does not continue the column partition");
+    next += sn_ncol[s];
+    if (sn_parent[s] != -1 && (sn_parent[s] <= s || sn_parent[s] >= ns))
+      throw std::runtime_error("supernode " + std::to_string(s) + ": parent must be a later supernode or -1");
+  }
+  if (next != n) throw std::runtime_error("the supernode partition does not cover columns 0..n-1");
+  Pattern full = symmetric_pattern(n, colp, rowi);
+  S.permuted = permute_pattern(full, S.perm);
+  S.parent = etree_from_pattern(S.permuted);
+  S.col_counts = column_counts(S.permuted, S.parent);
+  for (idx c : S.col_counts) S.factor_nnz += c;
+  S.snodes.resize(ns);
+  for (idx s = 0; s < ns; ++s) {
+    Supernode& sn = S.snodes[s];
+    sn.first_col = sn_first[s];
+    sn.ncol = sn_ncol[s];
+    sn.nrow = std::max(S.col_counts[sn_first[s]], sn_ncol[s]);
+  }
+  for (idx s = 0; s < ns; ++s)
+    if (sn_parent[s] >= 0) S.snodes[sn_parent[s]].children.push_back(s);  // ascending s
+  // every off-diagonal row of a supernode must lie in one of its ancestors:
+  // the left-looking schedule then finishes every updater of a supernode
+  // (a descendant) before it, as the reference's recursion does
+  std::vector<std::vector<idx>> rs = supernode_row_structures(S);
+  std::vector<idx> col2s(n, 0);
+  for (idx s = 0; s < ns; ++s)
+    for (idx j = 0; j < sn_ncol[s]; ++j) col2s[sn_first[s] + j] = s;
+  for (idx s = 0; s < ns; ++s) {
+    idx last_t = -1;
+    for (size_t q = (size_t)sn_ncol[s]; q < rs[s].size(); ++q) {
+      const idx t = col2s[rs[s][q]];
+      if (t == last_t) continue;
+      last_t = t;
+      idx a = sn_parent[s];
+      while (a != -1 && a < t) a = sn_parent[a];
+      if (a != t)
+        throw std::runtime_error("supernode " + std::to_string(s) + " has rows in supernode " + std::to_string(t) +
+                                 ", which is not one of its ancestors");
+    }
+  }
+  return S;
+}
+
 std::vector<std::vector<idx>> supernode_row_structures(const Symbolic& S) {
   // multifrontal_ldlt.rs:2121-2152
   const idx ns = (idx)S.snodes.size();

@@ -105,6 +105,15 @@ Symbolic analyze(idx n, const off* colp, const idx* rowi, Ordering method,
 // every block keeps exactly its own analysis.
 Symbolic compose_block_diagonal(const std::vector<Symbolic>& parts);
 
+// Symbolic from a caller's own analysis: permutation (new -> old) and the
+// supernode partition of the permuted columns in postorder with its tree
+// (parent a later supernode, or -1). Throws std::runtime_error when the
+// permutation or the partition is malformed, or when a supernode's rows
+// reach a supernode that is not one of its ancestors.
+Symbolic from_partition(idx n, const off* colp, const idx* rowi, const std::vector<idx>& perm,
+                        const std::vector<idx>& sn_first, const std::vector<idx>& sn_ncol,
+                        const std::vector<idx>& sn_parent);
+
 // Filled row structure per supernode: own columns, then sorted trailing rows.
 std::vector<std::vector<idx>> supernode_row_structures(const Symbolic& s);
 // update_list[s]: descendants k whose off-diagonal rows hit s's columns,

@@ -215,8 +215,13 @@ class SolverSettings:
     # cmod product-buffer cap in elements (None: automatic); levels needing
     # more take the gathered-tile cmod, 0 forces it everywhere
     cmod_cap: Optional[int] = None
+    # same-pattern refactors replay the factor's launch sequence as a HIP graph
+    graphs: bool = True
 
     # builders (multifrontal_ldlt.rs:616-779)
+    def with_graphs(self, on: bool):
+        return dataclasses.replace(self, graphs=bool(on))
+
     def with_drop_tol(self, tau: float):
         """Incomplete-factor threshold dropping (with_drop_tol, multifrontal_ldlt.rs:640-643):
         stored L entries below tau times the largest in their column are dropped."""
@@ -273,6 +278,7 @@ class SolverSettings:
             raise InvalidInput(f"drop_tol must be a non-negative number, got {self.drop_tol!r}")
         s.drop_tol = -1.0 if self.drop_tol is None else float(self.drop_tol)
         s.cmod_cap = -1 if self.cmod_cap is None else int(self.cmod_cap)
+        s.graphs = 1 if self.graphs else 0
         return s
 
 
@@ -305,6 +311,21 @@ class LdltFactors:
     n_perturbed: int
     inertia: Inertia
 
+    def save(self, path):
+        """The arrays of LdltFactors (ldlt_generic.rs:31-66) in one .npz
+        (plain arrays, no pickled objects)."""
+        np.savez(path, n=np.int64(self.n), l_col_ptr=self.l_col_ptr, l_row_idx=self.l_row_idx,
+                 l_values=self.l_values, d_diag=self.d_diag, d_subdiag=self.d_subdiag,
+                 two_by_two=self.two_by_two, perm=self.perm, n_perturbed=np.int64(self.n_perturbed),
+                 inertia=np.array(self.inertia.as_tuple(), dtype=np.int64))
+
+    @staticmethod
+    def load(path) -> "LdltFactors":
+        with np.load(path, allow_pickle=False) as z:
+            return LdltFactors(int(z["n"]), z["l_col_ptr"], z["l_row_idx"], z["l_values"], z["d_diag"],
+                               z["d_subdiag"], z["two_by_two"], z["perm"], int(z["n_perturbed"]),
+                               Inertia(*map(int, z["inertia"])))
+
 
 @dataclasses.dataclass
 class MixedInfo:
@@ -329,6 +350,13 @@ class SymbolicExport:
     pp_colp: np.ndarray
     pp_rowi: np.ndarray
 
+    def sn_parent(self) -> np.ndarray:
+        """Supernode tree parent (-1 for roots) from the children lists."""
+        par = np.full(len(self.sn_first), -1, dtype=np.int32)
+        for s in range(len(self.sn_first)):
+            par[self.ch_idx[self.ch_ptr[s]:self.ch_ptr[s + 1]]] = s
+        return par
+
 
 # ---------------------------------------------------------------- symbolic
 class LdltSymbolic:
@@ -350,6 +378,28 @@ class LdltSymbolic:
         s = opts.to_c()
         _check(lib.rsl_analyze(a.n, ptr(a.col_ptr, C.c_int64), ptr(a.row_idx, C.c_int64),
                                C.byref(s), C.byref(h)))
+        return LdltSymbolic(h, a.n, a.nnz, opts)
+
+    @staticmethod
+    def from_symbolic(a: CscMatrix, perm, sn_first, sn_ncol, sn_parent,
+                      opts: Optional[SolverSettings] = None) -> "LdltSymbolic":
+        """LdltSymbolic from the caller's own analysis (what factor_numeric
+        takes, multifrontal_ldlt.rs:1851-1855): permutation (new -> old), the
+        supernode partition of the permuted columns in postorder and its tree
+        (parent a later supernode, or -1). Raises InvalidInput when they do not
+        form a valid supernodal elimination structure for a's pattern."""
+        opts = opts or SolverSettings()
+        ar = lambda v: np.ascontiguousarray(v, dtype=np.int32)  # noqa: E731
+        perm, sn_first, sn_ncol, sn_parent = ar(perm), ar(sn_first), ar(sn_ncol), ar(sn_parent)
+        if not (sn_first.size == sn_ncol.size == sn_parent.size):
+            raise InvalidInput("supernode arrays must have equal length")
+        if perm.size != a.n:
+            raise InvalidInput(f"perm must hold {a.n} entries, got {perm.size}")
+        h = C.c_void_p()
+        s = opts.to_c()
+        _check(lib.rsl_plan_create(a.n, ptr(a.col_ptr, C.c_int64), ptr(a.row_idx, C.c_int64),
+                                   ptr(perm, C.c_int32), int(sn_first.size), ptr(sn_first, C.c_int32),
+                                   ptr(sn_ncol, C.c_int32), ptr(sn_parent, C.c_int32), C.byref(s), C.byref(h)))
         return LdltSymbolic(h, a.n, a.nnz, opts)
 
     @staticmethod
@@ -418,6 +468,23 @@ class LdltSymbolic:
         _check(lib.rsl_factor(self._h, field, ptr(vals, C.c_double), C.byref(s), C.byref(h)))
         return LdltSolver(h, self, a.n, a.is_complex)
 
+    def load_factor(self, path) -> "LdltSolver":
+        """A factor written by LdltSolver.save for a symbolic of the same plan
+        (same pattern and analysis), without a numeric factorization."""
+        import os
+        p = os.fsencode(path)
+        try:
+            with open(p, "rb") as fh:
+                head = fh.read(20)
+        except OSError as e:
+            raise InvalidInput(f"factor file: {e}") from None
+        if len(head) < 20 or head[:8] != b"RSLFACT1":
+            raise InvalidInput("factor file: not a factor file")
+        cplx = int.from_bytes(head[16:20], "little") == 1
+        h = C.c_void_p()
+        _check(lib.rsl_factor_load(self._h, p, C.byref(h)))
+        return LdltSolver(h, self, self._n, cplx)
+
 
 # ---------------------------------------------------------------- solver
 class LdltSolver:
@@ -442,6 +509,11 @@ class LdltSolver:
 
     def n(self) -> int:
         return self._n
+
+    def save(self, path):
+        """Write this factor (everything a solve reads) for LdltSymbolic.load_factor."""
+        import os
+        _check(lib.rsl_factor_save(self._h, os.fsencode(path)))
 
     def refactor(self, a: CscMatrix):
         """Same-pattern numeric refactorization (plan + device storage reused)."""

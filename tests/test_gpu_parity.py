@@ -363,6 +363,66 @@ def test_config1_matches_oracle():
     xo = fo.solve_many(b * s) * s
     bg, bo = O.backward_error(a, x, b)[0], O.backward_error(a, xo, b)[0]
     assert bg <= 10 * bo and bo <= 10 * bg
+    # Sylvester: only the rounding-level pivots' signs are open, so the closed
+    # form (8163, 1837, 0) lies inside [neg, neg + ntiny] for the device factor
+    from test_oracle import rounding_level_bracket
+    neg, ntiny = rounding_level_bracket(g)
+    assert ntiny > 0 and neg <= 1837 <= neg + ntiny, (neg, ntiny, g.inertia)
+    # a clean gap between the rounding-level pivots (the device's summation
+    # order leaves them a few ulps of 1 rather than the oracle's exact 2^-54)
+    # and the O(1e-2) ones
+    d = np.abs(g.d_diag[~g.two_by_two])
+    assert np.all((d <= 1e-12) | (d >= 1e-3)), np.sort(d[(d > 1e-12) & (d < 1e-3)])[:8]
+    # the ladder cannot repair such a factor, on the device as in the oracle:
+    # uncertified after the full FGMRES budget, the flag consistent with the
+    # backward error; forward error against x_true reported
+    xr, info = R.MixedLdltSolver(fg, a).solve_to(a, b, 1e-14)
+    _, ir_o, gm_o, be_o, cert_o = O.refine_ladder(a, fo, s, b, 1e-14)
+    assert not info.certified and not cert_o
+    assert info.gmres_iters == gm_o == 120
+    assert info.backward_error == pytest.approx(O.backward_error(a, xr, b)[0], rel=1e-9)
+    print(f"config 1: device inertia {g.inertia.as_tuple()} (oracle {fo.inertia}, closed form "
+          f"(8163, 1837, 0), {ntiny} rounding-level pivots); ladder berr {info.backward_error:.2e} "
+          f"(oracle {be_o:.2e}), forward error {np.abs(xr - x_true).max() / np.abs(x_true).max():.2e}")
+
+
+@pytest.mark.parametrize("prec", [R.Precision.FP64, R.Precision.FP32_SCHUR, R.Precision.BF16_SCHUR])
+def test_ladder_matches_oracle_ladder_config3_shape(prec):
+    """The device's refinement ladder against the oracle's restatement
+    (refine_ladder, mixed.rs:140-221; FGMRES, iterative.rs:822-1021) on the
+    oracle's factor with the same reduced-precision Schur products (operands
+    rounded to FP32 / BF16, FP32 sums): same inertia, the same IR and FGMRES
+    step counts and certificate per column, backward errors both at the
+    target, solutions equal to what the conditioning allows, and a forward
+    error against the FP64 oracle solution."""
+    a, B = matgen.config3(m=24, nrhs=4)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND).with_precision(prec)
+    sym = R.LdltSymbolic.analyze_with(a, st)
+    fg = sym.factor(a, st)
+    e = sym.export()
+    s = O.onepass_scale(a)
+    fo = O.factor(e, O.scaled(a, s), prec=int(prec))
+    f64 = O.factor(e, O.scaled(a, s))
+    if prec == R.Precision.BF16_SCHUR:
+        # BF16 products (8-bit significands) perturb the factored matrix by
+        # ~1e-3 relative: eigenvalues inside the +-0.01 band of the shift can
+        # change sign (the emulation's own inertia is one off the FP64 one),
+        # and the device's FP32 sums run in another order than the oracle's
+        assert abs(fg.inertia().negative - fo.inertia[1]) <= 2
+        assert abs(fg.inertia().negative - f64.inertia[1]) <= 2
+    else:
+        assert fg.inertia().as_tuple() == fo.inertia == f64.inertia
+    X, infos = R.MixedLdltSolver(fg, a).solve_to(a, B, 1e-14)
+    Xref = f64.solve_many(B * s[:, None]) * s[:, None]
+    for c, info in enumerate(infos):
+        xo, ir, gm, be, cert = O.refine_ladder(a, fo, s, B[:, c], 1e-14)
+        assert info.certified == cert, (c, info, ir, gm, be)
+        gm_slack = 2 if prec != R.Precision.BF16_SCHUR else max(4, gm // 4)
+        assert abs(info.ir_iters - ir) <= 1 and abs(info.gmres_iters - gm) <= gm_slack, (c, info, ir, gm)
+        assert info.backward_error <= 1e-14 and be <= 1e-14
+        assert np.abs(X[:, c] - xo).max() <= 1e-9 * np.abs(xo).max()
+        fwd = np.abs(X[:, c] - Xref[:, c]).max() / np.abs(Xref[:, c]).max()
+        assert fwd <= 1e-9, fwd
 
 
 def test_config3_shape_nd_matches_oracle():

@@ -105,6 +105,60 @@ def test_gathered_cmod_matches_oracle(cap):
     check_solution(a, fg, fo, s, B)
 
 
+def test_factor_from_caller_symbolic_equals_analysis_factor():
+    """rsl_plan_create with the permutation, partition and tree of an
+    analysis (as the reference's factor_numeric takes its symbolic) drives
+    the device factor to the same bits, and to the oracle's factor."""
+    a, B = matgen.config3(m=20, nrhs=3)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
+    sym = R.LdltSymbolic.analyze_with(a, st)
+    e = sym.export()
+    sym2 = R.LdltSymbolic.from_symbolic(a, e.perm, e.sn_first, e.sn_ncol, e.sn_parent(), st)
+    f1, f2 = sym.factor(a), sym2.factor(a)
+    g1, g2 = f1.factors(), f2.factors()
+    for k in ("l_col_ptr", "l_row_idx", "l_values", "d_diag", "d_subdiag", "two_by_two", "perm"):
+        assert np.array_equal(getattr(g1, k), getattr(g2, k)), k
+    assert np.array_equal(f1.solve_many(B), f2.solve_many(B))
+    s = O.onepass_scale(a)
+    assert_same_factor(g2, O.factor(sym2.export(), O.scaled(a, s)))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_factor_file_round_trip(tmp_path, cplx):
+    """Factor once, solve elsewhere: a factor written to a file and read into
+    a handle of a second, independent analysis of the same pattern solves to
+    the original's bits without a numeric factorization; the host LdltFactors
+    arrays round-trip through .npz; a file for another plan is rejected."""
+    if cplx:
+        a, B = matgen.config2(m=12, nrhs=2)
+    else:
+        a, B = matgen.config3(m=14, nrhs=2)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
+    f = R.LdltSymbolic.analyze_with(a, st).factor(a)
+    X = f.solve_many(B)
+    path = tmp_path / "factor.rslf"
+    f.save(path)
+    sym2 = R.LdltSymbolic.analyze_with(a, st)   # as another process would
+    g = sym2.load_factor(path)
+    assert g.inertia() == f.inertia() and g.n_perturbed() == f.n_perturbed()
+    assert np.array_equal(g.solve_many(B), X)
+    fa = f.factors()
+    fa.save(tmp_path / "factors.npz")
+    fb = R.LdltFactors.load(tmp_path / "factors.npz")
+    for k in ("l_col_ptr", "l_row_idx", "l_values", "d_diag", "d_subdiag", "two_by_two", "perm"):
+        assert np.array_equal(getattr(fa, k), getattr(fb, k)), k
+    assert fb.inertia == fa.inertia
+    other = R.LdltSymbolic.analyze_with(a, st.with_ordering(R.OrderingMethod.Amd))
+    with pytest.raises(R.InvalidInput, match="another symbolic"):
+        other.load_factor(path)
+    # the loaded handle refactors and refines like the original
+    if not cplx:
+        a2 = a.with_values(a.values * 0.9 + 0.01)
+        g.refactor(a2)
+        f.refactor(a2)
+        assert np.array_equal(g.solve_many(B), f.solve_many(B))
+
+
 def test_rank_deficiency_names_supernode_and_column():
     a = R.CscMatrix.from_triplets(3, [0, 1, 2], [0, 1, 2], [2.0, 0.0, 3.0])
     with pytest.raises(R.NumericallyRankDeficient) as ei:

@@ -291,3 +291,170 @@ def test_backward_error_known_answer():
     be = O.backward_error(a, np.array([1.0, 1.0]), np.array([3.0, -1.0]))
     assert be[0] == pytest.approx(1.0 / 7.0, rel=1e-15)
     assert O.sym_norm_inf(a) == 4.0
+
+
+# ------------------------------------------------------------ ladder, FGMRES, reduced-precision emulation
+def _krylov_min_residual(A, b, k):
+    """min over x in K_k(A, b) of ||b - A x||_2, by least squares on an
+    orthonormal basis of the Krylov space (x0 = 0, no preconditioner)."""
+    Q = np.zeros((b.size, k))
+    v = b / np.linalg.norm(b)
+    for j in range(k):
+        for i in range(j):  # twice-orthogonalised power basis
+            v = v - (Q[:, i] @ v) * Q[:, i]
+        for i in range(j):
+            v = v - (Q[:, i] @ v) * Q[:, i]
+        Q[:, j] = v / np.linalg.norm(v)
+        v = A @ Q[:, j]
+    y = np.linalg.lstsq(A @ Q, b, rcond=None)[0]
+    return np.linalg.norm(b - A @ Q @ y)
+
+
+@pytest.mark.parametrize("k", [1, 4, 9])
+def test_gmres_iterate_minimizes_residual_over_krylov_space(k):
+    """GMRES's defining property: after k steps (no restart, x0 = 0, identity
+    preconditioner) its iterate minimizes ||b - A x|| over K_k(A, b)."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((40, 40)) + 6 * np.eye(40)
+    b = rng.standard_normal(40)
+    x, its, _ = O.gmres(lambda v: A @ v, b, lambda v: v, 1e-300, max_iter=k, restart=30)
+    assert its == k
+    assert np.linalg.norm(b - A @ x) == pytest.approx(_krylov_min_residual(A, b, k), rel=1e-9)
+    # an exact preconditioner solves in one iteration; a restart keeps the iterate
+    Ai = np.linalg.inv(A)
+    x1, its1, conv = O.gmres(lambda v: A @ v, b, lambda v: Ai @ v, 1e-12)
+    assert conv and its1 == 1 and np.linalg.norm(b - A @ x1) <= 1e-12 * np.linalg.norm(b)
+    x2, its2, _ = O.gmres(lambda v: A @ v, b, lambda v: v, 1e-300, max_iter=12, restart=4)
+    assert its2 == 12 and np.linalg.norm(b - A @ x2) < np.linalg.norm(b)
+
+
+def test_gmres_complex_symmetric_system():
+    """The complex field: GMRES orthogonalises with the conjugated inner product
+    (dotc) even for a complex-symmetric operator."""
+    rng = np.random.default_rng(8)
+    M = rng.standard_normal((30, 30)) + 1j * rng.standard_normal((30, 30))
+    A = M + M.T + 10 * np.eye(30)
+    b = rng.standard_normal(30) + 1j * rng.standard_normal(30)
+    x, its, conv = O.gmres(lambda v: A @ v, b, lambda v: v, 1e-12, max_iter=60, restart=30)
+    assert conv and np.linalg.norm(b - A @ x) <= 1e-12 * np.linalg.norm(b) * 1.01
+    x5, _, _ = O.gmres(lambda v: A @ v, b, lambda v: v, 1e-300, max_iter=5, restart=30)
+    assert np.linalg.norm(b - A @ x5) == pytest.approx(_krylov_min_residual_c(A, b, 5), rel=1e-9)
+
+
+def _krylov_min_residual_c(A, b, k):
+    K = np.zeros((b.size, k), dtype=complex)
+    v = b.copy()
+    for j in range(k):
+        K[:, j] = v
+        v = A @ v
+    Q, _ = np.linalg.qr(K)
+    y = np.linalg.lstsq(A @ Q, b, rcond=None)[0]
+    return np.linalg.norm(b - A @ Q @ y)
+
+
+def _oracle_factor(a, prec=0, pivot=0, abs_floor=0.0, drop_tol=None, **kw):
+    e = analyze(a, **kw)
+    s = O.onepass_scale(a)
+    return O.factor(e, O.scaled(a, s), pivot=pivot, abs_floor=abs_floor, drop_tol=drop_tol, prec=prec), s
+
+
+def test_ladder_certifies_laplacian_and_matches_direct_solve():
+    """mixed.rs:415-439 (lap2d(40)): certified, and equal to the FP64 direct
+    solve to 1e-10; the FP64 factor certifies without any GMRES step."""
+    a = matgen.grid_laplacian((40, 40))
+    b = np.array([(i % 13) - 6.0 for i in range(a.n)])
+    for prec in (O.PREC_FP64, O.PREC_FP32, O.PREC_BF16):
+        fo, s = _oracle_factor(a, prec=prec)
+        x, ir, gm, be, cert = O.refine_ladder(a, fo, s, b, O.CERT_FACTOR * np.finfo(float).eps)
+        assert cert and be <= 64 * np.finfo(float).eps, (prec, ir, gm, be)
+        f64, s64 = _oracle_factor(a)
+        xd = f64.solve_many(b * s64) * s64
+        assert np.abs(x - xd).max() <= 1e-10 * np.abs(xd).max()
+        if prec == O.PREC_FP64:
+            assert gm == 0 and ir <= 1
+
+
+def test_ladder_reports_uncertified_for_crippled_factor():
+    """mixed.rs:504-523: lap2d(90) with PerturbToEps(1e-12) and drop
+    tolerance 0.9: IR stalls, FGMRES spends its 120 iterations, not certified,
+    and the flag agrees with the backward error."""
+    a = matgen.grid_laplacian((90, 90))
+    b = np.array([(i % 5) - 2.0 for i in range(a.n)])
+    fo, s = _oracle_factor(a, prec=O.PREC_FP32, pivot=1, abs_floor=1e-12, drop_tol=0.9)
+    x, ir, gm, be, cert = O.refine_ladder(a, fo, s, b, 64 * np.finfo(float).eps)
+    assert not cert and gm == 120
+    assert cert == (be <= 64 * np.finfo(float).eps)
+    assert be == pytest.approx(O.backward_error(a, x, b)[0], rel=1e-12)
+
+
+def test_lowp_rounding_known_answers():
+    """FP32 and BF16 operand rounding (round to nearest, ties to even)."""
+    assert O.round_lowp(1.0 + 2.0 ** -30, 1) == 1.0
+    assert O.round_lowp(0.1, 1) == float(np.float32(0.1))
+    assert O.round_lowp(1.0 + 2.0 ** -8, 2) == 1.0               # tie -> even (1.0)
+    assert O.round_lowp(1.0 + 2.0 ** -7 + 2.0 ** -8, 2) == 1.0 + 2.0 ** -6  # tie -> even
+    assert O.round_lowp(1.0 + 3 * 2.0 ** -9, 2) == 1.0 + 2.0 ** -7
+    assert O.round_lowp(-3.0, 2) == -3.0
+    assert O.round_lowp(1.0 + 2.0 ** -30, 0) == float(np.float32(1.0 + 2.0 ** -30))
+
+
+def test_lowp_emulation_reduces_only_the_schur_products():
+    """One dense front narrower than a panel has no Schur products: the
+    reduced-precision factor equals the FP64 one bit for bit. On a sparse
+    multi-level tree the products are reduced: with FP32 the pivot sequence
+    is the FP64 one and D moves by a few FP32 roundoffs, with BF16 (8-bit
+    significands) some Bunch-Kaufman choices change; the inertia holds."""
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((48, 48))
+    a1 = dense_lower_csc(M + M.T)
+    f_hi, _ = _oracle_factor(a1, ordering=R.OrderingMethod.Natural)
+    for prec in (O.PREC_FP32, O.PREC_BF16):
+        f_lo, _ = _oracle_factor(a1, prec=prec, ordering=R.OrderingMethod.Natural)
+        assert np.array_equal(f_lo.l_values, f_hi.l_values) and np.array_equal(f_lo.d_diag, f_hi.d_diag)
+    a, _ = matgen.config3(m=12, nrhs=1)
+    f_hi, _ = _oracle_factor(a, ordering=R.OrderingMethod.MetisND)
+    f32, _ = _oracle_factor(a, prec=O.PREC_FP32, ordering=R.OrderingMethod.MetisND)
+    assert f32.inertia == f_hi.inertia and np.array_equal(f32.perm, f_hi.perm)
+    rel = np.abs(f32.d_diag - f_hi.d_diag).max() / np.abs(f_hi.d_diag).max()
+    assert 1e-9 < rel < 1e-4, rel
+    f16, _ = _oracle_factor(a, prec=O.PREC_BF16, ordering=R.OrderingMethod.MetisND)
+    assert f16.inertia == f_hi.inertia
+    assert not np.array_equal(f16.d_diag, f_hi.d_diag)
+
+
+def rounding_level_bracket(f, tiny=1e-10):
+    """(negatives among the pivots above `tiny`, number of 1x1 pivots at or
+    below it): Sylvester's law pins the closed-form negative count inside
+    [neg, neg + ntiny] when only the rounding-level pivots' signs are open."""
+    two = np.asarray(f.two_by_two, dtype=bool)
+    first = np.nonzero(two)[0]
+    one = np.ones(f.n, dtype=bool)
+    one[first] = False
+    one[first + 1] = False
+    d = np.real(f.d_diag)
+    dd = d[first] * d[first + 1] - np.real(f.d_subdiag[first]) ** 2
+    tr = d[first] + d[first + 1]
+    neg2 = int((dd < 0).sum() + 2 * ((dd > 0) & (tr < 0)).sum() + ((dd == 0) & (tr < 0)).sum())
+    big = one & (np.abs(d) > tiny)
+    return int((d[big] < 0).sum()) + neg2, int((one & (np.abs(d) <= tiny)).sum())
+
+
+def test_config1_inertia_is_fixed_up_to_rounding_level_pivots():
+    """Config 1 (A = L - 2I on the 100x100 grid, AMD): after equilibration a
+    node whose four neighbours were eliminated as 1x1 pivots is left with
+    1 - 4/4 = 0 up to rounding, and the panel-bounded Bunch-Kaufman (no
+    delayed pivots in the left-looking schedule) takes it as a 1x1 pivot of
+    size 2^-54. Every other pivot is O(1e-2) or larger, and the closed-form
+    inertia (8163, 1837, 0) is reachable by the rounding-level pivots' signs
+    alone. Refinement cannot repair such a factor: the ladder's FGMRES spends
+    its 120 iterations and reports an uncertified backward error of ~0.2."""
+    a, b, xt = matgen.config1()
+    fo, s = _oracle_factor(a)
+    neg, ntiny = rounding_level_bracket(fo)
+    pos_cf, neg_cf, _, _ = O.laplacian2d_inertia(100, 2.0)
+    assert (pos_cf, neg_cf) == (8163, 1837)
+    assert ntiny > 0 and neg <= neg_cf <= neg + ntiny
+    d = np.abs(fo.d_diag[~fo.two_by_two])
+    assert np.all((d <= 1e-15) | (d >= 1e-3))  # a clean gap: tiny pivots are 2^-54
+    x, ir, gm, be, cert = O.refine_ladder(a, fo, s, b, 1e-14)
+    assert not cert and gm == 120 and be > 1e-3

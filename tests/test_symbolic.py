@@ -246,3 +246,66 @@ def test_batch_symbolic_is_per_system_symbolic():
         off += m.n
         soff += ns
     assert off == eb.n and soff == len(eb.sn_first)
+
+
+def _export_arrays(sym):
+    e = sym.export()
+    return e, e.sn_parent()
+
+
+@pytest.mark.parametrize("ordering", [R.OrderingMethod.Amd, R.OrderingMethod.MetisND])
+def test_plan_from_caller_symbolic_round_trips(ordering):
+    """rsl_plan_create on the permutation, supernode partition and tree that
+    an analysis exported gives back the same symbolic: permutation, partition,
+    panel heights, children, permuted pattern and work figures."""
+    a, _ = matgen.config3(m=12, nrhs=1)
+    st = R.SolverSettings().with_ordering(ordering)
+    sym = R.LdltSymbolic.analyze_with(a, st)
+    e, par = _export_arrays(sym)
+    sym2 = R.LdltSymbolic.from_symbolic(a, e.perm, e.sn_first, e.sn_ncol, par, st)
+    e2 = sym2.export()
+    for k in ("perm", "perm_inv", "sn_first", "sn_ncol", "sn_nrow", "ch_ptr", "ch_idx", "pp_colp", "pp_rowi"):
+        assert np.array_equal(getattr(e, k), getattr(e2, k)), k
+    i1, i2 = sym.info(), sym2.info()
+    for k in ("nsuper", "panel_entries", "levels", "nnzl_padded", "factor_flops", "colcount_nnz"):
+        assert i1[k] == i2[k], k
+
+
+def test_plan_from_caller_symbolic_without_amalgamation():
+    """A different valid partition (every column its own supernode, the
+    column elimination tree as the supernode tree) is accepted and gives the
+    scalar structure: panel heights equal the column counts."""
+    a = matgen.grid_laplacian((9, 9), shift=-2.0)
+    sym = R.LdltSymbolic.analyze(a)
+    e = sym.export()
+    n = a.n
+    # column etree of the permuted pattern
+    parent = O.brute_force_etree(n, e.pp_colp, e.pp_rowi)
+    sym2 = R.LdltSymbolic.from_symbolic(a, e.perm, np.arange(n), np.ones(n), parent)
+    cc = O.brute_force_col_counts(n, e.pp_colp, e.pp_rowi)
+    assert np.array_equal(sym2.export().sn_nrow, cc)
+
+
+@pytest.mark.parametrize("case", ["perm_dup", "perm_len", "gap", "parent_earlier", "not_ancestor"])
+def test_plan_from_caller_symbolic_rejects_invalid(case):
+    a, _ = matgen.config3(m=8, nrhs=1)
+    sym = R.LdltSymbolic.analyze_with(a, R.SolverSettings().with_ordering(R.OrderingMethod.MetisND))
+    e, par = _export_arrays(sym)
+    perm, first, ncol = e.perm.copy(), e.sn_first.copy(), e.sn_ncol.copy()
+    if case == "perm_dup":
+        perm[1] = perm[0]
+    elif case == "perm_len":
+        perm = perm[:-1]
+    elif case == "gap":
+        first[1] += 1
+    elif case == "parent_earlier":
+        par = par.copy()
+        par[-1] = 0
+    else:
+        # re-hang a leaf supernode with off-diagonal rows under another root
+        # branch: its rows then reach supernodes outside its new ancestors
+        par = par.copy()
+        leaf = next(s for s in range(len(first)) if e.sn_nrow[s] > e.sn_ncol[s] and par[s] != len(first) - 1)
+        par[leaf] = len(first) - 1
+    with pytest.raises(R.InvalidInput):
+        R.LdltSymbolic.from_symbolic(a, perm, first, ncol, par)
