@@ -169,6 +169,21 @@ int rsl_solve(rsl_solver *f, const double *b, int64_t nrhs, double *x);
 int rsl_solve_device(rsl_solver *f, const void *d_b, int64_t nrhs, void *d_x,
                      void *stream);
 
+/* rsl_solve_device without any allocation. No counterpart in rslab: for
+ * callers that cannot allocate device memory during the call (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, result and stream ordering as rsl_solve_device, but the solve's
+ * scratch (y and z: n x nrhs each, u: one row per off-diagonal panel row x
+ * nrhs, each part 256-byte aligned) lives in one fixed device array of
+ * RSL_SOLVE_NOALLOC_SCRATCH_BYTES per GPU, held from the library's first use
+ * on that GPU until the process ends. A solve needing more returns
+ * RSL_INVALID_INPUT with the sizes. Every call on one GPU shares that array:
+ * calls that overlap in time on one GPU (two handles, two streams) race;
+ * calls on one handle are ordered on its stream and do not. */
+#define RSL_SOLVE_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int rsl_solve_device_noalloc(rsl_solver *f, const void *d_b, int64_t nrhs, void *d_x,
+                             void *stream);
+
 /* inertia[3] = (positive, negative, zero) (LdltSolver::inertia,
  * src/numeric/sparse_solver.rs:89-91); n_perturbed (:76-78); t_ms = last
  * numeric factor wall time. Any pointer may be NULL. */

@@ -60,6 +60,7 @@ SIGNATURES = {
     "rsl_refactor": (C.c_int, [_P, _DP]),
     "rsl_solve": (C.c_int, [_P, _DP, C.c_int64, _DP]),
     "rsl_solve_device": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "rsl_solve_device_noalloc": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "rsl_factor_info": (C.c_int, [_P, _I64P, _I64P, _DP]),
     "rsl_factor_nnz": (C.c_int, [_P, _I64P]),
     "rsl_factor_download": (C.c_int, [_P, _I64P, _I32P, _DP, _DP, _DP, _U8P, _I32P]),

@@ -365,6 +365,14 @@ int rsl_solve_device(rsl_solver* f, const void* d_b, int64_t nrhs, void* d_x, vo
   });
 }
 
+int rsl_solve_device_noalloc(rsl_solver* f, const void* d_b, int64_t nrhs, void* d_x, void* stream) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs <= 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "nrhs must be positive");
+    f->f->solve_noalloc(d_b, nrhs, d_x, static_cast<hipStream_t>(stream));
+  });
+}
+
 int rsl_factor_info(const rsl_solver* f, int64_t inertia[3], int64_t* n_perturbed, double* t_ms) {
   return guard([&] {
     if (inertia)

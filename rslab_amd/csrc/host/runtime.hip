@@ -263,6 +263,37 @@ void DeviceSymbolic::ensure_full_csr() {
   full_ready = true;
 }
 
+// ---- fixed scratch for the no-allocation solve (rsl_solve_device_noalloc) ----
+// One array per GPU, part of the library's code object: the loader places it
+// in device memory when the library's kernels are first loaded on a GPU and
+// it stays until the process ends. Host code never uses its name as an
+// address: the device address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+constexpr size_t kNoallocScratchBytes = RSL_SOLVE_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up on the first
+// call for that device, then cached).
+void* noalloc_scratch_current_device() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw Error(DEVICE_ERROR, "device: device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw Error(DEVICE_ERROR, "device: hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 namespace {
 
 template <class T>
@@ -363,6 +394,7 @@ class DeviceFactor final : public DeviceFactorBase {
   hipEvent_t ev_caller = nullptr, ev_done = nullptr;
   bool ev_ready = false;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
+  void* noalloc_base = nullptr;  // device address of the fixed scratch on this factor's device
   DevFac<T> fv{};
   std::vector<double> scale;
   bool configured = false;
@@ -425,7 +457,10 @@ class DeviceFactor final : public DeviceFactorBase {
     fv.sn_inertia = sn_inert.as<int32_t>();
     fv.rs_e = rows_e.as<int32_t>();
     fv.status = status.as<int32_t>();
+    // set-up for the calls that must not allocate later: the ordering events
+    // and the fixed scratch's address on this (current) device
     make_events();
+    noalloc_base = noalloc_scratch_current_device();
   }
 
   int field() const override { return FieldOf<T>::id; }
@@ -680,6 +715,37 @@ class DeviceFactor final : public DeviceFactorBase {
     if (work_b.bytes < needu) work_b.alloc(needu);
     if (work_z.bytes < need) work_z.alloc(need);
     solve_core(d_b, nrhs, d_x, st, work_y.as<T>(), work_b.as<T>(), work_z.as<T>(), prof_on ? &prof : nullptr);
+  }
+
+  // Scratch bytes of a solve with nrhs right-hand sides, laid out as
+  // y (n x nrhs) | u (off-diagonal panel rows x nrhs) | z (n x nrhs), each
+  // part 256-byte aligned.
+  void solve_scratch_layout(int64_t nrhs, size_t& ny, size_t& nu) const {
+    const Plan& P = sym->plan;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    ny = al(sizeof(T) * (size_t)std::max<int64_t>(P.n * nrhs, 1));
+    nu = al(sizeof(T) * (size_t)std::max<int64_t>(P.u_total * nrhs, 1));
+  }
+
+  // rsl_solve_device without allocation: the scratch is the library's
+  // fixed device array (noalloc_scratch, set up when the factor was made),
+  // ordered like solve_async. No hipMalloc, no event creation, no profiling.
+  void solve_noalloc(const void* d_b, int64_t nrhs, void* d_x, hipStream_t caller) override {
+    if (!valid) throw Error(INVALID_INPUT, "the last factorization of this handle failed; refactor first");
+    size_t ny = 0, nu = 0;
+    solve_scratch_layout(nrhs, ny, nu);
+    const size_t need = 2 * ny + nu;
+    if (need > kNoallocScratchBytes)
+      throw Error(INVALID_INPUT, "rsl_solve_device_noalloc: this solve needs " + std::to_string(need) +
+                                     " bytes of scratch (n = " + std::to_string(sym->plan.n) + ", nrhs = " +
+                                     std::to_string(nrhs) + "); the fixed scratch holds " +
+                                     std::to_string(kNoallocScratchBytes) + " bytes");
+    if (!noalloc_base) throw Error(DEVICE_ERROR, "device: no-allocation scratch is not set up");
+    wait_for(caller);
+    unsigned char* base = static_cast<unsigned char*>(noalloc_base);
+    solve_core(d_b, nrhs, d_x, sym->stream, reinterpret_cast<T*>(base), reinterpret_cast<T*>(base + ny),
+               reinterpret_cast<T*>(base + ny + nu), nullptr);
+    signal_to(caller);
   }
 
   // the solve's launch sequence on `st` with caller-provided scratch:

@@ -135,6 +135,8 @@ class DeviceFactorBase {
   virtual void solve_device(const void* d_b, int64_t nrhs, void* d_x, hipStream_t st) = 0;
   // solve_device on the library stream, ordered after and before `caller`'s work
   virtual void solve_async(const void* d_b, int64_t nrhs, void* d_x, hipStream_t caller) = 0;
+  // solve_async with the library's fixed scratch array: allocates nothing
+  virtual void solve_noalloc(const void* d_b, int64_t nrhs, void* d_x, hipStream_t caller) = 0;
   virtual void download(std::vector<int64_t>& colp, std::vector<int32_t>& rowi,
                         std::vector<double>& lval, std::vector<double>& d,
                         std::vector<double>& ds, std::vector<uint8_t>& two,

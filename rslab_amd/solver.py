@@ -563,6 +563,15 @@ class LdltSolver:
         _check(lib.rsl_solve_device(self._h, C.c_void_p(d_b), int(nrhs), C.c_void_p(d_x),
                                     C.c_void_p(stream) if stream else None))
 
+    def solve_device_noalloc(self, d_b: int, nrhs: int, d_x: int, stream: int = 0):
+        """solve_device without allocating: the scratch is the library's fixed
+        device array (RSL_SOLVE_NOALLOC_SCRATCH_BYTES per GPU); a solve needing
+        more raises InvalidInput. Overlapping calls on one GPU race."""
+        if nrhs <= 0:
+            raise DimensionMismatch("nrhs must be positive")
+        _check(lib.rsl_solve_device_noalloc(self._h, C.c_void_p(d_b), int(nrhs), C.c_void_p(d_x),
+                                            C.c_void_p(stream) if stream else None))
+
     def profile(self, on: bool = True):
         """Enable (and clear) or disable per-kernel event timing."""
         _check(lib.rsl_profile_enable(self._h, 1 if on else 0))
