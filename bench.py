@@ -159,6 +159,59 @@ def rooflines(kern, prec, info, n, nrhs, solves):
     return out
 
 
+def sweep_line(R, matgen, torch, m=48, count=20, nrhs=32, target=1e-12):
+    """Config 5, the same-pattern refactor sweep at scale: 3 dof on m^3 nodes,
+    27-point vector stencil shifted by -omega^2 I for `count` values of omega^2
+    evenly spaced in [0.05, 0.5]; one analysis (nested dissection), then per
+    omega^2 a numeric refactor with FP32 Schur products (replayed as a
+    captured graph) and the refinement ladder to backward error <= 1e-12 on
+    32 right-hand sides (seed 7 + index). Every value array and right-hand
+    side is in HBM before the timed region."""
+    import numpy as np
+    oms = matgen.config5_omegas(count)
+    base = matgen.vector_stencil27(m=m, omega2=0.01)  # diagonal blocks carry + 0.0
+    cols = np.repeat(np.arange(base.n), np.diff(base.col_ptr))
+    on_diag = base.row_idx == cols
+    t0 = time.perf_counter()
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND).with_precision(R.Precision.FP32_SCHUR)
+    sym = R.LdltSymbolic.analyze_with(base, st)
+    t_an = time.perf_counter() - t0
+    info = sym.info()
+    # A(omega^2) exactly as the generator forms it: diagonal + (0.01 - omega^2)
+    vals = [base.values + np.where(on_diag, 0.01 - w2, 0.0) for w2 in oms]
+    dvals = [torch.from_numpy(v).to("cuda") for v in vals]
+    dbs = [torch.from_numpy(np.ascontiguousarray(matgen.config5_rhs(base.n, k, nrhs=nrhs))).to("cuda")
+           for k in range(count)]
+    dx = torch.empty_like(dbs[0])
+    f = sym.factor(base.with_values(vals[0]), st)
+    mixed = R.MixedLdltSolver(f, base.with_values(vals[0]))
+
+    def one(k):
+        f.refactor_device(dvals[k].data_ptr())
+        return mixed.solve_to_device(dvals[k].data_ptr(), dbs[k].data_ptr(), nrhs, dx.data_ptr(), target)
+
+    one(0)  # warm-up: graph capture, workspaces
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos, inert = [], []
+    for k in range(count):
+        infos.append(one(k))
+        inert.append(f.inertia().as_tuple())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del dvals, dbs
+    return dict(
+        workload=f"3 dof on {m}^3 nodes (n={base.n}), 27-point vector stencil - omega^2 I, {count} omega^2 in "
+                 f"[0.05, 0.5], nested dissection, FP32-Schur refactor + IR to berr <= {target:g}, {nrhs} RHS",
+        s_total=round(dt, 3), s_per_omega2=round(dt / count, 4), analyze_s=round(t_an, 2),
+        factor_flops=info["factor_flops"],
+        certified=all(i.certified for inf in infos for i in inf),
+        berr_max=max(i.backward_error for inf in infos for i in inf),
+        ir_iters_max=max(i.ir_iters for inf in infos for i in inf),
+        gmres_iters_max=max(i.gmres_iters for inf in infos for i in inf),
+        negatives_first_last=[inert[0][1], inert[-1][1]])
+
+
 def cpu_baseline(R, matgen, a, B, sym):
     """The oracle (C restatement of the reference's left-looking factor,
     solve and refinement ladder) on the host cores of this machine, on the
@@ -218,6 +271,8 @@ def main():
     ap.add_argument("--grid", type=int, default=60)
     ap.add_argument("--paths", default="fp64,fp32,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the config-5 refactor sweep line")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default plan knobs)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -239,6 +294,17 @@ def main():
     t_an = time.perf_counter() - t0
     info = sym.info()
     log(f"analyze {t_an:.2f}s  n={a.n} nsuper={info['nsuper']} F={info['factor_flops']:.3e}")
+    tuned = None
+    if not args.no_tune:
+        from rslab_amd.autotune import autotune
+        t0 = time.perf_counter()
+        tr = autotune(a, sym, st, nrhs=B.shape[1], reps=2)
+        st = tr.settings
+        sym = sym.replan(st)
+        tuned = dict(wide_flops=st.wide_flops, solve_wide=st.solve_wide, seconds=round(time.perf_counter() - t0, 2),
+                     factor_ms={str(k): round(v, 2) for k, v in tr.factor_ms.items()},
+                     solve_ms={str(k): round(v, 2) for k, v in tr.solve_ms.items()})
+        log(f"autotune: {tuned}")
 
     precs = {"fp64": R.Precision.FP64, "fp32": R.Precision.FP32_SCHUR, "bf16": R.Precision.BF16_SCHUR}
     paths, kerns, xs = {}, {}, {}
@@ -265,6 +331,11 @@ def main():
                   tflops=round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 3) if v["flops"] else None,
                   gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["bytes"] else None)
           for k, v in sorted(kerns[best].items(), key=lambda kv: -kv[1]["ms"])}
+    sweep = None
+    if not args.no_sweep:
+        log("config-5 refactor sweep")
+        sweep = sweep_line(R, matgen, torch)
+        log(f"sweep: {sweep}")
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on the host cores")
@@ -289,10 +360,11 @@ def main():
         "config": {"workload": f"3D 7-point L - 1.5I + diag(U[-0.01,0.01]), {m}^3 (n={a.n}), "
                                f"nested dissection, 8 RHS, refactor + refine to berr<=1e-14",
                    "path": best, "n": a.n, "nrhs": 8, "factor_flops": info["factor_flops"],
-                   "analyze_s": round(t_an, 2)},
+                   "analyze_s": round(t_an, 2), "tuned": tuned},
         "roofline": roof,
         "rooflines": roofs,
         "cpu_baseline": cpu,
+        "sweep": sweep,
         "paths": paths,
         "kernels": kt,
     }

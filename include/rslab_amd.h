@@ -49,6 +49,14 @@ extern "C" {
 #define RSL_PREC_FP32_SCHUR 1
 #define RSL_PREC_BF16_SCHUR 2
 
+/* Numeric schedule (FactorMethod, src/numeric/multifrontal_ldlt.rs:257-295).
+ * Both give the same factor: left-looking pulls each level's updates before
+ * its cdiv, right-looking pushes a level's updates into every ancestor right
+ * after its cdiv. FactorMethod::Multifrontal is not provided (rejected). */
+#define RSL_METHOD_LEFT_LOOKING 0
+#define RSL_METHOD_RIGHT_LOOKING 1
+#define RSL_METHOD_MULTIFRONTAL 2
+
 /* SolverSettings (src/numeric/multifrontal_ldlt.rs:302-394, defaults :583-614)
  * restricted to the knobs the device path honours. */
 typedef struct {
@@ -75,6 +83,11 @@ typedef struct {
   int32_t graphs;         /* 1 (default): a same-pattern refactor captures the
                              factor's launch sequence once as a HIP graph and
                              replays it; 0: every factor launches eagerly */
+  int32_t solve_wide;     /* supernodes wider than this solve their own block in
+                             multi-workgroup 64-column steps (tuning knob,
+                             default 32; narrower ones use one workgroup) */
+  int32_t method;         /* RSL_METHOD_*, default RSL_METHOD_LEFT_LOOKING
+                             (SolverSettings::with_method); a plan setting */
 } rsl_settings;
 
 typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */
@@ -105,6 +118,12 @@ int rsl_plan_create(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
                     const int32_t *perm, int32_t nsuper, const int32_t *sn_first,
                     const int32_t *sn_ncol, const int32_t *sn_parent,
                     const rsl_settings *s, rsl_symbolic **out);
+
+/* The same analysis (permutation, partition) with a device plan built for
+ * other plan settings (panel_nb, wide_flops, solve_wide, method; the analysis fields
+ * of s are unused) -- what the autotuner times candidates with, without
+ * repeating the ordering. No counterpart in rslab. */
+int rsl_symbolic_replan(const rsl_symbolic *sym, const rsl_settings *s, rsl_symbolic **out);
 
 /* Batch of independent systems (no counterpart in rslab, which factors one
  * matrix per call; for callers with many small systems of uneven size):
@@ -137,9 +156,12 @@ int rsl_symbolic_export(const rsl_symbolic *sym, int32_t *perm, int32_t *perm_in
                         int32_t *pp_rowi);
 
 /* LdltSymbolic::factor (src/numeric/sparse_solver.rs:677-711) -> factor_numeric
- * (src/numeric/multifrontal_ldlt.rs:1851-1893), left-looking branch. values are
- * the CSC values in the analysed pattern (interleaved for RSL_C64). s may be
- * NULL (defaults). The symbolic handle must outlive the factor. */
+ * (src/numeric/multifrontal_ldlt.rs:1851-1893), left-looking branch (or the
+ * right-looking one, :1950-1977, for a symbolic planned with
+ * RSL_METHOD_RIGHT_LOOKING). values are the CSC values in the analysed
+ * pattern (interleaved for RSL_C64). s may be NULL (defaults); its method
+ * must be the one the symbolic was planned for. The symbolic handle must
+ * outlive the factor. */
 int rsl_factor(rsl_symbolic *sym, int32_t field, const double *values,
                const rsl_settings *s, rsl_solver **out);
 

@@ -376,6 +376,90 @@ def gmres(op, b, pre, tol, max_iter=GMRES_MAX, restart=GMRES_RESTART, x0=None):
     return x, total, final <= tol
 
 
+def _stop(converged, iters, max_iter):
+    """stop_reason (iterative.rs:351-363)."""
+    return "converged" if converged else ("breakdown" if iters < max_iter else "max_iter")
+
+
+def cocg(op, b, pre, tol, max_iter):
+    """Preconditioned COCG (iterative.rs:402-489) for complex-symmetric A,
+    x0 = 0, unconjugated products r^T z and p^T A p, breakdown on a zero
+    denominator. Returns (x, iters, converged, final_res, stop)."""
+    b = np.asarray(b)
+    x = np.zeros_like(b)
+    r = b.copy()
+    bnorm = _nrm2(b)
+    if bnorm == 0.0:
+        return x, 0, True, 0.0, "converged"
+    z = pre(r)
+    p = z.copy()
+    rho = np.sum(r * z)
+    final, conv, it = _nrm2(r) / bnorm, False, 0
+    while it < max_iter:
+        q = op(p)
+        pq = np.sum(p * q)
+        if pq == 0:
+            break
+        alpha = rho / pq
+        x = x + alpha * p
+        r = r - alpha * q
+        it += 1
+        final = _nrm2(r) / bnorm
+        if final <= tol:
+            conv = True
+            break
+        z = pre(r)
+        rho_new = np.sum(r * z)
+        if rho == 0:
+            break
+        beta = rho_new / rho
+        p = z + beta * p
+        rho = rho_new
+    return x, it, conv, final, _stop(conv, it, max_iter)
+
+
+def cocr(op, b, pre, tol, max_iter):
+    """Preconditioned COCR (iterative.rs:495-583), x0 = 0: w = M^-1 A p,
+    alpha = gamma / ((A p)^T w), the four coupled updates, beta from
+    z^T A z. Returns (x, iters, converged, final_res, stop)."""
+    b = np.asarray(b)
+    x = np.zeros_like(b)
+    r = b.copy()
+    bnorm = _nrm2(b)
+    if bnorm == 0.0:
+        return x, 0, True, 0.0, "converged"
+    z = pre(r)
+    p = z.copy()
+    ap = op(p)
+    az = ap.copy()
+    gamma = np.sum(z * az)
+    final, conv, it = _nrm2(r) / bnorm, False, 0
+    while it < max_iter:
+        w = pre(ap)
+        aw = op(w)
+        denom = np.sum(ap * w)
+        if denom == 0:
+            break
+        alpha = gamma / denom
+        x = x + alpha * p
+        r = r - alpha * ap
+        z = z - alpha * w
+        az = az - alpha * aw
+        it += 1
+        final = _nrm2(r) / bnorm
+        if final <= tol:
+            conv = True
+            break
+        gamma_new = np.sum(z * az)
+        if gamma == 0:
+            break
+        beta = gamma_new / gamma
+        p = z + beta * p
+        ap = az + beta * ap
+        gamma = gamma_new
+    return x, it, conv, final, _stop(conv, it, max_iter)
+
+
 def refine_ladder(a, fo, s, b, target, anorm=None):
     """refine_ladder (mixed.rs:140-221) for one right-hand side: x0 = the
     factor's solve of b; plain IR with the FP64 residual against the original

@@ -39,6 +39,8 @@ class Settings(C.Structure):
         ("drop_tol", C.c_double),
         ("cmod_cap", C.c_int64),
         ("graphs", C.c_int32),
+        ("solve_wide", C.c_int32),
+        ("method", C.c_int32),
     ]
 
 
@@ -55,6 +57,7 @@ SIGNATURES = {
     "rsl_symbolic_info": (C.c_int, [_P, _I64P, _DP]),
     "rsl_plan_create": (C.c_int, [C.c_int64, _I64P, _I64P, _I32P, C.c_int32, _I32P, _I32P, _I32P,
                                   C.POINTER(Settings), C.POINTER(_P)]),
+    "rsl_symbolic_replan": (C.c_int, [_P, C.POINTER(Settings), C.POINTER(_P)]),
     "rsl_symbolic_export": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _I64P, _I32P]),
     "rsl_factor": (C.c_int, [_P, C.c_int32, _DP, C.POINTER(Settings), C.POINTER(_P)]),
     "rsl_refactor": (C.c_int, [_P, _DP]),

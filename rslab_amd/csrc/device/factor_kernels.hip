@@ -34,9 +34,9 @@ static_assert(STG >= KC * TM && STG >= tile_stage_elems<TP_F64>() && STG * 8 >= 
               "staging buffer too small");
 
 // ---------------------------------------------------------------------------
-// Tile core: acc[ii][jj] += sum_c  L[ar + i, cb + c] * G[br + j, c]
+// Tile products acc(i, j) += sum_c  L[ar + i, cb + c] * G[br + j, c]
 // with G = L[:, cb..cb+K) * D  (block diagonal 1x1 / 2x2, symmetric: no
-// conjugation). Thread t owns rows ty + 16*ii, columns tx + 16*jj.
+// conjugation); g_entry forms one entry of G.
 // ---------------------------------------------------------------------------
 template <class T>
 __device__ inline T g_entry(const T* __restrict__ L, int64_t ld, int64_t row, int64_t col,
@@ -54,21 +54,40 @@ __device__ inline T g_entry(const T* __restrict__ L, int64_t ld, int64_t row, in
   return L[row + col * ld] * d[c];
 }
 
-template <class T>
-__device__ void ldl_tile_gemm(const T* __restrict__ L, int64_t ld, int64_t cb, int K,
-                              int64_t ar, int ma, int64_t br, int mb,
-                              const T* __restrict__ d, const T* __restrict__ ds,
-                              const uint8_t* __restrict__ two, T (&acc)[4][4], T* As, T* Bs) {
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  constexpr int PER = KC * TM / NT;
-  T va[PER], vb[PER];
+// ---------------------------------------------------------------------------
+// Complex-symmetric tile on the FP64 matrix cores: the same product as
+// ldl_tile_gemm<cplx> (acc(i, j) = sum_c L[ar + i, cb + c] * G[br + j, c],
+// no conjugation) as four real products per complex one,
+//   re += Ar Br - Ai Bi,   im += Ar Bi + Ai Br,
+// on v_mfma_f64_16x16x4f64. Each K-chunk of the operands is staged as four
+// LDS planes (real and imaginary parts of A and B, k-major like the real
+// FP64 tile); the next chunk's global loads are in flight during the MFMAs.
+// Waves own 32x32 quadrants as in the real tile (tile_for_each's layout).
+// stage holds 4 * KC * LDA doubles.
+// ---------------------------------------------------------------------------
+struct TileAccC {
+  TileAcc<TP_F64> re, im;
+};
+
+__device__ void ldl_tile_mfma_c64(const cplx* __restrict__ L, int64_t ld, int64_t cb, int K, int64_t ar, int ma,
+                                  int64_t br, int mb, const cplx* __restrict__ d, const cplx* __restrict__ ds,
+                                  const uint8_t* __restrict__ two, TileAccC& acc, double* stage) {
+  typedef TileCfg<TP_F64> C;
+  constexpr int KCC = C::KC, LDA = C::LDA, PER = KCC * 64 / NT;
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  double* Ar = stage;
+  double* Ai = Ar + KCC * LDA;
+  double* Br = Ai + KCC * LDA;
+  double* Bi = Br + KCC * LDA;
+  cplx va[PER], vb[PER];
   auto fetch = [&](int c0) {
-    const int kc = min(KC, K - c0);
+    const int kc = min(KCC, K - c0);
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * NT, r = e % TM, c = e / TM;
-      va[u] = F<T>::zero();
-      vb[u] = F<T>::zero();
+      const int e = tid + NT * u, r = e & 63, c = e >> 6;
+      va[u] = F<cplx>::zero();
+      vb[u] = F<cplx>::zero();
       if (c < kc) {
         const int64_t col = cb + c0 + c;
         if (r < ma) va[u] = L[(ar + r) + col * ld];
@@ -77,30 +96,56 @@ __device__ void ldl_tile_gemm(const T* __restrict__ L, int64_t ld, int64_t cb, i
     }
   };
   fetch(0);
-  for (int c0 = 0; c0 < K; c0 += KC) {
-    const int kc = min(KC, K - c0);
+  for (int c0 = 0; c0 < K; c0 += KCC) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * NT, r = e % TM, c = e / TM;
-      As[c * TM + r] = va[u];
-      Bs[c * TN + r] = vb[u];
+      const int e = tid + NT * u, r = e & 63, c = e >> 6;
+      Ar[c * LDA + r] = va[u].re;
+      Ai[c * LDA + r] = va[u].im;
+      Br[c * LDA + r] = vb[u].re;
+      Bi[c * LDA + r] = vb[u].im;
     }
     __syncthreads();
-    if (c0 + KC < K) fetch(c0 + KC);
-    for (int c = 0; c < kc; ++c) {
-      T a[4], b[4];
+    if (c0 + KCC < K) fetch(c0 + KCC);
+#pragma unroll 2
+    for (int kk = 0; kk < KCC; kk += 4) {
+      const int k = kk + (l >> 4);
+      double a_r[2], a_i[2], a_n[2], b_r[2], b_i[2];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        a[q] = As[c * TM + ty + 16 * q];
-        b[q] = Bs[c * TN + tx + 16 * q];
+      for (int q = 0; q < 2; ++q) {
+        a_r[q] = Ar[k * LDA + wr + q * 16 + (l & 15)];
+        a_i[q] = Ai[k * LDA + wr + q * 16 + (l & 15)];
+        a_n[q] = -a_i[q];
+        b_r[q] = Br[k * LDA + wc + q * 16 + (l & 15)];
+        b_i[q] = Bi[k * LDA + wc + q * 16 + (l & 15)];
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + a[i] * b[j];
+        for (int bj = 0; bj < 2; ++bj) {
+          acc.re.v[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[bi], b_r[bj], acc.re.v[bi][bj], 0, 0, 0);
+          acc.re.v[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_n[bi], b_i[bj], acc.re.v[bi][bj], 0, 0, 0);
+          acc.im.v[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[bi], b_i[bj], acc.im.v[bi][bj], 0, 0, 0);
+          acc.im.v[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[bi], b_r[bj], acc.im.v[bi][bj], 0, 0, 0);
+        }
     }
     __syncthreads();
   }
+}
+
+// Visit every (i, j, value) of this thread's complex accumulator.
+template <class Fn>
+__device__ inline void tile_for_each_c64(const TileAccC& acc, Fn&& fn) {
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        fn(wr + tacc_row<TP_F64>(bi, l, r), wc + bj * 16 + (l & 15),
+           cplx{acc.re.v[bi][bj][r], acc.im.v[bi][bj][r]});
 }
 
 // ---------------------------------------------------------------------------
@@ -157,7 +202,7 @@ __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
   __shared__ T acc_s[TM * TN];
   __shared__ __align__(16) T stage[2 * STG];
   __shared__ int32_t trow[TM], rpos[TM], cpos[TN];
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int tid = threadIdx.x;
   const CItem it = items[blockIdx.x];
   const int s = it.s;
   const int64_t h = S.sn_nrow[s];
@@ -208,26 +253,18 @@ __global__ __launch_bounds__(NT) void k_cmod(DevSym S, DevFac<T> Fc,
         }
       });
     } else {
-      T acc[4][4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
-      ldl_tile_gemm(Pk, hk, 0, nck, nck + tk.a0, ma, nck + tk.b0, mb, Fc.d + eo, Fc.ds + eo,
-                    Fc.two + eo, acc, stage, stage + STG);
-#pragma unroll
-      for (int ii = 0; ii < 4; ++ii) {
-        int i = ty + 16 * ii;
-        if (i >= ma) continue;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          int j = tx + 16 * jj;
-          if (j < mb && tk.a0 + i >= tk.b0 + j) {
-            int e = rpos[i] + cpos[j] * TM;
-            acc_s[e] = acc_s[e] - acc[ii][jj];
-          }
+      // complex symmetric: four real FP64 matrix-core products per complex one
+      TileAccC acc;
+      tile_zero(acc.re);
+      tile_zero(acc.im);
+      ldl_tile_mfma_c64(Pk, hk, 0, nck, nck + tk.a0, ma, nck + tk.b0, mb, Fc.d + eo, Fc.ds + eo, Fc.two + eo, acc,
+                        reinterpret_cast<double*>(stage));
+      tile_for_each_c64(acc, [&](int i, int j, cplx v) {
+        if (i < ma && j < mb && tk.a0 + i >= tk.b0 + j) {
+          const int e = rpos[i] + cpos[j] * TM;
+          acc_s[e] = acc_s[e] - v;
         }
-      }
+      });
     }
     __syncthreads();
   }
@@ -1212,7 +1249,6 @@ __device__ void panel_store(const SnCtx<T>& c, int kb, int ke, T* Ab, const int3
 // Deferred Schur update of one 64x64 tile of P[ke.., ke..w).
 template <int PR, class T>
 __device__ void panel_schur_tile(const SnCtx<T>& c, int kb, int ke, int rr0, int cc0, T* stage) {
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int64_t mt = c.h - ke;
   const int cw = c.w - ke, pw = ke - kb;
   const int ma = (int)min<int64_t>(TM, mt - rr0), mb = min(TN, cw - cc0);
@@ -1229,26 +1265,18 @@ __device__ void panel_schur_tile(const SnCtx<T>& c, int kb, int ke, int rr0, int
       }
     });
   } else {
-    T acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = F<T>::zero();
-    ldl_tile_gemm(c.P, c.h, kb, pw, ke + rr0, ma, ke + cc0, mb, c.d + kb, c.ds + kb, c.two + kb,
-                  acc, stage, stage + STG);
-#pragma unroll
-    for (int ii = 0; ii < 4; ++ii) {
-      int i = ty + 16 * ii;
-      if (i >= ma) continue;
-#pragma unroll
-      for (int jj = 0; jj < 4; ++jj) {
-        int j = tx + 16 * jj;
-        if (j < mb && rr0 + i >= cc0 + j) {
-          T* dst = c.P + (ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h;
-          *dst = *dst - acc[ii][jj];
-        }
+    // complex symmetric: four real FP64 matrix-core products per complex one
+    TileAccC acc;
+    tile_zero(acc.re);
+    tile_zero(acc.im);
+    ldl_tile_mfma_c64(c.P, c.h, kb, pw, ke + rr0, ma, ke + cc0, mb, c.d + kb, c.ds + kb, c.two + kb, acc,
+                      reinterpret_cast<double*>(stage));
+    tile_for_each_c64(acc, [&](int i, int j, cplx v) {
+      if (i < ma && j < mb && rr0 + i >= cc0 + j) {
+        T* dst = c.P + (ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h;
+        *dst = *dst - v;
       }
-    }
+    });
   }
 }
 

@@ -78,6 +78,12 @@ rsl::PlanParams plan_params(const rsl_settings& st) {
   rsl::PlanParams pp;
   pp.panel_nb = st.panel_nb < 8 ? 8 : st.panel_nb;
   pp.wide_flops = st.wide_flops > 0 ? st.wide_flops : 2.5e5;
+  pp.solve_wide = st.solve_wide >= 0 ? st.solve_wide : 32;
+  if (st.method == RSL_METHOD_MULTIFRONTAL)
+    throw rsl::Error(RSL_INVALID_INPUT, "FactorMethod::Multifrontal is not provided; use left- or right-looking");
+  if (st.method != RSL_METHOD_LEFT_LOOKING && st.method != RSL_METHOD_RIGHT_LOOKING)
+    throw rsl::Error(RSL_INVALID_INPUT, "unknown factor method");
+  pp.schedule = st.method;
   return pp;
 }
 }  // namespace
@@ -101,6 +107,8 @@ void rsl_settings_default(rsl_settings* s) {
   s->drop_tol = -1.0;
   s->cmod_cap = -1;
   s->graphs = 1;
+  s->solve_wide = 32;
+  s->method = RSL_METHOD_LEFT_LOOKING;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }
@@ -212,6 +220,27 @@ int rsl_plan_create(int64_t n, const int64_t* col_ptr, const int64_t* row_idx, c
   });
 }
 
+int rsl_symbolic_replan(const rsl_symbolic* sym, const rsl_settings* s, rsl_symbolic** out) {
+  *out = nullptr;
+  return guard([&] {
+    rsl_settings st;
+    if (s) st = *s;
+    else rsl_settings_default(&st);
+    if (st.panel_nb > 128) throw rsl::Error(RSL_INVALID_INPUT, "panel_nb above 128 is not supported");
+    const rsl::DeviceSymbolic& src = *sym->ds;
+    auto ds = std::make_shared<rsl::DeviceSymbolic>();
+    ds->plan.sym = src.plan.sym;
+    ds->h_colp = src.h_colp;
+    ds->h_rowi = src.h_rowi;
+    ds->batch_off = src.batch_off;
+    ds->batch_snode_off = src.batch_snode_off;
+    rsl::build_plan(ds->plan, ds->h_colp.data(), ds->h_rowi.data(), plan_params(st));
+    auto* h = new rsl_symbolic;
+    h->ds = std::move(ds);
+    *out = h;
+  });
+}
+
 int rsl_analyze_batch(int32_t count, const int64_t* n, const int64_t* const* col_ptr,
                       const int64_t* const* row_idx, const rsl_settings* s, rsl_symbolic** out) {
   *out = nullptr;
@@ -317,6 +346,11 @@ int rsl_factor(rsl_symbolic* sym, int32_t field, const double* values, const rsl
     else rsl_settings_default(&st);
     if (field == RSL_C64 && st.precision != RSL_PREC_FP64)
       throw rsl::Error(RSL_INVALID_INPUT, "reduced-precision Schur updates are built for the real field only");
+    // the schedule is fixed by the plan: a settings struct asking for the
+    // other one is a caller mistake, not something to ignore
+    if (s && st.method != sym->ds->plan.params.schedule)
+      throw rsl::Error(RSL_INVALID_INPUT,
+                       "factor method differs from the one this symbolic was planned for; analyze or replan with it");
     auto f = rsl::make_factor(sym->ds, field, values, factor_settings(st));
     auto* h = new rsl_solver;
     h->f = std::move(f);

@@ -11,7 +11,6 @@ namespace rsl {
 
 namespace {
 constexpr int TILE = 64;
-constexpr int SOLVE_WIDE = 32;  // own blocks wider than this are solved in 64-column steps
 }
 
 void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanParams& prm) {
@@ -106,19 +105,42 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     int32_t tile, lo, hi;
   };
   std::vector<Run> cruns, rruns;
-  for (LevelPlan& L : P.levels) {
+  // The cmod work of a level: (target s, its updaters k ascending) groups.
+  // Left-looking: a level's targets with all their updaters (pulled before
+  // the level's cdiv). Right-looking: every target of the level's updaters
+  // with the updaters of that level (pushed after the level's cdiv).
+  const int32_t nlev = (int32_t)P.levels.size();
+  std::vector<std::vector<std::pair<int32_t, std::vector<int32_t>>>> groups(nlev);
+  if (prm.schedule == 0) {
+    for (int32_t l = 0; l < nlev; ++l)
+      for (int32_t s : P.levels[l].all) {
+        if (P.upd_ptr[s] == P.upd_ptr[s + 1]) continue;
+        groups[l].push_back({s, std::vector<int32_t>(P.upd_idx.begin() + P.upd_ptr[s],
+                                                      P.upd_idx.begin() + P.upd_ptr[s + 1])});
+      }
+  } else {
+    for (int32_t s = 0; s < ns; ++s)
+      for (int32_t u = P.upd_ptr[s]; u < P.upd_ptr[s + 1]; ++u) {
+        const int32_t k = P.upd_idx[u], l = P.level[k];
+        auto& g = groups[l];
+        if (g.empty() || g.back().first != s) g.push_back({s, {}});
+        g.back().second.push_back(k);  // ascending k within the level
+      }
+  }
+  for (int32_t l = 0; l < nlev; ++l) {
+    LevelPlan& L = P.levels[l];
     L.item_beg = (int64_t)P.items.size();
     L.ptile_beg = (int64_t)P.ptiles.size();
     int64_t coff = 0;  // product blocks of the level, one after the other
-    for (int32_t s : L.all) {
-      if (P.upd_ptr[s] == P.upd_ptr[s + 1]) continue;
+    for (const auto& grp : groups[l]) {
+      const int32_t s = grp.first;
       const int32_t h = P.sn_nrow[s], w = P.sn_ncol[s], first = P.sn_first[s];
       const int32_t* rss = P.rs_idx.data() + P.rs_off[s];
       for (int32_t i = 0; i < h; ++i) gloc[rss[i]] = i;
       const int32_t nrt = (h + TILE - 1) / TILE, nct = (w + TILE - 1) / TILE;
       tt.assign((size_t)nrt * nct, {});
-      for (int32_t u = P.upd_ptr[s]; u < P.upd_ptr[s + 1]; ++u) {
-        const int32_t k = P.upd_idx[u], nck = P.sn_ncol[k];
+      for (const int32_t k : grp.second) {
+        const int32_t nck = P.sn_ncol[k];
         const int32_t* ok = P.rs_idx.data() + P.rs_off[k] + nck;
         const int32_t nok = P.sn_nrow[k] - nck;
         const int32_t p0 = (int32_t)(std::lower_bound(ok, ok + nok, first) - ok);
@@ -275,7 +297,7 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
         any = P.gat_ptr[P.sn_first[s] + c + 1] > P.gat_ptr[P.sn_first[s] + c];
       if (any)
         for (int32_t r0 = 0; r0 < P.sn_ncol[s]; r0 += 64) P.gat_items.push_back({s, r0});
-      if (P.sn_ncol[s] > SOLVE_WIDE) L.swide.push_back(s);
+      if (P.sn_ncol[s] > prm.solve_wide) L.swide.push_back(s);
       else L.snarrow.push_back(s);
     }
     L.fwd_end = (int64_t)P.fwd_items.size();

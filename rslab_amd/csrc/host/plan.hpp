@@ -11,6 +11,9 @@ namespace rsl {
 struct PlanParams {
   int panel_nb = 64;            // SolverSettings::panel_nb default (gemm_tuning.rs:27)
   double wide_flops = 2.5e5;    // supernodes with h*w*w above this get multi-workgroup cdiv
+  int solve_wide = 32;          // own blocks wider than this are solved in 64-column steps
+  int schedule = 0;             // 0 left-looking (cmod pulled at the target's level),
+                                // 1 right-looking (pushed after the updater's level)
 };
 
 struct LevelPlan {

@@ -671,20 +671,27 @@ class DeviceFactor final : public DeviceFactorBase {
     const int nb = P.params.panel_nb < 8 ? 8 : P.params.panel_nb;
     const int prec = static_cast<int>(settings.precision);
     const int32_t* lists = sym->level_lists.as<int32_t>();
-    for (size_t L = 0; L < P.levels.size(); ++L) {
-      const LevelPlan& lv = P.levels[L];
-      bool done = false;
+    // left-looking: a level's cmod (its targets' updates, pulled) before its
+    // cdiv; right-looking: after its cdiv, its supernodes' updates of every
+    // ancestor (pushed), the same product blocks grouped by updater level
+    const bool rl = P.params.schedule == 1;
+    auto cmod_level = [&](const LevelPlan& lv) {
+      if (lv.item_end == lv.item_beg) return;
       if constexpr (std::is_same<T, double>::value) {
         if (lv.c_elems <= c_cap) {
           launch_cmod_two_phase(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
                                 sym->tasks.as<CTask>(), sym->ptiles.as<PTile>() + lv.ptile_beg,
-                                lv.ptile_end - lv.ptile_beg, cbuf.p, lv.c_elems, gpool.as<double>(), lv.cmod_flops, prec, st, pf);
-          done = true;
+                                lv.ptile_end - lv.ptile_beg, cbuf.p, lv.c_elems, gpool.as<double>(), lv.cmod_flops,
+                                prec, st, pf);
+          return;
         }
       }
-      if (!done)
-        launch_cmod<T>(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
-                       sym->tasks.as<CTask>(), lv.cmod_flops, prec, st, pf);
+      launch_cmod<T>(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
+                     sym->tasks.as<CTask>(), lv.cmod_flops, prec, st, pf);
+    };
+    for (size_t L = 0; L < P.levels.size(); ++L) {
+      const LevelPlan& lv = P.levels[L];
+      if (!rl) cmod_level(lv);
       launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
                            lv.small_flops, prec, st, pf);
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
@@ -697,6 +704,7 @@ class DeviceFactor final : public DeviceFactorBase {
           launch_make_g(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), maxw, gpool.as<double>(), st);
         }
       }
+      if (rl) cmod_level(lv);
     }
     launch_rows_e<T>(S, fv, P.rs_off[P.ns], st);
     // drop tolerance at emit (ldlt_emit_and_free, multifrontal_ldlt.rs:2358-2366):

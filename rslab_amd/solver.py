@@ -170,6 +170,17 @@ class OrderingMethod(enum.IntEnum):
     Natural = 2
 
 
+class FactorMethod(enum.IntEnum):
+    """FactorMethod (multifrontal_ldlt.rs:257-295): both schedules give the same
+    factor. LeftLooking pulls each level's updates before its cdiv;
+    RightLooking pushes a level's updates into its ancestors right after its
+    cdiv. Multifrontal is named for API parity and rejected by the device path."""
+
+    LeftLooking = 0
+    RightLooking = 1
+    Multifrontal = 2
+
+
 class Precision(enum.IntEnum):
     """Precision of the Schur / contribution GEMMs; pivots, D and residuals stay FP64."""
 
@@ -217,8 +228,15 @@ class SolverSettings:
     cmod_cap: Optional[int] = None
     # same-pattern refactors replay the factor's launch sequence as a HIP graph
     graphs: bool = True
+    # own blocks wider than this are solved in multi-workgroup steps (tuning knob)
+    solve_wide: int = 32
+    method: FactorMethod = FactorMethod.LeftLooking
 
     # builders (multifrontal_ldlt.rs:616-779)
+    def with_method(self, m):
+        """SolverSettings::with_method (multifrontal_ldlt.rs:257-295)."""
+        return dataclasses.replace(self, method=FactorMethod(m))
+
     def with_graphs(self, on: bool):
         return dataclasses.replace(self, graphs=bool(on))
 
@@ -279,6 +297,10 @@ class SolverSettings:
         s.drop_tol = -1.0 if self.drop_tol is None else float(self.drop_tol)
         s.cmod_cap = -1 if self.cmod_cap is None else int(self.cmod_cap)
         s.graphs = 1 if self.graphs else 0
+        s.solve_wide = int(self.solve_wide)
+        if FactorMethod(self.method) == FactorMethod.Multifrontal:
+            raise InvalidInput("FactorMethod.Multifrontal is not provided; use LeftLooking or RightLooking")
+        s.method = int(self.method)
         return s
 
 
@@ -401,6 +423,17 @@ class LdltSymbolic:
                                    ptr(perm, C.c_int32), int(sn_first.size), ptr(sn_first, C.c_int32),
                                    ptr(sn_ncol, C.c_int32), ptr(sn_parent, C.c_int32), C.byref(s), C.byref(h)))
         return LdltSymbolic(h, a.n, a.nnz, opts)
+
+    def replan(self, opts: SolverSettings) -> "LdltSymbolic":
+        """This analysis with a device plan built for `opts`' plan settings
+        (panel_nb, wide_flops, solve_wide); the ordering is not repeated."""
+        h = C.c_void_p()
+        s = opts.to_c()
+        _check(lib.rsl_symbolic_replan(self._h, C.byref(s), C.byref(h)))
+        out = LdltSymbolic(h, self._n, self._nnz, opts)
+        if hasattr(self, "batch"):
+            out.batch = self.batch
+        return out
 
     @staticmethod
     def analyze_batch(mats, opts: Optional[SolverSettings] = None) -> "LdltSymbolic":

@@ -91,6 +91,35 @@ def test_factor_rejects_pattern_mismatch_before_device_work():
         sym.factor(b)
 
 
+def test_factor_method_is_validated_and_fixed_by_the_plan():
+    """FactorMethod (multifrontal_ldlt.rs:257-295): left- and right-looking
+    plan on the host; Multifrontal and unknown values are InvalidInput; a
+    factor asking for the other schedule than its symbolic's is InvalidInput
+    before any device work."""
+    a = matgen.grid_laplacian((6, 6), shift=-0.3)
+    st = R.SolverSettings()
+    assert st.method == R.FactorMethod.LeftLooking
+    with pytest.raises(R.InvalidInput):
+        R.LdltSymbolic.analyze_with(a, st.with_method(R.FactorMethod.Multifrontal))
+    s = st.to_c()
+    s.method = 7
+    h = C.c_void_p()
+    cp = np.ascontiguousarray(a.col_ptr, dtype=np.int64)
+    ri = np.ascontiguousarray(a.row_idx, dtype=np.int64)
+    assert _lib.lib.rsl_analyze(a.n, cp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                ri.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(s), C.byref(h)) == 2
+    assert not h.value
+    rl = st.with_method(R.FactorMethod.RightLooking)
+    sym = R.LdltSymbolic.analyze_with(a, rl)
+    with pytest.raises(R.InvalidInput, match="method"):
+        sym.factor(a, st)
+    # a replan switches the schedule without repeating the ordering
+    sym_ll = sym.replan(st)
+    i_rl, i_ll = sym.info(), sym_ll.info()
+    for k in ("n", "nsuper", "panel_entries", "levels", "nnzl_padded", "factor_flops"):
+        assert i_rl[k] == i_ll[k], k
+
+
 def test_from_triplets_sums_duplicates_and_rejects_upper():
     # CscMatrix::from_triplets (csc.rs:41-106)
     a = R.CscMatrix.from_triplets(3, [0, 2, 2, 1], [0, 0, 0, 1], [1.0, 2.0, 3.0, 4.0])
