@@ -301,7 +301,8 @@ def main():
         tr = autotune(a, sym, st, nrhs=B.shape[1], reps=2)
         st = tr.settings
         sym = sym.replan(st)
-        tuned = dict(wide_flops=st.wide_flops, solve_wide=st.solve_wide, seconds=round(time.perf_counter() - t0, 2),
+        tuned = dict(wide_flops=st.wide_flops, solve_dep=st.solve_dep, solve_wide=st.solve_wide,
+                     seconds=round(time.perf_counter() - t0, 2),
                      factor_ms={str(k): round(v, 2) for k, v in tr.factor_ms.items()},
                      solve_ms={str(k): round(v, 2) for k, v in tr.solve_ms.items()})
         log(f"autotune: {tuned}")

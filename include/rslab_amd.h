@@ -88,6 +88,12 @@ typedef struct {
                              default 32; narrower ones use one workgroup) */
   int32_t method;         /* RSL_METHOD_*, default RSL_METHOD_LEFT_LOOKING
                              (SolverSettings::with_method); a plan setting */
+  int32_t solve_dep;      /* 1 (default): a solve of up to 8 right-hand sides
+                             runs each sweep as one launch whose work units
+                             order themselves through device counters; wider
+                             solves, and every solve when 0, use one launch
+                             per level and block step (tuning knob, a plan
+                             setting) */
 } rsl_settings;
 
 typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */

@@ -41,6 +41,7 @@ class Settings(C.Structure):
         ("graphs", C.c_int32),
         ("solve_wide", C.c_int32),
         ("method", C.c_int32),
+        ("solve_dep", C.c_int32),
     ]
 
 

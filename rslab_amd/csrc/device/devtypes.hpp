@@ -26,7 +26,21 @@ struct DevSym {
   const int32_t* a_row;      // user nnz -> user row
   const int32_t* a_col;      // user nnz -> user column
   const int32_t* nb_eff;     // panel width of s (64, or 128 on supernodes >= 512 wide)
+  const int32_t* anc_ptr;    // distinct supernodes owning s's off-diagonal rows (ns+1)
+  const int32_t* anc_idx;
   int64_t nnz;
+};
+
+// Dependency-driven solve: per-supernode progress counters, cleared before
+// every solve, and control words. A unit waits on counters with bounded
+// spins; a wait that exceeds its bound sets abort (every later wait returns
+// at once, so the launch always drains) and counts itself in ctl[3].
+struct SolveSync {
+  int32_t* cnt;    // supernode s at cnt[32 s + kind] (one 128-byte line each): kind 0 forward
+                   // own 64-blocks solved (in block order), 1 forward off-diagonal 64-row
+                   // chunks written to the scratch, 2 backward own blocks solved (from the
+                   // last), 3 backward 16-column off-diagonal units done
+  uint32_t* ctl;   // [0] forward ticket [1] backward ticket [2] abort [3] timed-out waits (sticky)
 };
 
 // Numeric factor storage for field T.

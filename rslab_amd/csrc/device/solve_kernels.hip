@@ -361,20 +361,13 @@ __global__ __launch_bounds__(SNT) void k_fwd_update(DevSym S, DevFac<T> Fc,
   }
 }
 
-// Backward, part 1: y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]] for 16-column
-// chunks (items (supernode, column offset)). Lanes run over rows, 512 rows
-// per pass (their x rows gathered into LDS; each pass costs a dependent
-// gather, so few large passes), each wave owns four columns; the lane
-// partial sums (lane l: rows = l mod 64, ascending) are reduced once at the
-// end.
+// y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]] for the 16 columns p0.. of
+// supernode s (k_bwd_update's unit); xs holds RB * RCH values of LDS.
+constexpr int BW_RB = 512, BW_CW = 4;  // rows per pass, columns per wave
 template <class T>
-__global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
-                                                    const int2* __restrict__ items, int32_t nrhs,
-                                                    T* __restrict__ y) {
-  constexpr int RB = 512, CW = 4;  // rows per pass, columns per wave
-  __shared__ T xs[RB * RCH];
-  const int2 it = items[blockIdx.x];
-  const int s = it.x, p0 = it.y;
+__device__ void bwd_update_cols(const DevSym& S, const DevFac<T>& Fc, int s, int p0, int32_t nrhs,
+                                T* __restrict__ y, T* xs) {
+  constexpr int RB = BW_RB, CW = BW_CW;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
   const T* P = Fc.pool + S.panel_off[s];
@@ -440,6 +433,21 @@ __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
         }
       }
   }
+}
+
+// Backward, part 1: y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]] for 16-column
+// chunks (items (supernode, column offset)). Lanes run over rows, 512 rows
+// per pass (their x rows gathered into LDS; each pass costs a dependent
+// gather, so few large passes), each wave owns four columns; the lane
+// partial sums (lane l: rows = l mod 64, ascending) are reduced once at the
+// end.
+template <class T>
+__global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
+                                                    const int2* __restrict__ items, int32_t nrhs,
+                                                    T* __restrict__ y) {
+  __shared__ T xs[BW_RB * RCH];
+  const int2 it = items[blockIdx.x];
+  bwd_update_cols(S, Fc, it.x, it.y, nrhs, y, xs);
 }
 
 // Backward own-block solve of a narrow supernode (w <= 32): one workgroup
@@ -732,6 +740,403 @@ __global__ void k_dsolve(int32_t n, int32_t nrhs, const T* __restrict__ d,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dependency-driven sweeps: one launch per sweep, one workgroup per unit.
+// A unit's workgroup takes a ticket (an atomic counter, so units start in
+// plan order whatever order the hardware dispatches workgroups in) and waits
+// only on units with earlier tickets, which belong to workgroups that are
+// already running or finished: the launch makes progress whatever the
+// residency. A producer publishes a counter with an agent-scope release after
+// its stores; a consumer polls it and acquires before reading.
+//   forward unit (s, c), c < nblk: own 64-row block c -- gather the
+//     descendants' products (waits for every updater's off-diagonal chunks),
+//     subtract L[c, j] y_j for the earlier blocks j (waiting for each), solve
+//     the unit-lower diagonal block, publish its block counter = c + 1;
+//   forward unit (s, c), c >= nblk: 64 off-diagonal rows of s, u = L21 y_own
+//     over every own block, publish by its chunk counter += 1;
+//   backward off unit (s, -1 - q): once every ancestor owning the
+//     off-diagonal rows is solved, subtract L21^T x from the own entries of
+//     the 16 columns 16q.. (one writer per entry), publish by its counter += 1;
+//   backward own unit (s, c): own 64-column block c -- once the off units of
+//     s are done, subtract L[j, c]^T x_j for the later own blocks j (last
+//     first, waiting for each), back-substitute with the diagonal block,
+//     publish its backward counter = nblk - c.
+// Sums run in a fixed order per output entry: run-to-run identical.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kSpinTicks = 200000000ull;  // 2 s of the 100 MHz constant clock
+// counters of supernode s: cnt[s * CSTRIDE + kind], one 128-byte line per supernode
+constexpr int CSTRIDE = 32;
+constexpr int CF_BLK = 0, CF_U = 1, CB_BLK = 2, CB_OFF = 3;
+
+__device__ inline int32_t* counter(const SolveSync& sy, int s, int kind) {
+  return sy.cnt + (int64_t)s * CSTRIDE + kind;
+}
+
+// Poll *p until it reaches target and return the value read. Returns target
+// at once when the solve is aborted: this wait or an earlier one of the same
+// solve ran past kSpinTicks.
+__device__ inline int32_t wait_ge(int32_t* p, int32_t target, uint32_t* ctl) {
+  int32_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (v >= target) return v;
+  const uint64_t t0 = (uint64_t)wall_clock64();
+  for (;;) {
+    __builtin_amdgcn_s_sleep(2);
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v >= target) return v;
+    if (__hip_atomic_load(ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return target;
+    if ((uint64_t)wall_clock64() - t0 > kSpinTicks) {
+      __hip_atomic_store(ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(ctl + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return target;
+    }
+  }
+}
+
+// The workgroup's stores (ordered before thread 0 by the barrier), then one
+// agent-scope release store / add of the counter.
+__device__ inline void wg_publish(int32_t* p, int32_t v, bool add) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (add) __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    else __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__device__ inline int take_ticket(uint32_t* ctr, int* slot) {
+  if (threadIdx.x == 0) *slot = (int)__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return *slot;
+}
+
+__global__ void k_sync_clear(int32_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <class T>
+__global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, const int2* __restrict__ items,
+                                                 SolveSync sy, int32_t nrhs, T* __restrict__ y,
+                                                 T* __restrict__ u) {
+  __shared__ T Ls[TB * (TB + 1)];  // Ls[p * (TB+1) + r] = L[row0 + r][pj + p]; then the diagonal block
+  __shared__ T vs[TB * RCH];       // vs[p * RCH + k] = y[pj + p][c0 + k]
+  __shared__ int s_t, s_ready;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int2 it = items[take_ticket(sy.ctl, &s_t)];
+  const int s = it.x, c = it.y;
+  const int w = S.sn_ncol[s], eo = S.e_off[s];
+  const int64_t h = S.sn_nrow[s];
+  const T* P = Fc.pool + S.panel_off[s];
+  const uint8_t* two = Fc.two + eo;
+  const int nblk = (w + TB - 1) / TB;
+  const bool own = c < nblk;
+  const int64_t row0 = own ? (int64_t)c * TB : w + (int64_t)(c - nblk) * TB;
+  const int nr = (int)min<int64_t>(TB, (own ? (int64_t)w : h) - row0);
+  const int jn = own ? c : nblk;  // own blocks feeding this unit
+  const int k0 = wv, k1 = wv + NWV;
+  // own block: the strict lower part of its diagonal block (stage_block_lower's
+  // entries), loaded while the last product runs
+  T dv[TPER];
+  auto load_diag = [&]() {
+#pragma unroll
+    for (int q = 0; q < TPER; ++q) {
+      const int t = tid + q * SNT, i = t % TB, p = t / TB;
+      const bool in = p < nr && i < nr && i > p && !(i == p + 1 && two[row0 + p]);
+      dv[q] = in ? P[(row0 + i) + (row0 + p) * h] : F<T>::zero();
+    }
+  };
+  if (own) {
+    if (wv == 0) {
+      for (int q = S.upd_ptr[s] + lane; q < S.upd_ptr[s + 1]; q += 64) {
+        const int k = S.upd_idx[q];
+        (void)wait_ge(counter(sy, k, CF_U), (S.sn_nrow[k] - S.sn_ncol[k] + TB - 1) / TB, sy.ctl);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+  }
+  T lv[TPER];
+  auto load_tile = [&](int j) {
+    const int pj = j * TB, nbj = min(TB, w - pj);
+#pragma unroll
+    for (int m = 0; m < TPER; ++m) {
+      const int p = wv + NWV * m;
+      lv[m] = (lane < nr && p < nbj) ? P[(row0 + lane) + (int64_t)(pj + p) * h] : F<T>::zero();
+    }
+    // the 2x2 coupling entry (pj + 64, pj + 63) is held in D, not in L
+    if (own && j == c - 1 && lane == 0 && wv == NWV - 1 && two[pj + TB - 1]) lv[TPER - 1] = F<T>::zero();
+  };
+  int ready = 0;  // own blocks of s seen solved (and acquired)
+  const int64_t e = eo + row0 + lane;  // own unit: this lane's row (elimination order)
+  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+    const int nc = min(RCH, nrhs - c0);
+    T a0 = F<T>::zero(), a1 = F<T>::zero();
+    // block j's y entries, fetched with its tile once the block is known solved
+    T yv[TB * RCH / SNT];
+    bool ypre = false;
+    auto load_y = [&](int j) {
+      const int pj = j * TB, nbj = min(TB, w - pj);
+#pragma unroll
+      for (int v = 0; v < TB * RCH / SNT; ++v) {
+        const int t = tid + v * SNT, p = t / RCH, k = t % RCH;
+        yv[v] = (p < nbj && k < nc) ? y[(int64_t)(eo + pj + p) * nrhs + c0 + k] : F<T>::zero();
+      }
+    };
+    if (jn > 0) {
+      load_tile(0);
+      if (ready > 0) {
+        load_y(0);
+        ypre = true;
+      }
+    } else if (own) {
+      load_diag();
+    }
+    // own unit: b minus the descendants' products, before waiting on the
+    // earlier blocks (the gather lists are long on separators: loads in
+    // batches of four)
+    T b0 = F<T>::zero(), b1 = F<T>::zero();
+    if (own && lane < nr) {
+      const int g = S.sn_first[s] + Fc.lperm[e];
+      if (k0 < nc) b0 = y[e * nrhs + c0 + k0];
+      if (k1 < nc) b1 = y[e * nrhs + c0 + k1];
+      int64_t q = S.gat_ptr[g];
+      const int64_t qe = S.gat_ptr[g + 1];
+      for (; q + 4 <= qe; q += 4) {
+        int64_t ix[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) ix[m] = S.gat_idx[q + m] * nrhs + c0;
+        T v0[4], v1[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          v0[m] = k0 < nc ? u[ix[m] + k0] : F<T>::zero();
+          v1[m] = k1 < nc ? u[ix[m] + k1] : F<T>::zero();
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          b0 = b0 - v0[m];
+          b1 = b1 - v1[m];
+        }
+      }
+      for (; q < qe; ++q) {
+        const int64_t ur = S.gat_idx[q] * nrhs + c0;
+        if (k0 < nc) b0 = b0 - u[ur + k0];
+        if (k1 < nc) b1 = b1 - u[ur + k1];
+      }
+    }
+    for (int j = 0; j < jn; ++j) {
+      const bool must = j >= ready;
+      if (must && tid == 0) {
+        s_ready = wait_ge(counter(sy, s, CF_BLK), j + 1, sy.ctl);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      __syncthreads();  // also: the previous products are done with Ls / vs
+      if (must) ready = s_ready;
+#pragma unroll
+      for (int m = 0; m < TPER; ++m) Ls[(wv + NWV * m) * (TB + 1) + lane] = lv[m];
+      if (!ypre) load_y(j);
+#pragma unroll
+      for (int v = 0; v < TB * RCH / SNT; ++v) vs[tid + v * SNT] = yv[v];
+      __syncthreads();
+      ypre = false;
+      if (j + 1 < jn) {
+        load_tile(j + 1);
+        if (j + 1 < ready) {
+          load_y(j + 1);
+          ypre = true;
+        }
+      } else if (own) {
+        load_diag();
+      }
+#pragma unroll 8
+      for (int p = 0; p < TB; ++p) {
+        const T l = Ls[p * (TB + 1) + lane];
+        a0 = a0 + l * vs[p * RCH + k0];
+        a1 = a1 + l * vs[p * RCH + k1];
+      }
+    }
+    if (own) {
+      T y0 = b0 - a0, y1 = b1 - a1;
+      __syncthreads();  // the products are done with Ls
+#pragma unroll
+      for (int q = 0; q < TPER; ++q) Ls[tid + q * SNT] = dv[q];  // Ld[p * TB + i]
+      __syncthreads();
+#pragma unroll 8
+      for (int p = 0; p + 1 < nr; ++p) {
+        const T l = Ls[p * TB + lane];
+        const T q0 = lane_bcast(y0, p), q1 = lane_bcast(y1, p);
+        y0 = y0 - l * q0;
+        y1 = y1 - l * q1;
+      }
+      if (lane < nr) {
+        if (k0 < nc) y[e * nrhs + c0 + k0] = y0;
+        if (k1 < nc) y[e * nrhs + c0 + k1] = y1;
+      }
+    } else if (lane < nr) {
+      const int64_t ur = (S.u_off[s] + (row0 - w) + lane) * nrhs + c0;
+      if (k0 < nc) u[ur + k0] = a0;
+      if (k1 < nc) u[ur + k1] = a1;
+    }
+    __syncthreads();  // before the next right-hand-side chunk reuses Ls / vs
+  }
+  if (own) wg_publish(counter(sy, s, CF_BLK), c + 1, false);
+  else wg_publish(counter(sy, s, CF_U), 1, true);
+}
+
+template <class T>
+__global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, const int2* __restrict__ items,
+                                                 SolveSync sy, int32_t nrhs, T* __restrict__ y) {
+  // own unit: Ls[i * (TB+1) + p] = L[rb + i][j0 + p], then the diagonal block,
+  // and xs[i * RCH + k] = x[rb + i][c0 + k]; off unit: its pass's x rows
+  constexpr int NOWN = TB * (TB + 1) + TB * RCH, NOFF = BW_RB * RCH;
+  __shared__ T sm[NOWN > NOFF ? NOWN : NOFF];
+  T* Ls = sm;
+  T* xs = sm + TB * (TB + 1);
+  __shared__ int s_t, s_ready;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int2 it = items[take_ticket(sy.ctl + 1, &s_t)];
+  const int s = it.x, c = it.y;
+  const int w = S.sn_ncol[s], eo = S.e_off[s];
+  const int64_t h = S.sn_nrow[s];
+  const int nblk = (w + TB - 1) / TB;
+  if (c < 0) {
+    // off unit: the off-diagonal rows' terms of 16 own columns, once every
+    // ancestor holding those rows is solved
+    if (wv == 0) {
+      for (int q = S.anc_ptr[s] + lane; q < S.anc_ptr[s + 1]; q += 64) {
+        const int a = S.anc_idx[q];
+        (void)wait_ge(counter(sy, a, CB_BLK), (S.sn_ncol[a] + TB - 1) / TB, sy.ctl);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    bwd_update_cols(S, Fc, s, (-1 - c) * 16, nrhs, y, sm);
+    wg_publish(counter(sy, s, CB_OFF), 1, true);
+    return;
+  }
+  const T* P = Fc.pool + S.panel_off[s];
+  const uint8_t* two = Fc.two + eo;
+  const int j0 = c * TB, nbk = min(TB, w - j0);
+  const int nm = nblk - 1 - c;  // later own blocks nblk-1 .. c+1
+  const int k0 = wv, k1 = wv + NWV;
+  // the transposed strict lower part of the diagonal block (stage_block_upper's
+  // entries), loaded while the last product runs
+  T dv[TPER];
+  auto load_diag = [&]() {
+#pragma unroll
+    for (int q = 0; q < TPER; ++q) {
+      const int t = tid + q * SNT, p = t % TB, i = t / TB;
+      const bool in = p < nbk && i < nbk && p > i && !(p == i + 1 && two[j0 + i]);
+      dv[q] = in ? P[(int64_t)(j0 + p) + (int64_t)(j0 + i) * h] : F<T>::zero();
+    }
+  };
+  if (h > w) {
+    // the off units of s have applied the off-diagonal rows to every own entry
+    if (tid == 0) {
+      (void)wait_ge(counter(sy, s, CB_OFF), (w + 15) / 16, sy.ctl);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+  }
+  T lv[TPER];
+  auto load_tile = [&](int m) {
+    const int j = nblk - 1 - m;
+    const int64_t rb = (int64_t)j * TB;
+    const int nrm = min(TB, w - j * TB);
+#pragma unroll
+    for (int v = 0; v < TPER; ++v) {
+      const int p = wv + NWV * v;
+      lv[v] = (lane < nrm && p < nbk) ? P[(rb + lane) + (int64_t)(j0 + p) * h] : F<T>::zero();
+    }
+    // the 2x2 coupling entry (j0 + 64, j0 + 63) is held in D, not in L
+    if (j == c + 1 && lane == 0 && wv == NWV - 1 && two[j0 + TB - 1]) lv[TPER - 1] = F<T>::zero();
+  };
+  int ready = 0;  // own blocks of s seen solved, counted from the last (and acquired)
+  for (int c0 = 0; c0 < nrhs; c0 += RCH) {
+    const int nc = min(RCH, nrhs - c0);
+    T a0 = F<T>::zero(), a1 = F<T>::zero();
+    // block j's x entries, fetched with its tile once the block is known solved
+    T xv[TB * RCH / SNT];
+    bool xpre = false;
+    auto load_x = [&](int m) {
+      const int j = nblk - 1 - m;
+      const int64_t rb = (int64_t)j * TB;
+      const int nrm = min(TB, w - j * TB);
+#pragma unroll
+      for (int v = 0; v < TB * RCH / SNT; ++v) {
+        const int t = tid + v * SNT, i = t / RCH, k = t % RCH;
+        xv[v] = (i < nrm && k < nc) ? y[((int64_t)eo + rb + i) * nrhs + c0 + k] : F<T>::zero();
+      }
+    };
+    if (nm > 0) {
+      load_tile(0);
+      if (ready >= 1) {
+        load_x(0);
+        xpre = true;
+      }
+    } else {
+      load_diag();
+    }
+    for (int m = 0; m < nm; ++m) {
+      const int j = nblk - 1 - m;
+      const int need = nblk - j;
+      const bool must = need > ready;
+      if (must && tid == 0) {
+        s_ready = wait_ge(counter(sy, s, CB_BLK), need, sy.ctl);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      __syncthreads();  // also: the previous products are done with Ls / xs
+      if (must) ready = s_ready;
+#pragma unroll
+      for (int v = 0; v < TPER; ++v) Ls[lane * (TB + 1) + wv + NWV * v] = lv[v];
+      if (!xpre) load_x(m);
+#pragma unroll
+      for (int v = 0; v < TB * RCH / SNT; ++v) xs[tid + v * SNT] = xv[v];
+      __syncthreads();
+      xpre = false;
+      if (m + 1 < nm) {
+        load_tile(m + 1);
+        if (nblk - (j - 1) <= ready) {
+          load_x(m + 1);
+          xpre = true;
+        }
+      } else {
+        load_diag();
+      }
+#pragma unroll 8
+      for (int i = 0; i < TB; ++i) {
+        const T l = Ls[i * (TB + 1) + lane];
+        a0 = a0 + l * xs[i * RCH + k0];
+        a1 = a1 + l * xs[i * RCH + k1];
+      }
+    }
+    const int64_t e = (int64_t)eo + j0 + lane;
+    T y0 = F<T>::zero(), y1 = F<T>::zero();
+    if (lane < nbk) {
+      if (k0 < nc) y0 = y[e * nrhs + c0 + k0] - a0;
+      if (k1 < nc) y1 = y[e * nrhs + c0 + k1] - a1;
+    }
+    __syncthreads();  // the products are done with Ls
+#pragma unroll
+    for (int q = 0; q < TPER; ++q) {
+      const int t = tid + q * SNT, p = t % TB, i = t / TB;
+      Ls[p * (TB + 1) + i] = dv[q];  // Lt[p * (TB+1) + i]
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int p = nbk - 1; p > 0; --p) {
+      const T l = Ls[p * (TB + 1) + lane];
+      const T q0 = lane_bcast(y0, p), q1 = lane_bcast(y1, p);
+      y0 = y0 - l * q0;
+      y1 = y1 - l * q1;
+    }
+    if (lane < nbk) {
+      if (k0 < nc) y[e * nrhs + c0 + k0] = y0;
+      if (k1 < nc) y[e * nrhs + c0 + k1] = y1;
+    }
+    __syncthreads();  // before the next right-hand-side chunk reuses Ls / xs
+  }
+  wg_publish(counter(sy, s, CB_BLK), nblk - c, false);
+}
+
 // ---- launchers ----
 template <class T>
 void launch_gather_rhs(int32_t n, int32_t nrhs, const int32_t* perm_e, const double* scale,
@@ -818,6 +1223,25 @@ void launch_bwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, i
   hipLaunchKernelGGL(k_bwd_tri<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y);
 }
 
+void launch_sync_clear(int32_t* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sync_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+template <class T>
+void launch_fwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count, const SolveSync& sy,
+                    int32_t nrhs, T* y, T* u, double bytes, hipStream_t st, LaunchProfiler* pf) {
+  if (count <= 0) return;
+  ProfScope ps(pf, LK_FWD_TRI, 0.0, bytes, st);
+  hipLaunchKernelGGL(k_fwd_dep<T>, dim3((unsigned)count), dim3(SNT), 0, st, S, Fc, items, sy, nrhs, y, u);
+}
+template <class T>
+void launch_bwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count, const SolveSync& sy,
+                    int32_t nrhs, T* y, double bytes, hipStream_t st, LaunchProfiler* pf) {
+  if (count <= 0) return;
+  ProfScope ps(pf, LK_BWD_TRI, 0.0, bytes, st);
+  hipLaunchKernelGGL(k_bwd_dep<T>, dim3((unsigned)count), dim3(SNT), 0, st, S, Fc, items, sy, nrhs, y);
+}
+
 #define RSL_SINST(T)                                                                           \
   template void launch_gather_rhs<T>(int32_t, int32_t, const int32_t*, const double*, const T*, \
                                      T*, hipStream_t, LaunchProfiler*);                        \
@@ -839,7 +1263,11 @@ void launch_bwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, i
   template void launch_bwd_update<T>(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, \
                                      T*, double, hipStream_t, LaunchProfiler*);                \
   template void launch_bwd_tri<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
-                                  T*, double, hipStream_t, LaunchProfiler*);
+                                  T*, double, hipStream_t, LaunchProfiler*);                   \
+  template void launch_fwd_dep<T>(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, \
+                                  int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);      \
+  template void launch_bwd_dep<T>(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, \
+                                  int32_t, T*, double, hipStream_t, LaunchProfiler*);
 
 RSL_SINST(double)
 RSL_SINST(cplx)

@@ -84,6 +84,7 @@ rsl::PlanParams plan_params(const rsl_settings& st) {
   if (st.method != RSL_METHOD_LEFT_LOOKING && st.method != RSL_METHOD_RIGHT_LOOKING)
     throw rsl::Error(RSL_INVALID_INPUT, "unknown factor method");
   pp.schedule = st.method;
+  pp.solve_sched = st.solve_dep != 0 ? 1 : 0;
   return pp;
 }
 }  // namespace
@@ -109,6 +110,7 @@ void rsl_settings_default(rsl_settings* s) {
   s->graphs = 1;
   s->solve_wide = 32;
   s->method = RSL_METHOD_LEFT_LOOKING;
+  s->solve_dep = 1;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }

@@ -342,6 +342,78 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     L.z_end = (int64_t)P.zitems.size();
   }
 
+  // dependency-driven solve: every unit waits only on units taken before it
+  // (updaters and ancestors sit in earlier / later levels, own blocks in order)
+  P.dfwd_items.clear();
+  P.dbwd_items.clear();
+  P.tri_total = P.off_total = 0;
+  // Within a level the units that start chains come first (forward: own
+  // blocks; backward: the off units the own blocks wait for), so workgroups
+  // waiting on a chain do not hold the slots its links need.
+  for (const LevelPlan& L : P.levels) {
+    for (int32_t s : L.all) {
+      const int32_t w = P.sn_ncol[s], off = P.sn_nrow[s] - w;
+      for (int32_t c = 0; c < (w + 63) / 64; ++c) P.dfwd_items.push_back({s, c});
+      P.tri_total += 0.5 * (double)w * (w + 1);
+      P.off_total += (double)off * w;
+    }
+    for (int32_t s : L.all) {
+      const int32_t w = P.sn_ncol[s], nblk = (w + 63) / 64, nuch = (P.sn_nrow[s] - w + 63) / 64;
+      for (int32_t c = nblk; c < nblk + nuch; ++c) P.dfwd_items.push_back({s, c});
+    }
+  }
+  for (size_t l = P.levels.size(); l-- > 0;) {
+    // off units (s, -1 - q) for 16-column chunks q, then own blocks, last first
+    for (int32_t s : P.levels[l].all)
+      if (P.sn_nrow[s] > P.sn_ncol[s])
+        for (int32_t q = 0; q * 16 < P.sn_ncol[s]; ++q) P.dbwd_items.push_back({s, -1 - q});
+    for (int32_t s : P.levels[l].all)
+      for (int32_t c = (P.sn_ncol[s] + 63) / 64; c-- > 0;) P.dbwd_items.push_back({s, c});
+  }
+  {
+    std::vector<int32_t> owner(S.n, 0);
+    for (int32_t s = 0; s < ns; ++s)
+      for (int32_t c = 0; c < P.sn_ncol[s]; ++c) owner[P.sn_first[s] + c] = s;
+    P.anc_ptr.assign((size_t)ns + 1, 0);
+    P.anc_idx.clear();
+    for (int32_t s = 0; s < ns; ++s) {
+      // off-diagonal rows ascend and every owner holds a contiguous column
+      // range, so equal owners are adjacent
+      int32_t last = -1;
+      for (int64_t q = P.rs_off[s] + P.sn_ncol[s]; q < P.rs_off[s + 1]; ++q) {
+        const int32_t a = owner[P.rs_idx[q]];
+        if (a != last) P.anc_idx.push_back(a);
+        last = a;
+      }
+      P.anc_ptr[s + 1] = (int32_t)P.anc_idx.size();
+    }
+  }
+  // The device units wait only on units taken before them; a plan that broke
+  // this would make a unit wait forever, so it is checked here. Forward: a
+  // supernode's own blocks come after every off-diagonal chunk of each of its
+  // updaters. Backward: after the first own block (the last unit) of every
+  // ancestor owning its off-diagonal rows. Within a supernode the order is
+  // fixed by construction.
+  {
+    std::vector<int64_t> fu_last(ns, -1), f_first(ns, -1), b_last(ns, -1), b_first(ns, -1);
+    for (int64_t i = 0; i < (int64_t)P.dfwd_items.size(); ++i) {
+      const Int2 it = P.dfwd_items[i];
+      if (f_first[it.x] < 0) f_first[it.x] = i;
+      fu_last[it.x] = i;
+    }
+    for (int64_t i = 0; i < (int64_t)P.dbwd_items.size(); ++i) {
+      const Int2 it = P.dbwd_items[i];
+      if (b_first[it.x] < 0) b_first[it.x] = i;
+      b_last[it.x] = i;
+    }
+    for (int32_t s = 0; s < ns; ++s) {
+      for (int32_t q = P.upd_ptr[s]; q < P.upd_ptr[s + 1]; ++q)
+        if (fu_last[P.upd_idx[q]] >= f_first[s]) throw std::runtime_error("plan: solve unit ordered before a unit it waits on");
+      for (int32_t q = P.anc_ptr[s]; q < P.anc_ptr[s + 1]; ++q)
+        if (b_last[P.anc_idx[q]] >= b_first[s]) throw std::runtime_error("plan: solve unit ordered before a unit it waits on");
+    }
+  }
+
   // A scatter map: user lower entry (i >= j) -> slot of P^T A P in its panel
   P.nnz = colp[S.n];
   P.amap.resize(P.nnz);

@@ -14,6 +14,8 @@ struct PlanParams {
   int solve_wide = 32;          // own blocks wider than this are solved in 64-column steps
   int schedule = 0;             // 0 left-looking (cmod pulled at the target's level),
                                 // 1 right-looking (pushed after the updater's level)
+  int solve_sched = 1;          // 1 dependency-driven solve (one launch per sweep),
+                                // 0 level-by-level launches
 };
 
 struct LevelPlan {
@@ -69,6 +71,13 @@ struct Plan {
   std::vector<SItem> sitems;
   std::vector<SItem> bitems;    // backward step units: (s, j0, i0) pushes of block j0 into block i0
   std::vector<SItem> zitems;    // (s, j0) own blocks of the wide supernodes
+  // dependency-driven solve: work units in the order they are taken
+  std::vector<Int2> dfwd_items;  // (s, c): own 64-row blocks c < nblk of a level's supernodes,
+                                 // then their off-diagonal 64-row chunks (c >= nblk); levels ascending
+  std::vector<Int2> dbwd_items;  // (s, -1 - q): off-diagonal terms of own columns 16q..16q+15,
+                                 // then (s, c): own 64-column blocks, last first; levels descending
+  std::vector<int32_t> anc_ptr, anc_idx;  // distinct supernodes owning s's off-diagonal rows
+  double tri_total = 0, off_total = 0;    // own-block lower entries, off-diagonal entries
   int64_t panel_total = 0;      // elements in the panel pool
   // user lower-CSC entries -> panel slot
   int64_t nnz = 0;

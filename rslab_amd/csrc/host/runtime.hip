@@ -46,6 +46,9 @@ template <class T> void launch_fwd_update(const DevSym&, const DevFac<T>&, const
 template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, const uint8_t*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
+void launch_sync_clear(int32_t*, int64_t, hipStream_t);
+template <class T> void launch_fwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_bwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_expand(int64_t, const int64_t*, const T*, T*, hipStream_t);
 template <class T> void launch_residual(int32_t, int32_t, const int64_t*, const int32_t*, const T*, const T*, const T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_colmax(int32_t, int32_t, const T*, unsigned long long*, hipStream_t);
@@ -204,6 +207,10 @@ void DeviceSymbolic::upload() {
   bitems.upload(P.bitems, stream);
   zitems.upload(P.zitems, stream);
   nb_eff.upload(P.nb_eff, stream);
+  dfwd_items.upload(P.dfwd_items, stream);
+  dbwd_items.upload(P.dbwd_items, stream);
+  anc_ptr.upload(P.anc_ptr, stream);
+  anc_idx.upload(P.anc_idx, stream);
   hip_check(hipStreamSynchronize(stream), "plan upload");
   view.n = P.n;
   view.ns = P.ns;
@@ -225,6 +232,8 @@ void DeviceSymbolic::upload() {
   view.a_row = a_row.as<int32_t>();
   view.a_col = a_col.as<int32_t>();
   view.nb_eff = nb_eff.as<int32_t>();
+  view.anc_ptr = anc_ptr.as<int32_t>();
+  view.anc_idx = anc_idx.as<int32_t>();
   view.nnz = P.nnz;
   uploaded = true;
 }
@@ -314,6 +323,10 @@ class DeviceFactor final : public DeviceFactorBase {
  public:
   DBuf pool, d, ds, two, lperm, e_of_g, perm_e, deep, counters, status, scale_d, vals_d, sn_inert, rows_e,
       work_y, work_b, work_z, cbuf, gpool, dev_u64;
+  // dependency-driven solve: one 128-byte line of counters per supernode + 4 control words
+  static constexpr int64_t kSyncStride = 32;
+  DBuf solve_sync;
+  SolveSync sy{};
   // refinement workspace, kept across calls (grown on demand; calls on one
   // handle are serialized by the C ABI)
   DBuf rf_afull, rf_dx, rf_dbest, rf_dr, rf_ddx, rf_norms, rf_mask, rf_mask2;
@@ -381,6 +394,7 @@ class DeviceFactor final : public DeviceFactorBase {
       throw Error(INVALID_INPUT, "unknown Krylov method");
     hip_check(hipMemcpyAsync(x, hr_dxo.p, bytes, hipMemcpyDeviceToHost, st), "download x");
     hip_check(hipStreamSynchronize(st), "krylov");
+    check_solve_waits();
     for (int c = 0; c < k; ++c) {
       info[4 * c + 0] = res[c].iters;
       info[4 * c + 1] = res[c].final_res;
@@ -457,6 +471,10 @@ class DeviceFactor final : public DeviceFactorBase {
     fv.sn_inertia = sn_inert.as<int32_t>();
     fv.rs_e = rows_e.as<int32_t>();
     fv.status = status.as<int32_t>();
+    solve_sync.alloc(sizeof(int32_t) * (kSyncStride * (size_t)std::max<int64_t>(P.ns, 1) + 4));
+    hip_check(hipMemset(solve_sync.p, 0, solve_sync.bytes), "memset");
+    sy.cnt = solve_sync.as<int32_t>();
+    sy.ctl = reinterpret_cast<uint32_t*>(sy.cnt + kSyncStride * std::max<int64_t>(P.ns, 1));
     // set-up for the calls that must not allocate later: the ordering events
     // and the fixed scratch's address on this (current) device
     make_events();
@@ -771,6 +789,23 @@ class DeviceFactor final : public DeviceFactorBase {
     const int2* gi = sym->gat_items.as<int2>();
     launch_gather_rhs<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(),
                          static_cast<const T*>(d_b), y, st, pf);
+    // Dependency-driven sweeps for solves of up to 8 right-hand sides (one
+    // chunk per unit: a unit's block chain is walked once); wider solves use
+    // the level launches, which stream every chunk through each launch.
+    if (P.params.solve_sched == 1 && nrhs <= 8) {
+      // one launch per sweep; the units order themselves through counters
+      const double sweep_bytes = ts * (P.tri_total + P.off_total + 2.0 * P.n * nrhs);
+      launch_sync_clear(solve_sync.as<int32_t>(), kSyncStride * std::max<int64_t>(P.ns, 1) + 3, st);
+      launch_fwd_dep<T>(S, fv, sym->dfwd_items.as<int2>(), (int)P.dfwd_items.size(), sy, (int32_t)nrhs, y, u,
+                        sweep_bytes, st, pf);
+      launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
+      launch_bwd_dep<T>(S, fv, sym->dbwd_items.as<int2>(), (int)P.dbwd_items.size(), sy, (int32_t)nrhs, y,
+                        sweep_bytes, st, pf);
+      launch_scatter_sol<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(), y,
+                            static_cast<T*>(d_x), st, pf);
+      hip_check(hipGetLastError(), "solve launch");
+      return;
+    }
     const int32_t* lists = sym->level_lists.as<int32_t>();
     for (size_t L = 0; L < P.levels.size(); ++L) {
       const LevelPlan& lv = P.levels[L];
@@ -808,6 +843,20 @@ class DeviceFactor final : public DeviceFactorBase {
     solve_device(db.p, nrhs, dx.p, st);
     hip_check(hipMemcpyAsync(x, dx.p, bytes, hipMemcpyDeviceToHost, st), "download solution");
     hip_check(hipStreamSynchronize(st), "solve");
+    check_solve_waits();
+  }
+
+  // A dependency wait of a solve that ran past its bound leaves a count in
+  // the sticky control word; the synchronous calls report it (and clear it).
+  void check_solve_waits() {
+    hipStream_t st = sym->stream;
+    uint32_t v = 0;
+    hip_check(hipMemcpyAsync(&v, sy.ctl + 3, sizeof(v), hipMemcpyDeviceToHost, st), "solve status");
+    hip_check(hipStreamSynchronize(st), "solve status");
+    if (v == 0) return;
+    hip_check(hipMemsetAsync(sy.ctl + 3, 0, sizeof(v), st), "memset");
+    hip_check(hipStreamSynchronize(st), "solve status");
+    throw Error(DEVICE_ERROR, "device: a solve's dependency wait timed out; its result is invalid");
   }
 
   void download(std::vector<int64_t>& colp, std::vector<int32_t>& rowi, std::vector<double>& lval,
@@ -1179,6 +1228,7 @@ void DeviceFactor<T>::refine_core(const T* av, const T* dbp, int32_t k, double t
     }
   }
   hip_check(hipMemcpyAsync(xout, dbest.p, sizeof(T) * n * k, hipMemcpyDeviceToDevice, st), "copy x");
+  check_solve_waits();
   for (int c = 0; c < k; ++c) {
     info[4 * c + 0] = ir[c];
     info[4 * c + 1] = gm_iters[c];

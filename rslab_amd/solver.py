@@ -231,6 +231,10 @@ class SolverSettings:
     # own blocks wider than this are solved in multi-workgroup steps (tuning knob)
     solve_wide: int = 32
     method: FactorMethod = FactorMethod.LeftLooking
+    # solves of up to 8 right-hand sides run each sweep as one launch ordered
+    # by device counters (False, and wider solves: one launch per level and
+    # block step; tuning knob)
+    solve_dep: bool = True
 
     # builders (multifrontal_ldlt.rs:616-779)
     def with_method(self, m):
@@ -301,6 +305,7 @@ class SolverSettings:
         if FactorMethod(self.method) == FactorMethod.Multifrontal:
             raise InvalidInput("FactorMethod.Multifrontal is not provided; use LeftLooking or RightLooking")
         s.method = int(self.method)
+        s.solve_dep = 1 if self.solve_dep else 0
         return s
 
 

@@ -27,3 +27,5 @@ def test_settings_carry_the_solve_cutover():
     s = R.SolverSettings().to_c()
     assert s.solve_wide == 32 and s.graphs == 1 and s.drop_tol == -1.0 and s.cmod_cap == -1
     assert dataclasses.replace(R.SolverSettings(), solve_wide=0).to_c().solve_wide == 0
+    assert s.solve_dep == 1 and s.method == 0
+    assert dataclasses.replace(R.SolverSettings(), solve_dep=False).to_c().solve_dep == 0

@@ -8,7 +8,7 @@ import pytest
 import oracle as O
 import rslab_amd as R
 from rslab_amd import matgen
-from rslab_amd.autotune import SOLVE_WIDE, WIDE_FLOPS, autotune
+from rslab_amd.autotune import SOLVE_DEP, SOLVE_WIDE, WIDE_FLOPS, autotune
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,7 @@ def test_autotune_picks_a_candidate_and_keeps_the_factor():
     sym = R.LdltSymbolic.analyze_with(a, st)
     res = autotune(a, sym, st, nrhs=4, reps=2)
     assert res.settings.wide_flops in WIDE_FLOPS and res.settings.solve_wide in SOLVE_WIDE
-    assert set(res.factor_ms) == set(WIDE_FLOPS) and set(res.solve_ms) == set(SOLVE_WIDE)
+    assert set(res.factor_ms) == set(WIDE_FLOPS) and set(res.solve_ms) == {SOLVE_DEP, *SOLVE_WIDE}
     assert all(t > 0 for t in list(res.factor_ms.values()) + list(res.solve_ms.values()))
     tuned = sym.replan(res.settings).factor(a, res.settings)
     plain = sym.factor(a)
