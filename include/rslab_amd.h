@@ -175,11 +175,13 @@ int rsl_factor(rsl_symbolic *sym, int32_t field, const double *values,
  * (LdltSymbolic reuse, src/numeric/sparse_solver.rs:457-488). */
 int rsl_refactor(rsl_solver *f, const double *values);
 
-/* Device-resident variant of rsl_refactor (real field): d_values in device
- * memory, same layout. The equilibration and the ForceAccept floor are
- * computed on the device and give the host path's bits, so the factor equals
- * rsl_refactor's on the same values. Ordered after the work enqueued on
- * `stream` (a hipStream_t; NULL is the null stream); returns when done. */
+/* Device-resident variant of rsl_refactor: d_values in device memory, same
+ * layout (interleaved for RSL_C64). The equilibration and the ForceAccept
+ * floor are computed on the device; in the real field they give the host
+ * path's bits, so the factor equals rsl_refactor's on the same values (the
+ * complex field's modulus is the device's hypot, equal to the host's to
+ * rounding). Ordered after the work enqueued on `stream` (a hipStream_t; NULL
+ * is the null stream); returns when done. */
 int rsl_refactor_device(rsl_solver *f, const void *d_values, void *stream);
 
 /* LdltSolver::solve_many (src/numeric/sparse_solver.rs:302-327) with
@@ -244,10 +246,11 @@ int rsl_factor_load(rsl_symbolic *sym, const char *path, rsl_solver **out);
 int rsl_refine(rsl_solver *f, const double *a_values, const double *b, int64_t nrhs,
                double target, double *x, double *info);
 
-/* Device-resident variant of rsl_refine (real field): d_a_values, d_b, d_x
- * in device memory, info on the host. Takes the same steps as rsl_refine on
- * the same inputs. Ordered after the work enqueued on `stream`; returns when
- * done. */
+/* Device-resident variant of rsl_refine: d_a_values, d_b, d_x in device
+ * memory (interleaved for RSL_C64), info on the host. Takes the same steps as
+ * rsl_refine on the same inputs (real field; in the complex field ||A||inf
+ * is formed with the device's hypot). Ordered after the work enqueued on
+ * `stream`; returns when done. */
 int rsl_refine_device(rsl_solver *f, const void *d_a_values, const void *d_b, int64_t nrhs,
                       double target, void *d_x, double *info, void *stream);
 

@@ -134,17 +134,24 @@ void launch_copy_col(int32_t n, int32_t ka, int32_t ca, const T* A, int32_t kb, 
   hipLaunchKernelGGL(k_copy_col<T>, dim3(blocks_for(n)), dim3(RT), 0, st, n, ka, ca, A, kb, cb, B);
 }
 
-// ---- device-resident inputs (real field): the host computations of the
-// same quantities, in an order that gives the same bits ----
+// ---- device-resident inputs: the host computations of the same quantities,
+// in an order that gives the same bits (real field; the complex field's
+// modulus is the device's hypot, which may differ from the host's in the last
+// bit) ----
+
+// |a_q| of entry q: values interleaved (re, im) for the complex field
+__device__ inline double entry_mag(const double* __restrict__ v, int64_t q, int cplx) {
+  return cplx ? hypot(v[2 * q], v[2 * q + 1]) : fabs(v[q]);
+}
 
 // one-pass equilibration (onepass_scale, src/numeric/sparse_solver.rs:399-418):
 // row maxima of |a| over the symmetric matrix; maxima are order-free, so the
 // atomics give the host's value.
 __global__ void k_scale_max(int64_t nnz, const int32_t* __restrict__ a_row, const int32_t* __restrict__ a_col,
-                            const double* __restrict__ v, unsigned long long* __restrict__ smax) {
+                            const double* __restrict__ v, int cplx, unsigned long long* __restrict__ smax) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nnz) return;
-  const double m = fabs(v[q]);
+  const double m = entry_mag(v, q, cplx);
   if (!(m > 0.0)) return;  // zero cannot raise a maximum; NaN never compares greater
   const unsigned long long b = (unsigned long long)__double_as_longlong(m);
   const int32_t i = a_row[q], j = a_col[q];
@@ -167,11 +174,11 @@ __global__ void k_scale_final(int32_t n, const unsigned long long* __restrict__ 
 
 // max |a_ij| s_i s_j over the entries (the ForceAccept pivot floor's |A_hat|max)
 __global__ void k_scaled_amax(int64_t nnz, const int32_t* __restrict__ a_row, const int32_t* __restrict__ a_col,
-                              const double* __restrict__ v, const double* __restrict__ scale,
+                              const double* __restrict__ v, int cplx, const double* __restrict__ scale,
                               unsigned long long* __restrict__ out) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nnz) return;
-  const double m = fabs(v[q]) * scale[a_row[q]] * scale[a_col[q]];
+  const double m = entry_mag(v, q, cplx) * scale[a_row[q]] * scale[a_col[q]];
   if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
@@ -179,33 +186,34 @@ __global__ void k_scaled_amax(int64_t nnz, const int32_t* __restrict__ a_row, co
 // in the full symmetric CSR's order, which is the host loop's order of
 // accumulation for that row; the maximum over rows is order-free.
 __global__ void k_row_abs_max(int32_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ src,
-                              const double* __restrict__ v, unsigned long long* __restrict__ out) {
+                              const double* __restrict__ v, int cplx, unsigned long long* __restrict__ out) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
-  for (int64_t q = rp[i]; q < rp[i + 1]; ++q) acc += fabs(v[src[q]]);
+  for (int64_t q = rp[i]; q < rp[i + 1]; ++q) acc += entry_mag(v, src[q], cplx);
   if (acc > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(acc));
 }
 
 void launch_device_scale(int32_t n, int64_t nnz, const int32_t* a_row, const int32_t* a_col, const double* v,
-                         unsigned long long* smax, double* scale, hipStream_t st) {
+                         int cplx, unsigned long long* smax, double* scale, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(std::max<int64_t>(n, 1))), dim3(RT), 0, st, smax, (int64_t)n);
   if (nnz > 0)
-    hipLaunchKernelGGL(k_scale_max, dim3(blocks_for(nnz)), dim3(RT), 0, st, nnz, a_row, a_col, v, smax);
+    hipLaunchKernelGGL(k_scale_max, dim3(blocks_for(nnz)), dim3(RT), 0, st, nnz, a_row, a_col, v, cplx, smax);
   if (n > 0) hipLaunchKernelGGL(k_scale_final, dim3(blocks_for(n)), dim3(RT), 0, st, n, smax, scale);
 }
 
-void launch_scaled_amax(int64_t nnz, const int32_t* a_row, const int32_t* a_col, const double* v,
+void launch_scaled_amax(int64_t nnz, const int32_t* a_row, const int32_t* a_col, const double* v, int cplx,
                         const double* scale, unsigned long long* out, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(RT), 0, st, out, (int64_t)1);
   if (nnz > 0)
-    hipLaunchKernelGGL(k_scaled_amax, dim3(blocks_for(nnz)), dim3(RT), 0, st, nnz, a_row, a_col, v, scale, out);
+    hipLaunchKernelGGL(k_scaled_amax, dim3(blocks_for(nnz)), dim3(RT), 0, st, nnz, a_row, a_col, v, cplx, scale,
+                       out);
 }
 
-void launch_row_abs_max(int32_t n, const int64_t* rp, const int64_t* src, const double* v, unsigned long long* out,
-                        hipStream_t st) {
+void launch_row_abs_max(int32_t n, const int64_t* rp, const int64_t* src, const double* v, int cplx,
+                        unsigned long long* out, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(RT), 0, st, out, (int64_t)1);
-  if (n > 0) hipLaunchKernelGGL(k_row_abs_max, dim3(blocks_for(n)), dim3(RT), 0, st, n, rp, src, v, out);
+  if (n > 0) hipLaunchKernelGGL(k_row_abs_max, dim3(blocks_for(n)), dim3(RT), 0, st, n, rp, src, v, cplx, out);
 }
 
 #define RSL_INST(T)                                                                                     \

@@ -24,11 +24,12 @@ template <class T> void launch_drop(const DevSym&, const DevFac<T>&, double, hip
 template <class T> void launch_rows_e(const DevSym&, const DevFac<T>&, int64_t, hipStream_t);
 template <class T> void launch_assemble(const DevSym&, const DevFac<T>&, const T*, const double*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_cmod(const DevSym&, const DevFac<T>&, const CItem*, int, const CTask*, double, int, hipStream_t, LaunchProfiler*);
-void launch_device_scale(int32_t, int64_t, const int32_t*, const int32_t*, const double*, unsigned long long*,
+void launch_device_scale(int32_t, int64_t, const int32_t*, const int32_t*, const double*, int, unsigned long long*,
                          double*, hipStream_t);
-void launch_scaled_amax(int64_t, const int32_t*, const int32_t*, const double*, const double*, unsigned long long*,
+void launch_scaled_amax(int64_t, const int32_t*, const int32_t*, const double*, int, const double*,
+                        unsigned long long*, hipStream_t);
+void launch_row_abs_max(int32_t, const int64_t*, const int64_t*, const double*, int, unsigned long long*,
                         hipStream_t);
-void launch_row_abs_max(int32_t, const int64_t*, const int64_t*, const double*, unsigned long long*, hipStream_t);
 void launch_cmod_two_phase(const DevSym&, const DevFac<double>&, const CItem*, int, const CTask*, const PTile*,
                            int64_t, void*, int64_t, const double*, double, int, hipStream_t, LaunchProfiler*);
 void launch_make_g(const DevSym&, const DevFac<double>&, const int32_t*, int, int, double*, hipStream_t);
@@ -515,24 +516,22 @@ class DeviceFactor final : public DeviceFactorBase {
     run_factor(vals_d.as<T>(), pol, t0);
   }
 
-  // values already in device memory (real field): the equilibration and the
-  // pivot floor are computed on the device, to the host path's bits
+  // values already in device memory: the equilibration and the pivot floor
+  // are computed on the device, to the host path's bits in the real field
+  // (the complex field's modulus is the device's hypot: equal to rounding)
   void refactor_device(const void* d_values, hipStream_t caller) override {
-    if constexpr (FieldOf<T>::id != 0) {
-      (void)d_values;
-      (void)caller;
-      throw Error(INVALID_INPUT, "device-resident refactor supports the real field only");
-    } else {
+    {
       const Plan& P = sym->plan;
       hipStream_t st = sym->stream;
       auto t0 = std::chrono::steady_clock::now();
       wait_for(caller);
       const double* v = static_cast<const double*>(d_values);
+      const int cplx = FieldOf<T>::id;
       if (dev_u64.bytes < sizeof(unsigned long long) * (size_t)(P.n + 1))
         dev_u64.alloc(sizeof(unsigned long long) * (size_t)(P.n + 1));
       auto* u64 = dev_u64.as<unsigned long long>();
       if (settings.scaling == 1)
-        launch_device_scale(P.n, P.nnz, sym->a_row.as<int32_t>(), sym->a_col.as<int32_t>(), v, u64,
+        launch_device_scale(P.n, P.nnz, sym->a_row.as<int32_t>(), sym->a_col.as<int32_t>(), v, cplx, u64,
                             scale_d.as<double>(), st);
       else
         launch_fill<double>(scale_d.as<double>(), P.n, 1.0, st);
@@ -542,8 +541,8 @@ class DeviceFactor final : public DeviceFactorBase {
         pol.floor = std::max(settings.abs_floor, 0.0);
       } else if (settings.pivot == 2) {
         unsigned long long bits = 0;
-        launch_scaled_amax(P.nnz, sym->a_row.as<int32_t>(), sym->a_col.as<int32_t>(), v, scale_d.as<double>(),
-                           u64 + P.n, st);
+        launch_scaled_amax(P.nnz, sym->a_row.as<int32_t>(), sym->a_col.as<int32_t>(), v, cplx,
+                           scale_d.as<double>(), u64 + P.n, st);
         hip_check(hipMemcpyAsync(&bits, u64 + P.n, sizeof(bits), hipMemcpyDeviceToHost, st), "amax");
         hip_check(hipStreamSynchronize(st), "amax");
         double amax;
@@ -552,10 +551,11 @@ class DeviceFactor final : public DeviceFactorBase {
         pol.floor = std::max(amax, 1.0) * 2.220446049250313e-16;
       }
       // the replayed graph reads the handle's own values buffer
-      const double* src = v;
+      const T* src = static_cast<const T*>(d_values);
       if (settings.graphs && !prof_on) {
-        hip_check(hipMemcpyAsync(vals_d.p, v, sizeof(double) * P.nnz, hipMemcpyDeviceToDevice, st), "copy values");
-        src = vals_d.as<double>();
+        hip_check(hipMemcpyAsync(vals_d.p, d_values, sizeof(T) * P.nnz, hipMemcpyDeviceToDevice, st),
+                  "copy values");
+        src = vals_d.as<T>();
       }
       run_factor(src, pol, t0);
     }
@@ -1077,15 +1077,13 @@ void DeviceFactor<T>::refine(const double* a_values, const double* b, int64_t nr
   hip_check(hipStreamSynchronize(st), "refine");
 }
 
-// a_values, b and x in device memory (real field): ||A||_inf on the device in
-// the host loop's order of accumulation, so the ladder takes the same steps
+// a_values, b and x in device memory: ||A||_inf on the device in the host
+// loop's order of accumulation, so the ladder takes the same steps (real
+// field; the complex field's modulus is the device's hypot)
 template <class T>
 void DeviceFactor<T>::refine_device(const void* d_a, const void* d_b, int64_t nrhs, double target, void* d_x,
                                     double* info, hipStream_t caller) {
-  if constexpr (FieldOf<T>::id != 0) {
-    (void)d_a, (void)d_b, (void)nrhs, (void)target, (void)d_x, (void)info, (void)caller;
-    throw Error(INVALID_INPUT, "device-resident refinement supports the real field only");
-  } else {
+  {
     const Plan& P = sym->plan;
     hipStream_t st = sym->stream;
     sym->ensure_full_csr();
@@ -1094,7 +1092,7 @@ void DeviceFactor<T>::refine_device(const void* d_a, const void* d_b, int64_t nr
       dev_u64.alloc(sizeof(unsigned long long) * (size_t)(P.n + 1));
     unsigned long long bits = 0;
     launch_row_abs_max(P.n, sym->fa_rp.as<int64_t>(), sym->fa_src.as<int64_t>(), static_cast<const double*>(d_a),
-                       dev_u64.as<unsigned long long>(), st);
+                       FieldOf<T>::id, dev_u64.as<unsigned long long>(), st);
     hip_check(hipMemcpyAsync(&bits, dev_u64.p, sizeof(bits), hipMemcpyDeviceToHost, st), "anorm");
     hip_check(hipStreamSynchronize(st), "anorm");
     double anorm;

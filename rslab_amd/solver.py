@@ -589,7 +589,8 @@ class LdltSolver:
 
     def refactor_device(self, d_values: int, stream: int = 0):
         """Same-pattern refactorization from lower-CSC values already in
-        device memory (real field; d_values is their device address), ordered
+        device memory (d_values is their device address; interleaved for the
+        complex field), ordered
         after the work enqueued on `stream` (0 = the null stream)."""
         _check(lib.rsl_refactor_device(self._h, C.c_void_p(d_values), C.c_void_p(stream) if stream else None))
 
@@ -721,9 +722,9 @@ class MixedLdltSolver:
 
     def solve_to_device(self, d_a_values: int, d_b: int, nrhs: int, d_x: int, target_backward_error: float,
                         stream: int = 0):
-        """solve_to with the matrix values, b and x in device memory (real
-        field; device addresses, b and x row-major n x nrhs). Returns the per-column
-        MixedInfo list."""
+        """solve_to with the matrix values, b and x in device memory (device
+        addresses, b and x row-major n x nrhs, interleaved for the complex
+        field). Returns the per-column MixedInfo list."""
         if nrhs <= 0:
             raise DimensionMismatch("nrhs must be positive")
         info = np.zeros(4 * nrhs, dtype=np.float64)
