@@ -94,6 +94,11 @@ typedef struct {
                              solves, and every solve when 0, use one launch
                              per level and block step (tuning knob, a plan
                              setting) */
+  int32_t solve_split;    /* with solve_dep: the first tree level (leaves are
+                             level 0) in the dependency-driven launch, the
+                             levels below run one launch per level; < 0
+                             (default): the first level with at most 128
+                             supernodes (tuning knob, a plan setting) */
 } rsl_settings;
 
 typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */

@@ -42,6 +42,7 @@ class Settings(C.Structure):
         ("solve_wide", C.c_int32),
         ("method", C.c_int32),
         ("solve_dep", C.c_int32),
+        ("solve_split", C.c_int32),
     ]
 
 

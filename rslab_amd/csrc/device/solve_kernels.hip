@@ -808,9 +808,15 @@ __device__ inline int take_ticket(uint32_t* ctr, int* slot) {
   return *slot;
 }
 
-__global__ void k_sync_clear(int32_t* __restrict__ p, int64_t n) {
+// Counters at the start of a solve: zero, except that the off-diagonal chunk
+// counter of a supernode solved before the dependency-driven launch (below the
+// split level) already holds its full count; control words zero (the sticky
+// timeout count, ctl[3], is kept).
+__global__ void k_sync_init(int32_t* __restrict__ cnt, const int32_t* __restrict__ fu_init, int32_t ns) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+  const int64_t n = (int64_t)ns * CSTRIDE;
+  if (i < n) cnt[i] = (i % CSTRIDE == CF_U) ? fu_init[i / CSTRIDE] : 0;
+  else if (i < n + 3) cnt[i] = 0;
 }
 
 template <class T>
@@ -1223,9 +1229,9 @@ void launch_bwd_tri(const DevSym& S, const DevFac<T>& Fc, const int32_t* list, i
   hipLaunchKernelGGL(k_bwd_tri<T>, dim3(count), dim3(SNT), 0, st, S, Fc, list, nrhs, y);
 }
 
-void launch_sync_clear(int32_t* p, int64_t n, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_sync_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+void launch_sync_init(int32_t* cnt, const int32_t* fu_init, int32_t ns, hipStream_t st) {
+  const int64_t n = (int64_t)ns * CSTRIDE + 3;
+  hipLaunchKernelGGL(k_sync_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cnt, fu_init, ns);
 }
 template <class T>
 void launch_fwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count, const SolveSync& sy,

@@ -85,6 +85,7 @@ rsl::PlanParams plan_params(const rsl_settings& st) {
     throw rsl::Error(RSL_INVALID_INPUT, "unknown factor method");
   pp.schedule = st.method;
   pp.solve_sched = st.solve_dep != 0 ? 1 : 0;
+  pp.solve_split = st.solve_split;
   return pp;
 }
 }  // namespace
@@ -111,6 +112,7 @@ void rsl_settings_default(rsl_settings* s) {
   s->solve_wide = 32;
   s->method = RSL_METHOD_LEFT_LOOKING;
   s->solve_dep = 1;
+  s->solve_split = -1;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }

@@ -346,23 +346,45 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   // (updaters and ancestors sit in earlier / later levels, own blocks in order)
   P.dfwd_items.clear();
   P.dbwd_items.clear();
-  P.tri_total = P.off_total = 0;
+  // Split level: below it the solve runs level launches (many small
+  // supernodes), from it up one dependency-driven launch per sweep. Automatic:
+  // the first level with at most 128 supernodes.
+  {
+    const int32_t nl = (int32_t)P.levels.size();
+    int32_t sp = prm.solve_split;
+    if (sp < 0) {
+      sp = nl;
+      for (int32_t l = 0; l < nl; ++l)
+        if ((int64_t)P.levels[l].all.size() <= 128) {
+          sp = l;
+          break;
+        }
+    }
+    P.solve_split = std::min(sp, nl);
+  }
   // Within a level the units that start chains come first (forward: own
   // blocks; backward: the off units the own blocks wait for), so workgroups
   // waiting on a chain do not hold the slots its links need.
-  for (const LevelPlan& L : P.levels) {
+  P.fu_init.assign(ns, 0);
+  P.dep_tri = P.dep_off = 0;
+  for (size_t l = 0; l < P.levels.size(); ++l) {
+    const LevelPlan& L = P.levels[l];
+    if ((int32_t)l < P.solve_split) {
+      for (int32_t s : L.all) P.fu_init[s] = (P.sn_nrow[s] - P.sn_ncol[s] + 63) / 64;
+      continue;
+    }
     for (int32_t s : L.all) {
       const int32_t w = P.sn_ncol[s], off = P.sn_nrow[s] - w;
       for (int32_t c = 0; c < (w + 63) / 64; ++c) P.dfwd_items.push_back({s, c});
-      P.tri_total += 0.5 * (double)w * (w + 1);
-      P.off_total += (double)off * w;
+      P.dep_tri += 0.5 * (double)w * (w + 1);
+      P.dep_off += (double)off * w;
     }
     for (int32_t s : L.all) {
       const int32_t w = P.sn_ncol[s], nblk = (w + 63) / 64, nuch = (P.sn_nrow[s] - w + 63) / 64;
       for (int32_t c = nblk; c < nblk + nuch; ++c) P.dfwd_items.push_back({s, c});
     }
   }
-  for (size_t l = P.levels.size(); l-- > 0;) {
+  for (size_t l = P.levels.size(); l-- > (size_t)P.solve_split;) {
     // off units (s, -1 - q) for 16-column chunks q, then own blocks, last first
     for (int32_t s : P.levels[l].all)
       if (P.sn_nrow[s] > P.sn_ncol[s])
@@ -392,8 +414,9 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   // this would make a unit wait forever, so it is checked here. Forward: a
   // supernode's own blocks come after every off-diagonal chunk of each of its
   // updaters. Backward: after the first own block (the last unit) of every
-  // ancestor owning its off-diagonal rows. Within a supernode the order is
-  // fixed by construction.
+  // ancestor owning its off-diagonal rows. Updaters below the split have no
+  // units and their chunk counters start full. Within a supernode the order
+  // is fixed by construction.
   {
     std::vector<int64_t> fu_last(ns, -1), f_first(ns, -1), b_last(ns, -1), b_first(ns, -1);
     for (int64_t i = 0; i < (int64_t)P.dfwd_items.size(); ++i) {
@@ -407,10 +430,15 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
       b_last[it.x] = i;
     }
     for (int32_t s = 0; s < ns; ++s) {
-      for (int32_t q = P.upd_ptr[s]; q < P.upd_ptr[s + 1]; ++q)
-        if (fu_last[P.upd_idx[q]] >= f_first[s]) throw std::runtime_error("plan: solve unit ordered before a unit it waits on");
+      if (f_first[s] < 0) continue;  // below the split: no units
+      for (int32_t q = P.upd_ptr[s]; q < P.upd_ptr[s + 1]; ++q) {
+        const int32_t k = P.upd_idx[q];
+        if (fu_last[k] < 0 ? P.fu_init[k] != (P.sn_nrow[k] - P.sn_ncol[k] + 63) / 64 : fu_last[k] >= f_first[s])
+          throw std::runtime_error("plan: solve unit ordered before a unit it waits on");
+      }
       for (int32_t q = P.anc_ptr[s]; q < P.anc_ptr[s + 1]; ++q)
-        if (b_last[P.anc_idx[q]] >= b_first[s]) throw std::runtime_error("plan: solve unit ordered before a unit it waits on");
+        if (b_last[P.anc_idx[q]] < 0 || b_last[P.anc_idx[q]] >= b_first[s])
+          throw std::runtime_error("plan: solve unit ordered before a unit it waits on");
     }
   }
 

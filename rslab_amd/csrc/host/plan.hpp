@@ -14,8 +14,9 @@ struct PlanParams {
   int solve_wide = 32;          // own blocks wider than this are solved in 64-column steps
   int schedule = 0;             // 0 left-looking (cmod pulled at the target's level),
                                 // 1 right-looking (pushed after the updater's level)
-  int solve_sched = 1;          // 1 dependency-driven solve (one launch per sweep),
-                                // 0 level-by-level launches
+  int solve_sched = 1;          // 1 dependency-driven solve above the split level,
+                                // 0 level-by-level launches throughout
+  int solve_split = -1;         // first level of the dependency-driven launch (< 0: automatic)
 };
 
 struct LevelPlan {
@@ -77,7 +78,9 @@ struct Plan {
   std::vector<Int2> dbwd_items;  // (s, -1 - q): off-diagonal terms of own columns 16q..16q+15,
                                  // then (s, c): own 64-column blocks, last first; levels descending
   std::vector<int32_t> anc_ptr, anc_idx;  // distinct supernodes owning s's off-diagonal rows
-  double tri_total = 0, off_total = 0;    // own-block lower entries, off-diagonal entries
+  int32_t solve_split = 0;                // levels below run as launches, from it up as units
+  std::vector<int32_t> fu_init;           // off-diagonal chunk counter at solve start (full below the split)
+  double dep_tri = 0, dep_off = 0;        // own-block lower / off-diagonal entries from the split up
   int64_t panel_total = 0;      // elements in the panel pool
   // user lower-CSC entries -> panel slot
   int64_t nnz = 0;

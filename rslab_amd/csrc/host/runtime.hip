@@ -47,7 +47,7 @@ template <class T> void launch_fwd_update(const DevSym&, const DevFac<T>&, const
 template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, const uint8_t*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
-void launch_sync_clear(int32_t*, int64_t, hipStream_t);
+void launch_sync_init(int32_t*, const int32_t*, int32_t, hipStream_t);
 template <class T> void launch_fwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_expand(int64_t, const int64_t*, const T*, T*, hipStream_t);
@@ -210,6 +210,7 @@ void DeviceSymbolic::upload() {
   nb_eff.upload(P.nb_eff, stream);
   dfwd_items.upload(P.dfwd_items, stream);
   dbwd_items.upload(P.dbwd_items, stream);
+  fu_init.upload(P.fu_init, stream);
   anc_ptr.upload(P.anc_ptr, stream);
   anc_idx.upload(P.anc_idx, stream);
   hip_check(hipStreamSynchronize(stream), "plan upload");
@@ -789,25 +790,8 @@ class DeviceFactor final : public DeviceFactorBase {
     const int2* gi = sym->gat_items.as<int2>();
     launch_gather_rhs<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(),
                          static_cast<const T*>(d_b), y, st, pf);
-    // Dependency-driven sweeps for solves of up to 8 right-hand sides (one
-    // chunk per unit: a unit's block chain is walked once); wider solves use
-    // the level launches, which stream every chunk through each launch.
-    if (P.params.solve_sched == 1 && nrhs <= 8) {
-      // one launch per sweep; the units order themselves through counters
-      const double sweep_bytes = ts * (P.tri_total + P.off_total + 2.0 * P.n * nrhs);
-      launch_sync_clear(solve_sync.as<int32_t>(), kSyncStride * std::max<int64_t>(P.ns, 1) + 3, st);
-      launch_fwd_dep<T>(S, fv, sym->dfwd_items.as<int2>(), (int)P.dfwd_items.size(), sy, (int32_t)nrhs, y, u,
-                        sweep_bytes, st, pf);
-      launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
-      launch_bwd_dep<T>(S, fv, sym->dbwd_items.as<int2>(), (int)P.dbwd_items.size(), sy, (int32_t)nrhs, y,
-                        sweep_bytes, st, pf);
-      launch_scatter_sol<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(), y,
-                            static_cast<T*>(d_x), st, pf);
-      hip_check(hipGetLastError(), "solve launch");
-      return;
-    }
     const int32_t* lists = sym->level_lists.as<int32_t>();
-    for (size_t L = 0; L < P.levels.size(); ++L) {
+    auto fwd_level = [&](size_t L) {
       const LevelPlan& lv = P.levels[L];
       launch_fgather<T>(S, fv, gi + lv.gat_beg, (int)(lv.gat_end - lv.gat_beg), (int32_t)nrhs, y, u, st, pf);
       launch_fwd_tri<T>(S, fv, lists + sym->level_snarrow_off[L], (int)lv.snarrow.size(), (int32_t)nrhs, y,
@@ -816,9 +800,8 @@ class DeviceFactor final : public DeviceFactorBase {
       launch_zcopy<T>(S, zi + lv.z_beg, (int)(lv.z_end - lv.z_beg), (int32_t)nrhs, z, y, st);
       launch_fwd_update<T>(S, fv, fi + lv.fwd_beg, (int)(lv.fwd_end - lv.fwd_beg), (int32_t)nrhs,
                            y, u, ts * lv.off_entries, st, pf);
-    }
-    launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
-    for (size_t L = P.levels.size(); L-- > 0;) {
+    };
+    auto bwd_level = [&](size_t L) {
       const LevelPlan& lv = P.levels[L];
       launch_bwd_update<T>(S, fv, bi + lv.bwd_beg, (int)(lv.bwd_end - lv.bwd_beg), (int32_t)nrhs, y,
                            ts * lv.off_entries, st, pf);
@@ -826,7 +809,28 @@ class DeviceFactor final : public DeviceFactorBase {
                         ts * (lv.tri_entries + 2.0 * lv.ncols * nrhs), st, pf);
       for (const SolveStep& stp : lv.bsteps) launch_btri_step<T>(S, fv, bi2, stp, (int32_t)nrhs, y, z, st, pf);
       launch_zcopy<T>(S, zi + lv.z_beg, (int)(lv.z_end - lv.z_beg), (int32_t)nrhs, z, y, st);
+    };
+    // Solves of up to 8 right-hand sides: the levels below the split run one
+    // launch per level and block step (many small supernodes, bandwidth
+    // work), the levels from the split up run as one dependency-driven launch
+    // per sweep (few wide supernodes, chains of dependent blocks). Wider
+    // solves run every level as launches, which stream each chunk of
+    // right-hand sides through one launch instead of walking every chain once
+    // per chunk.
+    const size_t split = (P.params.solve_sched == 1 && nrhs <= 8) ? (size_t)P.solve_split : P.levels.size();
+    for (size_t L = 0; L < split; ++L) fwd_level(L);
+    if (split < P.levels.size()) {
+      const double sweep_bytes = ts * (P.dep_tri + P.dep_off + 2.0 * P.n * nrhs);
+      launch_sync_init(solve_sync.as<int32_t>(), sym->fu_init.as<int32_t>(), P.ns, st);
+      launch_fwd_dep<T>(S, fv, sym->dfwd_items.as<int2>(), (int)P.dfwd_items.size(), sy, (int32_t)nrhs, y, u,
+                        sweep_bytes, st, pf);
+      launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
+      launch_bwd_dep<T>(S, fv, sym->dbwd_items.as<int2>(), (int)P.dbwd_items.size(), sy, (int32_t)nrhs, y,
+                        sweep_bytes, st, pf);
+    } else {
+      launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
     }
+    for (size_t L = split; L-- > 0;) bwd_level(L);
     launch_scatter_sol<T>(P.n, (int32_t)nrhs, fv.perm_e, scale_d.as<double>(), y,
                           static_cast<T*>(d_x), st, pf);
     hip_check(hipGetLastError(), "solve launch");

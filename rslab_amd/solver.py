@@ -235,6 +235,8 @@ class SolverSettings:
     # by device counters (False, and wider solves: one launch per level and
     # block step; tuning knob)
     solve_dep: bool = True
+    # first tree level of the dependency-driven launch (None: automatic)
+    solve_split: Optional[int] = None
 
     # builders (multifrontal_ldlt.rs:616-779)
     def with_method(self, m):
@@ -306,6 +308,7 @@ class SolverSettings:
             raise InvalidInput("FactorMethod.Multifrontal is not provided; use LeftLooking or RightLooking")
         s.method = int(self.method)
         s.solve_dep = 1 if self.solve_dep else 0
+        s.solve_split = -1 if self.solve_split is None else int(self.solve_split)
         return s
 
 

@@ -43,8 +43,12 @@ CASES = {
 
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("nrhs", [1, 5, 8, 13])
-def test_dependency_solve_equals_level_solve_and_oracle(case, nrhs):
+@pytest.mark.parametrize("split", [None, 0, 2])
+def test_dependency_solve_equals_level_solve_and_oracle(case, nrhs, split):
+    """split: the first tree level of the dependency-driven launch (None:
+    automatic; 0: every level; 2: the two lowest levels as launches)."""
     a, st = CASES[case]()
+    st = dataclasses.replace(st, solve_split=split)
     sym, fg, fo, s = factor_pair(a, st)
     sym0 = sym.replan(dataclasses.replace(st, solve_dep=False))
     f0 = sym0.factor(a)
