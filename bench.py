@@ -316,7 +316,18 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the config-5 refactor sweep line")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default plan knobs)")
     ap.add_argument("--no-complex", action="store_true", help="skip the config-2 complex line")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 256 MiB float64 device buffer (the HBM counters' calibration run)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8}), flush=True)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
