@@ -547,7 +547,13 @@ def test_batch_factor_equals_per_system_factors():
         if np.abs(lam).min() > 1e-8:
             assert ib.negative == int((lam < 0).sum())
         xi = fi.solve(bb)
-        assert np.abs(x[off:off + a.n] - xi).max() <= 1e-12 * max(np.abs(xi).max(), 1.0)
+        # the batch plan and a system's own plan split the solve between level
+        # launches and the dependency-driven launch at different levels, so the
+        # two solves sum in different groupings: equal to the conditioning
+        # (the shifts sit next to eigenvalues), and equally backward stable
+        xs = x[off:off + a.n]
+        assert np.abs(xs - xi).max() <= 1e-10 * max(np.abs(xi).max(), 1.0)
+        assert O.backward_error(a, xs, bb)[0] <= 10 * max(O.backward_error(a, xi, bb)[0], 1e-15)
         off += a.n
     assert sum(i.total() for i in inert) == big.n
 
