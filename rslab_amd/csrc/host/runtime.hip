@@ -409,6 +409,10 @@ class DeviceFactor final : public DeviceFactorBase {
   }
   hipEvent_t ev_caller = nullptr, ev_done = nullptr;
   bool ev_ready = false;
+  // a level's one-workgroup cdiv runs on the side stream beside its wide
+  // panel steps (independent supernodes), joined before the next level
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
   void* noalloc_base = nullptr;  // device address of the fixed scratch on this factor's device
   DevFac<T> fv{};
@@ -421,6 +425,9 @@ class DeviceFactor final : public DeviceFactorBase {
       (void)hipEventDestroy(ev_caller);
       (void)hipEventDestroy(ev_done);
     }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
   }
 
   DeviceFactor(std::shared_ptr<DeviceSymbolic> s, const FactorSettings& fs) {
@@ -477,6 +484,9 @@ class DeviceFactor final : public DeviceFactorBase {
     hip_check(hipMemset(solve_sync.p, 0, solve_sync.bytes), "memset");
     sy.cnt = solve_sync.as<int32_t>();
     sy.ctl = reinterpret_cast<uint32_t*>(sy.cnt + kSyncStride * std::max<int64_t>(P.ns, 1));
+    hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
     // set-up for the calls that must not allocate later: the ordering events
     // and the fixed scratch's address on this (current) device
     make_events();
@@ -711,10 +721,21 @@ class DeviceFactor final : public DeviceFactorBase {
     for (size_t L = 0; L < P.levels.size(); ++L) {
       const LevelPlan& lv = P.levels[L];
       if (!rl) cmod_level(lv);
+      // the level's small and wide supernodes are independent: with both
+      // present, the one-workgroup cdiv runs on the side stream meanwhile
+      const bool fork = !lv.small.empty() && !lv.wide.empty();
+      if (fork) {
+        hip_check(hipEventRecord(ev_fork, st), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(side, ev_fork, 0), "hipStreamWaitEvent");
+      }
       launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
-                           lv.small_flops, prec, st, pf);
+                           lv.small_flops, prec, fork ? side : st, pf);
       launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
                            lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, st, pf);
+      if (fork) {
+        hip_check(hipEventRecord(ev_join, side), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(st, ev_join, 0), "hipStreamWaitEvent");
+      }
       if constexpr (std::is_same<T, double>::value) {
         if (c_cap > 0 && L + 1 < P.levels.size()) {
           int maxw = 0;
