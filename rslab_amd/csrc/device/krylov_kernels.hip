@@ -84,14 +84,15 @@ __global__ void k_axpy_cols(int32_t n, int32_t k, const T* __restrict__ a, const
 }
 
 // y[:, c] = s[c] * x[:, c] for active columns (s real, device memory); y may
-// be x
+// be x. s[c] == 0 writes zeros without reading x (clearing a block must not
+// keep a NaN or Inf of memory not yet written)
 template <class T>
 __global__ void k_scal_cols(int32_t n, int32_t k, const double* __restrict__ s, const uint8_t* __restrict__ act,
                             const T* x, T* y) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)n * k) return;
   const int c = (int)(e % k);
-  if (act[c]) y[e] = F<T>::scale(x[e], s[c]);
+  if (act[c]) y[e] = s[c] == 0.0 ? F<T>::zero() : F<T>::scale(x[e], s[c]);
 }
 
 // x[:, c] += sum_{q < jd[c]} Z_q[:, c] y[q, c] for active columns; the basis
