@@ -99,10 +99,6 @@ typedef struct {
                              levels below run one launch per level; < 0
                              (default): the first level with at most 128
                              supernodes (tuning knob, a plan setting) */
-  int32_t streams;        /* 1 (default): the factor's two largest independent
-                             subtree groups step through their levels on
-                             separate streams (left-looking); 0: one
-                             sequence (tuning knob, a plan setting) */
 } rsl_settings;
 
 typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */

@@ -43,7 +43,6 @@ class Settings(C.Structure):
         ("method", C.c_int32),
         ("solve_dep", C.c_int32),
         ("solve_split", C.c_int32),
-        ("streams", C.c_int32),
     ]
 
 

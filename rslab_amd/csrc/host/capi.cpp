@@ -86,7 +86,6 @@ rsl::PlanParams plan_params(const rsl_settings& st) {
   pp.schedule = st.method;
   pp.solve_sched = st.solve_dep != 0 ? 1 : 0;
   pp.solve_split = st.solve_split;
-  pp.groups = st.streams != 0 ? 1 : 0;
   return pp;
 }
 }  // namespace
@@ -114,7 +113,6 @@ void rsl_settings_default(rsl_settings* s) {
   s->method = RSL_METHOD_LEFT_LOOKING;
   s->solve_dep = 1;
   s->solve_split = -1;
-  s->streams = 1;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }

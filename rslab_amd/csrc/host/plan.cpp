@@ -58,64 +58,9 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   P.upd_idx.reserve(P.upd_ptr[ns]);
   for (int32_t s = 0; s < ns; ++s) P.upd_idx.insert(P.upd_idx.end(), ul[s].begin(), ul[s].end());
 
-  // Factor concurrency groups (left-looking only: a right-looking push from
-  // either group could meet the other's in a shared ancestor): the children
-  // of the first supernode with two or more (walking down from a single
-  // root), or the roots of a forest, binned by subtree work into two groups
-  // that step through their levels independently; the supernodes above them
-  // form the top group, factored after both.
-  P.sn_group.assign(ns, 0);
-  P.ngroups = 1;
-  if (prm.schedule == 0 && prm.groups && ns > 1) {
-    std::vector<int32_t> par(ns, -1);
-    std::vector<int64_t> size(ns, 1);
-    std::vector<double> work(ns, 0.0);
-    for (int32_t s = 0; s < ns; ++s) {
-      const double h = P.sn_nrow[s], w = P.sn_ncol[s];
-      work[s] += h * h * w;
-      for (idx c : S.snodes[s].children) {
-        par[c] = s;
-        size[s] += size[c];
-        work[s] += work[c];
-      }
-    }
-    std::vector<int32_t> heads, top;
-    for (int32_t s = 0; s < ns; ++s)
-      if (par[s] < 0) heads.push_back(s);
-    if (heads.size() == 1) {
-      int32_t s = heads[0];
-      heads.clear();
-      while (S.snodes[s].children.size() == 1) {
-        top.push_back(s);
-        s = (int32_t)S.snodes[s].children[0];
-      }
-      top.push_back(s);
-      for (idx c : S.snodes[s].children) heads.push_back((int32_t)c);
-    }
-    if (heads.size() >= 2) {
-      std::stable_sort(heads.begin(), heads.end(), [&](int32_t a, int32_t b) { return work[a] > work[b]; });
-      double bin[2] = {0.0, 0.0};
-      for (int32_t hd : heads) {
-        const int g = bin[1] < bin[0] ? 1 : 0;
-        bin[g] += work[hd];
-        // postorder: the subtree of hd is the index range [hd - size + 1, hd]
-        for (int64_t s = hd - size[hd] + 1; s <= hd; ++s) P.sn_group[s] = (int8_t)g;
-      }
-      for (int32_t s : top) P.sn_group[s] = 2;
-      P.ngroups = NGROUP;
-    }
-  }
-
   // levels; panels laid out level by level so a level's panels are contiguous
   P.levels.assign(ns ? maxlev + 1 : 0, LevelPlan());
   for (int32_t s = 0; s < ns; ++s) P.levels[P.level[s]].all.push_back(s);
-  for (LevelPlan& L : P.levels) {
-    std::stable_sort(L.all.begin(), L.all.end(),
-                     [&](int32_t a, int32_t b) { return P.sn_group[a] < P.sn_group[b]; });
-    for (int g = 0; g < NGROUP; ++g) L.gall[g + 1] = L.gall[g];
-    for (int32_t s : L.all)
-      for (int g = P.sn_group[s] + 1; g <= NGROUP; ++g) L.gall[g]++;
-  }
   P.panel_off.assign(ns, 0);
   int64_t poff = 0;
   P.factor_flops = P.geom_flops = 0;
@@ -127,10 +72,8 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
       poff += h * w;
       if ((double)h * w * w > prm.wide_flops && w > 32) {
         L.wide.push_back(s);
-        for (int g = P.sn_group[s] + 1; g <= NGROUP; ++g) L.gwide[g]++;
       } else {
         L.small.push_back(s);
-        for (int g = P.sn_group[s] + 1; g <= NGROUP; ++g) L.gsmall[g]++;
         for (int64_t k = 0; k < w; ++k) L.small_flops += (double)(h - k) * (h - k) - (double)(h - w) * (h - w);
       }
       L.tri_entries += 0.5 * (double)w * (w + 1);
@@ -188,21 +131,9 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     LevelPlan& L = P.levels[l];
     L.item_beg = (int64_t)P.items.size();
     L.ptile_beg = (int64_t)P.ptiles.size();
-    int64_t coff = 0;  // product blocks of the level's group, one after the other
-    int cg = 0;        // group of the targets being laid out (targets ascend by group)
-    L.gitem[0] = L.item_beg;
-    L.gptile[0] = L.ptile_beg;
-    auto close_groups = [&](int upto) {  // groups cg .. upto-1 end here
-      for (; cg < upto; ++cg) {
-        L.gcel[cg] = coff;
-        coff = 0;
-        L.gitem[cg + 1] = (int64_t)P.items.size();
-        L.gptile[cg + 1] = (int64_t)P.ptiles.size();
-      }
-    };
+    int64_t coff = 0;  // product blocks of the level, one after the other
     for (const auto& grp : groups[l]) {
       const int32_t s = grp.first;
-      if (P.ngroups > 1) close_groups(P.sn_group[s]);
       const int32_t h = P.sn_nrow[s], w = P.sn_ncol[s], first = P.sn_first[s];
       const int32_t* rss = P.rs_idx.data() + P.rs_off[s];
       for (int32_t i = 0; i < h; ++i) gloc[rss[i]] = i;
@@ -252,7 +183,6 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
             double pairs = 0;
             for (int32_t cc = c.lo; cc < c.hi; ++cc) pairs += std::max(0, r.hi - std::max(a0, cc));
             L.cmod_flops += 2.0 * nck * pairs;
-            L.gcmod_flops[P.ngroups > 1 ? P.sn_group[s] : 0] += 2.0 * nck * pairs;
           }
       }
       for (int32_t rb = 0; rb < nrt; ++rb)
@@ -270,86 +200,58 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
         }
       for (int32_t i = 0; i < h; ++i) gloc[rss[i]] = -1;
     }
-    close_groups(NGROUP);
     L.item_end = (int64_t)P.items.size();
     L.ptile_end = (int64_t)P.ptiles.size();
     if (L.ptile_end - L.ptile_beg > INT32_MAX) throw std::runtime_error("plan: too many cmod product tiles");
-    L.c_elems = 0;
-    for (int g = 0; g < NGROUP; ++g) L.c_elems = std::max(L.c_elems, L.gcel[g]);
-  }
-  // each group's product blocks get their own region of the buffer (the
-  // groups' levels run at the same time); offsets were laid out from 0 per
-  // (level, group) and are moved to the group's region here
-  {
-    int64_t gmax[NGROUP] = {};
-    for (const LevelPlan& L : P.levels)
-      for (int g = 0; g < NGROUP; ++g) gmax[g] = std::max(gmax[g], L.gcel[g]);
-    int64_t acc = 0;
-    for (int g = 0; g < NGROUP; ++g) {
-      P.gbase[g] = acc;
-      acc += gmax[g];
-    }
-    P.c_elems_max = acc;
-    for (LevelPlan& L : P.levels)
-      for (int g = 1; g < NGROUP; ++g) {
-        if (P.gbase[g] == 0) continue;
-        for (int64_t t = L.gptile[g]; t < L.gptile[g + 1]; ++t) P.ptiles[t].c_off += P.gbase[g];
-        for (int64_t i = L.gitem[g]; i < L.gitem[g + 1]; ++i)
-          for (int32_t t = P.items[i].t_beg; t < P.items[i].t_end; ++t) P.tasks[t].cbase += P.gbase[g];
-      }
+    L.c_elems = coff;
+    P.c_elems_max = std::max(P.c_elems_max, coff);
   }
   if ((int64_t)P.tasks.size() > INT32_MAX) throw std::runtime_error("plan: too many cmod tasks");
 
-  // batched wide path: per level and group, one step per panel index
+  // batched wide path: per level, one step per panel index
   P.witems.clear();
   for (LevelPlan& L : P.levels) {
     L.wsteps.clear();
-    for (int g = 0; g < NGROUP; ++g) {
-      std::vector<WideStep>& steps = L.gsteps[g];
-      steps.clear();
-      const std::vector<int32_t> wide(L.wide.begin() + L.gwide[g], L.wide.begin() + L.gwide[g + 1]);
-      int maxp = 0;
-      for (int32_t s : wide) maxp = std::max(maxp, (P.sn_ncol[s] + P.nb_eff[s] - 1) / P.nb_eff[s]);
-      for (int p = 0; p < maxp; ++p) {
-        WideStep ws;
-        ws.g_beg = (int64_t)P.witems.size();
-        for (int32_t s : wide) {
-          const int32_t kb = p * P.nb_eff[s];
-          if (kb < P.sn_ncol[s]) {
-            P.witems.push_back({s, kb, 0, 0});
-            ws.max_kb = std::max(ws.max_kb, kb);
-            const double pw = std::min(P.nb_eff[s], P.sn_ncol[s] - kb);
-            ws.getf2_flops += pw * pw * pw / 3.0;
-          }
+    int maxp = 0;
+    for (int32_t s : L.wide) maxp = std::max(maxp, (P.sn_ncol[s] + P.nb_eff[s] - 1) / P.nb_eff[s]);
+    for (int p = 0; p < maxp; ++p) {
+      WideStep ws;
+      ws.g_beg = (int64_t)P.witems.size();
+      for (int32_t s : L.wide) {
+        const int32_t kb = p * P.nb_eff[s];
+        if (kb < P.sn_ncol[s]) {
+          P.witems.push_back({s, kb, 0, 0});
+          ws.max_kb = std::max(ws.max_kb, kb);
+          const double pw = std::min(P.nb_eff[s], P.sn_ncol[s] - kb);
+          ws.getf2_flops += pw * pw * pw / 3.0;
         }
-        ws.g_end = ws.r_beg = (int64_t)P.witems.size();
-        for (int32_t s : wide) {
-          const int32_t kb = p * P.nb_eff[s];
-          if (kb >= P.sn_ncol[s]) continue;
-          const int32_t ke = std::min(kb + P.nb_eff[s], P.sn_ncol[s]);
-          const int64_t mt = (int64_t)P.sn_nrow[s] - ke;
-          for (int64_t r0 = 0; r0 < mt; r0 += REPLAY_ROWS) P.witems.push_back({s, kb, (int32_t)r0, 0});
-          const double pw = ke - kb;
-          ws.replay_flops += (double)mt * pw * (pw - 1);
-        }
-        ws.r_end = ws.t_beg = (int64_t)P.witems.size();
-        for (int32_t s : wide) {
-          const int32_t kb = p * P.nb_eff[s];
-          if (kb >= P.sn_ncol[s]) continue;
-          const int32_t ke = std::min(kb + P.nb_eff[s], P.sn_ncol[s]);
-          const int64_t mt = (int64_t)P.sn_nrow[s] - ke;
-          const int32_t cw = P.sn_ncol[s] - ke;
-          if (cw <= 0 || mt <= 0) continue;
-          const int32_t nrt = (int32_t)((mt + TILE - 1) / TILE), nct = (cw + TILE - 1) / TILE;
-          for (int32_t cb = 0; cb < nct; ++cb)
-            for (int32_t rb = cb; rb < nrt; ++rb) P.witems.push_back({s, kb, rb, cb});
-          const double pw = ke - kb;
-          ws.schur_flops += 2.0 * pw * ((double)cw * mt - 0.5 * (double)cw * (cw - 1));
-        }
-        ws.t_end = (int64_t)P.witems.size();
-        steps.push_back(ws);
       }
-      L.wsteps.insert(L.wsteps.end(), steps.begin(), steps.end());
+      ws.g_end = ws.r_beg = (int64_t)P.witems.size();
+      for (int32_t s : L.wide) {
+        const int32_t kb = p * P.nb_eff[s];
+        if (kb >= P.sn_ncol[s]) continue;
+        const int32_t ke = std::min(kb + P.nb_eff[s], P.sn_ncol[s]);
+        const int64_t mt = (int64_t)P.sn_nrow[s] - ke;
+        for (int64_t r0 = 0; r0 < mt; r0 += REPLAY_ROWS) P.witems.push_back({s, kb, (int32_t)r0, 0});
+        const double pw = ke - kb;
+        ws.replay_flops += (double)mt * pw * (pw - 1);
+      }
+      ws.r_end = ws.t_beg = (int64_t)P.witems.size();
+      for (int32_t s : L.wide) {
+        const int32_t kb = p * P.nb_eff[s];
+        if (kb >= P.sn_ncol[s]) continue;
+        const int32_t ke = std::min(kb + P.nb_eff[s], P.sn_ncol[s]);
+        const int64_t mt = (int64_t)P.sn_nrow[s] - ke;
+        const int32_t cw = P.sn_ncol[s] - ke;
+        if (cw <= 0 || mt <= 0) continue;
+        const int32_t nrt = (int32_t)((mt + TILE - 1) / TILE), nct = (cw + TILE - 1) / TILE;
+        for (int32_t cb = 0; cb < nct; ++cb)
+          for (int32_t rb = cb; rb < nrt; ++rb) P.witems.push_back({s, kb, rb, cb});
+        const double pw = ke - kb;
+        ws.schur_flops += 2.0 * pw * ((double)cw * mt - 0.5 * (double)cw * (cw - 1));
+      }
+      ws.t_end = (int64_t)P.witems.size();
+      L.wsteps.push_back(ws);
     }
   }
 

@@ -17,17 +17,10 @@ struct PlanParams {
   int solve_sched = 1;          // 1 dependency-driven solve above the split level,
                                 // 0 level-by-level launches throughout
   int solve_split = -1;         // first level of the dependency-driven launch (< 0: automatic)
-  int groups = 1;               // 1: the factor's levels run per subtree group on their own
-                                // streams (left-looking only), 0: one sequence
 };
 
-// The factor's concurrency groups: two bins of independent subtrees, each
-// stepping through its own levels on its own streams, then the supernodes
-// above them (the path from the root down to the first branching).
-constexpr int NGROUP = 3;
-
 struct LevelPlan {
-  std::vector<int32_t> all;     // supernodes of the level, ordered by group
+  std::vector<int32_t> all;     // supernodes of the level (solve)
   std::vector<int32_t> small;   // one workgroup per supernode (cdiv)
   std::vector<int32_t> wide;    // multi-launch panel path (cdiv)
   int64_t item_beg = 0, item_end = 0;  // cmod items of the level
@@ -47,13 +40,6 @@ struct LevelPlan {
   double off_entries = 0;  // off-diagonal entries sum (h-w) w (solve)
   int64_t ncols = 0;       // columns eliminated in the level
   std::vector<WideStep> wsteps;
-  // per factor group g: all / small / wide list entries [g[i], g[i+1]),
-  // cmod items and product tiles, product-buffer elements, wide steps
-  int32_t gall[NGROUP + 1] = {}, gsmall[NGROUP + 1] = {}, gwide[NGROUP + 1] = {};
-  int64_t gitem[NGROUP + 1] = {}, gptile[NGROUP + 1] = {};
-  int64_t gcel[NGROUP] = {};
-  double gcmod_flops[NGROUP] = {};
-  std::vector<WideStep> gsteps[NGROUP];
 };
 
 struct Int2 {
@@ -72,9 +58,6 @@ struct Plan {
   std::vector<int32_t> rs_idx;
   std::vector<int32_t> upd_ptr, upd_idx;
   std::vector<int32_t> level;   // level of each supernode (leaves 0)
-  std::vector<int8_t> sn_group; // factor concurrency group of each supernode
-  int ngroups = 1;              // 1: every supernode in group 0; 3: groups 0, 1 and the top (2)
-  int64_t gbase[NGROUP] = {};   // product-buffer offset of each group's region
   std::vector<LevelPlan> levels;
   std::vector<CItem> items;
   std::vector<CTask> tasks;

@@ -413,9 +413,6 @@ class DeviceFactor final : public DeviceFactorBase {
   // panel steps (independent supernodes), joined before the next level
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  // the second factor group's streams (main and side) and its fork / join
-  hipStream_t g1 = nullptr, g1side = nullptr;
-  hipEvent_t ev_g1fork = nullptr, ev_g1join = nullptr, ev_g1sfork = nullptr, ev_g1sjoin = nullptr;
   int64_t c_cap = 0;  // product-buffer elements: levels needing more use the gathered cmod
   void* noalloc_base = nullptr;  // device address of the fixed scratch on this factor's device
   DevFac<T> fv{};
@@ -428,10 +425,9 @@ class DeviceFactor final : public DeviceFactorBase {
       (void)hipEventDestroy(ev_caller);
       (void)hipEventDestroy(ev_done);
     }
-    for (hipEvent_t e : {ev_fork, ev_join, ev_g1fork, ev_g1join, ev_g1sfork, ev_g1sjoin})
-      if (e) (void)hipEventDestroy(e);
-    for (hipStream_t q : {side, g1, g1side})
-      if (q) (void)hipStreamDestroy(q);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
   }
 
   DeviceFactor(std::shared_ptr<DeviceSymbolic> s, const FactorSettings& fs) {
@@ -488,10 +484,9 @@ class DeviceFactor final : public DeviceFactorBase {
     hip_check(hipMemset(solve_sync.p, 0, solve_sync.bytes), "memset");
     sy.cnt = solve_sync.as<int32_t>();
     sy.ctl = reinterpret_cast<uint32_t*>(sy.cnt + kSyncStride * std::max<int64_t>(P.ns, 1));
-    for (hipStream_t* q : {&side, &g1, &g1side})
-      hip_check(hipStreamCreateWithFlags(q, hipStreamNonBlocking), "hipStreamCreate");
-    for (hipEvent_t* e : {&ev_fork, &ev_join, &ev_g1fork, &ev_g1join, &ev_g1sfork, &ev_g1sjoin})
-      hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
     // set-up for the calls that must not allocate later: the ordering events
     // and the fixed scratch's address on this (current) device
     make_events();
@@ -709,64 +704,47 @@ class DeviceFactor final : public DeviceFactorBase {
     // cdiv; right-looking: after its cdiv, its supernodes' updates of every
     // ancestor (pushed), the same product blocks grouped by updater level
     const bool rl = P.params.schedule == 1;
-    // the part of level L in factor group g, on stream q (small cdiv forked to
-    // qs while the wide panel steps run on q, when both are present)
-    auto level_group = [&](size_t L, int g, hipStream_t q, hipStream_t qs, hipEvent_t efork, hipEvent_t ejoin) {
-      const LevelPlan& lv = P.levels[L];
-      auto cmod = [&]() {
-        const int64_t i0 = lv.gitem[g], i1 = lv.gitem[g + 1];
-        if (i1 == i0) return;
-        if constexpr (std::is_same<T, double>::value) {
-          if (P.gbase[g] + lv.gcel[g] <= c_cap) {
-            launch_cmod_two_phase(S, fv, sym->items.as<CItem>() + i0, (int)(i1 - i0), sym->tasks.as<CTask>(),
-                                  sym->ptiles.as<PTile>() + lv.gptile[g], lv.gptile[g + 1] - lv.gptile[g],
-                                  cbuf.p, lv.gcel[g], gpool.as<double>(), lv.gcmod_flops[g], prec, q, pf);
-            return;
-          }
+    auto cmod_level = [&](const LevelPlan& lv) {
+      if (lv.item_end == lv.item_beg) return;
+      if constexpr (std::is_same<T, double>::value) {
+        if (lv.c_elems <= c_cap) {
+          launch_cmod_two_phase(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
+                                sym->tasks.as<CTask>(), sym->ptiles.as<PTile>() + lv.ptile_beg,
+                                lv.ptile_end - lv.ptile_beg, cbuf.p, lv.c_elems, gpool.as<double>(), lv.cmod_flops,
+                                prec, st, pf);
+          return;
         }
-        launch_cmod<T>(S, fv, sym->items.as<CItem>() + i0, (int)(i1 - i0), sym->tasks.as<CTask>(),
-                       lv.gcmod_flops[g], prec, q, pf);
-      };
-      if (!rl) cmod();
-      const int nsm = lv.gsmall[g + 1] - lv.gsmall[g], nwd = lv.gwide[g + 1] - lv.gwide[g];
-      const bool fork = nsm > 0 && nwd > 0;
-      if (fork) {
-        hip_check(hipEventRecord(efork, q), "hipEventRecord");
-        hip_check(hipStreamWaitEvent(qs, efork, 0), "hipStreamWaitEvent");
       }
-      launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L] + lv.gsmall[g], nsm, nb, pol, lv.small_flops,
-                           prec, fork ? qs : q, pf);
-      launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.gsteps[g].data(), (int)lv.gsteps[g].size(),
-                           lists + sym->level_wide_off[L] + lv.gwide[g], nwd, pol, prec, q, pf);
+      launch_cmod<T>(S, fv, sym->items.as<CItem>() + lv.item_beg, (int)(lv.item_end - lv.item_beg),
+                     sym->tasks.as<CTask>(), lv.cmod_flops, prec, st, pf);
+    };
+    for (size_t L = 0; L < P.levels.size(); ++L) {
+      const LevelPlan& lv = P.levels[L];
+      if (!rl) cmod_level(lv);
+      // the level's small and wide supernodes are independent: with both
+      // present, the one-workgroup cdiv runs on the side stream meanwhile
+      const bool fork = !lv.small.empty() && !lv.wide.empty();
       if (fork) {
-        hip_check(hipEventRecord(ejoin, qs), "hipEventRecord");
-        hip_check(hipStreamWaitEvent(q, ejoin, 0), "hipStreamWaitEvent");
+        hip_check(hipEventRecord(ev_fork, st), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(side, ev_fork, 0), "hipStreamWaitEvent");
+      }
+      launch_cdiv_small<T>(S, fv, lists + sym->level_small_off[L], (int)lv.small.size(), nb, pol,
+                           lv.small_flops, prec, fork ? side : st, pf);
+      launch_wide_level<T>(S, fv, sym->witems.as<WItem>(), lv.wsteps.data(), (int)lv.wsteps.size(),
+                           lists + sym->level_wide_off[L], (int)lv.wide.size(), pol, prec, st, pf);
+      if (fork) {
+        hip_check(hipEventRecord(ev_join, side), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(st, ev_join, 0), "hipStreamWaitEvent");
       }
       if constexpr (std::is_same<T, double>::value) {
-        const int na = lv.gall[g + 1] - lv.gall[g];
-        if (c_cap > 0 && L + 1 < P.levels.size() && na > 0) {
+        if (c_cap > 0 && L + 1 < P.levels.size()) {
           int maxw = 0;
-          for (int i = lv.gall[g]; i < lv.gall[g + 1]; ++i) maxw = std::max(maxw, P.sn_ncol[lv.all[i]]);
-          ProfScope ps(pf, LK_MAKE_G, 0.0, 0.0, q);
-          launch_make_g(S, fv, lists + sym->level_all_off[L] + lv.gall[g], na, maxw, gpool.as<double>(), q);
+          for (int32_t sn : lv.all) maxw = std::max(maxw, P.sn_ncol[sn]);
+          ProfScope ps(pf, LK_MAKE_G, 0.0, 0.0, st);
+          launch_make_g(S, fv, lists + sym->level_all_off[L], (int)lv.all.size(), maxw, gpool.as<double>(), st);
         }
       }
-      if (rl) cmod();
-    };
-    if (P.ngroups == 1) {
-      for (size_t L = 0; L < P.levels.size(); ++L) level_group(L, 0, st, side, ev_fork, ev_join);
-    } else {
-      // groups 0 and 1 step through their levels at the same time, group 1 on
-      // its own streams; the top group follows once both are done
-      hip_check(hipEventRecord(ev_g1fork, st), "hipEventRecord");
-      hip_check(hipStreamWaitEvent(g1, ev_g1fork, 0), "hipStreamWaitEvent");
-      for (size_t L = 0; L < P.levels.size(); ++L) {
-        level_group(L, 0, st, side, ev_fork, ev_join);
-        level_group(L, 1, g1, g1side, ev_g1sfork, ev_g1sjoin);
-      }
-      hip_check(hipEventRecord(ev_g1join, g1), "hipEventRecord");
-      hip_check(hipStreamWaitEvent(st, ev_g1join, 0), "hipStreamWaitEvent");
-      for (size_t L = 0; L < P.levels.size(); ++L) level_group(L, 2, st, side, ev_fork, ev_join);
+      if (rl) cmod_level(lv);
     }
     launch_rows_e<T>(S, fv, P.rs_off[P.ns], st);
     // drop tolerance at emit (ldlt_emit_and_free, multifrontal_ldlt.rs:2358-2366):

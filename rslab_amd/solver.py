@@ -237,8 +237,6 @@ class SolverSettings:
     solve_dep: bool = True
     # first tree level of the dependency-driven launch (None: automatic)
     solve_split: Optional[int] = None
-    # the factor's two largest independent subtree groups run on separate streams
-    streams: bool = True
 
     # builders (multifrontal_ldlt.rs:616-779)
     def with_method(self, m):
@@ -311,7 +309,6 @@ class SolverSettings:
         s.method = int(self.method)
         s.solve_dep = 1 if self.solve_dep else 0
         s.solve_split = -1 if self.solve_split is None else int(self.solve_split)
-        s.streams = 1 if self.streams else 0
         return s
 
 
