@@ -543,3 +543,162 @@ def laplacian2d_inertia(m: int, sigma: float):
     t = 2.0 - 2.0 * np.cos(np.arange(1, m + 1) * np.pi / (m + 1))
     lam = (t[:, None] + t[None, :]).ravel() - sigma
     return int((lam > 0).sum()), int((lam < 0).sum()), int((lam == 0).sum()), float(np.abs(lam).min())
+
+
+# ------------------------------------------------------------------ LU twin
+class OruFactors(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("cplx", C.c_int32), ("status", C.c_int32),
+        ("n_perturbed", C.c_int64), ("nnz_l", C.c_int64), ("nnz_u", C.c_int64),
+        ("l_col_ptr", C.POINTER(C.c_int64)), ("l_row_idx", C.POINTER(C.c_int32)),
+        ("l_values", C.POINTER(C.c_double)),
+        ("u_row_ptr", C.POINTER(C.c_int64)), ("u_col_idx", C.POINTER(C.c_int32)),
+        ("u_values", C.POINTER(C.c_double)),
+        ("perm", C.POINTER(C.c_int32)), ("perm_row", C.POINTER(C.c_int32)),
+        ("d_row", C.POINTER(C.c_double)), ("d_col", C.POINTER(C.c_double)),
+    ]
+
+
+_lib.oru_factor.restype = C.POINTER(OruFactors)
+_lib.oru_factor.argtypes = [C.POINTER(OraSym), C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                            C.POINTER(C.c_double), C.c_int, C.c_double, C.c_double, C.c_double]
+_lib.oru_free.argtypes = [C.POINTER(OruFactors)]
+_lib.oru_solve_many.restype = C.c_int
+_lib.oru_solve_many.argtypes = [C.POINTER(OruFactors), C.POINTER(C.c_double), C.c_int32,
+                                C.POINTER(C.c_double)]
+
+
+def symmetrized_lower(n, col_ptr, row_idx):
+    """Lower triangle of the pattern A + A^T as (col_ptr, row_idx)
+    (symmetrized_lower_pattern, multifrontal_lu.rs:364-409)."""
+    cols = np.repeat(np.arange(n, dtype=np.int64), np.diff(np.asarray(col_ptr, dtype=np.int64)))
+    rows = np.asarray(row_idx, dtype=np.int64)
+    hi, lo = np.maximum(rows, cols), np.minimum(rows, cols)
+    key = np.unique(lo * n + hi)
+    cp = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(cp, key // n + 1, 1)
+    return np.cumsum(cp), key % n
+
+
+def general_matvec(a, x):
+    """y = A x for a full (unsymmetric) CSC matrix `a` (GeneralCsc::matvec)."""
+    X = np.asarray(x)
+    cols = np.repeat(np.arange(a.n), np.diff(a.col_ptr))
+    vals = np.asarray(a.values)
+    y = np.zeros((a.n,) + X.shape[1:], dtype=np.result_type(vals, X))
+    contrib = vals[:, None] * X[cols] if X.ndim == 2 else vals * X[cols]
+    np.add.at(y, np.asarray(a.row_idx), contrib)
+    return y
+
+
+def general_norm_inf(a) -> float:
+    """max_i sum_j |A_ij| of a full CSC matrix."""
+    r = np.zeros(a.n)
+    np.add.at(r, np.asarray(a.row_idx), np.abs(np.asarray(a.values)))
+    return float(r.max()) if a.n else 0.0
+
+
+def lu_backward_error(a, x, b):
+    """Normwise backward error |b - A x|inf / (|A|inf |x|inf + |b|inf) per column."""
+    an = general_norm_inf(a)
+    X = np.asarray(x).reshape(a.n, -1)
+    B = np.asarray(b).reshape(a.n, -1)
+    R = B - general_matvec(a, X)
+    out = []
+    for c in range(X.shape[1]):
+        den = max(an * np.abs(X[:, c]).max() + np.abs(B[:, c]).max(), np.finfo(float).tiny)
+        out.append(float(np.abs(R[:, c]).max() / den))
+    return np.array(out)
+
+
+@dataclasses.dataclass
+class OracleLu:
+    """LuFactors (multifrontal_lu.rs:321-352): L CSC unit lower, U CSR upper,
+    perm / perm_row (position -> original column / row), d_row, d_col."""
+    n: int
+    l_col_ptr: np.ndarray
+    l_row_idx: np.ndarray
+    l_values: np.ndarray
+    u_row_ptr: np.ndarray
+    u_col_idx: np.ndarray
+    u_values: np.ndarray
+    perm: np.ndarray
+    perm_row: np.ndarray
+    d_row: np.ndarray
+    d_col: np.ndarray
+    n_perturbed: int
+    cplx: bool
+    _handle: object = None
+
+    def factor_nnz(self) -> int:
+        return int(self.l_values.size + self.u_values.size)
+
+    def solve_many(self, b):
+        """solve_lu_block on the unscaled system; b row-major n x k."""
+        b = np.asarray(b)
+        single = b.ndim == 1
+        B = np.ascontiguousarray(b.reshape(self.n, -1), dtype=np.complex128 if self.cplx else np.float64)
+        X = np.zeros_like(B)
+        bv = B.view(np.float64) if self.cplx else B
+        xv = X.view(np.float64) if self.cplx else X
+        if _lib.oru_solve_many(self._handle, _p(bv, C.c_double), B.shape[1], _p(xv, C.c_double)):
+            raise RankDeficient("numerically rank deficient")
+        return X.reshape(self.n) if single else X
+
+    def dense_lu(self):
+        """(L, U) as dense matrices in factorization order."""
+        n = self.n
+        L = np.zeros((n, n), dtype=self.l_values.dtype)
+        U = np.zeros((n, n), dtype=self.u_values.dtype)
+        for e in range(n):
+            s, t = self.l_col_ptr[e], self.l_col_ptr[e + 1]
+            L[self.l_row_idx[s:t], e] = self.l_values[s:t]
+            s, t = self.u_row_ptr[e], self.u_row_ptr[e + 1]
+            U[e, self.u_col_idx[s:t]] = self.u_values[s:t]
+        return L, U
+
+    def __del__(self):
+        if self._handle is not None:
+            _lib.oru_free(self._handle)
+            self._handle = None
+
+
+def lu_factor(sym, a, pivot=0, abs_floor=0.0, pivot_u=0.1, drop_tol=None) -> OracleLu:
+    """factor_general_lu_numeric, left-looking path, on the full matrix `a`
+    (unscaled) given the export of the symmetrized pattern's analysis."""
+    keep = dict(
+        perm=np.ascontiguousarray(sym.perm, dtype=np.int32),
+        perm_inv=np.ascontiguousarray(sym.perm_inv, dtype=np.int32),
+        sn_first=np.ascontiguousarray(sym.sn_first, dtype=np.int32),
+        sn_ncol=np.ascontiguousarray(sym.sn_ncol, dtype=np.int32),
+        ch_ptr=np.ascontiguousarray(sym.ch_ptr, dtype=np.int32),
+        ch_idx=np.ascontiguousarray(sym.ch_idx, dtype=np.int32),
+        pp_colp=np.ascontiguousarray(sym.pp_colp, dtype=np.int64),
+        pp_rowi=np.ascontiguousarray(sym.pp_rowi, dtype=np.int32),
+    )
+    s = OraSym(sym.n, len(sym.sn_first), _p(keep["perm"], C.c_int32), _p(keep["perm_inv"], C.c_int32),
+               _p(keep["sn_first"], C.c_int32), _p(keep["sn_ncol"], C.c_int32),
+               _p(keep["ch_ptr"], C.c_int32), _p(keep["ch_idx"], C.c_int32),
+               _p(keep["pp_colp"], C.c_int64), _p(keep["pp_rowi"], C.c_int32))
+    v, cp = _vals(a)
+    ri = np.ascontiguousarray(a.row_idx, dtype=np.int32)
+    cpt = np.ascontiguousarray(a.col_ptr, dtype=np.int64)
+    h = _lib.oru_factor(C.byref(s), cp, _p(cpt, C.c_int64), _p(ri, C.c_int32), _p(v, C.c_double),
+                        int(pivot), float(abs_floor), float(pivot_u), -1.0 if drop_tol is None else float(drop_tol))
+    f = h.contents
+    if f.status != 0:
+        _lib.oru_free(h)
+        raise RankDeficient("numerically rank deficient")
+    n, nd = f.n, (2 if cp else 1)
+
+    def arr(p, k, dt):
+        return np.ctypeslib.as_array(p, shape=(k,)).astype(dt, copy=True) if k else np.zeros(0, dt)
+
+    lv = arr(f.l_values, f.nnz_l * nd, np.float64)
+    uv = arr(f.u_values, f.nnz_u * nd, np.float64)
+    if cp:
+        lv, uv = lv.view(np.complex128), uv.view(np.complex128)
+    return OracleLu(n, arr(f.l_col_ptr, n + 1, np.int64), arr(f.l_row_idx, f.nnz_l, np.int32), lv,
+                    arr(f.u_row_ptr, n + 1, np.int64), arr(f.u_col_idx, f.nnz_u, np.int32), uv,
+                    arr(f.perm, n, np.int32), arr(f.perm_row, n, np.int32), arr(f.d_row, n, np.float64),
+                    arr(f.d_col, n, np.float64), int(f.n_perturbed), bool(cp), h)

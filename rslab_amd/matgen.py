@@ -161,3 +161,48 @@ def config5_omegas(count: int = 20):
 
 def config5_rhs(n: int, k: int, nrhs: int = 32):
     return rng_uniform(7 + k, -1.0, 1.0, n * nrhs).reshape(n, nrhs)
+
+
+# ------------------------------------------------------- unsymmetric (LU)
+def convection_diffusion(dims, velocity=(0.8, 0.4, 0.2), shift=0.0, delta=None, eta=None):
+    """Full CSC (GeneralCsc) of the 5/7-point diffusion + central-difference
+    convection operator on `dims`: for each grid edge (i, i + e_d) the
+    entries A[i + e_d, i] = -1 - v_d/2 and A[i, i + e_d] = -1 + v_d/2 (v_d the
+    cell Peclet number of direction d), diagonal 2*ndim + shift (+ delta)
+    (+ i*eta for a complex matrix). Unsymmetric values on a symmetric
+    pattern, indefinite for negative shifts."""
+    from .lu import GeneralCsc
+    dims = tuple(int(d) for d in dims)
+    n = int(np.prod(dims))
+    stride = [1]
+    for d in dims[:-1]:
+        stride.append(stride[-1] * d)
+    idx = np.arange(n, dtype=np.int64)
+    diag = np.full(n, 2.0 * len(dims) + shift)
+    if delta is not None:
+        diag = diag + np.asarray(delta, dtype=np.float64)
+    cplx = eta is not None
+    dt = np.complex128 if cplx else np.float64
+    rows, cols = [idx], [idx]
+    vals = [diag.astype(dt) + (1j * np.asarray(eta) if cplx else 0.0)]
+    for dd, st in enumerate(stride):
+        v = float(velocity[dd]) if dd < len(velocity) else 0.0
+        coord = (idx // st) % dims[dd]
+        m = coord + 1 < dims[dd]
+        lo = idx[m]
+        rows += [lo + st, lo]
+        cols += [lo, lo + st]
+        vals += [np.full(lo.size, -1.0 - 0.5 * v, dtype=dt), np.full(lo.size, -1.0 + 0.5 * v, dtype=dt)]
+    return GeneralCsc.from_triplets(n, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+
+
+def lu_workload(m: int = 60, nrhs: int = 8, cplx: bool = False):
+    """Unsymmetric workload of the LU twin: convection_diffusion on m^3 with
+    velocity (0.8, 0.4, 0.2), shift -1.5 and delta ~ U[-0.01, 0.01] from
+    Rng(3) (the grid, shift and perturbation of config 3), plus i*0.1 on the
+    diagonal when complex; B ~ U[-1, 1] from Rng(4), n x nrhs row-major."""
+    n = m ** 3
+    delta = rng_uniform(3, -0.01, 0.01, n)
+    a = convection_diffusion((m, m, m), shift=-1.5, delta=delta, eta=np.full(n, 0.1) if cplx else None)
+    B = rng_uniform(4, -1.0, 1.0, n * nrhs).reshape(n, nrhs)
+    return a, (B.astype(np.complex128) if cplx else B)
