@@ -99,6 +99,10 @@ typedef struct {
                              levels below run one launch per level; < 0
                              (default): the first level with at most 128
                              supernodes (tuning knob, a plan setting) */
+  double pivot_u;         /* LU twin's pivot threshold u (SolverSettings::
+                             pivot_u, multifrontal_ldlt.rs:382, default 0.1,
+                             clamped to [0, 1] as with_pivot_u :742-745): the
+                             diagonal is kept unless |a_kk| < u max |a_ik| */
 } rsl_settings;
 
 typedef struct rsl_symbolic rsl_symbolic; /* LdltSymbolic + device plan */
@@ -304,6 +308,60 @@ const char *rsl_last_error(void);
  * AMD stats[0..6] = ncmpa, ndense, lnz, ndiv, nms_lu, nms_ldl, dmax. */
 int rsl_order(int32_t method, int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
               uint64_t seed, int32_t *perm, double *stats);
+
+/* ---- LU twin (src/numeric/multifrontal_lu.rs): unsymmetric matrices given
+ * as the full CSC (GeneralCsc, src/sparse/general.rs; rows strictly
+ * increasing within each column). The analysis runs on the pattern A + A^T
+ * (symmetrized_lower_pattern :364-409) with the settings' ordering and
+ * amalgamation; the numeric factor is the supernodal left-looking LU with
+ * threshold partial pivoting restricted to each supernode's own rows
+ * (factor_lu_left_looking :2428-2671, lu_ll_factor_node :1629-2347) after the
+ * two-sided equilibration D_r A D_c (:2733-2758). RSL_METHOD_LEFT_LOOKING
+ * only; the multifrontal LU is not provided. */
+typedef struct rsl_lu_symbolic rsl_lu_symbolic; /* LuSymbolic + device plan */
+typedef struct rsl_lu_solver rsl_lu_solver;     /* LuSolver<T>: device factor */
+
+/* LuSymbolic::analyze_with (:850-869). */
+int rsl_lu_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, const rsl_settings *s,
+                   rsl_lu_symbolic **out);
+/* out[8] = {n, nnz, supernodes, tree levels, stored L entries, stored U
+ * entries (both with the diagonal, before dropping), cmod tasks, panel-step
+ * units}; flops[0] = sum over supernodes and own columns k of
+ * 2 (h-k-1)^2 + (h-k-1) (real arithmetic). Either pointer may be NULL. */
+int rsl_lu_symbolic_info(const rsl_lu_symbolic *sym, int64_t *out, double *flops);
+/* The analysis of A + A^T, as rsl_symbolic_export. */
+int rsl_lu_symbolic_export(const rsl_lu_symbolic *sym, int32_t *perm, int32_t *perm_inv, int32_t *sn_first,
+                           int32_t *sn_ncol, int32_t *sn_nrow, int32_t *ch_ptr, int32_t *ch_idx,
+                           int64_t *pp_colp, int32_t *pp_rowi);
+/* LuSymbolic::factor (:880-907) -> factor_general_lu_numeric (:2673-2800):
+ * values in the analysed pattern (interleaved for RSL_C64); s may be NULL;
+ * uses pivot (ZeroPivotAction), abs_floor, pivot_u, drop_tol (applied when
+ * the factors are emitted, :1562-1602) and method. The symbolic handle must
+ * outlive the factor. */
+int rsl_lu_factor(rsl_lu_symbolic *sym, int32_t field, const double *values, const rsl_settings *s,
+                  rsl_lu_solver **out);
+/* Same-pattern refactor in place (storage and plan reused). */
+int rsl_lu_refactor(rsl_lu_solver *f, const double *values);
+/* LuSolver::solve_many (:1212-1217) -> solve_lu_many (:3158-3207): b, x host
+ * buffers, row-major n x nrhs (interleaved for RSL_C64). */
+int rsl_lu_solve(rsl_lu_solver *f, const double *b, int64_t nrhs, double *x);
+/* solve_lu_many with b, x in device memory, ordered after the work enqueued
+ * on `stream` (hipStream_t; NULL the default stream), which then waits for
+ * the solve; returns without waiting. */
+int rsl_lu_solve_device(rsl_lu_solver *f, const void *d_b, int64_t nrhs, void *d_x, void *stream);
+/* out[6] = {n, n_perturbed (LuSolver::n_perturbed :1234), stored L entries,
+ * stored U entries (LuFactors, after drop_tol), field, 0}; t[0] = the last
+ * factor's wall time in ms (host to host). Either pointer may be NULL. */
+int rsl_lu_info(rsl_lu_solver *f, int64_t *out, double *t);
+/* LuFactors (:321-352): L CSC (n+1, nnz_l; unit diagonal first in each
+ * column), U CSR (n+1, nnz_u; the pivot first in each row), perm and
+ * perm_row (position -> original column / row), d_row, d_col; sizes from
+ * rsl_lu_info. Any pointer may be NULL. */
+int rsl_lu_factor_download(rsl_lu_solver *f, int64_t *l_col_ptr, int32_t *l_row_idx, double *l_values,
+                           int64_t *u_row_ptr, int32_t *u_col_idx, double *u_values, int32_t *perm,
+                           int32_t *perm_row, double *d_row, double *d_col);
+void rsl_lu_free_factor(rsl_lu_solver *f);
+void rsl_lu_free_symbolic(rsl_lu_symbolic *s);
 
 /* Synthetic inputs: `count` draws of rslab's xorshift64* Rng::range(lo, hi)
  * from Rng::new(seed) (src/matgen/mod.rs:63-96). */

@@ -43,6 +43,7 @@ class Settings(C.Structure):
         ("method", C.c_int32),
         ("solve_dep", C.c_int32),
         ("solve_split", C.c_int32),
+        ("pivot_u", C.c_double),
     ]
 
 
@@ -89,6 +90,17 @@ SIGNATURES = {
     "rsl_factor_load": (C.c_int, [_P, C.c_char_p, C.POINTER(_P)]),
     "rsl_profile_read": (C.c_int, [_P, C.c_int32, _DP]),
     "rsl_profile_kind_name": (C.c_char_p, [C.c_int32]),
+    "rsl_lu_analyze": (C.c_int, [C.c_int64, _I64P, _I64P, C.POINTER(Settings), C.POINTER(_P)]),
+    "rsl_lu_symbolic_info": (C.c_int, [_P, _I64P, _DP]),
+    "rsl_lu_symbolic_export": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _I64P, _I32P]),
+    "rsl_lu_factor": (C.c_int, [_P, C.c_int32, _DP, C.POINTER(Settings), C.POINTER(_P)]),
+    "rsl_lu_refactor": (C.c_int, [_P, _DP]),
+    "rsl_lu_solve": (C.c_int, [_P, _DP, C.c_int64, _DP]),
+    "rsl_lu_solve_device": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "rsl_lu_info": (C.c_int, [_P, _I64P, _DP]),
+    "rsl_lu_factor_download": (C.c_int, [_P, _I64P, _I32P, _DP, _I64P, _I32P, _DP, _I32P, _I32P, _DP, _DP]),
+    "rsl_lu_free_factor": (None, [_P]),
+    "rsl_lu_free_symbolic": (None, [_P]),
 }
 
 

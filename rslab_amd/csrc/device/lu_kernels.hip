@@ -1,0 +1,612 @@
+// Device kernels of the LU twin: supernodal left-looking LU with threshold
+// partial pivoting restricted to each supernode's own rows
+// (lu_ll_factor_node, src/numeric/multifrontal_lu.rs:1629-2347), and its
+// supernodal forward / backward solve (the arithmetic of solve_lu_block,
+// :3209-3283, grouped by supernode).
+//
+// Storage per supernode s (h rows, w own columns, rows in rs[s] order with the
+// own rows first): L panel h x w column-major (unit-lower L below the
+// diagonal, U11 on and above it) and U12 w x (h - w) column-major. The own
+// rows of the panel are interchanged by the pivoting (rperm records which
+// row-structure row sits at each position); the deep rows never are.
+//
+// Launch sequence of a tree level: cmod (every target tile of the level's
+// supernodes sums its updaters' products L_k[rows] U_k[:, cols]), then per
+// panel index: getf2 (one workgroup per supernode panel), apply (the panel's
+// interchanges on the other columns and the triangular solve of the later
+// own columns and of U12, the deep rows' transform), and the trailing GEMM.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "lu_types.hpp"
+#include "scalar.hpp"
+
+namespace rsl {
+
+namespace {
+
+template <class T>
+__device__ inline T lu_fma(T a, T b, T c) {
+  return c + a * b;
+}
+template <>
+__device__ inline double lu_fma<double>(double a, double b, double c) {
+  return fma(a, b, c);
+}
+
+// perturb_pivot (multifrontal_ldlt.rs:786-795)
+template <class T>
+__device__ inline T lu_perturb(T d, double floor) {
+  const double mag = F<T>::mag(d);
+  if (mag >= floor) return d;
+  if (mag == 0.0) return F<T>::from_real(floor);
+  return F<T>::scale(d, floor / mag);
+}
+
+// ---- 64x64 tile products on the vector units: thread (ty, tx) owns rows
+// 4 ty .. 4 ty + 3 and columns 4 tx .. 4 tx + 3; operands staged through LDS
+// 16 columns of K at a time.
+constexpr int KC = 16;
+constexpr int LDT = 64 + 4;
+
+template <class T>
+struct LuAcc {
+  T c[4][4];
+};
+
+template <class T>
+__device__ inline void lu_acc_zero(LuAcc<T>& a) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a.c[r][q] = F<T>::zero();
+}
+
+// acc += As(64 x kc) Bs(kc x 64) from the staged chunk
+template <class T>
+__device__ inline void lu_acc_chunk(const T (*As)[LDT], const T (*Bs)[LDT], int kc, LuAcc<T>& acc) {
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  for (int c = 0; c < kc; ++c) {
+    T a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[r] = As[c][ty * 4 + r];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = Bs[c][tx * 4 + q];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc.c[r][q] = lu_fma(a[r], b[q], acc.c[r][q]);
+  }
+}
+
+}  // namespace
+
+// Scatter of the equilibrated, permuted values into the zeroed L and U12
+// storage (the assembly of lu_ll_factor_node, multifrontal_lu.rs:1670-1686).
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_assemble(LuSymDev S, LuFacDev<T> Fc, const T* __restrict__ vals) {
+  const int64_t k = (int64_t)blockIdx.x * LU_NT + threadIdx.x;
+  if (k >= S.nnz) return;
+  const int64_t d = S.amap[k];
+  if (d >= 0) Fc.l[d] = vals[k];
+  else Fc.u[-1 - d] = vals[k];
+}
+
+// cmod: one workgroup per target tile; every updater's product (ascending)
+// is accumulated, the sum is subtracted once.
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_cmod(LuSymDev S, LuFacDev<T> Fc, const LuItem* __restrict__ items,
+                                                   const CTask* __restrict__ tasks) {
+  __shared__ T As[KC][LDT];
+  __shared__ T Bs[KC][LDT];
+  const int tid = threadIdx.x;
+  const LuItem it = items[blockIdx.x];
+  LuAcc<T> acc;
+  lu_acc_zero(acc);
+  for (int t = it.t_beg; t < it.t_end; ++t) {
+    const CTask tk = tasks[t];
+    const int k = tk.k, nck = S.sn_ncol[k];
+    const int64_t hk = S.sn_nrow[k];
+    const T* Lk = Fc.l + S.l_off[k];
+    const T* Uk = Fc.u + S.u_off[k];
+    for (int c0 = 0; c0 < nck; c0 += KC) {
+      const int kc = min(KC, nck - c0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = tid + LU_NT * u, r = e & 63, c = e >> 6;
+        const uint64_t below = (1ull << r) - 1ull;
+        T va = F<T>::zero(), vb = F<T>::zero();
+        if (c < kc) {
+          if ((tk.rmask >> r) & 1ull) va = Lk[(nck + tk.a0 + __popcll(tk.rmask & below)) + (int64_t)(c0 + c) * hk];
+          if ((tk.cmask >> r) & 1ull) vb = Uk[(c0 + c) + (int64_t)(tk.b0 + __popcll(tk.cmask & below)) * nck];
+        }
+        As[c][r] = va;
+        Bs[c][r] = vb;
+      }
+      __syncthreads();
+      lu_acc_chunk<T>(As, Bs, kc, acc);
+      __syncthreads();
+    }
+  }
+  const int s = it.s;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s];
+  T* P;
+  int64_t ld, mr, mc;
+  if (it.region == 0) {
+    P = Fc.l + S.l_off[s];
+    ld = h;
+    mr = h - it.r0;
+    mc = w - it.c0;
+  } else {
+    P = Fc.u + S.u_off[s];
+    ld = w;
+    mr = w - it.r0;
+    mc = h - w - it.c0;
+  }
+  const int ty = tid >> 4, tx = tid & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = ty * 4 + r, j = tx * 4 + q;
+      if (i < mr && j < mc) {
+        T* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * ld;
+        *dst = *dst - acc.c[r][q];
+      }
+    }
+}
+
+// getf2 on the own rows of one panel (multifrontal_lu.rs:2064-2111): per
+// column, the pivot is the first row of largest magnitude in [k, w); the rows
+// are interchanged only when the diagonal is below u times it; the
+// interchange is applied to the panel's columns here and recorded (ipiv,
+// rperm) for the other columns; then the column is scaled and the panel's
+// later columns get the rank-1 update.
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units) {
+  __shared__ double red_m[LU_NT / 64];
+  __shared__ int red_i[LU_NT / 64];
+  __shared__ T ukj[128];
+  __shared__ int sh_p;
+  __shared__ T sh_pinv;
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, kb = un.kb;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s], ke = min(kb + S.nb[s], w);
+  T* L = Fc.l + S.l_off[s];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int k = kb; k < ke; ++k) {
+    T* ck = L + (int64_t)k * h;
+    int p = k;
+    if (Fc.thresh_sq > 0.0) {
+      double m = -1.0;
+      int mi = INT_MAX;
+      for (int i = k + tid; i < w; i += LU_NT) {
+        const double v = F<T>::magsq(ck[i]);
+        if (v > m) {
+          m = v;
+          mi = i;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double om = __shfl_xor(m, off);
+        const int oi = __shfl_xor(mi, off);
+        if (om > m || (om == m && oi < mi)) {
+          m = om;
+          mi = oi;
+        }
+      }
+      if (lane == 0) {
+        red_m[wv] = m;
+        red_i[wv] = mi;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double bm = red_m[0];
+        int bi = red_i[0];
+        for (int q = 1; q < LU_NT / 64; ++q)
+          if (red_m[q] > bm || (red_m[q] == bm && red_i[q] < bi)) {
+            bm = red_m[q];
+            bi = red_i[q];
+          }
+        const double dsq = F<T>::magsq(ck[k]);
+        sh_p = (bi != k && dsq < Fc.thresh_sq * bm) ? bi : k;
+      }
+      __syncthreads();
+      p = sh_p;
+    }
+    if (p != k) {
+      for (int j = kb + tid; j < ke; j += LU_NT) {
+        T* cj = L + (int64_t)j * h;
+        const T t = cj[k];
+        cj[k] = cj[p];
+        cj[p] = t;
+      }
+      if (tid == 0) {
+        int32_t* rp = Fc.rperm + first;
+        const int32_t t = rp[k];
+        rp[k] = rp[p];
+        rp[p] = t;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      Fc.ipiv[first + k] = p;
+      T piv = ck[k];
+      if (Fc.pol.have_floor) {
+        if (F<T>::mag(piv) < Fc.pol.floor) {
+          piv = lu_perturb(piv, Fc.pol.floor);
+          atomicAdd(&Fc.counters[0], 1ull);
+        }
+      } else if (F<T>::is_zero(piv)) {
+        Fc.status[0] = 1;
+        atomicMin(&Fc.counters[1], (unsigned long long)(first + k));
+      }
+      ck[k] = piv;
+      sh_pinv = F<T>::recip(piv);
+    }
+    for (int j = k + 1 + tid; j < ke; j += LU_NT) ukj[j - kb] = L[(int64_t)j * h + k];
+    __syncthreads();
+    const T pinv = sh_pinv;
+    for (int i = k + 1 + tid; i < w; i += LU_NT) {
+      const T lik = ck[i] * pinv;
+      ck[i] = lik;
+      for (int j = k + 1; j < ke; ++j) {
+        const T m = ukj[j - kb];
+        if (!F<T>::is_zero(m)) {
+          T* cj = L + (int64_t)j * h;
+          cj[i] = cj[i] - lik * m;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// After a panel's getf2: column units apply the panel's interchanges to the
+// columns outside the panel (the finished ones left of it, the later own
+// columns and U12) and, on the later ones, the unit-lower triangular solve
+// with L11 (the TRSM of :2157-2199); deep-row units apply the panel transform
+// to the rows below the own rows (apply_panel_trailing, :1357-1387).
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_apply(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units) {
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, kb = un.kb;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s], ke = min(kb + S.nb[s], w), pw = ke - kb;
+  T* L = Fc.l + S.l_off[s];
+  if (un.b == 0) {
+    const int64_t c = (int64_t)un.a + threadIdx.x;
+    const int64_t nlater = w - ke, ncols = kb + nlater + (h - w);
+    if (c >= ncols) return;
+    T* col;
+    bool later;
+    if (c < kb) {
+      col = L + c * h;
+      later = false;
+    } else if (c < kb + nlater) {
+      col = L + (ke + c - kb) * h;
+      later = true;
+    } else {
+      col = Fc.u + S.u_off[s] + (c - kb - nlater) * w;
+      later = true;
+    }
+    for (int k = kb; k < ke; ++k) {
+      const int p = Fc.ipiv[first + k];
+      if (p != k) {
+        const T t = col[k];
+        col[k] = col[p];
+        col[p] = t;
+      }
+    }
+    if (later)
+      for (int r = kb + 1; r < ke; ++r) {
+        T acc = col[r];
+        for (int i = kb; i < r; ++i) acc = acc - L[(int64_t)i * h + r] * col[i];
+        col[r] = acc;
+      }
+  } else {
+    const int64_t i = (int64_t)w + un.a + threadIdx.x;
+    if (i >= h) return;
+    for (int q = 0; q < pw; ++q) {
+      const int k = kb + q;
+      T* ck = L + (int64_t)k * h;
+      const T x = ck[i] * F<T>::recip(ck[k]);
+      ck[i] = x;
+      for (int jj = q + 1; jj < pw; ++jj) {
+        const T m = L[(int64_t)(kb + jj) * h + k];
+        if (!F<T>::is_zero(m)) {
+          T* cj = L + (int64_t)(kb + jj) * h;
+          cj[i] = cj[i] - x * m;
+        }
+      }
+    }
+  }
+}
+
+// Trailing GEMMs of a panel (:2206-2280): L[ke.., ke..w) -= L[ke.., kb..ke)
+// U[kb..ke, ke..w) and U12[ke..w, :] -= L[ke..w, kb..ke) U12[kb..ke, :], one
+// 64x64 tile per workgroup.
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_gemm(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units) {
+  __shared__ T As[KC][LDT];
+  __shared__ T Bs[KC][LDT];
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, kb = un.kb;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], ke = min(kb + S.nb[s], w), pw = ke - kb;
+  const T* L = Fc.l + S.l_off[s];
+  const bool upart = un.a < 0;
+  const int64_t r0 = (int64_t)ke + (int64_t)(upart ? -1 - un.a : un.a) * 64;
+  const int64_t c0 = (int64_t)un.b * 64;
+  T* P;
+  int64_t ld, mr, mc;
+  const T* B;
+  if (!upart) {
+    P = Fc.l + S.l_off[s] + (int64_t)ke * h;  // column ke + c0 + j at P + (c0 + j) h
+    ld = h;
+    mr = h - r0;
+    mc = w - ke - c0;
+    B = L + (int64_t)ke * h + kb;
+  } else {
+    P = Fc.u + S.u_off[s];
+    ld = w;
+    mr = w - r0;
+    mc = h - w - c0;
+    B = Fc.u + S.u_off[s] + kb;
+  }
+  const int tid = threadIdx.x;
+  LuAcc<T> acc;
+  lu_acc_zero(acc);
+  for (int q0 = 0; q0 < pw; q0 += KC) {
+    const int kc = min(KC, pw - q0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + LU_NT * u, r = e & 63, c = e >> 6;
+      T va = F<T>::zero(), vb = F<T>::zero();
+      if (c < kc) {
+        if (r < mr) va = L[(int64_t)(kb + q0 + c) * h + r0 + r];
+        if (r < mc) vb = B[(q0 + c) + (c0 + r) * ld];
+      }
+      As[c][r] = va;
+      Bs[c][r] = vb;
+    }
+    __syncthreads();
+    lu_acc_chunk<T>(As, Bs, kc, acc);
+    __syncthreads();
+  }
+  const int ty = tid >> 4, tx = tid & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = ty * 4 + r, j = tx * 4 + q;
+      if (i < mr && j < mc) {
+        T* dst = P + (r0 + i) + (c0 + j) * ld;
+        *dst = *dst - acc.c[r][q];
+      }
+    }
+}
+
+// ---- solve (row-major n x nrhs vectors)
+
+// bh[g] = d_row[o] b[o] for the original row o = perm[g]
+template <class T>
+__global__ void k_lu_sprep(int32_t n, int32_t nrhs, const int32_t* __restrict__ perm, const double* __restrict__ d_row,
+                           const T* __restrict__ b, T* __restrict__ bh) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * nrhs) return;
+  const int64_t g = e / nrhs, c = e - g * nrhs, o = perm[g];
+  bh[e] = F<T>::scale(b[o * nrhs + c], d_row[o]);
+}
+
+// x[o] = d_col[o] y[e] for the original column o = perm[e]
+template <class T>
+__global__ void k_lu_sfinish(int32_t n, int32_t nrhs, const int32_t* __restrict__ perm,
+                             const double* __restrict__ d_col, const T* __restrict__ y, T* __restrict__ x) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * nrhs) return;
+  const int64_t g = e / nrhs, c = e - g * nrhs, o = perm[g];
+  x[o * nrhs + c] = F<T>::scale(y[e], d_col[o]);
+}
+
+// Forward, own rows of one supernode per workgroup: gather the pivot rows
+// (b minus the descendants' off-diagonal products), then the unit-lower
+// solve with L11 in 64-column blocks (each row's terms in ascending column).
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_fwd_own(LuSymDev S, LuFacDev<T> Fc, const int32_t* __restrict__ list,
+                                                      int32_t nrhs, const T* __restrict__ bh, const T* __restrict__ scr,
+                                                      T* __restrict__ y) {
+  const int s = list[blockIdx.x];
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s];
+  const T* L = Fc.l + S.l_off[s];
+  const int tid = threadIdx.x;
+  T* ys = y + (int64_t)first * nrhs;
+  for (int64_t e = tid; e < (int64_t)w * nrhs; e += LU_NT) {
+    const int64_t p = e / nrhs, c = e - p * nrhs;
+    const int64_t g = first + Fc.rperm[first + p];
+    T v = bh[g * nrhs + c];
+    for (int64_t q = S.gat_ptr[g]; q < S.gat_ptr[g + 1]; ++q) v = v - scr[S.gat_idx[q] * nrhs + c];
+    ys[e] = v;
+  }
+  __syncthreads();
+  for (int j0 = 0; j0 < w; j0 += 64) {
+    const int j1 = min(j0 + 64, w);
+    for (int c = tid; c < nrhs; c += LU_NT)
+      for (int r = j0 + 1; r < j1; ++r) {
+        T acc = ys[(int64_t)r * nrhs + c];
+        for (int q = j0; q < r; ++q) acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
+        ys[(int64_t)r * nrhs + c] = acc;
+      }
+    __syncthreads();
+    for (int64_t e = tid; e < (int64_t)(w - j1) * nrhs; e += LU_NT) {
+      const int64_t r = j1 + e / nrhs, c = e % nrhs;
+      T acc = ys[r * nrhs + c];
+      for (int q = j0; q < j1; ++q) acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
+      ys[r * nrhs + c] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Forward, off-diagonal rows: scr[s_off[s] + i] = L21[i, :] y_s for 64 rows
+// per unit (x = s, y = first row).
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_fwd_off(LuSymDev S, LuFacDev<T> Fc, const int2* __restrict__ units,
+                                                      int32_t nrhs, const T* __restrict__ y, T* __restrict__ scr) {
+  const int2 un = units[blockIdx.x];
+  const int s = un.x;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s];
+  const T* L = Fc.l + S.l_off[s];
+  const int64_t m = min<int64_t>(64, h - w - un.y);
+  for (int64_t e = threadIdx.x; e < m * nrhs; e += LU_NT) {
+    const int64_t i = un.y + e / nrhs, c = e % nrhs;
+    T acc = F<T>::zero();
+    for (int p = 0; p < w; ++p) acc = lu_fma(L[(int64_t)p * h + w + i], y[(int64_t)(first + p) * nrhs + c], acc);
+    scr[(S.s_off[s] + i) * nrhs + c] = acc;
+  }
+}
+
+// Backward, U12 terms: y_s[p] -= U12[p, :] x[rs[w..]] for 64 own rows per
+// unit (the ancestors' x are final).
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_bwd_off(LuSymDev S, LuFacDev<T> Fc, const int2* __restrict__ units,
+                                                      int32_t nrhs, T* __restrict__ y) {
+  const int2 un = units[blockIdx.x];
+  const int s = un.x;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s];
+  const T* U = Fc.u + S.u_off[s];
+  const int32_t* rs = S.rs_idx + S.rs_off[s] + w;
+  const int64_t m = min(64, w - un.y);
+  for (int64_t e = threadIdx.x; e < m * nrhs; e += LU_NT) {
+    const int64_t p = un.y + e / nrhs, c = e % nrhs;
+    T acc = y[(first + p) * nrhs + c];
+    for (int64_t t = 0; t < h - w; ++t) acc = lu_fma(F<T>::zero() - U[p + t * w], y[(int64_t)rs[t] * nrhs + c], acc);
+    y[(first + p) * nrhs + c] = acc;
+  }
+}
+
+// Backward, own block of one supernode per workgroup: back substitution
+// with U11 in 64-column blocks from the last.
+template <class T>
+__global__ __launch_bounds__(LU_NT) void k_lu_bwd_own(LuSymDev S, LuFacDev<T> Fc, const int32_t* __restrict__ list,
+                                                      int32_t nrhs, T* __restrict__ y) {
+  const int s = list[blockIdx.x];
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s];
+  const T* L = Fc.l + S.l_off[s];
+  const int tid = threadIdx.x;
+  T* ys = y + (int64_t)first * nrhs;
+  for (int j1 = w; j1 > 0; j1 -= 64) {
+    const int j0 = max(0, j1 - 64);
+    for (int c = tid; c < nrhs; c += LU_NT)
+      for (int r = j1 - 1; r >= j0; --r) {
+        T acc = ys[(int64_t)r * nrhs + c];
+        for (int q = r + 1; q < j1; ++q)
+          acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
+        ys[(int64_t)r * nrhs + c] = acc * F<T>::recip(L[(int64_t)r * h + r]);
+      }
+    __syncthreads();
+    for (int64_t e = tid; e < (int64_t)j0 * nrhs; e += LU_NT) {
+      const int64_t r = e / nrhs, c = e % nrhs;
+      T acc = ys[r * nrhs + c];
+      for (int q = j0; q < j1; ++q) acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
+      ys[r * nrhs + c] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- launchers
+template <class T>
+void launch_lu_assemble(const LuSymDev& S, const LuFacDev<T>& Fc, const T* vals, hipStream_t st) {
+  if (S.nnz <= 0) return;
+  hipLaunchKernelGGL(k_lu_assemble<T>, dim3((unsigned)((S.nnz + LU_NT - 1) / LU_NT)), dim3(LU_NT), 0, st, S, Fc, vals);
+}
+template <class T>
+void launch_lu_cmod(const LuSymDev& S, const LuFacDev<T>& Fc, const LuItem* items, int nitems, const CTask* tasks,
+                    hipStream_t st) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_lu_cmod<T>, dim3((unsigned)nitems), dim3(LU_NT), 0, st, S, Fc, items, tasks);
+}
+template <class T>
+void launch_lu_getf2(const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lu_getf2<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units);
+}
+template <class T>
+void launch_lu_apply(const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lu_apply<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units);
+}
+template <class T>
+void launch_lu_gemm(const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lu_gemm<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units);
+}
+template <class T>
+void launch_lu_sprep(int32_t n, int32_t nrhs, const int32_t* perm, const double* d_row, const T* b, T* bh,
+                     hipStream_t st) {
+  const int64_t m = (int64_t)n * nrhs;
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_lu_sprep<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, n, nrhs, perm, d_row, b, bh);
+}
+template <class T>
+void launch_lu_sfinish(int32_t n, int32_t nrhs, const int32_t* perm, const double* d_col, const T* y, T* x,
+                       hipStream_t st) {
+  const int64_t m = (int64_t)n * nrhs;
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_lu_sfinish<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, n, nrhs, perm, d_col, y, x);
+}
+template <class T>
+void launch_lu_fwd_own(const LuSymDev& S, const LuFacDev<T>& Fc, const int32_t* list, int n, int32_t nrhs,
+                       const T* bh, const T* scr, T* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lu_fwd_own<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, list, nrhs, bh, scr, y);
+}
+template <class T>
+void launch_lu_fwd_off(const LuSymDev& S, const LuFacDev<T>& Fc, const int2* units, int n, int32_t nrhs, const T* y,
+                       T* scr, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lu_fwd_off<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units, nrhs, y, scr);
+}
+template <class T>
+void launch_lu_bwd_off(const LuSymDev& S, const LuFacDev<T>& Fc, const int2* units, int n, int32_t nrhs, T* y,
+                       hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lu_bwd_off<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units, nrhs, y);
+}
+template <class T>
+void launch_lu_bwd_own(const LuSymDev& S, const LuFacDev<T>& Fc, const int32_t* list, int n, int32_t nrhs, T* y,
+                       hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lu_bwd_own<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, list, nrhs, y);
+}
+
+#define RSL_LU_INST(T)                                                                                          \
+  template void launch_lu_assemble<T>(const LuSymDev&, const LuFacDev<T>&, const T*, hipStream_t);              \
+  template void launch_lu_cmod<T>(const LuSymDev&, const LuFacDev<T>&, const LuItem*, int, const CTask*,         \
+                                  hipStream_t);                                                                 \
+  template void launch_lu_getf2<T>(const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, hipStream_t);       \
+  template void launch_lu_apply<T>(const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, hipStream_t);       \
+  template void launch_lu_gemm<T>(const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, hipStream_t);        \
+  template void launch_lu_sprep<T>(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);  \
+  template void launch_lu_sfinish<T>(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t); \
+  template void launch_lu_fwd_own<T>(const LuSymDev&, const LuFacDev<T>&, const int32_t*, int, int32_t, const T*, \
+                                     const T*, T*, hipStream_t);                                                \
+  template void launch_lu_fwd_off<T>(const LuSymDev&, const LuFacDev<T>&, const int2*, int, int32_t, const T*,  \
+                                     T*, hipStream_t);                                                          \
+  template void launch_lu_bwd_off<T>(const LuSymDev&, const LuFacDev<T>&, const int2*, int, int32_t, T*,        \
+                                     hipStream_t);                                                              \
+  template void launch_lu_bwd_own<T>(const LuSymDev&, const LuFacDev<T>&, const int32_t*, int, int32_t, T*,     \
+                                     hipStream_t);
+RSL_LU_INST(double)
+RSL_LU_INST(cplx)
+
+}  // namespace rsl

@@ -7,10 +7,18 @@
 #include <string>
 
 #include "../../../include/rslab_amd.h"
+#include "lu_runtime.hpp"
 #include "runtime.hpp"
 
 struct rsl_symbolic {
   std::shared_ptr<rsl::DeviceSymbolic> ds;
+};
+struct rsl_lu_symbolic {
+  std::shared_ptr<rsl::LuDeviceSymbolic> ds;
+};
+struct rsl_lu_solver {
+  std::unique_ptr<rsl::LuFactorBase> f;
+  std::mutex mu;  // calls on one handle share its device workspace
 };
 struct rsl_solver {
   std::unique_ptr<rsl::DeviceFactorBase> f;
@@ -113,6 +121,7 @@ void rsl_settings_default(rsl_settings* s) {
   s->method = RSL_METHOD_LEFT_LOOKING;
   s->solve_dep = 1;
   s->solve_split = -1;
+  s->pivot_u = 0.1;
 }
 
 const char* rsl_last_error(void) { return g_err.c_str(); }
@@ -559,5 +568,187 @@ int rsl_factor_inertia_batch(rsl_solver* f, int32_t count, int64_t* inertia) {
 
 void rsl_free_factor(rsl_solver* f) { delete f; }
 void rsl_free_symbolic(rsl_symbolic* s) { delete s; }
+
+}  // extern "C"
+
+// ---- LU twin
+namespace {
+void check_general_csc(int64_t n, const int64_t* col_ptr, const int64_t* row_idx, std::vector<int32_t>& ri) {
+  if (n < 0 || n >= INT32_MAX) throw rsl::Error(RSL_INVALID_INPUT, "dimension out of range");
+  if (!col_ptr || (n > 0 && col_ptr[n] > 0 && !row_idx)) throw rsl::Error(RSL_INVALID_INPUT, "null pattern");
+  if (col_ptr[0] != 0) throw rsl::Error(RSL_INVALID_INPUT, "col_ptr[0] must be 0");
+  for (int64_t j = 0; j < n; ++j)
+    if (col_ptr[j + 1] < col_ptr[j]) throw rsl::Error(RSL_INVALID_INPUT, "col_ptr is not non-decreasing");
+  ri.resize(col_ptr[n]);
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t k = col_ptr[j]; k < col_ptr[j + 1]; ++k) {
+      if (row_idx[k] < 0 || row_idx[k] >= n) throw rsl::Error(RSL_INVALID_INPUT, "row index out of bounds");
+      if (k > col_ptr[j] && row_idx[k] <= row_idx[k - 1])
+        throw rsl::Error(RSL_INVALID_INPUT, "rows not strictly increasing within a column");
+      ri[k] = (int32_t)row_idx[k];
+    }
+}
+
+rsl::LuFactorSettings lu_settings(const rsl_settings& s) {
+  if (s.method != RSL_METHOD_LEFT_LOOKING)
+    throw rsl::Error(RSL_INVALID_INPUT, "the LU twin is built for FactorMethod::LeftLooking only");
+  if (std::isnan(s.drop_tol) || std::isnan(s.pivot_u)) throw rsl::Error(RSL_INVALID_INPUT, "NaN setting");
+  rsl::LuFactorSettings fs;
+  fs.pivot = s.pivot;
+  fs.abs_floor = s.abs_floor;
+  fs.pivot_u = std::min(1.0, std::max(0.0, s.pivot_u));
+  fs.drop_tol = s.drop_tol;
+  return fs;
+}
+}  // namespace
+
+extern "C" {
+
+int rsl_lu_analyze(int64_t n, const int64_t* col_ptr, const int64_t* row_idx, const rsl_settings* s,
+                   rsl_lu_symbolic** out) {
+  *out = nullptr;
+  return guard([&] {
+    rsl_settings st;
+    if (s) st = *s;
+    else rsl_settings_default(&st);
+    std::vector<int32_t> ri;
+    check_general_csc(n, col_ptr, row_idx, ri);
+    (void)lu_settings(st);
+    std::vector<int64_t> scp;
+    std::vector<int32_t> sri;
+    rsl::symmetrized_lower((int32_t)n, col_ptr, ri.data(), scp, sri);
+    auto ds = std::make_shared<rsl::LuDeviceSymbolic>();
+    ds->plan.sym = rsl::analyze((int32_t)n, scp.data(), sri.data(), ordering_of(st), symbolic_params(st));
+    rsl::build_lu_plan(ds->plan, col_ptr, ri.data());
+    auto* h = new rsl_lu_symbolic;
+    h->ds = std::move(ds);
+    *out = h;
+  });
+}
+
+int rsl_lu_symbolic_info(const rsl_lu_symbolic* sym, int64_t* out, double* flops) {
+  return guard([&] {
+    const rsl::LuPlan& P = sym->ds->plan;
+    if (out) {
+      out[0] = P.n;
+      out[1] = P.nnz;
+      out[2] = P.ns;
+      out[3] = (int64_t)P.levels.size();
+      out[4] = P.l_entries;
+      out[5] = P.u_entries;
+      out[6] = (int64_t)P.tasks.size();
+      out[7] = (int64_t)P.units.size();
+    }
+    if (flops) flops[0] = P.factor_flops;
+  });
+}
+
+int rsl_lu_symbolic_export(const rsl_lu_symbolic* sym, int32_t* perm, int32_t* perm_inv, int32_t* sn_first,
+                           int32_t* sn_ncol, int32_t* sn_nrow, int32_t* ch_ptr, int32_t* ch_idx, int64_t* pp_colp,
+                           int32_t* pp_rowi) {
+  return guard([&] {
+    const rsl::LuPlan& P = sym->ds->plan;
+    const rsl::Symbolic& S = P.sym;
+    if (perm) std::memcpy(perm, S.perm.data(), sizeof(int32_t) * S.n);
+    if (perm_inv) std::memcpy(perm_inv, S.perm_inv.data(), sizeof(int32_t) * S.n);
+    if (sn_first) std::memcpy(sn_first, P.sn_first.data(), sizeof(int32_t) * P.ns);
+    if (sn_ncol) std::memcpy(sn_ncol, P.sn_ncol.data(), sizeof(int32_t) * P.ns);
+    if (sn_nrow) std::memcpy(sn_nrow, P.sn_nrow.data(), sizeof(int32_t) * P.ns);
+    if (ch_ptr) {
+      int32_t acc = 0;
+      for (int32_t s = 0; s < P.ns; ++s) {
+        ch_ptr[s] = acc;
+        if (ch_idx)
+          for (size_t c = 0; c < S.snodes[s].children.size(); ++c) ch_idx[acc + c] = S.snodes[s].children[c];
+        acc += (int32_t)S.snodes[s].children.size();
+      }
+      ch_ptr[P.ns] = acc;
+    }
+    if (pp_colp) std::memcpy(pp_colp, S.permuted.colp.data(), sizeof(int64_t) * (S.n + 1));
+    if (pp_rowi) std::memcpy(pp_rowi, S.permuted.rowi.data(), sizeof(int32_t) * S.permuted.rowi.size());
+  });
+}
+
+int rsl_lu_factor(rsl_lu_symbolic* sym, int32_t field, const double* values, const rsl_settings* s,
+                  rsl_lu_solver** out) {
+  *out = nullptr;
+  return guard([&] {
+    rsl_settings st;
+    if (s) st = *s;
+    else rsl_settings_default(&st);
+    if (field != RSL_F64 && field != RSL_C64) throw rsl::Error(RSL_INVALID_INPUT, "unknown field");
+    if (sym->ds->plan.nnz > 0 && !values) throw rsl::Error(RSL_INVALID_INPUT, "null values");
+    auto f = rsl::make_lu_factor(sym->ds, field == RSL_C64 ? 1 : 0, values, lu_settings(st));
+    auto* h = new rsl_lu_solver;
+    h->f = std::move(f);
+    *out = h;
+  });
+}
+
+int rsl_lu_refactor(rsl_lu_solver* f, const double* values) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (f->f->sym->plan.nnz > 0 && !values) throw rsl::Error(RSL_INVALID_INPUT, "null values");
+    f->f->refactor(values);
+  });
+}
+
+int rsl_lu_solve(rsl_lu_solver* f, const double* b, int64_t nrhs, double* x) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs < 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "negative nrhs");
+    f->f->solve_host(b, nrhs, x);
+  });
+}
+
+int rsl_lu_solve_device(rsl_lu_solver* f, const void* d_b, int64_t nrhs, void* d_x, void* stream) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs < 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "negative nrhs");
+    f->f->solve_device(d_b, nrhs, d_x, static_cast<hipStream_t>(stream));
+  });
+}
+
+int rsl_lu_info(rsl_lu_solver* f, int64_t* out, double* t) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (out) {
+      const rsl::LuEmitted& e = f->f->emitted();
+      out[0] = f->f->sym->plan.n;
+      out[1] = f->f->n_perturbed;
+      out[2] = (int64_t)e.l_row_idx.size();
+      out[3] = (int64_t)e.u_col_idx.size();
+      out[4] = f->f->field();
+      out[5] = 0;
+    }
+    if (t) t[0] = f->f->t_factor_ms;
+  });
+}
+
+int rsl_lu_factor_download(rsl_lu_solver* f, int64_t* l_col_ptr, int32_t* l_row_idx, double* l_values,
+                           int64_t* u_row_ptr, int32_t* u_col_idx, double* u_values, int32_t* perm,
+                           int32_t* perm_row, double* d_row, double* d_col) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    const rsl::LuEmitted& e = f->f->emitted();
+    const size_t n = (size_t)f->f->sym->plan.n;
+    auto cp = [](auto* dst, const auto& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
+    };
+    cp(l_col_ptr, e.l_col_ptr);
+    cp(l_row_idx, e.l_row_idx);
+    cp(l_values, e.l_values);
+    cp(u_row_ptr, e.u_row_ptr);
+    cp(u_col_idx, e.u_col_idx);
+    cp(u_values, e.u_values);
+    cp(perm, e.perm);
+    cp(perm_row, e.perm_row);
+    if (d_row && n) std::memcpy(d_row, f->f->d_row.data(), sizeof(double) * n);
+    if (d_col && n) std::memcpy(d_col, f->f->d_col.data(), sizeof(double) * n);
+  });
+}
+
+void rsl_lu_free_factor(rsl_lu_solver* f) { delete f; }
+void rsl_lu_free_symbolic(rsl_lu_symbolic* s) { delete s; }
 
 }  // extern "C"

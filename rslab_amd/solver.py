@@ -237,6 +237,8 @@ class SolverSettings:
     solve_dep: bool = True
     # first tree level of the dependency-driven launch (None: automatic)
     solve_split: Optional[int] = None
+    # LU twin's threshold partial-pivoting tolerance u (multifrontal_ldlt.rs:382)
+    pivot_u: float = 0.1
 
     # builders (multifrontal_ldlt.rs:616-779)
     def with_method(self, m):
@@ -245,6 +247,10 @@ class SolverSettings:
 
     def with_graphs(self, on: bool):
         return dataclasses.replace(self, graphs=bool(on))
+
+    def with_pivot_u(self, u: float):
+        """SolverSettings::with_pivot_u (multifrontal_ldlt.rs:742-745): clamped to [0, 1]."""
+        return dataclasses.replace(self, pivot_u=min(1.0, max(0.0, float(u))))
 
     def with_drop_tol(self, tau: float):
         """Incomplete-factor threshold dropping (with_drop_tol, multifrontal_ldlt.rs:640-643):
@@ -309,6 +315,7 @@ class SolverSettings:
         s.method = int(self.method)
         s.solve_dep = 1 if self.solve_dep else 0
         s.solve_split = -1 if self.solve_split is None else int(self.solve_split)
+        s.pivot_u = min(1.0, max(0.0, float(self.pivot_u)))
         return s
 
 
