@@ -254,6 +254,64 @@ def complex_line(R, matgen, torch, m=40, nrhs=16, steps=3, target=1e-14):
                                  "factor_plus_solve_ms": 659, "source": "BASELINE.md"})
 
 
+def lu_line(R, matgen, torch, m=60, nrhs=8, steps=3, max_iter=2, cpu=True):
+    """The LU twin (src/numeric/multifrontal_lu.rs) on an unsymmetric matrix:
+    3D 7-point convection-diffusion with cell Peclet numbers (0.8, 0.4, 0.2),
+    shift -1.5 and diag(U[-0.01, 0.01]) on m^3 nodes (the grid of config 3),
+    nested dissection on A + A^T, 8 right-hand sides. A step is the numeric
+    refactor (two-sided equilibration on the host, the factor on the device)
+    and solve_refined with up to `max_iter` corrections, values and
+    right-hand sides in host memory (the LU entry points take host buffers).
+    The CPU baseline is the oracle's LU (one thread) on a 24^3 sample, scaled
+    by the factor flop ratio."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    a, B = matgen.lu_workload(m=m, nrhs=nrhs)
+    st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
+    t0 = time.perf_counter()
+    sym = R.LuSymbolic.analyze_with(a, st)
+    t_an = time.perf_counter() - t0
+    inf = sym.info()
+    f = sym.factor(a, st)
+    f.solve_refined(a, B, max_iter)
+    ts, fts = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        f.refactor(a)
+        X, its = f.solve_refined(a, B, max_iter, info=True)
+        ts.append(time.perf_counter() - t0)
+        fts.append(f.info()["factor_ms"])
+    berr = O.lu_backward_error(a, X, B)
+    # solve alone (device buffers resident), for the solve rate
+    t_solve = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        f.solve_many(B)
+        t_solve.append(time.perf_counter() - t0)
+    out = dict(
+        workload=f"unsymmetric 3D convection-diffusion, cell Peclet (0.8, 0.4, 0.2), shift -1.5 + "
+                 f"diag(U[-0.01,0.01]), {m}^3 (n={a.n}), nested dissection on A + A^T, {nrhs} RHS, "
+                 f"refactor + solve_refined (<= {max_iter} corrections), host buffers",
+        s_per_step=round(min(ts), 4), factor_ms=round(min(fts), 2), solve_ms_host=round(min(t_solve) * 1e3, 2),
+        analyze_s=round(t_an, 2), factor_flops=inf["factor_flops"],
+        factor_tflops=round(inf["factor_flops"] / (min(fts) * 1e-3) / 1e12, 3),
+        berr_max=float(berr.max()), corrections=max(c for c, _ in its), n_perturbed=f.n_perturbed(),
+        factor_nnz=f.factor_nnz())
+    if cpu:
+        mc = 24
+        ac, Bc = matgen.lu_workload(m=mc, nrhs=nrhs)
+        sc = R.LuSymbolic.analyze_with(ac, st)
+        fl = sc.info()["factor_flops"]
+        t0 = time.perf_counter()
+        fo = O.lu_factor(sc.export(), ac)
+        fo.solve_many(Bc)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=round(dt * inf["factor_flops"] / fl, 2), unit="s", cores=1, kind="port",
+                                   sample=f"oracle LU factor + {nrhs}-RHS solve on {mc}^3 (n={ac.n}), "
+                                          f"{dt:.2f} s measured, scaled by factor flops {inf['factor_flops']:.3e}/{fl:.3e}")
+    return out
+
+
 def cpu_baseline(R, matgen, a, B, sym):
     """The oracle (C restatement of the reference's left-looking factor,
     solve and refinement ladder) on the host cores of this machine, on the
@@ -316,6 +374,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the config-5 refactor sweep line")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default plan knobs)")
     ap.add_argument("--no-complex", action="store_true", help="skip the config-2 complex line")
+    ap.add_argument("--no-lu", action="store_true", help="skip the LU twin line")
+    ap.add_argument("--lu-only", action="store_true", help="only the LU twin line (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB float64 device buffer (the HBM counters' calibration run)")
     args = ap.parse_args()
@@ -339,6 +399,10 @@ def main():
     import rslab_amd as R
     from rslab_amd import matgen
     torch.cuda.init()
+    if args.lu_only:
+        res = lu_line(R, matgen, torch, steps=args.steps, cpu=not args.no_cpu_baseline)
+        print(json.dumps({"lu": res}), flush=True)
+        return
 
     m = args.grid
     a, B = matgen.config3(m=m, nrhs=8)
@@ -396,6 +460,11 @@ def main():
         log("config-2 complex line")
         cplx = complex_line(R, matgen, torch)
         log(f"complex: {cplx}")
+    lu = None
+    if not args.no_lu:
+        log("LU twin line")
+        lu = lu_line(R, matgen, torch, cpu=not args.no_cpu_baseline)
+        log(f"lu: {lu}")
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on the host cores")
@@ -426,6 +495,7 @@ def main():
         "cpu_baseline": cpu,
         "sweep": sweep,
         "complex": cplx,
+        "lu": lu,
         "paths": paths,
         "kernels": kt,
     }

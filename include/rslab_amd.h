@@ -349,6 +349,15 @@ int rsl_lu_solve(rsl_lu_solver *f, const double *b, int64_t nrhs, double *x);
  * on `stream` (hipStream_t; NULL the default stream), which then waits for
  * the solve; returns without waiting. */
 int rsl_lu_solve_device(rsl_lu_solver *f, const void *d_b, int64_t nrhs, void *d_x, void *stream);
+/* LuSolver::solve_refined (:1219-1227) -> solve_lu_refined (:3285-3322), per
+ * column, on the device: x = (LU)^-1 b, then up to max_iter corrections
+ * x += (LU)^-1 (b - A x) with the residual against a_values (A's values in
+ * the analysed pattern), every correction evaluated; each column returns its
+ * iterate of least max |b - A x| and stops early at a zero residual. b, x
+ * host buffers, row-major n x nrhs (interleaved for RSL_C64), nrhs <= 4096;
+ * info (may be NULL) = 2 per column: {corrections applied, least max |r|}. */
+int rsl_lu_refine(rsl_lu_solver *f, const double *a_values, const double *b, int64_t nrhs, int32_t max_iter,
+                  double *x, double *info);
 /* out[6] = {n, n_perturbed (LuSolver::n_perturbed :1234), stored L entries,
  * stored U entries (LuFactors, after drop_tol), field, 0}; t[0] = the last
  * factor's wall time in ms (host to host). Either pointer may be NULL. */

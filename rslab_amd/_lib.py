@@ -97,6 +97,7 @@ SIGNATURES = {
     "rsl_lu_refactor": (C.c_int, [_P, _DP]),
     "rsl_lu_solve": (C.c_int, [_P, _DP, C.c_int64, _DP]),
     "rsl_lu_solve_device": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "rsl_lu_refine": (C.c_int, [_P, _DP, _DP, C.c_int64, C.c_int32, _DP, _DP]),
     "rsl_lu_info": (C.c_int, [_P, _I64P, _DP]),
     "rsl_lu_factor_download": (C.c_int, [_P, _I64P, _I32P, _DP, _I64P, _I32P, _DP, _I32P, _I32P, _DP, _DP]),
     "rsl_lu_free_factor": (None, [_P]),

@@ -35,6 +35,10 @@ __device__ inline double lu_fma<double>(double a, double b, double c) {
   return fma(a, b, c);
 }
 
+// value of lane q across the wave
+__device__ inline double lu_bcast(double v, int q) { return __shfl(v, q, 64); }
+__device__ inline cplx lu_bcast(cplx v, int q) { return {__shfl(v.re, q, 64), __shfl(v.im, q, 64)}; }
+
 // perturb_pivot (multifrontal_ldlt.rs:786-795)
 template <class T>
 __device__ inline T lu_perturb(T d, double floor) {
@@ -413,113 +417,163 @@ __global__ void k_lu_sfinish(int32_t n, int32_t nrhs, const int32_t* __restrict_
   x[o * nrhs + c] = F<T>::scale(y[e], d_col[o]);
 }
 
-// Forward, own rows of one supernode per workgroup: gather the pivot rows
-// (b minus the descendants' off-diagonal products), then the unit-lower
-// solve with L11 in 64-column blocks (each row's terms in ascending column).
+// Forward gather, 64 own rows of one supernode per unit (s, -, first row):
+// y_s[p] = b[pivot row of p] minus the descendants' off-diagonal products
+// that land on that row.
 template <class T>
-__global__ __launch_bounds__(LU_NT) void k_lu_fwd_own(LuSymDev S, LuFacDev<T> Fc, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(LU_NT) void k_lu_fgather(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
                                                       int32_t nrhs, const T* __restrict__ bh, const T* __restrict__ scr,
                                                       T* __restrict__ y) {
-  const int s = list[blockIdx.x];
-  const int64_t h = S.sn_nrow[s];
-  const int w = S.sn_ncol[s], first = S.sn_first[s];
-  const T* L = Fc.l + S.l_off[s];
-  const int tid = threadIdx.x;
-  T* ys = y + (int64_t)first * nrhs;
-  for (int64_t e = tid; e < (int64_t)w * nrhs; e += LU_NT) {
-    const int64_t p = e / nrhs, c = e - p * nrhs;
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, w = S.sn_ncol[s], first = S.sn_first[s];
+  const int64_t m = min(64, w - un.a);
+  for (int64_t e = threadIdx.x; e < m * nrhs; e += LU_NT) {
+    const int64_t p = un.a + e / nrhs, c = e % nrhs;
     const int64_t g = first + Fc.rperm[first + p];
     T v = bh[g * nrhs + c];
     for (int64_t q = S.gat_ptr[g]; q < S.gat_ptr[g + 1]; ++q) v = v - scr[S.gat_idx[q] * nrhs + c];
-    ys[e] = v;
+    y[(first + p) * nrhs + c] = v;
+  }
+}
+
+// Forward diagonal block [j0, j0 + m) of L11 of one supernode per unit
+// (s, j0): unit-lower substitution, one wave per right-hand side with lane r
+// holding row r; each solved value is broadcast across the wave and the rows
+// below it take its term (each row's terms in ascending column).
+template <class T, int BS>
+__global__ __launch_bounds__(LU_NT) void k_lu_fdiag(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
+                                                    int32_t nrhs, T* __restrict__ y) {
+  __shared__ T Lb[BS][BS + 1];
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, j0 = un.kb;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
+  const T* L = Fc.l + S.l_off[s];
+  for (int e = threadIdx.x; e < BS * BS; e += LU_NT) {
+    const int r = e % BS, q = e / BS;
+    Lb[r][q] = (r < m && q < r) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
   }
   __syncthreads();
-  for (int j0 = 0; j0 < w; j0 += 64) {
-    const int j1 = min(j0 + 64, w);
-    for (int c = tid; c < nrhs; c += LU_NT)
-      for (int r = j0 + 1; r < j1; ++r) {
-        T acc = ys[(int64_t)r * nrhs + c];
-        for (int q = j0; q < r; ++q) acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
-        ys[(int64_t)r * nrhs + c] = acc;
-      }
-    __syncthreads();
-    for (int64_t e = tid; e < (int64_t)(w - j1) * nrhs; e += LU_NT) {
-      const int64_t r = j1 + e / nrhs, c = e % nrhs;
-      T acc = ys[r * nrhs + c];
-      for (int q = j0; q < j1; ++q) acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
-      ys[r * nrhs + c] = acc;
+  const int lane = threadIdx.x & 63;
+  for (int c = threadIdx.x >> 6; c < nrhs; c += LU_NT / 64) {
+    T* yc = y + (int64_t)(first + j0) * nrhs + c;
+    T v = lane < m ? yc[(int64_t)lane * nrhs] : F<T>::zero();
+    for (int q = 0; q < m; ++q) {
+      const T yq = lu_bcast(v, q);
+      if (lane > q && lane < m) v = lu_fma(F<T>::zero() - Lb[lane][q], yq, v);
     }
-    __syncthreads();
+    if (lane < m) yc[(int64_t)lane * nrhs] = v;
   }
 }
 
-// Forward, off-diagonal rows: scr[s_off[s] + i] = L21[i, :] y_s for 64 rows
-// per unit (x = s, y = first row).
-template <class T>
-__global__ __launch_bounds__(LU_NT) void k_lu_fwd_off(LuSymDev S, LuFacDev<T> Fc, const int2* __restrict__ units,
-                                                      int32_t nrhs, const T* __restrict__ y, T* __restrict__ scr) {
-  const int2 un = units[blockIdx.x];
-  const int s = un.x;
+// Forward update by the solved block [j0, j0 + m) of 64 rows per unit
+// (s, j0, first row): own rows below the block get y -= L y_block; the
+// off-diagonal rows accumulate L21 y_s into the supernode's scratch rows
+// (written by the first block, added to by the later ones).
+template <class T, int BS>
+__global__ __launch_bounds__(LU_NT) void k_lu_fupd(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
+                                                   int32_t nrhs, T* __restrict__ y, T* __restrict__ scr) {
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, j0 = un.kb;
   const int64_t h = S.sn_nrow[s];
-  const int w = S.sn_ncol[s], first = S.sn_first[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
   const T* L = Fc.l + S.l_off[s];
-  const int64_t m = min<int64_t>(64, h - w - un.y);
-  for (int64_t e = threadIdx.x; e < m * nrhs; e += LU_NT) {
-    const int64_t i = un.y + e / nrhs, c = e % nrhs;
+  const T* yb = y + (int64_t)(first + j0) * nrhs;
+  for (int64_t e = threadIdx.x; e < 64 * (int64_t)nrhs; e += LU_NT) {
+    const int64_t r = un.a + (e & 63), c = e >> 6;
+    if (r >= h) continue;
     T acc = F<T>::zero();
-    for (int p = 0; p < w; ++p) acc = lu_fma(L[(int64_t)p * h + w + i], y[(int64_t)(first + p) * nrhs + c], acc);
-    scr[(S.s_off[s] + i) * nrhs + c] = acc;
+    for (int q = 0; q < m; ++q) acc = lu_fma(L[(int64_t)(j0 + q) * h + r], yb[(int64_t)q * nrhs + c], acc);
+    if (r < w) {
+      T* d = y + (first + r) * nrhs + c;
+      *d = *d - acc;
+    } else {
+      T* d = scr + (S.s_off[s] + r - w) * nrhs + c;
+      *d = j0 == 0 ? acc : *d + acc;
+    }
   }
 }
 
-// Backward, U12 terms: y_s[p] -= U12[p, :] x[rs[w..]] for 64 own rows per
-// unit (the ancestors' x are final).
+// Backward U12 terms, 16 own rows per unit (s, -, first row), the trailing
+// columns split over 16 slices summed in LDS: y_s[p] -= U12[p, :] x[rs[w..]]
+// (the ancestors' x are final).
 template <class T>
-__global__ __launch_bounds__(LU_NT) void k_lu_bwd_off(LuSymDev S, LuFacDev<T> Fc, const int2* __restrict__ units,
-                                                      int32_t nrhs, T* __restrict__ y) {
-  const int2 un = units[blockIdx.x];
-  const int s = un.x;
+__global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
+                                                   int32_t nrhs, T* __restrict__ y) {
+  __shared__ T part[16][17];
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s;
   const int64_t h = S.sn_nrow[s];
   const int w = S.sn_ncol[s], first = S.sn_first[s];
   const T* U = Fc.u + S.u_off[s];
   const int32_t* rs = S.rs_idx + S.rs_off[s] + w;
-  const int64_t m = min(64, w - un.y);
-  for (int64_t e = threadIdx.x; e < m * nrhs; e += LU_NT) {
-    const int64_t p = un.y + e / nrhs, c = e % nrhs;
-    T acc = y[(first + p) * nrhs + c];
-    for (int64_t t = 0; t < h - w; ++t) acc = lu_fma(F<T>::zero() - U[p + t * w], y[(int64_t)rs[t] * nrhs + c], acc);
-    y[(first + p) * nrhs + c] = acc;
+  const int64_t nt = h - w;
+  const int i = threadIdx.x & 15, sl = threadIdx.x >> 4;
+  const int64_t p = un.a + i;
+  for (int c = 0; c < nrhs; ++c) {
+    T acc = F<T>::zero();
+    if (p < w)
+      for (int64_t t = sl; t < nt; t += 16) acc = lu_fma(U[p + t * w], y[(int64_t)rs[t] * nrhs + c], acc);
+    part[sl][i] = acc;
+    __syncthreads();
+    if (sl == 0 && p < w) {
+      T sum = part[0][i];
+      for (int q = 1; q < 16; ++q) sum = sum + part[q][i];
+      T* d = y + (first + p) * nrhs + c;
+      *d = *d - sum;
+    }
+    __syncthreads();
   }
 }
 
-// Backward, own block of one supernode per workgroup: back substitution
-// with U11 in 64-column blocks from the last.
-template <class T>
-__global__ __launch_bounds__(LU_NT) void k_lu_bwd_own(LuSymDev S, LuFacDev<T> Fc, const int32_t* __restrict__ list,
-                                                      int32_t nrhs, T* __restrict__ y) {
-  const int s = list[blockIdx.x];
+// Backward diagonal block [j0, j0 + m) of U11 of one supernode per unit
+// (s, j0): back substitution with the pivots, one wave per right-hand side
+// with lane r holding row r.
+template <class T, int BS>
+__global__ __launch_bounds__(LU_NT) void k_lu_bdiag(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
+                                                    int32_t nrhs, T* __restrict__ y) {
+  __shared__ T Ub[BS][BS + 1];
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, j0 = un.kb;
   const int64_t h = S.sn_nrow[s];
-  const int w = S.sn_ncol[s], first = S.sn_first[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
   const T* L = Fc.l + S.l_off[s];
-  const int tid = threadIdx.x;
-  T* ys = y + (int64_t)first * nrhs;
-  for (int j1 = w; j1 > 0; j1 -= 64) {
-    const int j0 = max(0, j1 - 64);
-    for (int c = tid; c < nrhs; c += LU_NT)
-      for (int r = j1 - 1; r >= j0; --r) {
-        T acc = ys[(int64_t)r * nrhs + c];
-        for (int q = r + 1; q < j1; ++q)
-          acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
-        ys[(int64_t)r * nrhs + c] = acc * F<T>::recip(L[(int64_t)r * h + r]);
-      }
-    __syncthreads();
-    for (int64_t e = tid; e < (int64_t)j0 * nrhs; e += LU_NT) {
-      const int64_t r = e / nrhs, c = e % nrhs;
-      T acc = ys[r * nrhs + c];
-      for (int q = j0; q < j1; ++q) acc = lu_fma(F<T>::zero() - L[(int64_t)q * h + r], ys[(int64_t)q * nrhs + c], acc);
-      ys[r * nrhs + c] = acc;
+  for (int e = threadIdx.x; e < BS * BS; e += LU_NT) {
+    const int r = e % BS, q = e / BS;
+    Ub[r][q] = (q < m && r <= q) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int c = threadIdx.x >> 6; c < nrhs; c += LU_NT / 64) {
+    T* yc = y + (int64_t)(first + j0) * nrhs + c;
+    T v = lane < m ? yc[(int64_t)lane * nrhs] : F<T>::zero();
+    for (int q = m - 1; q >= 0; --q) {
+      if (lane == q) v = v * F<T>::recip(Ub[q][q]);
+      const T xq = lu_bcast(v, q);
+      if (lane < q) v = lu_fma(F<T>::zero() - Ub[lane][q], xq, v);
     }
-    __syncthreads();
+    if (lane < m) yc[(int64_t)lane * nrhs] = v;
+  }
+}
+
+// Backward update by the solved block [j0, j0 + m), 64 own rows above it per
+// unit (s, j0, first row): y[r] -= U[r, block] y_block.
+template <class T, int BS>
+__global__ __launch_bounds__(LU_NT) void k_lu_bupd(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
+                                                   int32_t nrhs, T* __restrict__ y) {
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, j0 = un.kb;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
+  const T* L = Fc.l + S.l_off[s];
+  const T* yb = y + (int64_t)(first + j0) * nrhs;
+  for (int64_t e = threadIdx.x; e < 64 * (int64_t)nrhs; e += LU_NT) {
+    const int64_t r = un.a + (e & 63), c = e >> 6;
+    if (r >= j0) continue;
+    T acc = F<T>::zero();
+    for (int q = 0; q < m; ++q) acc = lu_fma(L[(int64_t)(j0 + q) * h + r], yb[(int64_t)q * nrhs + c], acc);
+    T* d = y + (first + r) * nrhs + c;
+    *d = *d - acc;
   }
 }
 
@@ -565,28 +619,25 @@ void launch_lu_sfinish(int32_t n, int32_t nrhs, const int32_t* perm, const doubl
   hipLaunchKernelGGL(k_lu_sfinish<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, n, nrhs, perm, d_col, y, x);
 }
 template <class T>
-void launch_lu_fwd_own(const LuSymDev& S, const LuFacDev<T>& Fc, const int32_t* list, int n, int32_t nrhs,
-                       const T* bh, const T* scr, T* y, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_lu_fwd_own<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, list, nrhs, bh, scr, y);
+int lu_solve_bs() {
+  return sizeof(T) == 16 ? 32 : 64;
 }
+// kind: 0 gather, 1 forward diagonal, 2 forward update, 3 backward U12,
+// 4 backward diagonal, 5 backward update
 template <class T>
-void launch_lu_fwd_off(const LuSymDev& S, const LuFacDev<T>& Fc, const int2* units, int n, int32_t nrhs, const T* y,
-                       T* scr, hipStream_t st) {
+void launch_lu_sstep(int kind, const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, int32_t nrhs,
+                     const T* bh, T* scr, T* y, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_lu_fwd_off<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units, nrhs, y, scr);
-}
-template <class T>
-void launch_lu_bwd_off(const LuSymDev& S, const LuFacDev<T>& Fc, const int2* units, int n, int32_t nrhs, T* y,
-                       hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_lu_bwd_off<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units, nrhs, y);
-}
-template <class T>
-void launch_lu_bwd_own(const LuSymDev& S, const LuFacDev<T>& Fc, const int32_t* list, int n, int32_t nrhs, T* y,
-                       hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_lu_bwd_own<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, list, nrhs, y);
+  constexpr int BS = sizeof(T) == 16 ? 32 : 64;
+  const dim3 g((unsigned)n), b(LU_NT);
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(k_lu_fgather<T>, g, b, 0, st, S, Fc, units, nrhs, bh, scr, y); break;
+    case 1: hipLaunchKernelGGL((k_lu_fdiag<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y); break;
+    case 2: hipLaunchKernelGGL((k_lu_fupd<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y, scr); break;
+    case 3: hipLaunchKernelGGL(k_lu_boff<T>, g, b, 0, st, S, Fc, units, nrhs, y); break;
+    case 4: hipLaunchKernelGGL((k_lu_bdiag<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y); break;
+    default: hipLaunchKernelGGL((k_lu_bupd<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y); break;
+  }
 }
 
 #define RSL_LU_INST(T)                                                                                          \
@@ -598,14 +649,9 @@ void launch_lu_bwd_own(const LuSymDev& S, const LuFacDev<T>& Fc, const int32_t* 
   template void launch_lu_gemm<T>(const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, hipStream_t);        \
   template void launch_lu_sprep<T>(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);  \
   template void launch_lu_sfinish<T>(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t); \
-  template void launch_lu_fwd_own<T>(const LuSymDev&, const LuFacDev<T>&, const int32_t*, int, int32_t, const T*, \
-                                     const T*, T*, hipStream_t);                                                \
-  template void launch_lu_fwd_off<T>(const LuSymDev&, const LuFacDev<T>&, const int2*, int, int32_t, const T*,  \
-                                     T*, hipStream_t);                                                          \
-  template void launch_lu_bwd_off<T>(const LuSymDev&, const LuFacDev<T>&, const int2*, int, int32_t, T*,        \
-                                     hipStream_t);                                                              \
-  template void launch_lu_bwd_own<T>(const LuSymDev&, const LuFacDev<T>&, const int32_t*, int, int32_t, T*,     \
-                                     hipStream_t);
+  template int lu_solve_bs<T>();                                                                              \
+  template void launch_lu_sstep<T>(int, const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, int32_t, const T*, \
+                                   T*, T*, hipStream_t);
 RSL_LU_INST(double)
 RSL_LU_INST(cplx)
 

@@ -709,6 +709,17 @@ int rsl_lu_solve_device(rsl_lu_solver* f, const void* d_b, int64_t nrhs, void* d
   });
 }
 
+int rsl_lu_refine(rsl_lu_solver* f, const double* a_values, const double* b, int64_t nrhs, int32_t max_iter,
+                  double* x, double* info) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (nrhs < 0) throw rsl::Error(RSL_DIMENSION_MISMATCH, "negative nrhs");
+    if (max_iter < 0) throw rsl::Error(RSL_INVALID_INPUT, "negative max_iter");
+    if (f->f->sym->plan.nnz > 0 && !a_values) throw rsl::Error(RSL_INVALID_INPUT, "null values");
+    f->f->refine(a_values, b, nrhs, max_iter, x, info);
+  });
+}
+
 int rsl_lu_info(rsl_lu_solver* f, int64_t* out, double* t) {
   return guard([&] {
     std::lock_guard<std::mutex> lk(f->mu);

@@ -1,7 +1,8 @@
 // Execution plan of the LU twin: storage offsets, the A -> L / U12 scatter
 // map, the cmod target tiles of both regions with their updater tasks, the
 // per-level panel steps (the reference's panel widths, 32 or 128 from 512
-// columns up, multifrontal_lu.rs:2020), and the solve's gather lists.
+// columns up, multifrontal_lu.rs:2020), and the solve's gather lists (the
+// solve's block steps depend on the field and are laid out by the factor).
 #include "lu_plan.hpp"
 
 #include <algorithm>
@@ -139,6 +140,22 @@ void build_lu_plan(LuPlan& P, const int64_t* colp, const int32_t* rowi) {
       P.amap[k] = -1 - (P.u_off[si] + (gi - P.sn_first[si]) + (lc - P.sn_ncol[si]) * P.sn_ncol[si]);
     }
 
+  // CSR of A in the user numbering, for the refinement's residuals
+  P.csr_ptr.assign((size_t)n + 1, 0);
+  for (int64_t k = 0; k < P.nnz; ++k) P.csr_ptr[rowi[k] + 1]++;
+  for (int32_t i = 0; i < n; ++i) P.csr_ptr[i + 1] += P.csr_ptr[i];
+  P.csr_col.resize(P.nnz);
+  P.csr_src.resize(P.nnz);
+  {
+    std::vector<int64_t> fill(P.csr_ptr.begin(), P.csr_ptr.end() - 1);
+    for (int32_t j = 0; j < n; ++j)
+      for (int64_t k = colp[j]; k < colp[j + 1]; ++k) {
+        const int64_t q = fill[rowi[k]]++;
+        P.csr_col[q] = j;
+        P.csr_src[q] = k;
+      }
+  }
+
   // levels
   P.levels.assign(ns ? maxlev + 1 : 0, LuLevelPlan());
   for (int32_t s = 0; s < ns; ++s) P.levels[level[s]].all.push_back(s);
@@ -261,8 +278,7 @@ void build_lu_plan(LuPlan& P, const int64_t* colp, const int32_t* rowi) {
   }
   if (P.units.size() > (size_t)INT32_MAX) throw std::runtime_error("lu plan: too many panel units");
 
-  // solve: scratch rows that update each permuted row (ascending updater),
-  // forward off-diagonal units and backward U12 units per level
+  // solve: the scratch rows that update each permuted row (ascending updater)
   P.gat_ptr.assign((size_t)n + 1, 0);
   for (int32_t k = 0; k < ns; ++k)
     for (int64_t q = P.rs_off[k] + P.sn_ncol[k]; q < P.rs_off[k + 1]; ++q) P.gat_ptr[P.rs_idx[q] + 1]++;
@@ -273,23 +289,6 @@ void build_lu_plan(LuPlan& P, const int64_t* colp, const int32_t* rowi) {
     for (int32_t k = 0; k < ns; ++k)
       for (int64_t q = P.rs_off[k] + P.sn_ncol[k]; q < P.rs_off[k + 1]; ++q)
         P.gat_idx[fill[P.rs_idx[q]]++] = P.s_off[k] + (q - P.rs_off[k] - P.sn_ncol[k]);
-  }
-  P.sunits.clear();
-  for (LuLevelPlan& L : P.levels) {
-    L.fo_beg = (int64_t)P.sunits.size() / 2;
-    for (int32_t s : L.all)
-      for (int32_t r0 = 0; r0 < P.sn_nrow[s] - P.sn_ncol[s]; r0 += TILE) {
-        P.sunits.push_back(s);
-        P.sunits.push_back(r0);
-      }
-    L.fo_end = L.bo_beg = (int64_t)P.sunits.size() / 2;
-    for (int32_t s : L.all)
-      if (P.sn_nrow[s] > P.sn_ncol[s])
-        for (int32_t p0 = 0; p0 < P.sn_ncol[s]; p0 += TILE) {
-          P.sunits.push_back(s);
-          P.sunits.push_back(p0);
-        }
-    L.bo_end = (int64_t)P.sunits.size() / 2;
   }
 }
 

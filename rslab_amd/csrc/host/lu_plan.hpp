@@ -22,8 +22,6 @@ struct LuLevelPlan {
   int64_t item_beg = 0, item_end = 0;  // cmod target tiles
   double cmod_flops = 0;
   std::vector<LuStepPlan> steps;   // one per panel index
-  int64_t fo_beg = 0, fo_end = 0;  // solve: forward off-diagonal units (s, first row)
-  int64_t bo_beg = 0, bo_end = 0;  // solve: backward U12 units (s, first own row)
 };
 
 struct LuPlan {
@@ -38,6 +36,8 @@ struct LuPlan {
   std::vector<int64_t> l_off, u_off, s_off;
   int64_t l_total = 0, u_total = 0, s_total = 0;
   std::vector<int64_t> amap;            // user nnz -> L slot, or -1 - U slot
+  std::vector<int64_t> csr_ptr, csr_src;  // CSR of A (residuals): row pointers, CSC index of each entry
+  std::vector<int32_t> csr_col;
   std::vector<int64_t> gat_ptr, gat_idx;
   std::vector<int32_t> rperm_init;      // identity own-row positions (first + p -> p)
   std::vector<LuLevelPlan> levels;
@@ -46,7 +46,6 @@ struct LuPlan {
   std::vector<LuItem> items;
   std::vector<CTask> tasks;
   std::vector<LuUnit> units;
-  std::vector<int32_t> sunits;          // solve units as (s, row) int pairs
   double factor_flops = 0;              // sum_s sum_{k<w} 2 (h-k-1)^2 + (h-k-1) (real field)
   int64_t l_entries = 0, u_entries = 0; // stored L (incl. diagonal) and U (incl. diagonal) of the supernodes
 };

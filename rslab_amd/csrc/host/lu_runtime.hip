@@ -16,13 +16,15 @@ template <class T> void launch_lu_apply(const LuSymDev&, const LuFacDev<T>&, con
 template <class T> void launch_lu_gemm(const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, hipStream_t);
 template <class T> void launch_lu_sprep(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);
 template <class T> void launch_lu_sfinish(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);
-template <class T> void launch_lu_fwd_own(const LuSymDev&, const LuFacDev<T>&, const int32_t*, int, int32_t, const T*,
-                                          const T*, T*, hipStream_t);
-template <class T> void launch_lu_fwd_off(const LuSymDev&, const LuFacDev<T>&, const int2*, int, int32_t, const T*, T*,
-                                          hipStream_t);
-template <class T> void launch_lu_bwd_off(const LuSymDev&, const LuFacDev<T>&, const int2*, int, int32_t, T*, hipStream_t);
-template <class T> void launch_lu_bwd_own(const LuSymDev&, const LuFacDev<T>&, const int32_t*, int, int32_t, T*,
-                                          hipStream_t);
+template <class T> int lu_solve_bs();
+template <class T> void launch_lu_sstep(int, const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, int32_t, const T*,
+                                        T*, T*, hipStream_t);
+template <class T> void launch_expand(int64_t, const int64_t*, const T*, T*, hipStream_t);
+template <class T> void launch_residual(int32_t, int32_t, const int64_t*, const int32_t*, const T*, const T*, const T*, T*,
+                                        hipStream_t, LaunchProfiler*);
+template <class T> void launch_colmax(int32_t, int32_t, const T*, unsigned long long*, hipStream_t);
+template <class T> void launch_update_cols(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);
+template <class T> void launch_copy_cols(int32_t, int32_t, const uint8_t*, T*, const T*, hipStream_t);
 
 LuDeviceSymbolic::~LuDeviceSymbolic() {
   if (stream) (void)hipStreamDestroy(stream);
@@ -51,8 +53,10 @@ void LuDeviceSymbolic::upload() {
   items.upload(P.items, st);
   tasks.upload(P.tasks, st);
   units.upload(P.units, st);
-  sunits.upload(P.sunits, st);
   rperm_init.upload(P.rperm_init, st);
+  csr_ptr.upload(P.csr_ptr, st);
+  csr_col.upload(P.csr_col, st);
+  csr_src.upload(P.csr_src, st);
   hip_check(hipStreamSynchronize(st), "upload");
   view.n = P.n;
   view.ns = P.ns;
@@ -101,6 +105,16 @@ template <class T>
 class LuDeviceFactor final : public LuFactorBase {
  public:
   DBuf l, u, rperm, ipiv, status, counters, vals, drow, dcol, bh, yv, scr, bx;
+  DBuf r_a, r_acsr, r_b, r_x, r_best, r_r, r_dx, r_max, r_mask;
+  // solve work units (their block width depends on the field): per level the
+  // gather, the forward block steps (diagonal, update), the U12 units and the
+  // backward block steps
+  struct SolveLevel {
+    int64_t g0 = 0, g1 = 0, o0 = 0, o1 = 0;
+    std::vector<int64_t> fs, bs;  // per block step: diagonal begin, end, update begin, end
+  };
+  std::vector<SolveLevel> slev;
+  DBuf su;
   std::vector<T> h_vals;
   LuFacDev<T> fv{};
   hipEvent_t ev_caller = nullptr, ev_done = nullptr;
@@ -131,6 +145,61 @@ class LuDeviceFactor final : public LuFactorBase {
     fv.thresh_sq = pu * pu;
     hip_check(hipEventCreateWithFlags(&ev_caller, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming), "hipEventCreate");
+    plan_solve();
+  }
+
+  void plan_solve() {
+    const LuPlan& P = sym->plan;
+    const int BS = lu_solve_bs<T>();
+    std::vector<LuUnit> u;
+    slev.assign(P.levels.size(), SolveLevel());
+    for (size_t l = 0; l < P.levels.size(); ++l) {
+      const LuLevelPlan& L = P.levels[l];
+      SolveLevel& sl = slev[l];
+      sl.g0 = (int64_t)u.size();
+      for (int32_t s : L.all)
+        for (int32_t p0 = 0; p0 < P.sn_ncol[s]; p0 += 64) u.push_back({s, 0, p0, 0});
+      sl.g1 = (int64_t)u.size();
+      int nblk = 0;
+      for (int32_t s : L.all) nblk = std::max(nblk, (P.sn_ncol[s] + BS - 1) / BS);
+      for (int i = 0; i < nblk; ++i) {
+        sl.fs.push_back((int64_t)u.size());
+        for (int32_t s : L.all)
+          if (i * BS < P.sn_ncol[s]) u.push_back({s, i * BS, 0, 0});
+        sl.fs.push_back((int64_t)u.size());
+        sl.fs.push_back((int64_t)u.size());
+        for (int32_t s : L.all) {
+          const int32_t w = P.sn_ncol[s], h = P.sn_nrow[s], j0 = i * BS;
+          if (j0 >= w) continue;
+          for (int32_t r0 = j0 + std::min(BS, w - j0); r0 < h; r0 += 64) u.push_back({s, j0, r0, 0});
+        }
+        sl.fs.push_back((int64_t)u.size());
+      }
+      sl.o0 = (int64_t)u.size();
+      for (int32_t s : L.all)
+        if (P.sn_nrow[s] > P.sn_ncol[s])
+          for (int32_t p0 = 0; p0 < P.sn_ncol[s]; p0 += 16) u.push_back({s, 0, p0, 0});
+      sl.o1 = (int64_t)u.size();
+      for (int i = 0; i < nblk; ++i) {
+        sl.bs.push_back((int64_t)u.size());
+        for (int32_t s : L.all) {
+          const int32_t nb_s = (P.sn_ncol[s] + BS - 1) / BS;
+          if (i < nb_s) u.push_back({s, (nb_s - 1 - i) * BS, 0, 0});
+        }
+        sl.bs.push_back((int64_t)u.size());
+        sl.bs.push_back((int64_t)u.size());
+        for (int32_t s : L.all) {
+          const int32_t nb_s = (P.sn_ncol[s] + BS - 1) / BS;
+          if (i >= nb_s) continue;
+          const int32_t j0 = (nb_s - 1 - i) * BS;
+          for (int32_t r0 = 0; r0 < j0; r0 += 64) u.push_back({s, j0, r0, 0});
+        }
+        sl.bs.push_back((int64_t)u.size());
+      }
+    }
+    if (u.size() > (size_t)INT32_MAX) throw Error(INVALID_INPUT, "lu solve: too many work units");
+    su.upload(u, sym->stream);
+    hip_check(hipStreamSynchronize(sym->stream), "upload");
   }
   ~LuDeviceFactor() override {
     if (ev_caller) (void)hipEventDestroy(ev_caller);
@@ -236,18 +305,26 @@ class LuDeviceFactor final : public LuFactorBase {
     T* bhp = bh.as<T>();
     T* y = yv.as<T>();
     T* sc = scr.as<T>();
-    const int32_t* lists = sym->level_lists.as<int32_t>();
-    const int2* su = sym->sunits.as<int2>();
+    const LuUnit* u = su.as<LuUnit>();
     launch_lu_sprep<T>(P.n, nrhs, S.perm, drow.as<double>(), b, bhp, st);
-    for (size_t l = 0; l < P.levels.size(); ++l) {
-      const LuLevelPlan& L = P.levels[l];
-      launch_lu_fwd_own<T>(S, fv, lists + P.level_off[l], (int)L.all.size(), nrhs, bhp, sc, y, st);
-      launch_lu_fwd_off<T>(S, fv, su + L.fo_beg, (int)(L.fo_end - L.fo_beg), nrhs, y, sc, st);
+    auto step = [&](int kind, int64_t a, int64_t e) {
+      launch_lu_sstep<T>(kind, S, fv, u + a, (int)(e - a), nrhs, bhp, sc, y, st);
+    };
+    for (size_t l = 0; l < slev.size(); ++l) {
+      const SolveLevel& sl = slev[l];
+      step(0, sl.g0, sl.g1);
+      for (size_t i = 0; i < sl.fs.size(); i += 4) {
+        step(1, sl.fs[i], sl.fs[i + 1]);
+        step(2, sl.fs[i + 2], sl.fs[i + 3]);
+      }
     }
-    for (size_t l = P.levels.size(); l-- > 0;) {
-      const LuLevelPlan& L = P.levels[l];
-      launch_lu_bwd_off<T>(S, fv, su + L.bo_beg, (int)(L.bo_end - L.bo_beg), nrhs, y, st);
-      launch_lu_bwd_own<T>(S, fv, lists + P.level_off[l], (int)L.all.size(), nrhs, y, st);
+    for (size_t l = slev.size(); l-- > 0;) {
+      const SolveLevel& sl = slev[l];
+      step(3, sl.o0, sl.o1);
+      for (size_t i = 0; i < sl.bs.size(); i += 4) {
+        step(4, sl.bs[i], sl.bs[i + 1]);
+        step(5, sl.bs[i + 2], sl.bs[i + 3]);
+      }
     }
     launch_lu_sfinish<T>(P.n, nrhs, S.perm, dcol.as<double>(), y, x, st);
     hip_check(hipGetLastError(), "lu solve launch");
@@ -277,6 +354,71 @@ class LuDeviceFactor final : public LuFactorBase {
     enqueue_solve(static_cast<const T*>(d_b), (int32_t)nrhs, static_cast<T*>(d_x));
     hip_check(hipEventRecord(ev_done, sym->stream), "hipEventRecord");
     hip_check(hipStreamWaitEvent(caller, ev_done, 0), "hipStreamWaitEvent");
+  }
+
+  void refine(const double* a_values, const double* b, int64_t nrhs, int max_iter, double* x,
+              double* info) override {
+    const LuPlan& P = sym->plan;
+    const int32_t n = P.n;
+    if (nrhs <= 0 || n == 0) return;
+    if (nrhs > 4096 || nrhs > INT32_MAX / n) throw Error(INVALID_INPUT, "too many right-hand sides");
+    const int32_t k = (int32_t)nrhs;
+    ensure_solve(nrhs);
+    const hipStream_t st = sym->stream;
+    const size_t vb = sizeof(T) * (size_t)n * k, ab = sizeof(T) * (size_t)std::max<int64_t>(P.nnz, 1);
+    if (r_a.bytes < ab) r_a.alloc(ab);
+    if (r_acsr.bytes < ab) r_acsr.alloc(ab);
+    for (DBuf* d : {&r_b, &r_x, &r_best, &r_r, &r_dx})
+      if (d->bytes < vb) d->alloc(vb);
+    if (r_max.bytes < sizeof(unsigned long long) * k) r_max.alloc(sizeof(unsigned long long) * k);
+    if (r_mask.bytes < (size_t)2 * k) r_mask.alloc((size_t)2 * k);
+    if (P.nnz) hip_check(hipMemcpyAsync(r_a.p, a_values, sizeof(T) * P.nnz, hipMemcpyHostToDevice, st), "A");
+    launch_expand<T>(P.nnz, sym->csr_src.as<int64_t>(), r_a.as<T>(), r_acsr.as<T>(), st);
+    hip_check(hipMemcpyAsync(r_b.p, b, vb, hipMemcpyHostToDevice, st), "b");
+    T* xd = r_x.as<T>();
+    enqueue_solve(r_b.as<T>(), k, xd);
+    std::vector<double> best(k, INFINITY);
+    std::vector<int32_t> iters(k, 0);
+    std::vector<uint8_t> keep(k), active(k, 1);
+    std::vector<unsigned long long> mx(k);
+    uint8_t* keep_d = r_mask.as<uint8_t>();
+    uint8_t* act_d = keep_d + k;
+    for (int it = 0; it <= max_iter; ++it) {
+      launch_residual<T>(n, k, sym->csr_ptr.as<int64_t>(), sym->csr_col.as<int32_t>(), r_acsr.as<T>(), xd,
+                         r_b.as<T>(), r_r.as<T>(), st, nullptr);
+      hip_check(hipMemsetAsync(r_max.p, 0, sizeof(unsigned long long) * k, st), "memset");
+      launch_colmax<T>(n, k, r_r.as<T>(), r_max.as<unsigned long long>(), st);
+      hip_check(hipMemcpyAsync(mx.data(), r_max.p, sizeof(unsigned long long) * k, hipMemcpyDeviceToHost, st), "max");
+      hip_check(hipStreamSynchronize(st), "refine");
+      bool any = false;
+      for (int c = 0; c < k; ++c) {
+        keep[c] = 0;
+        if (!active[c]) continue;
+        double res;
+        std::memcpy(&res, &mx[c], sizeof(res));
+        if (res < best[c]) {
+          best[c] = res;
+          keep[c] = 1;
+        }
+        if (res == 0.0 || it == max_iter) active[c] = 0;
+        any = any || active[c];
+      }
+      hip_check(hipMemcpyAsync(keep_d, keep.data(), k, hipMemcpyHostToDevice, st), "mask");
+      hip_check(hipMemcpyAsync(act_d, active.data(), k, hipMemcpyHostToDevice, st), "mask");
+      launch_copy_cols<T>(n, k, keep_d, r_best.as<T>(), xd, st);
+      if (!any) break;
+      enqueue_solve(r_r.as<T>(), k, r_dx.as<T>());
+      launch_update_cols<T>(n, k, act_d, xd, r_dx.as<T>(), st);
+      for (int c = 0; c < k; ++c) iters[c] += active[c];
+      hip_check(hipStreamSynchronize(st), "refine");  // host masks are rewritten next step
+    }
+    hip_check(hipMemcpyAsync(x, r_best.p, vb, hipMemcpyDeviceToHost, st), "x");
+    hip_check(hipStreamSynchronize(st), "refine");
+    if (info)
+      for (int c = 0; c < k; ++c) {
+        info[2 * c] = iters[c];
+        info[2 * c + 1] = best[c];
+      }
   }
 
   // emit_and_free (multifrontal_lu.rs:1525-1618) over the downloaded panels

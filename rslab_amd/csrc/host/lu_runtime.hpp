@@ -15,7 +15,7 @@ struct LuDeviceSymbolic {
   LuPlan plan;
   hipStream_t stream = nullptr;
   DBuf sn_first, sn_ncol, sn_nrow, nb, rs_off, rs_idx, l_off, u_off, s_off, gat_ptr, gat_idx, perm, amap,
-      level_lists, items, tasks, units, sunits, rperm_init;
+      level_lists, items, tasks, units, rperm_init, csr_ptr, csr_col, csr_src;
   LuSymDev view{};
   bool uploaded = false;
   ~LuDeviceSymbolic();
@@ -48,6 +48,12 @@ class LuFactorBase {
   // and `caller` waits for the solve
   virtual void solve_device(const void* d_b, int64_t nrhs, void* d_x, hipStream_t caller) = 0;
   virtual const LuEmitted& emitted() = 0;
+  // solve_lu_refined (multifrontal_lu.rs:3285-3322) per column, on the
+  // device: x += (LU)^-1 (b - A x) with the residual against a_values (the
+  // analysed pattern's values, host), keeping each column's iterate of least
+  // max |r|; info[2c..2c+2) = {corrections applied, that least max |r|}
+  virtual void refine(const double* a_values, const double* b, int64_t nrhs, int max_iter, double* x,
+                      double* info) = 0;
   std::vector<double> d_row, d_col;
   int64_t n_perturbed = 0;
   double t_factor_ms = 0;

@@ -273,21 +273,28 @@ class LuSolver:
         _check(lib.rsl_lu_solve_device(self._h, C.c_void_p(d_b), int(nrhs), C.c_void_p(d_x),
                                        C.c_void_p(stream) if stream else None))
 
-    def solve_refined(self, a: GeneralCsc, b, max_iter: int):
-        """solve_lu_refined (:3285-3322): x += (LU)^-1 (b - A x), keeping the
-        iterate of least |r|inf; every correction is evaluated."""
+    def solve_refined(self, a: GeneralCsc, b, max_iter: int, info: bool = False):
+        """solve_lu_refined (:3285-3322) on the device, per column: x =
+        (LU)^-1 b, then up to max_iter corrections x += (LU)^-1 (b - A x),
+        every correction evaluated, keeping each column's iterate of least
+        max |b - A x|. info=True also returns per column (corrections
+        applied, that least max |r|)."""
+        if a.n != self._n:
+            raise DimensionMismatch(f"expected a matrix of dimension {self._n}, got {a.n}")
         b = np.asarray(b)
-        x = self.solve_many(b)
-        best, best_res = x.copy(), np.inf
-        for it in range(max_iter + 1):
-            r = b - a.matvec(x)
-            res = float(np.abs(r).max()) if r.size else 0.0
-            if res < best_res:
-                best_res, best = res, x.copy()
-            if res == 0.0 or it == max_iter:
-                break
-            x = x + self.solve_many(r)
-        return best
+        single = b.ndim == 1
+        B = np.ascontiguousarray(b.reshape(self._n, -1), dtype=self._dtype())
+        X = np.zeros_like(B)
+        inf = np.zeros(2 * B.shape[1])
+        bv = B.view(np.float64) if self._cplx else B
+        xv = X.view(np.float64) if self._cplx else X
+        vals = a.values_f64()
+        _check(lib.rsl_lu_refine(self._h, ptr(vals, C.c_double), ptr(bv, C.c_double), B.shape[1], int(max_iter),
+                                 ptr(xv, C.c_double), ptr(inf, C.c_double)))
+        X = X.reshape(self._n) if single else X
+        if info:
+            return X, [(int(inf[2 * c]), float(inf[2 * c + 1])) for c in range(B.shape[1])]
+        return X
 
     def info(self) -> dict:
         out = np.zeros(6, dtype=np.int64)
