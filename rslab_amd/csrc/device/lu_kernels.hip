@@ -21,6 +21,7 @@
 
 #include "lu_types.hpp"
 #include "scalar.hpp"
+#include "tile_mfma.hpp"
 
 namespace rsl {
 
@@ -162,6 +163,90 @@ __global__ __launch_bounds__(LU_NT) void k_lu_cmod(LuSymDev S, LuFacDev<T> Fc, c
     }
 }
 
+// cmod of the real field on the matrix cores (v_mfma_f64_16x16x4f64): the
+// same target tiles and task order as k_lu_cmod, operands gathered into LDS
+// 32 columns of K at a time (the next chunk's loads issued before the current
+// chunk's products), every updater's product accumulated in FP64.
+__global__ __launch_bounds__(LU_NT) void k_lu_cmod_mfma(LuSymDev S, LuFacDev<double> Fc,
+                                                        const LuItem* __restrict__ items,
+                                                        const CTask* __restrict__ tasks) {
+  typedef TileCfgW<TP_F64> C;
+  constexpr int KCW = C::KC, PER = KCW * 64 / LU_NT;
+  __shared__ __align__(16) double As[C::ELEMS];
+  __shared__ __align__(16) double Bs[C::ELEMS];
+  const int tid = threadIdx.x;
+  const LuItem it = items[blockIdx.x];
+  double va[PER], vb[PER];
+  auto fetch = [&](int tt, int cc) {
+    const CTask tk = tasks[tt];
+    const int k = tk.k, nck = S.sn_ncol[k];
+    const int64_t hk = S.sn_nrow[k];
+    const double* Lk = Fc.l + S.l_off[k];
+    const double* Uk = Fc.u + S.u_off[k];
+    const int kc = min(KCW, nck - cc);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + LU_NT * u, r = e & 63, c = e >> 6;
+      const uint64_t below = (1ull << r) - 1ull;
+      va[u] = 0.0;
+      vb[u] = 0.0;
+      if (c < kc) {
+        if ((tk.rmask >> r) & 1ull) va[u] = Lk[(nck + tk.a0 + __popcll(tk.rmask & below)) + (int64_t)(cc + c) * hk];
+        if ((tk.cmask >> r) & 1ull) vb[u] = Uk[(cc + c) + (int64_t)(tk.b0 + __popcll(tk.cmask & below)) * nck];
+      }
+    }
+  };
+  int t = it.t_beg, c0 = 0;
+  int nck_t = S.sn_ncol[tasks[t].k];
+  fetch(t, 0);
+  TileAcc<TP_F64> acc;
+  tile_zero(acc);
+  for (;;) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + LU_NT * u, r = e & 63, c = e >> 6;
+      stage_store_w<TP_F64>(As, r, c, va[u]);
+      stage_store_w<TP_F64>(Bs, r, c, vb[u]);
+    }
+    __syncthreads();
+    int tn = t, cn = c0 + KCW, nck_n = nck_t;
+    if (cn >= nck_t) {
+      tn = t + 1;
+      cn = 0;
+      if (tn < it.t_end) nck_n = S.sn_ncol[tasks[tn].k];
+    }
+    if (tn < it.t_end) fetch(tn, cn);
+    tile_mma_w<TP_F64>(As, Bs, acc);
+    __syncthreads();
+    if (tn >= it.t_end) break;
+    t = tn;
+    c0 = cn;
+    nck_t = nck_n;
+  }
+  const int s = it.s;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s];
+  double* P;
+  int64_t ld, mr, mc;
+  if (it.region == 0) {
+    P = Fc.l + S.l_off[s];
+    ld = h;
+    mr = h - it.r0;
+    mc = w - it.c0;
+  } else {
+    P = Fc.u + S.u_off[s];
+    ld = w;
+    mr = w - it.r0;
+    mc = h - w - it.c0;
+  }
+  tile_for_each<TP_F64>(acc, [&](int i, int j, double v) {
+    if (i < mr && j < mc) {
+      double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * ld;
+      *dst = *dst - v;
+    }
+  });
+}
+
 // getf2 on the own rows of one panel (multifrontal_lu.rs:2064-2111): per
 // column, the pivot is the first row of largest magnitude in [k, w); the rows
 // are interchanged only when the diagonal is below u times it; the
@@ -169,9 +254,9 @@ __global__ __launch_bounds__(LU_NT) void k_lu_cmod(LuSymDev S, LuFacDev<T> Fc, c
 // rperm) for the other columns; then the column is scaled and the panel's
 // later columns get the rank-1 update.
 template <class T>
-__global__ __launch_bounds__(LU_NT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units) {
-  __shared__ double red_m[LU_NT / 64];
-  __shared__ int red_i[LU_NT / 64];
+__global__ __launch_bounds__(LU_GT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units) {
+  __shared__ double red_m[LU_GT / 64];
+  __shared__ int red_i[LU_GT / 64];
   __shared__ T ukj[128];
   __shared__ int sh_p;
   __shared__ T sh_pinv;
@@ -187,7 +272,7 @@ __global__ __launch_bounds__(LU_NT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, 
     if (Fc.thresh_sq > 0.0) {
       double m = -1.0;
       int mi = INT_MAX;
-      for (int i = k + tid; i < w; i += LU_NT) {
+      for (int i = k + tid; i < w; i += LU_GT) {
         const double v = F<T>::magsq(ck[i]);
         if (v > m) {
           m = v;
@@ -211,7 +296,7 @@ __global__ __launch_bounds__(LU_NT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, 
       if (tid == 0) {
         double bm = red_m[0];
         int bi = red_i[0];
-        for (int q = 1; q < LU_NT / 64; ++q)
+        for (int q = 1; q < LU_GT / 64; ++q)
           if (red_m[q] > bm || (red_m[q] == bm && red_i[q] < bi)) {
             bm = red_m[q];
             bi = red_i[q];
@@ -223,7 +308,7 @@ __global__ __launch_bounds__(LU_NT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, 
       p = sh_p;
     }
     if (p != k) {
-      for (int j = kb + tid; j < ke; j += LU_NT) {
+      for (int j = kb + tid; j < ke; j += LU_GT) {
         T* cj = L + (int64_t)j * h;
         const T t = cj[k];
         cj[k] = cj[p];
@@ -252,10 +337,10 @@ __global__ __launch_bounds__(LU_NT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, 
       ck[k] = piv;
       sh_pinv = F<T>::recip(piv);
     }
-    for (int j = k + 1 + tid; j < ke; j += LU_NT) ukj[j - kb] = L[(int64_t)j * h + k];
+    for (int j = k + 1 + tid; j < ke; j += LU_GT) ukj[j - kb] = L[(int64_t)j * h + k];
     __syncthreads();
     const T pinv = sh_pinv;
-    for (int i = k + 1 + tid; i < w; i += LU_NT) {
+    for (int i = k + 1 + tid; i < w; i += LU_GT) {
       const T lik = ck[i] * pinv;
       ck[i] = lik;
       for (int j = k + 1; j < ke; ++j) {
@@ -495,12 +580,13 @@ __global__ __launch_bounds__(LU_NT) void k_lu_fupd(LuSymDev S, LuFacDev<T> Fc, c
 }
 
 // Backward U12 terms, 16 own rows per unit (s, -, first row), the trailing
-// columns split over 16 slices summed in LDS: y_s[p] -= U12[p, :] x[rs[w..]]
-// (the ancestors' x are final).
+// columns split over 16 slices summed in LDS, 8 right-hand sides at a time:
+// y_s[p] -= U12[p, :] x[rs[w..]] (the ancestors' x are final).
 template <class T>
 __global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
                                                    int32_t nrhs, T* __restrict__ y) {
-  __shared__ T part[16][17];
+  constexpr int CW = 8;
+  __shared__ T part[16][16][CW + 1];
   const LuUnit un = units[blockIdx.x];
   const int s = un.s;
   const int64_t h = S.sn_nrow[s];
@@ -510,17 +596,31 @@ __global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, c
   const int64_t nt = h - w;
   const int i = threadIdx.x & 15, sl = threadIdx.x >> 4;
   const int64_t p = un.a + i;
-  for (int c = 0; c < nrhs; ++c) {
-    T acc = F<T>::zero();
+  for (int c0 = 0; c0 < nrhs; c0 += CW) {
+    const int cw = min(CW, nrhs - c0);
+    T acc[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c) acc[c] = F<T>::zero();
     if (p < w)
-      for (int64_t t = sl; t < nt; t += 16) acc = lu_fma(U[p + t * w], y[(int64_t)rs[t] * nrhs + c], acc);
-    part[sl][i] = acc;
+      for (int64_t t = sl; t < nt; t += 16) {
+        const T u = U[p + t * w];
+        const T* xr = y + (int64_t)rs[t] * nrhs + c0;
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+          if (c < cw) acc[c] = lu_fma(u, xr[c], acc[c]);
+      }
+#pragma unroll
+    for (int c = 0; c < CW; ++c) part[sl][i][c] = acc[c];
     __syncthreads();
-    if (sl == 0 && p < w) {
-      T sum = part[0][i];
-      for (int q = 1; q < 16; ++q) sum = sum + part[q][i];
-      T* d = y + (first + p) * nrhs + c;
-      *d = *d - sum;
+    if (threadIdx.x < 16 * CW) {
+      const int ii = threadIdx.x & 15, c = threadIdx.x >> 4;
+      const int64_t pp = un.a + ii;
+      if (pp < w && c < cw) {
+        T sum = part[0][ii][c];
+        for (int q = 1; q < 16; ++q) sum = sum + part[q][ii][c];
+        T* d = y + (first + pp) * nrhs + c0 + c;
+        *d = *d - sum;
+      }
     }
     __syncthreads();
   }
@@ -587,12 +687,15 @@ template <class T>
 void launch_lu_cmod(const LuSymDev& S, const LuFacDev<T>& Fc, const LuItem* items, int nitems, const CTask* tasks,
                     hipStream_t st) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL(k_lu_cmod<T>, dim3((unsigned)nitems), dim3(LU_NT), 0, st, S, Fc, items, tasks);
+  if constexpr (sizeof(T) == 8)
+    hipLaunchKernelGGL(k_lu_cmod_mfma, dim3((unsigned)nitems), dim3(LU_NT), 0, st, S, Fc, items, tasks);
+  else
+    hipLaunchKernelGGL(k_lu_cmod<T>, dim3((unsigned)nitems), dim3(LU_NT), 0, st, S, Fc, items, tasks);
 }
 template <class T>
 void launch_lu_getf2(const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_lu_getf2<T>, dim3((unsigned)n), dim3(LU_NT), 0, st, S, Fc, units);
+  hipLaunchKernelGGL(k_lu_getf2<T>, dim3((unsigned)n), dim3(LU_GT), 0, st, S, Fc, units);
 }
 template <class T>
 void launch_lu_apply(const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, hipStream_t st) {

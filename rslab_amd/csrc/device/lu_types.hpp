@@ -13,7 +13,7 @@ struct LuSymDev {
   const int32_t* sn_first;   // first column (permuted numbering)
   const int32_t* sn_ncol;    // own columns w
   const int32_t* sn_nrow;    // rows h = |rs[s]| (own rows first)
-  const int32_t* nb;         // panel width of s: 32, or 128 from 512 columns up
+  const int32_t* nb;         // device panel width of s
   const int64_t* rs_off;     // row structures
   const int32_t* rs_idx;
   const int64_t* l_off;      // h x w column-major: L below the diagonal, U11 on and above
@@ -55,7 +55,8 @@ struct LuUnit {
   int32_t s, kb, a, b;
 };
 
-constexpr int LU_NT = 256;     // threads per workgroup of every LU kernel
+constexpr int LU_NT = 256;     // threads per workgroup of the LU kernels but getf2
+constexpr int LU_GT = 1024;    // threads of the getf2 workgroup (one per supernode panel)
 constexpr int LU_COLS = 256;   // columns (rows) per apply unit
 
 }  // namespace rsl

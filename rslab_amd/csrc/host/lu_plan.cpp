@@ -1,8 +1,15 @@
 // Execution plan of the LU twin: storage offsets, the A -> L / U12 scatter
 // map, the cmod target tiles of both regions with their updater tasks, the
-// per-level panel steps (the reference's panel widths, 32 or 128 from 512
-// columns up, multifrontal_lu.rs:2020), and the solve's gather lists (the
-// solve's block steps depend on the field and are laid out by the factor).
+// per-level panel steps, and the solve's gather lists (the solve's block
+// steps depend on the field and are laid out by the factor).
+//
+// Panel width: the reference blocks its cdiv in panels of 32 columns, 128
+// from 512 columns up (multifrontal_lu.rs:2020). The pivot search spans every
+// own row below the diagonal, so each column's rank-1 update inside a panel
+// sweeps all those rows for every later column of the panel; on the device
+// one workgroup runs a panel, and 16-column panels keep that sweep short
+// while the trailing updates run as multi-workgroup GEMMs. The blocking
+// changes only the order of the sums, not the pivot rule.
 #include "lu_plan.hpp"
 
 #include <algorithm>
@@ -87,7 +94,7 @@ void build_lu_plan(LuPlan& P, const int64_t* colp, const int32_t* rowi) {
     P.sn_first[s] = sn.first_col;
     P.sn_ncol[s] = sn.ncol;
     P.sn_nrow[s] = (int32_t)h;
-    P.nb[s] = sn.ncol >= 512 ? 128 : 32;
+    P.nb[s] = LU_PANEL;
     P.rs_off[s + 1] = P.rs_off[s] + h;
     P.l_off[s] = P.l_total;
     P.l_total += h * w;

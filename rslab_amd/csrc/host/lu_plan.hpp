@@ -10,6 +10,8 @@
 
 namespace rsl {
 
+constexpr int LU_PANEL = 16;  // device panel width of the LU cdiv
+
 struct LuStepPlan {
   int64_t g_beg = 0, g_end = 0;  // getf2 units
   int64_t a_beg = 0, a_end = 0;  // apply units (columns, deep rows)
@@ -30,7 +32,7 @@ struct LuPlan {
   int64_t nnz = 0;
   std::vector<int64_t> colp;  // the full (unsymmetric) user pattern
   std::vector<int32_t> rowi;
-  std::vector<int32_t> sn_first, sn_ncol, sn_nrow, nb;
+  std::vector<int32_t> sn_first, sn_ncol, sn_nrow, nb;  // nb: LU_PANEL
   std::vector<int64_t> rs_off;
   std::vector<int32_t> rs_idx;
   std::vector<int64_t> l_off, u_off, s_off;

@@ -84,8 +84,8 @@ def test_lu_matches_oracle_on_convection_diffusion(cplx, ordering, u):
 
 
 def test_lu_larger_grid_exercises_wide_panels():
-    """16^3 (n = 4096): the top separators are wider than the 32-column
-    panels, and pivoting with u = 1 moves rows in most panels."""
+    """16^3 (n = 4096): the top separators span several panels, and
+    pivoting with u = 1 moves rows in most panels."""
     a, B = matgen.lu_workload(m=16, nrhs=4)
     st = R.SolverSettings().with_ordering(ND).with_pivot_u(1.0)
     sym, fd, fo = lu_pair(a, st)
@@ -98,7 +98,8 @@ def test_lu_larger_grid_exercises_wide_panels():
 @pytest.mark.parametrize("n", [200, 600])
 def test_dense_front_pivots_equal_lapack(n):
     """One dense supernode (Natural ordering): with u = 1 the row order is
-    LAPACK getrf's partial pivoting; 600 columns take the 128-column panels."""
+    LAPACK getrf's partial pivoting (the device's 16-column panels, the
+    oracle's 32 or 128, LAPACK's own blocking: the same pivots)."""
     rng = np.random.default_rng(n)
     M = rng.standard_normal((n, n))
     a = GeneralCsc.from_dense(M)
