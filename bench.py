@@ -254,10 +254,11 @@ def complex_line(R, matgen, torch, m=40, nrhs=16, steps=3, target=1e-14):
                                  "factor_plus_solve_ms": 659, "source": "BASELINE.md"})
 
 
-def lu_line(R, matgen, torch, m=60, nrhs=8, steps=3, max_iter=2, cpu=True):
+def lu_line(R, matgen, torch, m=60, nrhs=8, steps=3, max_iter=2, cpu=True, cplx=False):
     """The LU twin (src/numeric/multifrontal_lu.rs) on an unsymmetric matrix:
     3D 7-point convection-diffusion with cell Peclet numbers (0.8, 0.4, 0.2),
     shift -1.5 and diag(U[-0.01, 0.01]) on m^3 nodes (the grid of config 3),
+    plus i*0.1 on the diagonal for the complex line (c64),
     nested dissection on A + A^T, 8 right-hand sides. A step is the numeric
     refactor (two-sided equilibration on the host, the factor on the device)
     and solve_refined with up to `max_iter` corrections, values and
@@ -266,7 +267,7 @@ def lu_line(R, matgen, torch, m=60, nrhs=8, steps=3, max_iter=2, cpu=True):
     by the factor flop ratio."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    a, B = matgen.lu_workload(m=m, nrhs=nrhs)
+    a, B = matgen.lu_workload(m=m, nrhs=nrhs, cplx=cplx)
     st = R.SolverSettings().with_ordering(R.OrderingMethod.MetisND)
     t0 = time.perf_counter()
     sym = R.LuSymbolic.analyze_with(a, st)
@@ -290,16 +291,17 @@ def lu_line(R, matgen, torch, m=60, nrhs=8, steps=3, max_iter=2, cpu=True):
         t_solve.append(time.perf_counter() - t0)
     out = dict(
         workload=f"unsymmetric 3D convection-diffusion, cell Peclet (0.8, 0.4, 0.2), shift -1.5 + "
-                 f"diag(U[-0.01,0.01]), {m}^3 (n={a.n}), nested dissection on A + A^T, {nrhs} RHS, "
-                 f"refactor + solve_refined (<= {max_iter} corrections), host buffers",
+                 f"diag(U[-0.01,0.01]){' + 0.1i' if cplx else ''}, {m}^3 (n={a.n}), nested dissection on A + A^T, "
+                 f"{nrhs} RHS, refactor + solve_refined (<= {max_iter} corrections), host buffers"
+                 f"{', c64' if cplx else ''}",
         s_per_step=round(min(ts), 4), factor_ms=round(min(fts), 2), solve_ms_host=round(min(t_solve) * 1e3, 2),
-        analyze_s=round(t_an, 2), factor_flops=inf["factor_flops"],
-        factor_tflops=round(inf["factor_flops"] / (min(fts) * 1e-3) / 1e12, 3),
+        analyze_s=round(t_an, 2), factor_flops=inf["factor_flops"] * (4 if cplx else 1),
+        factor_tflops=round(inf["factor_flops"] * (4 if cplx else 1) / (min(fts) * 1e-3) / 1e12, 3),
         berr_max=float(berr.max()), corrections=max(c for c, _ in its), n_perturbed=f.n_perturbed(),
         factor_nnz=f.factor_nnz())
     if cpu:
-        mc = 24
-        ac, Bc = matgen.lu_workload(m=mc, nrhs=nrhs)
+        mc = 24 if not cplx else 20
+        ac, Bc = matgen.lu_workload(m=mc, nrhs=nrhs, cplx=cplx)
         sc = R.LuSymbolic.analyze_with(ac, st)
         fl = sc.info()["factor_flops"]
         t0 = time.perf_counter()
@@ -401,7 +403,9 @@ def main():
     torch.cuda.init()
     if args.lu_only:
         res = lu_line(R, matgen, torch, steps=args.steps, cpu=not args.no_cpu_baseline)
-        print(json.dumps({"lu": res}), flush=True)
+        log(f"lu: {res}")
+        resc = lu_line(R, matgen, torch, m=40, steps=args.steps, cpu=not args.no_cpu_baseline, cplx=True)
+        print(json.dumps({"lu": res, "lu_c64": resc}), flush=True)
         return
 
     m = args.grid
@@ -465,6 +469,9 @@ def main():
         log("LU twin line")
         lu = lu_line(R, matgen, torch, cpu=not args.no_cpu_baseline)
         log(f"lu: {lu}")
+        lu_c = lu_line(R, matgen, torch, m=40, cpu=not args.no_cpu_baseline, cplx=True)
+        log(f"lu c64: {lu_c}")
+        lu["c64_40cube"] = lu_c
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on the host cores")
