@@ -247,6 +247,102 @@ __global__ __launch_bounds__(LU_NT) void k_lu_cmod_mfma(LuSymDev S, LuFacDev<dou
   });
 }
 
+// cmod of the complex field on the FP64 matrix cores as four real products
+// per complex one (C = (Ar Br - Ai Bi) + i (Ar Bi + Ai Br)), the operands'
+// real and imaginary parts gathered into registers once per 32-column chunk
+// and staged through LDS pairwise.
+__global__ __launch_bounds__(LU_NT) void k_lu_cmod_mfma_c64(LuSymDev S, LuFacDev<cplx> Fc,
+                                                            const LuItem* __restrict__ items,
+                                                            const CTask* __restrict__ tasks) {
+  typedef TileCfgW<TP_F64> C;
+  constexpr int KCW = C::KC, PER = KCW * 64 / LU_NT;
+  __shared__ __align__(16) double As[C::ELEMS];
+  __shared__ __align__(16) double Bs[C::ELEMS];
+  const int tid = threadIdx.x;
+  const LuItem it = items[blockIdx.x];
+  cplx va[PER], vb[PER];
+  TileAcc<TP_F64> rr, ii, ri, ir;
+  tile_zero(rr);
+  tile_zero(ii);
+  tile_zero(ri);
+  tile_zero(ir);
+  auto stage = [&](bool aim, bool bim) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + LU_NT * u, r = e & 63, c = e >> 6;
+      stage_store_w<TP_F64>(As, r, c, aim ? va[u].im : va[u].re);
+      stage_store_w<TP_F64>(Bs, r, c, bim ? vb[u].im : vb[u].re);
+    }
+  };
+  for (int t = it.t_beg; t < it.t_end; ++t) {
+    const CTask tk = tasks[t];
+    const int k = tk.k, nck = S.sn_ncol[k];
+    const int64_t hk = S.sn_nrow[k];
+    const cplx* Lk = Fc.l + S.l_off[k];
+    const cplx* Uk = Fc.u + S.u_off[k];
+    for (int c0 = 0; c0 < nck; c0 += KCW) {
+      const int kc = min(KCW, nck - c0);
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int e = tid + LU_NT * u, r = e & 63, c = e >> 6;
+        const uint64_t below = (1ull << r) - 1ull;
+        va[u] = {0.0, 0.0};
+        vb[u] = {0.0, 0.0};
+        if (c < kc) {
+          if ((tk.rmask >> r) & 1ull) va[u] = Lk[(nck + tk.a0 + __popcll(tk.rmask & below)) + (int64_t)(c0 + c) * hk];
+          if ((tk.cmask >> r) & 1ull) vb[u] = Uk[(c0 + c) + (int64_t)(tk.b0 + __popcll(tk.cmask & below)) * nck];
+        }
+      }
+      stage(false, false);
+      __syncthreads();
+      tile_mma_w<TP_F64>(As, Bs, rr);
+      __syncthreads();
+      stage(true, true);
+      __syncthreads();
+      tile_mma_w<TP_F64>(As, Bs, ii);
+      __syncthreads();
+      stage(false, true);
+      __syncthreads();
+      tile_mma_w<TP_F64>(As, Bs, ri);
+      __syncthreads();
+      stage(true, false);
+      __syncthreads();
+      tile_mma_w<TP_F64>(As, Bs, ir);
+      __syncthreads();
+    }
+  }
+  const int s = it.s;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s];
+  cplx* P;
+  int64_t ld, mr, mc;
+  if (it.region == 0) {
+    P = Fc.l + S.l_off[s];
+    ld = h;
+    mr = h - it.r0;
+    mc = w - it.c0;
+  } else {
+    P = Fc.u + S.u_off[s];
+    ld = w;
+    mr = w - it.r0;
+    mc = h - w - it.c0;
+  }
+  const int l = tid & 63, wv = tid >> 6, wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = wr + tacc_row<TP_F64>(bi, l, q), j = wc + bj * 16 + (l & 15);
+        if (i < mr && j < mc) {
+          cplx* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * ld;
+          const double re = rr.v[bi][bj][q] - ii.v[bi][bj][q], im = ri.v[bi][bj][q] + ir.v[bi][bj][q];
+          *dst = {dst->re - re, dst->im - im};
+        }
+      }
+}
+
 // getf2 on the own rows of one panel (multifrontal_lu.rs:2064-2111): per
 // column, the pivot is the first row of largest magnitude in [k, w); the rows
 // are interchanged only when the diagonal is below u times it; the
@@ -690,7 +786,7 @@ void launch_lu_cmod(const LuSymDev& S, const LuFacDev<T>& Fc, const LuItem* item
   if constexpr (sizeof(T) == 8)
     hipLaunchKernelGGL(k_lu_cmod_mfma, dim3((unsigned)nitems), dim3(LU_NT), 0, st, S, Fc, items, tasks);
   else
-    hipLaunchKernelGGL(k_lu_cmod<T>, dim3((unsigned)nitems), dim3(LU_NT), 0, st, S, Fc, items, tasks);
+    hipLaunchKernelGGL(k_lu_cmod_mfma_c64, dim3((unsigned)nitems), dim3(LU_NT), 0, st, S, Fc, items, tasks);
 }
 template <class T>
 void launch_lu_getf2(const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, hipStream_t st) {
