@@ -617,70 +617,12 @@ __global__ __launch_bounds__(LU_NT) void k_lu_fgather(LuSymDev S, LuFacDev<T> Fc
   }
 }
 
-// Forward diagonal block [j0, j0 + m) of L11 of one supernode per unit
-// (s, j0): unit-lower substitution, one wave per right-hand side with lane r
-// holding row r; each solved value is broadcast across the wave and the rows
-// below it take its term (each row's terms in ascending column).
-template <class T, int BS>
-__global__ __launch_bounds__(LU_NT) void k_lu_fdiag(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
-                                                    int32_t nrhs, T* __restrict__ y) {
-  __shared__ T Lb[BS][BS + 1];
-  const LuUnit un = units[blockIdx.x];
-  const int s = un.s, j0 = un.kb;
-  const int64_t h = S.sn_nrow[s];
-  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
-  const T* L = Fc.l + S.l_off[s];
-  for (int e = threadIdx.x; e < BS * BS; e += LU_NT) {
-    const int r = e % BS, q = e / BS;
-    Lb[r][q] = (r < m && q < r) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  for (int c = threadIdx.x >> 6; c < nrhs; c += LU_NT / 64) {
-    T* yc = y + (int64_t)(first + j0) * nrhs + c;
-    T v = lane < m ? yc[(int64_t)lane * nrhs] : F<T>::zero();
-    for (int q = 0; q < m; ++q) {
-      const T yq = lu_bcast(v, q);
-      if (lane > q && lane < m) v = lu_fma(F<T>::zero() - Lb[lane][q], yq, v);
-    }
-    if (lane < m) yc[(int64_t)lane * nrhs] = v;
-  }
-}
-
-// Forward update by the solved block [j0, j0 + m) of 64 rows per unit
-// (s, j0, first row): own rows below the block get y -= L y_block; the
-// off-diagonal rows accumulate L21 y_s into the supernode's scratch rows
-// (written by the first block, added to by the later ones).
-template <class T, int BS>
-__global__ __launch_bounds__(LU_NT) void k_lu_fupd(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
-                                                   int32_t nrhs, T* __restrict__ y, T* __restrict__ scr) {
-  const LuUnit un = units[blockIdx.x];
-  const int s = un.s, j0 = un.kb;
-  const int64_t h = S.sn_nrow[s];
-  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
-  const T* L = Fc.l + S.l_off[s];
-  const T* yb = y + (int64_t)(first + j0) * nrhs;
-  for (int64_t e = threadIdx.x; e < 64 * (int64_t)nrhs; e += LU_NT) {
-    const int64_t r = un.a + (e & 63), c = e >> 6;
-    if (r >= h) continue;
-    T acc = F<T>::zero();
-    for (int q = 0; q < m; ++q) acc = lu_fma(L[(int64_t)(j0 + q) * h + r], yb[(int64_t)q * nrhs + c], acc);
-    if (r < w) {
-      T* d = y + (first + r) * nrhs + c;
-      *d = *d - acc;
-    } else {
-      T* d = scr + (S.s_off[s] + r - w) * nrhs + c;
-      *d = j0 == 0 ? acc : *d + acc;
-    }
-  }
-}
-
 // Backward U12 terms, 16 own rows per unit (s, -, first row), the trailing
 // columns split over 16 slices summed in LDS, 8 right-hand sides at a time:
-// y_s[p] -= U12[p, :] x[rs[w..]] (the ancestors' x are final).
+// y_s[p] -= U12[p, :] x[rs[w..]] (the ancestors' x, final, in x).
 template <class T>
 __global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
-                                                   int32_t nrhs, T* __restrict__ y) {
+                                                   int32_t nrhs, T* __restrict__ y, const T* __restrict__ x) {
   constexpr int CW = 8;
   __shared__ T part[16][16][CW + 1];
   const LuUnit un = units[blockIdx.x];
@@ -700,7 +642,7 @@ __global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, c
     if (p < w)
       for (int64_t t = sl; t < nt; t += 16) {
         const T u = U[p + t * w];
-        const T* xr = y + (int64_t)rs[t] * nrhs + c0;
+        const T* xr = x + (int64_t)rs[t] * nrhs + c0;
 #pragma unroll
         for (int c = 0; c < CW; ++c)
           if (c < cw) acc[c] = lu_fma(u, xr[c], acc[c]);
@@ -722,13 +664,91 @@ __global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, c
   }
 }
 
-// Backward diagonal block [j0, j0 + m) of U11 of one supernode per unit
-// (s, j0): back substitution with the pivots, one wave per right-hand side
-// with lane r holding row r.
+// One forward block step of a supernode's own block, units (s, j0, first
+// row, writer): every unit solves the diagonal block [j0, j0 + m) of L11 from
+// the updated, not yet solved values in y (unit-lower substitution, one wave
+// per right-hand side, lane per row, the block's L in LDS), then updates its
+// 64 rows below the block with it: own rows in y, off-diagonal rows into the
+// supernode's scratch rows (written by the first block, added to after). The
+// writer unit stores the solved block into yo; no unit of the launch writes
+// rows another one reads.
 template <class T, int BS>
-__global__ __launch_bounds__(LU_NT) void k_lu_bdiag(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
-                                                    int32_t nrhs, T* __restrict__ y) {
+__global__ __launch_bounds__(LU_NT) void k_lu_fstep(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
+                                                    int32_t nrhs, T* __restrict__ y, T* __restrict__ yo,
+                                                    T* __restrict__ scr) {
+  constexpr int CW = 16;
+  __shared__ T Lb[BS][BS + 1];
+  __shared__ T yb[BS][CW + 1];
+  const LuUnit un = units[blockIdx.x];
+  const int s = un.s, j0 = un.kb;
+  const int64_t h = S.sn_nrow[s];
+  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
+  const T* L = Fc.l + S.l_off[s];
+  for (int e = threadIdx.x; e < BS * BS; e += LU_NT) {
+    const int r = e % BS, q = e / BS;
+    Lb[r][q] = (r < m && q < r) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t r0 = un.a;
+  for (int c0 = 0; c0 < nrhs; c0 += CW) {
+    const int cw = min(CW, nrhs - c0);
+    // a wave's right-hand sides (columns wv, wv + 4, ...) as interleaved chains
+    constexpr int CPW = CW / (LU_NT / 64);
+    T v[CPW];
+    const T* yc = y + (int64_t)(first + j0 + lane) * nrhs + c0;
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      const int c = wv + k * (LU_NT / 64);
+      v[k] = (c < cw && lane < m) ? yc[c] : F<T>::zero();
+    }
+    for (int q = 0; q < m; ++q) {
+      const T lq = Lb[lane][q];
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        const T yq = lu_bcast(v[k], q);
+        if (lane > q && lane < m) v[k] = lu_fma(F<T>::zero() - lq, yq, v[k]);
+      }
+    }
+    if (lane < m)
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        const int c = wv + k * (LU_NT / 64);
+        if (c < cw) {
+          yb[lane][c] = v[k];
+          if (un.b) yo[(int64_t)(first + j0 + lane) * nrhs + c0 + c] = v[k];
+        }
+      }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * cw; e += LU_NT) {
+      const int64_t r = r0 + (e & 63);
+      const int c = e >> 6;
+      if (r >= h) continue;
+      T acc = F<T>::zero();
+#pragma unroll 8
+      for (int q = 0; q < m; ++q) acc = lu_fma(L[(int64_t)(j0 + q) * h + r], yb[q][c], acc);
+      if (r < w) {
+        T* d = y + (first + r) * nrhs + c0 + c;
+        *d = *d - acc;
+      } else {
+        T* d = scr + (S.s_off[s] + r - w) * nrhs + c0 + c;
+        *d = j0 == 0 ? acc : *d + acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One backward block step, units (s, j0, first row, writer): every unit
+// solves the diagonal block of U11 (back substitution with the pivots) from
+// the values in y, then updates its 64 own rows above the block; the writer
+// stores the solved block into xo.
+template <class T, int BS>
+__global__ __launch_bounds__(LU_NT) void k_lu_bstep(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
+                                                    int32_t nrhs, T* __restrict__ y, T* __restrict__ xo) {
+  constexpr int CW = 16;
   __shared__ T Ub[BS][BS + 1];
+  __shared__ T yb[BS][CW + 1];
   const LuUnit un = units[blockIdx.x];
   const int s = un.s, j0 = un.kb;
   const int64_t h = S.sn_nrow[s];
@@ -739,37 +759,49 @@ __global__ __launch_bounds__(LU_NT) void k_lu_bdiag(LuSymDev S, LuFacDev<T> Fc, 
     Ub[r][q] = (q < m && r <= q) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  for (int c = threadIdx.x >> 6; c < nrhs; c += LU_NT / 64) {
-    T* yc = y + (int64_t)(first + j0) * nrhs + c;
-    T v = lane < m ? yc[(int64_t)lane * nrhs] : F<T>::zero();
-    for (int q = m - 1; q >= 0; --q) {
-      if (lane == q) v = v * F<T>::recip(Ub[q][q]);
-      const T xq = lu_bcast(v, q);
-      if (lane < q) v = lu_fma(F<T>::zero() - Ub[lane][q], xq, v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t r0 = un.a;
+  for (int c0 = 0; c0 < nrhs; c0 += CW) {
+    const int cw = min(CW, nrhs - c0);
+    constexpr int CPW = CW / (LU_NT / 64);
+    T v[CPW];
+    const T* yc = y + (int64_t)(first + j0 + lane) * nrhs + c0;
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      const int c = wv + k * (LU_NT / 64);
+      v[k] = (c < cw && lane < m) ? yc[c] : F<T>::zero();
     }
-    if (lane < m) yc[(int64_t)lane * nrhs] = v;
-  }
-}
-
-// Backward update by the solved block [j0, j0 + m), 64 own rows above it per
-// unit (s, j0, first row): y[r] -= U[r, block] y_block.
-template <class T, int BS>
-__global__ __launch_bounds__(LU_NT) void k_lu_bupd(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units,
-                                                   int32_t nrhs, T* __restrict__ y) {
-  const LuUnit un = units[blockIdx.x];
-  const int s = un.s, j0 = un.kb;
-  const int64_t h = S.sn_nrow[s];
-  const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
-  const T* L = Fc.l + S.l_off[s];
-  const T* yb = y + (int64_t)(first + j0) * nrhs;
-  for (int64_t e = threadIdx.x; e < 64 * (int64_t)nrhs; e += LU_NT) {
-    const int64_t r = un.a + (e & 63), c = e >> 6;
-    if (r >= j0) continue;
-    T acc = F<T>::zero();
-    for (int q = 0; q < m; ++q) acc = lu_fma(L[(int64_t)(j0 + q) * h + r], yb[(int64_t)q * nrhs + c], acc);
-    T* d = y + (first + r) * nrhs + c;
-    *d = *d - acc;
+    for (int q = m - 1; q >= 0; --q) {
+      const T uq = Ub[lane][q];
+      const T dinv = F<T>::recip(Ub[q][q]);
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (lane == q) v[k] = v[k] * dinv;
+        const T xq = lu_bcast(v[k], q);
+        if (lane < q) v[k] = lu_fma(F<T>::zero() - uq, xq, v[k]);
+      }
+    }
+    if (lane < m)
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        const int c = wv + k * (LU_NT / 64);
+        if (c < cw) {
+          yb[lane][c] = v[k];
+          if (un.b) xo[(int64_t)(first + j0 + lane) * nrhs + c0 + c] = v[k];
+        }
+      }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * cw; e += LU_NT) {
+      const int64_t r = r0 + (e & 63);
+      const int c = e >> 6;
+      if (r >= j0) continue;
+      T acc = F<T>::zero();
+#pragma unroll 8
+      for (int q = 0; q < m; ++q) acc = lu_fma(L[(int64_t)(j0 + q) * h + r], yb[q][c], acc);
+      T* d = y + (first + r) * nrhs + c0 + c;
+      *d = *d - acc;
+    }
+    __syncthreads();
   }
 }
 
@@ -821,21 +853,19 @@ template <class T>
 int lu_solve_bs() {
   return sizeof(T) == 16 ? 32 : 64;
 }
-// kind: 0 gather, 1 forward diagonal, 2 forward update, 3 backward U12,
-// 4 backward diagonal, 5 backward update
+// kind: 0 gather (into y), 1 forward block step (y -> yo), 3 backward U12
+// terms (on y, x of the ancestors in yo), 4 backward block step (y -> yo)
 template <class T>
 void launch_lu_sstep(int kind, const LuSymDev& S, const LuFacDev<T>& Fc, const LuUnit* units, int n, int32_t nrhs,
-                     const T* bh, T* scr, T* y, hipStream_t st) {
+                     const T* bh, T* scr, T* y, T* yo, hipStream_t st) {
   if (n <= 0) return;
   constexpr int BS = sizeof(T) == 16 ? 32 : 64;
   const dim3 g((unsigned)n), b(LU_NT);
   switch (kind) {
     case 0: hipLaunchKernelGGL(k_lu_fgather<T>, g, b, 0, st, S, Fc, units, nrhs, bh, scr, y); break;
-    case 1: hipLaunchKernelGGL((k_lu_fdiag<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y); break;
-    case 2: hipLaunchKernelGGL((k_lu_fupd<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y, scr); break;
-    case 3: hipLaunchKernelGGL(k_lu_boff<T>, g, b, 0, st, S, Fc, units, nrhs, y); break;
-    case 4: hipLaunchKernelGGL((k_lu_bdiag<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y); break;
-    default: hipLaunchKernelGGL((k_lu_bupd<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y); break;
+    case 1: hipLaunchKernelGGL((k_lu_fstep<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y, yo, scr); break;
+    case 3: hipLaunchKernelGGL(k_lu_boff<T>, g, b, 0, st, S, Fc, units, nrhs, y, yo); break;
+    default: hipLaunchKernelGGL((k_lu_bstep<T, BS>), g, b, 0, st, S, Fc, units, nrhs, y, yo); break;
   }
 }
 
@@ -850,7 +880,7 @@ void launch_lu_sstep(int kind, const LuSymDev& S, const LuFacDev<T>& Fc, const L
   template void launch_lu_sfinish<T>(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t); \
   template int lu_solve_bs<T>();                                                                              \
   template void launch_lu_sstep<T>(int, const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, int32_t, const T*, \
-                                   T*, T*, hipStream_t);
+                                   T*, T*, T*, hipStream_t);
 RSL_LU_INST(double)
 RSL_LU_INST(cplx)
 

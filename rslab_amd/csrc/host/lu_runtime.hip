@@ -18,7 +18,7 @@ template <class T> void launch_lu_sprep(int32_t, int32_t, const int32_t*, const 
 template <class T> void launch_lu_sfinish(int32_t, int32_t, const int32_t*, const double*, const T*, T*, hipStream_t);
 template <class T> int lu_solve_bs();
 template <class T> void launch_lu_sstep(int, const LuSymDev&, const LuFacDev<T>&, const LuUnit*, int, int32_t, const T*,
-                                        T*, T*, hipStream_t);
+                                        T*, T*, T*, hipStream_t);
 template <class T> void launch_expand(int64_t, const int64_t*, const T*, T*, hipStream_t);
 template <class T> void launch_residual(int32_t, int32_t, const int64_t*, const int32_t*, const T*, const T*, const T*, T*,
                                         hipStream_t, LaunchProfiler*);
@@ -104,14 +104,14 @@ double inv_sqrt_guarded(double m) {
 template <class T>
 class LuDeviceFactor final : public LuFactorBase {
  public:
-  DBuf l, u, rperm, ipiv, status, counters, vals, drow, dcol, bh, yv, scr, bx;
+  DBuf l, u, rperm, ipiv, status, counters, vals, drow, dcol, bh, yv, y2, scr, bx;
   DBuf r_a, r_acsr, r_b, r_x, r_best, r_r, r_dx, r_max, r_mask;
   // solve work units (their block width depends on the field): per level the
   // gather, the forward block steps (diagonal, update), the U12 units and the
   // backward block steps
   struct SolveLevel {
     int64_t g0 = 0, g1 = 0, o0 = 0, o1 = 0;
-    std::vector<int64_t> fs, bs;  // per block step: diagonal begin, end, update begin, end
+    std::vector<int64_t> fs, bs;  // per block step: units begin, end
   };
   std::vector<SolveLevel> slev;
   DBuf su;
@@ -164,14 +164,12 @@ class LuDeviceFactor final : public LuFactorBase {
       for (int32_t s : L.all) nblk = std::max(nblk, (P.sn_ncol[s] + BS - 1) / BS);
       for (int i = 0; i < nblk; ++i) {
         sl.fs.push_back((int64_t)u.size());
-        for (int32_t s : L.all)
-          if (i * BS < P.sn_ncol[s]) u.push_back({s, i * BS, 0, 0});
-        sl.fs.push_back((int64_t)u.size());
-        sl.fs.push_back((int64_t)u.size());
         for (int32_t s : L.all) {
           const int32_t w = P.sn_ncol[s], h = P.sn_nrow[s], j0 = i * BS;
           if (j0 >= w) continue;
-          for (int32_t r0 = j0 + std::min(BS, w - j0); r0 < h; r0 += 64) u.push_back({s, j0, r0, 0});
+          const int32_t below = j0 + std::min(BS, w - j0);
+          if (below >= h) u.push_back({s, j0, h, 1});
+          for (int32_t r0 = below; r0 < h; r0 += 64) u.push_back({s, j0, r0, r0 == below ? 1 : 0});
         }
         sl.fs.push_back((int64_t)u.size());
       }
@@ -184,15 +182,10 @@ class LuDeviceFactor final : public LuFactorBase {
         sl.bs.push_back((int64_t)u.size());
         for (int32_t s : L.all) {
           const int32_t nb_s = (P.sn_ncol[s] + BS - 1) / BS;
-          if (i < nb_s) u.push_back({s, (nb_s - 1 - i) * BS, 0, 0});
-        }
-        sl.bs.push_back((int64_t)u.size());
-        sl.bs.push_back((int64_t)u.size());
-        for (int32_t s : L.all) {
-          const int32_t nb_s = (P.sn_ncol[s] + BS - 1) / BS;
           if (i >= nb_s) continue;
           const int32_t j0 = (nb_s - 1 - i) * BS;
-          for (int32_t r0 = 0; r0 < j0; r0 += 64) u.push_back({s, j0, r0, 0});
+          if (j0 == 0) u.push_back({s, 0, 0, 1});
+          for (int32_t r0 = 0; r0 < j0; r0 += 64) u.push_back({s, j0, r0, r0 == 0 ? 1 : 0});
         }
         sl.bs.push_back((int64_t)u.size());
       }
@@ -292,6 +285,7 @@ class LuDeviceFactor final : public LuFactorBase {
     const size_t need = sizeof(T) * (size_t)std::max<int64_t>((int64_t)P.n * nrhs, 1);
     if (bh.bytes < need) bh.alloc(need);
     if (yv.bytes < need) yv.alloc(need);
+    if (y2.bytes < need) y2.alloc(need);
     const size_t sneed = sizeof(T) * (size_t)std::max<int64_t>(P.s_total * nrhs, 1);
     if (scr.bytes < sneed) scr.alloc(sneed);
   }
@@ -307,24 +301,21 @@ class LuDeviceFactor final : public LuFactorBase {
     T* sc = scr.as<T>();
     const LuUnit* u = su.as<LuUnit>();
     launch_lu_sprep<T>(P.n, nrhs, S.perm, drow.as<double>(), b, bhp, st);
-    auto step = [&](int kind, int64_t a, int64_t e) {
-      launch_lu_sstep<T>(kind, S, fv, u + a, (int)(e - a), nrhs, bhp, sc, y, st);
+    // forward: gather into y, block steps solve from y into y2; backward:
+    // the U12 terms and block steps work on y2 and store x into y
+    T* y2p = y2.as<T>();
+    auto step = [&](int kind, int64_t a, int64_t e, T* yin, T* yout) {
+      launch_lu_sstep<T>(kind, S, fv, u + a, (int)(e - a), nrhs, bhp, sc, yin, yout, st);
     };
     for (size_t l = 0; l < slev.size(); ++l) {
       const SolveLevel& sl = slev[l];
-      step(0, sl.g0, sl.g1);
-      for (size_t i = 0; i < sl.fs.size(); i += 4) {
-        step(1, sl.fs[i], sl.fs[i + 1]);
-        step(2, sl.fs[i + 2], sl.fs[i + 3]);
-      }
+      step(0, sl.g0, sl.g1, y, y2p);
+      for (size_t i = 0; i < sl.fs.size(); i += 2) step(1, sl.fs[i], sl.fs[i + 1], y, y2p);
     }
     for (size_t l = slev.size(); l-- > 0;) {
       const SolveLevel& sl = slev[l];
-      step(3, sl.o0, sl.o1);
-      for (size_t i = 0; i < sl.bs.size(); i += 4) {
-        step(4, sl.bs[i], sl.bs[i + 1]);
-        step(5, sl.bs[i + 2], sl.bs[i + 3]);
-      }
+      step(3, sl.o0, sl.o1, y2p, y);
+      for (size_t i = 0; i < sl.bs.size(); i += 2) step(4, sl.bs[i], sl.bs[i + 1], y2p, y);
     }
     launch_lu_sfinish<T>(P.n, nrhs, S.perm, dcol.as<double>(), y, x, st);
     hip_check(hipGetLastError(), "lu solve launch");
