@@ -36,9 +36,13 @@ __device__ inline double lu_fma<double>(double a, double b, double c) {
   return fma(a, b, c);
 }
 
-// value of lane q across the wave
-__device__ inline double lu_bcast(double v, int q) { return __shfl(v, q, 64); }
-__device__ inline cplx lu_bcast(cplx v, int q) { return {__shfl(v.re, q, 64), __shfl(v.im, q, 64)}; }
+// value of lane q (wave-uniform) across the wave, read through the scalar unit
+__device__ inline double lu_bcast(double v, int q) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), q);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), q);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline cplx lu_bcast(cplx v, int q) { return {lu_bcast(v.re, q), lu_bcast(v.im, q)}; }
 
 // perturb_pivot (multifrontal_ldlt.rs:786-795)
 template <class T>
