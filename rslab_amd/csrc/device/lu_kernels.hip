@@ -616,6 +616,7 @@ __global__ __launch_bounds__(LU_NT) void k_lu_fgather(LuSymDev S, LuFacDev<T> Fc
     const int64_t p = un.a + e / nrhs, c = e % nrhs;
     const int64_t g = first + Fc.rperm[first + p];
     T v = bh[g * nrhs + c];
+#pragma unroll 4
     for (int64_t q = S.gat_ptr[g]; q < S.gat_ptr[g + 1]; ++q) v = v - scr[S.gat_idx[q] * nrhs + c];
     y[(first + p) * nrhs + c] = v;
   }
@@ -643,14 +644,31 @@ __global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, c
     T acc[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) acc[c] = F<T>::zero();
-    if (p < w)
-      for (int64_t t = sl; t < nt; t += 16) {
-        const T u = U[p + t * w];
-        const T* xr = x + (int64_t)rs[t] * nrhs + c0;
+    if (p < w) {
+      // four columns of the slice per pass, their loads issued together
+      int64_t t = sl;
+      for (; t + 48 < nt; t += 64) {
+        T u[4];
+        const T* xr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          u[q] = U[p + (t + 16 * q) * w];
+          xr[q] = x + (int64_t)rs[t + 16 * q] * nrhs + c0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int c = 0; c < CW; ++c)
+            if (c < cw) acc[c] = lu_fma(u[q], xr[q][c], acc[c]);
+      }
+      for (; t < nt; t += 16) {
+        const T u1 = U[p + t * w];
+        const T* x1 = x + (int64_t)rs[t] * nrhs + c0;
 #pragma unroll
         for (int c = 0; c < CW; ++c)
-          if (c < cw) acc[c] = lu_fma(u, xr[c], acc[c]);
+          if (c < cw) acc[c] = lu_fma(u1, x1[c], acc[c]);
       }
+    }
 #pragma unroll
     for (int c = 0; c < CW; ++c) part[sl][i][c] = acc[c];
     __syncthreads();
