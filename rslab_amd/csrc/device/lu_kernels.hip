@@ -459,14 +459,28 @@ __global__ __launch_bounds__(LU_GT) void k_lu_getf2(LuSymDev S, LuFacDev<T> Fc, 
 // columns outside the panel (the finished ones left of it, the later own
 // columns and U12) and, on the later ones, the unit-lower triangular solve
 // with L11 (the TRSM of :2157-2199); deep-row units apply the panel transform
-// to the rows below the own rows (apply_panel_trailing, :1357-1387).
+// to the rows below the own rows (apply_panel_trailing, :1357-1387). The
+// panel's diagonal block and pivot reciprocals sit in LDS, each thread's
+// column or row segment in registers (the same operations in the same order
+// as on the stored values).
 template <class T>
 __global__ __launch_bounds__(LU_NT) void k_lu_apply(LuSymDev S, LuFacDev<T> Fc, const LuUnit* __restrict__ units) {
+  __shared__ T Ls[LU_PANEL][LU_PANEL + 1];  // the panel's diagonal block: Ls[r][c] = entry (kb + r, kb + c)
+  __shared__ T pinv[LU_PANEL];
+  __shared__ int pv[LU_PANEL];
   const LuUnit un = units[blockIdx.x];
   const int s = un.s, kb = un.kb;
   const int64_t h = S.sn_nrow[s];
   const int w = S.sn_ncol[s], first = S.sn_first[s], ke = min(kb + S.nb[s], w), pw = ke - kb;
   T* L = Fc.l + S.l_off[s];
+  for (int e = threadIdx.x; e < LU_PANEL * LU_PANEL; e += LU_NT) {
+    const int r = e % LU_PANEL, c = e / LU_PANEL;
+    Ls[r][c] = (r < pw && c < pw) ? L[(int64_t)(kb + c) * h + kb + r] : F<T>::zero();
+  }
+  if (threadIdx.x < LU_PANEL) pv[threadIdx.x] = threadIdx.x < pw ? Fc.ipiv[first + kb + threadIdx.x] : 0;
+  __syncthreads();
+  if (threadIdx.x < LU_PANEL) pinv[threadIdx.x] = threadIdx.x < pw ? F<T>::recip(Ls[threadIdx.x][threadIdx.x]) : F<T>::zero();
+  __syncthreads();
   if (un.b == 0) {
     const int64_t c = (int64_t)un.a + threadIdx.x;
     const int64_t nlater = w - ke, ncols = kb + nlater + (h - w);
@@ -483,36 +497,58 @@ __global__ __launch_bounds__(LU_NT) void k_lu_apply(LuSymDev S, LuFacDev<T> Fc, 
       col = Fc.u + S.u_off[s] + (c - kb - nlater) * w;
       later = true;
     }
-    for (int k = kb; k < ke; ++k) {
-      const int p = Fc.ipiv[first + k];
-      if (p != k) {
-        const T t = col[k];
-        col[k] = col[p];
+    for (int q = 0; q < pw; ++q) {
+      const int p = pv[q];
+      if (p != kb + q) {
+        const T t = col[kb + q];
+        col[kb + q] = col[p];
         col[p] = t;
       }
     }
-    if (later)
-      for (int r = kb + 1; r < ke; ++r) {
-        T acc = col[r];
-        for (int i = kb; i < r; ++i) acc = acc - L[(int64_t)i * h + r] * col[i];
-        col[r] = acc;
+    if (later) {
+      T v[LU_PANEL];
+#pragma unroll
+      for (int r = 0; r < LU_PANEL; ++r) v[r] = r < pw ? col[kb + r] : F<T>::zero();
+      for (int r = 1; r < pw; ++r) {  // a runtime row: the block's entries are read per row, not hoisted
+        T acc = F<T>::zero();
+#pragma unroll
+        for (int i = 0; i < LU_PANEL; ++i)
+          if (i == r) acc = v[i];
+#pragma unroll
+        for (int i = 0; i < LU_PANEL; ++i)
+          if (i < r) acc = acc - Ls[r][i] * v[i];
+#pragma unroll
+        for (int i = 0; i < LU_PANEL; ++i)
+          if (i == r) v[i] = acc;
       }
+#pragma unroll
+      for (int r = 1; r < LU_PANEL; ++r)
+        if (r < pw) col[kb + r] = v[r];
+    }
   } else {
     const int64_t i = (int64_t)w + un.a + threadIdx.x;
     if (i >= h) return;
+    T x[LU_PANEL];
+#pragma unroll
+    for (int q = 0; q < LU_PANEL; ++q) x[q] = q < pw ? L[(int64_t)(kb + q) * h + i] : F<T>::zero();
     for (int q = 0; q < pw; ++q) {
-      const int k = kb + q;
-      T* ck = L + (int64_t)k * h;
-      const T x = ck[i] * F<T>::recip(ck[k]);
-      ck[i] = x;
-      for (int jj = q + 1; jj < pw; ++jj) {
-        const T m = L[(int64_t)(kb + jj) * h + k];
-        if (!F<T>::is_zero(m)) {
-          T* cj = L + (int64_t)(kb + jj) * h;
-          cj[i] = cj[i] - x * m;
+      T xq = F<T>::zero();
+#pragma unroll
+      for (int jj = 0; jj < LU_PANEL; ++jj)
+        if (jj == q) xq = x[jj];
+      xq = xq * pinv[q];
+#pragma unroll
+      for (int jj = 0; jj < LU_PANEL; ++jj) {
+        if (jj == q) x[jj] = xq;
+        if (jj > q) {
+          const T m = Ls[q][jj];
+          if (!F<T>::is_zero(m)) x[jj] = x[jj] - xq * m;
         }
       }
     }
+#pragma unroll
+    for (int q = 0; q < LU_PANEL; ++q)
+      if (q < pw) L[(int64_t)(kb + q) * h + i] = x[q];
   }
 }
 

@@ -58,5 +58,6 @@ struct LuUnit {
 constexpr int LU_NT = 256;     // threads per workgroup of the LU kernels but getf2
 constexpr int LU_GT = 1024;    // threads of the getf2 workgroup (one per supernode panel)
 constexpr int LU_COLS = 256;   // columns (rows) per apply unit
+constexpr int LU_PANEL = 16;   // device panel width of the LU cdiv
 
 }  // namespace rsl

@@ -10,8 +10,6 @@
 
 namespace rsl {
 
-constexpr int LU_PANEL = 16;  // device panel width of the LU cdiv
-
 struct LuStepPlan {
   int64_t g_beg = 0, g_end = 0;  // getf2 units
   int64_t a_beg = 0, a_end = 0;  // apply units (columns, deep rows)
