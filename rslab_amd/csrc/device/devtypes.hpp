@@ -5,6 +5,18 @@
 
 namespace rsl {
 
+// Backward dependency-driven solve: the off-diagonal terms of a supernode
+// with w own columns and `off` off-diagonal rows run as row chunks of
+// BWD_CHUNK rows, each chunk's partial sums (w x nrhs) stored in the
+// supernode's forward scratch rows (off x nrhs, no longer read), when they
+// fit; otherwise one chunk that subtracts its sums from the solution itself.
+constexpr int BWD_CHUNK = 1024;
+constexpr int bwd_row_chunks(int32_t w, int64_t off) {
+  return ((off + BWD_CHUNK - 1) / BWD_CHUNK > 1 && (int64_t)w * ((off + BWD_CHUNK - 1) / BWD_CHUNK) <= off)
+             ? (int)((off + BWD_CHUNK - 1) / BWD_CHUNK)
+             : 1;
+}
+
 // Value-free per-pattern arrays (uploaded once per analysis).
 struct DevSym {
   int32_t n, ns;

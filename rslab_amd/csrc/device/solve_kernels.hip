@@ -361,12 +361,18 @@ __global__ __launch_bounds__(SNT) void k_fwd_update(DevSym S, DevFac<T> Fc,
   }
 }
 
-// y_own[p] -= sum_{i>=w} L[i][p] x[rs[i]] for the 16 columns p0.. of
-// supernode s (k_bwd_update's unit); xs holds RB * RCH values of LDS.
+// y_own[p] -= sum_{i in [ib0, ie)} L[i][p] x[rs[i]] for the 16 columns p0..
+// of supernode s (w <= ib0 <= ie <= h; k_bwd_update's unit takes every
+// off-diagonal row); with part set the sums are stored to part[p * nrhs + k]
+// instead. xs holds RB * RCH values of LDS, is RB row indices. A pass's x
+// rows are gathered eight lanes to a row (its RCH values, one 64-byte line),
+// their indices shared through is.
 constexpr int BW_RB = 512, BW_CW = 4;  // rows per pass, columns per wave
+static_assert(RCH == 8, "a gathered x row is one lane octet");
 template <class T>
 __device__ void bwd_update_cols(const DevSym& S, const DevFac<T>& Fc, int s, int p0, int32_t nrhs,
-                                T* __restrict__ y, T* xs) {
+                                T* __restrict__ y, T* xs, int32_t* is, int64_t ib0, int64_t ie,
+                                T* __restrict__ part) {
   constexpr int RB = BW_RB, CW = BW_CW;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
@@ -381,26 +387,40 @@ __device__ void bwd_update_cols(const DevSym& S, const DevFac<T>& Fc, int s, int
     for (int a = 0; a < CW; ++a)
 #pragma unroll
       for (int k = 0; k < RCH; ++k) acc[a][k] = F<T>::zero();
-    for (int64_t ib = w; ib < h; ib += RB) {
-      const int nr = (int)min<int64_t>(RB, h - ib);
-      // x rows of the pass: thread t gathers rows t + SNT q (RCH values each)
-      {
-        int64_t e[RB / SNT];
+    // row indices of the next pass, loaded one pass ahead (real field; the
+    // complex field's registers hold no second set): a pass then waits on
+    // one round of loads (its x rows and L entries together)
+    constexpr bool AHEAD = sizeof(T) == sizeof(double);
+    int32_t en[RB / SNT];
+    auto load_idx = [&](int64_t ib) {
 #pragma unroll
-        for (int q = 0; q < RB / SNT; ++q) {
-          const int t = threadIdx.x + q * SNT;
-          e[q] = t < nr ? (int64_t)re[ib + t] : -1;
-        }
-        T xv[RB / SNT][RCH];
-#pragma unroll
-        for (int q = 0; q < RB / SNT; ++q)
-#pragma unroll
-          for (int k = 0; k < RCH; ++k) xv[q][k] = (e[q] >= 0 && k < nc) ? y[e[q] * nrhs + c0 + k] : F<T>::zero();
-#pragma unroll
-        for (int q = 0; q < RB / SNT; ++q)
-#pragma unroll
-          for (int k = 0; k < RCH; ++k) xs[k * RB + threadIdx.x + q * SNT] = xv[q][k];  // [k][row]
+      for (int q = 0; q < RB / SNT; ++q) {
+        const int64_t r = ib + threadIdx.x + q * SNT;
+        en[q] = r < ie ? re[r] : -1;
       }
+    };
+    if (AHEAD) load_idx(ib0);
+    for (int64_t ib = ib0; ib < ie; ib += RB) {
+      const int nr = (int)min<int64_t>(RB, ie - ib);
+      if (!AHEAD) load_idx(ib);
+#pragma unroll
+      for (int q = 0; q < RB / SNT; ++q) is[threadIdx.x + q * SNT] = en[q];
+      __syncthreads();
+      // x rows of the pass: thread t takes row t / 8 + 32 q, value t % 8
+      constexpr int XR = RB * RCH / SNT;
+      const int xk = threadIdx.x & 7, xr0 = threadIdx.x >> 3;
+      T xv[XR];
+#pragma unroll
+      for (int q = 0; q < XR; ++q) {
+        const int32_t ee = is[xr0 + 32 * q];
+        xv[q] = (ee >= 0 && xk < nc) ? y[(int64_t)ee * nrhs + c0 + xk] : F<T>::zero();
+      }
+      auto stage_x = [&]() {
+#pragma unroll
+        for (int q = 0; q < XR; ++q) xs[xk * RB + xr0 + 32 * q] = xv[q];  // [k][row]
+      };
+      // complex field: x staged before the L loads (registers)
+      if (!AHEAD) stage_x();
       T l[RB / 64][CW];
 #pragma unroll
       for (int sb = 0; sb < RB / 64; ++sb)
@@ -409,6 +429,8 @@ __device__ void bwd_update_cols(const DevSym& S, const DevFac<T>& Fc, int s, int
           const int ir = sb * 64 + lane, p = pw + a;
           l[sb][a] = (ir < nr && p < w) ? P[(ib + ir) + (int64_t)p * h] : F<T>::zero();
         }
+      if (AHEAD && ib + RB < ie) load_idx(ib + RB);
+      if (AHEAD) stage_x();
       __syncthreads();
 #pragma unroll
       for (int sb = 0; sb < RB / 64; ++sb) {
@@ -428,8 +450,12 @@ __device__ void bwd_update_cols(const DevSym& S, const DevFac<T>& Fc, int s, int
       for (int k = 0; k < RCH; ++k) {
         const T v = wave_sum_to63(acc[a][k]);
         if (lane == 63 && k < nc && pw + a < w && pw + a < p0 + 16) {
-          const int64_t o2 = (int64_t)(eo + pw + a) * nrhs + c0 + k;
-          y[o2] = y[o2] - v;
+          if (part) {
+            part[(int64_t)(pw + a) * nrhs + c0 + k] = v;
+          } else {
+            const int64_t o2 = (int64_t)(eo + pw + a) * nrhs + c0 + k;
+            y[o2] = y[o2] - v;
+          }
         }
       }
   }
@@ -446,8 +472,9 @@ __global__ __launch_bounds__(SNT) void k_bwd_update(DevSym S, DevFac<T> Fc,
                                                     const int2* __restrict__ items, int32_t nrhs,
                                                     T* __restrict__ y) {
   __shared__ T xs[BW_RB * RCH];
+  __shared__ int32_t is[BW_RB];
   const int2 it = items[blockIdx.x];
-  bwd_update_cols(S, Fc, it.x, it.y, nrhs, y, xs);
+  bwd_update_cols(S, Fc, it.x, it.y, nrhs, y, xs, is, S.sn_ncol[it.x], S.sn_nrow[it.x], (T*)nullptr);
 }
 
 // Backward own-block solve of a narrow supernode (w <= 32): one workgroup
@@ -792,13 +819,20 @@ __device__ inline int32_t wait_ge(int32_t* p, int32_t target, uint32_t* ctl) {
   }
 }
 
-// The workgroup's stores (ordered before thread 0 by the barrier), then one
-// agent-scope release store / add of the counter.
+// The workgroup's stores, then the counter. Every wave waits for its own
+// stores to complete before the barrier (the barrier does not wait for
+// them); then thread 0 writes the L2 back at agent scope and waits for that
+// write-back (an explicit wait: the fence alone is not followed by one when
+// an atomic comes next) before it stores / adds the counter. s_waitcnt
+// immediate: vmcnt 0, expcnt and lgkmcnt not waited on.
 __device__ inline void wg_publish(int32_t* p, int32_t v, bool add) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (add) __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    else __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (add) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -850,6 +884,27 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
       dv[q] = in ? P[(row0 + i) + (row0 + p) * h] : F<T>::zero();
     }
   };
+  // own unit: the static part of its rows' gather (the list bounds and its
+  // first GN scratch-row indices), loaded before the waits so that after
+  // them only the scratch values are fetched
+  constexpr int GN = 4;
+  int64_t gq = 0, gqe = 0, gix[GN];
+  if (own && lane < nr) {
+    const int g = S.sn_first[s] + Fc.lperm[eo + row0 + lane];
+    gq = S.gat_ptr[g];
+    gqe = S.gat_ptr[g + 1];
+#pragma unroll
+    for (int m = 0; m < GN; ++m) gix[m] = gq + m < gqe ? S.gat_idx[gq + m] : 0;
+  }
+  // the diagonal block's loads go out before the unit's last wait: here for
+  // a first block, before the last earlier block's wait otherwise (real
+  // field; the complex field's registers load it after that wait)
+  constexpr bool DEARLY = sizeof(T) == sizeof(double);
+  bool dpre = false;
+  if (DEARLY && own && jn == 0) {
+    load_diag();
+    dpre = true;
+  }
   if (own) {
     if (wv == 0) {
       for (int q = S.upd_ptr[s] + lane; q < S.upd_ptr[s + 1]; q += 64) {
@@ -893,43 +948,46 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
         load_y(0);
         ypre = true;
       }
-    } else if (own) {
+    } else if (own && !dpre) {
       load_diag();
+      dpre = true;
     }
-    // own unit: b minus the descendants' products, before waiting on the
-    // earlier blocks (the gather lists are long on separators: loads in
-    // batches of four)
+    // own unit: b minus the descendants' products (ascending list order),
+    // before waiting on the earlier blocks: GN terms per round, the first
+    // round's indices prefetched, each later round's loaded with the values
+    // of the round before
     T b0 = F<T>::zero(), b1 = F<T>::zero();
     if (own && lane < nr) {
-      const int g = S.sn_first[s] + Fc.lperm[e];
       if (k0 < nc) b0 = y[e * nrhs + c0 + k0];
       if (k1 < nc) b1 = y[e * nrhs + c0 + k1];
-      int64_t q = S.gat_ptr[g];
-      const int64_t qe = S.gat_ptr[g + 1];
-      for (; q + 4 <= qe; q += 4) {
-        int64_t ix[4];
+      int64_t ix[GN];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) ix[m] = S.gat_idx[q + m] * nrhs + c0;
-        T v0[4], v1[4];
+      for (int m = 0; m < GN; ++m) ix[m] = gix[m];
+      for (int64_t q = gq; q < gqe; q += GN) {
+        T v0[GN], v1[GN];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          v0[m] = k0 < nc ? u[ix[m] + k0] : F<T>::zero();
-          v1[m] = k1 < nc ? u[ix[m] + k1] : F<T>::zero();
+        for (int m = 0; m < GN; ++m) {
+          const bool in = q + m < gqe;
+          v0[m] = (in && k0 < nc) ? u[ix[m] * nrhs + c0 + k0] : F<T>::zero();
+          v1[m] = (in && k1 < nc) ? u[ix[m] * nrhs + c0 + k1] : F<T>::zero();
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          b0 = b0 - v0[m];
-          b1 = b1 - v1[m];
+        for (int m = 0; m < GN; ++m) ix[m] = q + GN + m < gqe ? S.gat_idx[q + GN + m] : 0;
+#pragma unroll
+        for (int m = 0; m < GN; ++m) {
+          if (q + m < gqe) {
+            b0 = b0 - v0[m];
+            b1 = b1 - v1[m];
+          }
         }
-      }
-      for (; q < qe; ++q) {
-        const int64_t ur = S.gat_idx[q] * nrhs + c0;
-        if (k0 < nc) b0 = b0 - u[ur + k0];
-        if (k1 < nc) b1 = b1 - u[ur + k1];
       }
     }
     for (int j = 0; j < jn; ++j) {
       const bool must = j >= ready;
+      if (DEARLY && own && j + 1 == jn && !dpre) {
+        load_diag();
+        dpre = true;
+      }
       if (must && tid == 0) {
         s_ready = wait_ge(counter(sy, s, CF_BLK), j + 1, sy.ctl);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -949,8 +1007,9 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
           load_y(j + 1);
           ypre = true;
         }
-      } else if (own) {
+      } else if (own && !dpre) {
         load_diag();
+        dpre = true;
       }
 #pragma unroll 8
       for (int p = 0; p < TB; ++p) {
@@ -964,6 +1023,7 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
       __syncthreads();  // the products are done with Ls
 #pragma unroll
       for (int q = 0; q < TPER; ++q) Ls[tid + q * SNT] = dv[q];  // Ld[p * TB + i]
+      dpre = false;
       __syncthreads();
 #pragma unroll 8
       for (int p = 0; p + 1 < nr; ++p) {
@@ -989,10 +1049,11 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
 
 template <class T>
 __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, const int2* __restrict__ items,
-                                                 SolveSync sy, int32_t nrhs, T* __restrict__ y) {
+                                                 SolveSync sy, int32_t nrhs, T* __restrict__ y,
+                                                 T* __restrict__ u) {
   // own unit: Ls[i * (TB+1) + p] = L[rb + i][j0 + p], then the diagonal block,
   // and xs[i * RCH + k] = x[rb + i][c0 + k]; off unit: its pass's x rows
-  constexpr int NOWN = TB * (TB + 1) + TB * RCH, NOFF = BW_RB * RCH;
+  constexpr int NOWN = TB * (TB + 1) + TB * RCH, NOFF = BW_RB * RCH + (BW_RB * 4 + sizeof(T) - 1) / sizeof(T);
   __shared__ T sm[NOWN > NOFF ? NOWN : NOFF];
   T* Ls = sm;
   T* xs = sm + TB * (TB + 1);
@@ -1003,9 +1064,10 @@ __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, cons
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
   const int nblk = (w + TB - 1) / TB;
+  const int ng = (w + 15) / 16, nch = bwd_row_chunks(w, h - w);
   if (c < 0) {
-    // off unit: the off-diagonal rows' terms of 16 own columns, once every
-    // ancestor holding those rows is solved
+    // off unit: the terms of 16 own columns from one chunk of the off-
+    // diagonal rows, once every ancestor holding those rows is solved
     if (wv == 0) {
       for (int q = S.anc_ptr[s] + lane; q < S.anc_ptr[s + 1]; q += 64) {
         const int a = S.anc_idx[q];
@@ -1014,7 +1076,11 @@ __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, cons
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    bwd_update_cols(S, Fc, s, (-1 - c) * 16, nrhs, y, sm);
+    const int g = (-1 - c) % ng, r = (-1 - c) / ng;
+    const int64_t ib0 = w + (int64_t)r * BWD_CHUNK;
+    bwd_update_cols(S, Fc, s, g * 16, nrhs, y, sm, reinterpret_cast<int32_t*>(sm + BW_RB * RCH), ib0,
+                    nch > 1 ? min<int64_t>(h, ib0 + BWD_CHUNK) : h,
+                    nch > 1 ? u + (S.u_off[s] + (int64_t)r * w) * nrhs : (T*)nullptr);
     wg_publish(counter(sy, s, CB_OFF), 1, true);
     return;
   }
@@ -1034,10 +1100,19 @@ __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, cons
       dv[q] = in ? P[(int64_t)(j0 + p) + (int64_t)(j0 + i) * h] : F<T>::zero();
     }
   };
+  // the diagonal block's loads go out before the unit's last wait: here for
+  // the last block, before the last later block's wait otherwise (real
+  // field; the complex field's registers load it after that wait)
+  constexpr bool DEARLY = sizeof(T) == sizeof(double);
+  bool dpre = false;
+  if (DEARLY && nm == 0) {
+    load_diag();
+    dpre = true;
+  }
   if (h > w) {
     // the off units of s have applied the off-diagonal rows to every own entry
     if (tid == 0) {
-      (void)wait_ge(counter(sy, s, CB_OFF), (w + 15) / 16, sy.ctl);
+      (void)wait_ge(counter(sy, s, CB_OFF), ng * nch, sy.ctl);
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
@@ -1078,13 +1153,37 @@ __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, cons
         load_x(0);
         xpre = true;
       }
-    } else {
+    } else if (!dpre) {
       load_diag();
+      dpre = true;
+    }
+    // this block's rows of y, final once the off units are done: minus the
+    // row chunks' partial sums (ascending chunks, summed first) when they
+    // were stored apart
+    const int64_t e = (int64_t)eo + j0 + lane;
+    T yb0 = F<T>::zero(), yb1 = F<T>::zero();
+    if (lane < nbk) {
+      if (k0 < nc) yb0 = y[e * nrhs + c0 + k0];
+      if (k1 < nc) yb1 = y[e * nrhs + c0 + k1];
+      if (nch > 1) {
+        T q0 = F<T>::zero(), q1 = F<T>::zero();
+        for (int r = 0; r < nch; ++r) {
+          const int64_t o = (S.u_off[s] + (int64_t)r * w + j0 + lane) * nrhs + c0;
+          if (k0 < nc) q0 = q0 + u[o + k0];
+          if (k1 < nc) q1 = q1 + u[o + k1];
+        }
+        yb0 = yb0 - q0;
+        yb1 = yb1 - q1;
+      }
     }
     for (int m = 0; m < nm; ++m) {
       const int j = nblk - 1 - m;
       const int need = nblk - j;
       const bool must = need > ready;
+      if (DEARLY && m + 1 == nm && !dpre) {
+        load_diag();
+        dpre = true;
+      }
       if (must && tid == 0) {
         s_ready = wait_ge(counter(sy, s, CB_BLK), need, sy.ctl);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1104,8 +1203,9 @@ __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, cons
           load_x(m + 1);
           xpre = true;
         }
-      } else {
+      } else if (!dpre) {
         load_diag();
+        dpre = true;
       }
 #pragma unroll 8
       for (int i = 0; i < TB; ++i) {
@@ -1114,11 +1214,10 @@ __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, cons
         a1 = a1 + l * xs[i * RCH + k1];
       }
     }
-    const int64_t e = (int64_t)eo + j0 + lane;
     T y0 = F<T>::zero(), y1 = F<T>::zero();
     if (lane < nbk) {
-      if (k0 < nc) y0 = y[e * nrhs + c0 + k0] - a0;
-      if (k1 < nc) y1 = y[e * nrhs + c0 + k1] - a1;
+      if (k0 < nc) y0 = yb0 - a0;
+      if (k1 < nc) y1 = yb1 - a1;
     }
     __syncthreads();  // the products are done with Ls
 #pragma unroll
@@ -1126,6 +1225,7 @@ __global__ __launch_bounds__(SNT, 2) void k_bwd_dep(DevSym S, DevFac<T> Fc, cons
       const int t = tid + q * SNT, p = t % TB, i = t / TB;
       Ls[p * (TB + 1) + i] = dv[q];  // Lt[p * (TB+1) + i]
     }
+    dpre = false;
     __syncthreads();
 #pragma unroll 8
     for (int p = nbk - 1; p > 0; --p) {
@@ -1242,10 +1342,10 @@ void launch_fwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int
 }
 template <class T>
 void launch_bwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count, const SolveSync& sy,
-                    int32_t nrhs, T* y, double bytes, hipStream_t st, LaunchProfiler* pf) {
+                    int32_t nrhs, T* y, T* u, double bytes, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
   ProfScope ps(pf, LK_BWD_TRI, 0.0, bytes, st);
-  hipLaunchKernelGGL(k_bwd_dep<T>, dim3((unsigned)count), dim3(SNT), 0, st, S, Fc, items, sy, nrhs, y);
+  hipLaunchKernelGGL(k_bwd_dep<T>, dim3((unsigned)count), dim3(SNT), 0, st, S, Fc, items, sy, nrhs, y, u);
 }
 
 #define RSL_SINST(T)                                                                           \
@@ -1273,7 +1373,7 @@ void launch_bwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int
   template void launch_fwd_dep<T>(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, \
                                   int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);      \
   template void launch_bwd_dep<T>(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, \
-                                  int32_t, T*, double, hipStream_t, LaunchProfiler*);
+                                  int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
 
 RSL_SINST(double)
 RSL_SINST(cplx)

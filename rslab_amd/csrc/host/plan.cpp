@@ -385,10 +385,14 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     }
   }
   for (size_t l = P.levels.size(); l-- > (size_t)P.solve_split;) {
-    // off units (s, -1 - q) for 16-column chunks q, then own blocks, last first
+    // off units (s, -1 - (q + ng r)) for 16-column chunks q of row chunk r,
+    // then own blocks, last first
     for (int32_t s : P.levels[l].all)
-      if (P.sn_nrow[s] > P.sn_ncol[s])
-        for (int32_t q = 0; q * 16 < P.sn_ncol[s]; ++q) P.dbwd_items.push_back({s, -1 - q});
+      if (P.sn_nrow[s] > P.sn_ncol[s]) {
+        const int32_t w = P.sn_ncol[s], ng = (w + 15) / 16, nch = bwd_row_chunks(w, P.sn_nrow[s] - w);
+        for (int32_t r = 0; r < nch; ++r)
+          for (int32_t q = 0; q < ng; ++q) P.dbwd_items.push_back({s, -1 - (q + ng * r)});
+      }
     for (int32_t s : P.levels[l].all)
       for (int32_t c = (P.sn_ncol[s] + 63) / 64; c-- > 0;) P.dbwd_items.push_back({s, c});
   }

@@ -49,7 +49,7 @@ template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const
 template <class T> void launch_bwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
 void launch_sync_init(int32_t*, const int32_t*, int32_t, hipStream_t);
 template <class T> void launch_fwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
-template <class T> void launch_bwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_bwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_expand(int64_t, const int64_t*, const T*, T*, hipStream_t);
 template <class T> void launch_residual(int32_t, int32_t, const int64_t*, const int32_t*, const T*, const T*, const T*, T*, hipStream_t, LaunchProfiler*);
 template <class T> void launch_colmax(int32_t, int32_t, const T*, unsigned long long*, hipStream_t);
@@ -846,7 +846,7 @@ class DeviceFactor final : public DeviceFactorBase {
       launch_fwd_dep<T>(S, fv, sym->dfwd_items.as<int2>(), (int)P.dfwd_items.size(), sy, (int32_t)nrhs, y, u,
                         sweep_bytes, st, pf);
       launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
-      launch_bwd_dep<T>(S, fv, sym->dbwd_items.as<int2>(), (int)P.dbwd_items.size(), sy, (int32_t)nrhs, y,
+      launch_bwd_dep<T>(S, fv, sym->dbwd_items.as<int2>(), (int)P.dbwd_items.size(), sy, (int32_t)nrhs, y, u,
                         sweep_bytes, st, pf);
     } else {
       launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
