@@ -801,20 +801,27 @@ __device__ inline int32_t* counter(const SolveSync& sy, int s, int kind) {
 
 // Poll *p until it reaches target and return the value read. Returns target
 // at once when the solve is aborted: this wait or an earlier one of the same
-// solve ran past kSpinTicks.
+// solve ran past kSpinTicks. A poll is one load; a counter more than one
+// step short of its target is polled about every 0.8 us (hundreds of
+// waiting workgroups polling at full rate slow every poll down), one step
+// short about every 30 ns. The abort word and the clock are read every 32nd
+// poll.
 __device__ inline int32_t wait_ge(int32_t* p, int32_t target, uint32_t* ctl) {
   int32_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (v >= target) return v;
   const uint64_t t0 = (uint64_t)wall_clock64();
-  for (;;) {
-    __builtin_amdgcn_s_sleep(2);
+  for (uint32_t it = 1;; ++it) {
+    if (target - v > 1) __builtin_amdgcn_s_sleep(31);
+    else __builtin_amdgcn_s_sleep(1);
     v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (v >= target) return v;
-    if (__hip_atomic_load(ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return target;
-    if ((uint64_t)wall_clock64() - t0 > kSpinTicks) {
-      __hip_atomic_store(ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(ctl + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return target;
+    if ((it & 31u) == 0) {
+      if (__hip_atomic_load(ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return target;
+      if ((uint64_t)wall_clock64() - t0 > kSpinTicks) {
+        __hip_atomic_store(ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(ctl + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return target;
+      }
     }
   }
 }
