@@ -14,6 +14,9 @@
 // run-to-run identical.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+#include <type_traits>
+
 #include "devtypes.hpp"
 #include "prof.hpp"
 #include "scalar.hpp"
@@ -793,7 +796,7 @@ __global__ void k_dsolve(int32_t n, int32_t nrhs, const T* __restrict__ d,
 constexpr uint64_t kSpinTicks = 200000000ull;  // 2 s of the 100 MHz constant clock
 // counters of supernode s: cnt[s * CSTRIDE + kind], one 128-byte line per supernode
 constexpr int CSTRIDE = 32;
-constexpr int CF_BLK = 0, CF_U = 1, CB_BLK = 2, CB_OFF = 3;
+constexpr int CF_BLK = 0, CF_U = 1, CB_BLK = 2, CB_OFF = 3, CF_H = 4;
 
 __device__ inline int32_t* counter(const SolveSync& sy, int s, int kind) {
   return sy.cnt + (int64_t)s * CSTRIDE + kind;
@@ -860,16 +863,218 @@ __global__ void k_sync_init(int32_t* __restrict__ cnt, const int32_t* __restrict
   else if (i < n + 3) cnt[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Forward chain unit (real field; the dependency-driven solve runs only for
+// up to RCH right-hand sides): one workgroup solves every own 64-row block of
+// supernode s in order, so consecutive blocks pass no counter between
+// workgroups. Block c takes t_c = b_c - sum_{j < c-FKC} L[c, j] y_j from its
+// helper unit (z rows of the block, published in block order through CF_H),
+// subtracts the near terms L[c, j] y_j for j = c-FKC .. c-1 (tiles in
+// registers, each wave holding a quarter of the columns for every right-hand
+// side, y_j from the LDS ring of the last FKC + 1 solved blocks; the four
+// waves' partial sums added in wave order) and solves the unit-lower diagonal
+// block. Block 0 gathers its own b.
+// The next block's tiles and diagonal block are loaded while the current one
+// is solved. No wait may fall between those loads and their use: the load
+// counter retires in order, so waiting on any later load would wait on them
+// too. Hence the 2x2 coupling flags are loaded beside them and applied when
+// the block is used, and the chain acquires the helper counter only when it
+// runs out of published helpers (keeping the highest count seen), loading a
+// block's t one block ahead otherwise.
+// ---------------------------------------------------------------------------
+constexpr int FKC = 3;
+constexpr int FRING = (FKC + 1) * TB * RCH;
+template <class T>
+__device__ void fwd_chain(const DevSym& S, const DevFac<T>& Fc, const SolveSync& sy, int32_t nrhs,
+                          T* __restrict__ y, const T* __restrict__ u, const T* __restrict__ z, int s, T* Ls,
+                          T* ring, T* red, int* s_flag) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int w = S.sn_ncol[s], eo = S.e_off[s];
+  const int64_t h = S.sn_nrow[s];
+  const T* P = Fc.pool + S.panel_off[s];
+  const uint8_t* two = Fc.two + eo;
+  const int nblk = (w + TB - 1) / TB;
+  const int nc = min(RCH, (int)nrhs);
+  const int k0 = wv, k1 = wv + NWV;
+  T lt[FKC][TPER];  // lt[q][m] = L[row0 + lane][(c - FKC + q) * TB + wv + NWV m]
+  T dv[TPER];       // the diagonal block's strict lower part (Ld[p * TB + i] order)
+  uint8_t cpl = 0;  // block c starts inside a 2x2 pivot: L[c TB][c TB - 1] is held in D
+  uint8_t dz = 0;   // this thread's diagonal entry on a 2x2 pivot's coupling (zero it)
+  // the one diagonal entry (i, p) of this thread with i == p + 1: p = wv + NWV qz
+  const int qz = (lane - wv - 1) % NWV == 0 && lane > wv ? (lane - wv - 1) / NWV : -1;
+  auto load_tiles = [&](int c) {
+    const int64_t row0 = (int64_t)c * TB;
+    const int nr = min(TB, w - c * TB);
+#pragma unroll
+    for (int q = 0; q < FKC; ++q) {
+      const int j = c - FKC + q;
+#pragma unroll
+      for (int m = 0; m < TPER; ++m) {
+        const int p = wv + NWV * m;
+        lt[q][m] = (j >= 0 && lane < nr) ? P[(row0 + lane) + (int64_t)(j * TB + p) * h] : F<T>::zero();
+      }
+    }
+    cpl = (c > 0 && lane == 0 && wv == NWV - 1) ? two[c * TB - 1] : 0;
+  };
+  auto load_diag = [&](int c) {
+    const int64_t row0 = (int64_t)c * TB;
+    const int nr = min(TB, w - c * TB);
+#pragma unroll
+    for (int q = 0; q < TPER; ++q) {
+      const int t = tid + q * SNT, i = t % TB, p = t / TB;
+      dv[q] = (p < nr && i < nr && i > p) ? P[(row0 + i) + (row0 + p) * h] : F<T>::zero();
+    }
+    dz = (qz >= 0 && wv + NWV * qz + 1 < nr) ? two[row0 + wv + NWV * qz] : 0;
+  };
+  // block 0: its rows' gather lists (static), then the updaters' chunks
+  constexpr int GN = 4;
+  const int nr0 = min(TB, w);
+  int64_t gq = 0, gqe = 0, ix[GN];
+  if (lane < nr0) {
+    const int g = S.sn_first[s] + Fc.lperm[eo + lane];
+    gq = S.gat_ptr[g];
+    gqe = S.gat_ptr[g + 1];
+#pragma unroll
+    for (int m = 0; m < GN; ++m) ix[m] = gq + m < gqe ? S.gat_idx[gq + m] : 0;
+  }
+  load_diag(0);
+  if (nblk > 1) load_tiles(1);
+  if (wv == 0) {
+    for (int q = S.upd_ptr[s] + lane; q < S.upd_ptr[s + 1]; q += 64) {
+      const int k = S.upd_idx[q];
+      (void)wait_ge(counter(sy, k, CF_U), (S.sn_nrow[k] - S.sn_ncol[k] + TB - 1) / TB, sy.ctl);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  T t0 = F<T>::zero(), t1 = F<T>::zero();
+  if (lane < nr0) {
+    const int64_t e = eo + lane;
+    if (k0 < nc) t0 = y[e * nrhs + k0];
+    if (k1 < nc) t1 = y[e * nrhs + k1];
+    for (int64_t q = gq; q < gqe; q += GN) {
+      T v0[GN], v1[GN];
+#pragma unroll
+      for (int m = 0; m < GN; ++m) {
+        const bool in = q + m < gqe;
+        v0[m] = (in && k0 < nc) ? u[ix[m] * nrhs + k0] : F<T>::zero();
+        v1[m] = (in && k1 < nc) ? u[ix[m] * nrhs + k1] : F<T>::zero();
+      }
+#pragma unroll
+      for (int m = 0; m < GN; ++m) ix[m] = q + GN + m < gqe ? S.gat_idx[q + GN + m] : 0;
+#pragma unroll
+      for (int m = 0; m < GN; ++m) {
+        if (q + m < gqe) {
+          t0 = t0 - v0[m];
+          t1 = t1 - v1[m];
+        }
+      }
+    }
+  }
+  int hseen = 0;  // helpers 1 .. hseen published and acquired
+  for (int c = 0; c < nblk; ++c) {
+    const int nr = min(TB, w - c * TB);
+    const int64_t e = (int64_t)eo + c * TB + lane;
+    const int nrn = c + 1 < nblk ? min(TB, w - (c + 1) * TB) : 0;
+    // the next block's t, when its helper is already known published
+    const bool tpre = c + 1 < nblk && c + 1 <= hseen;
+    T tn0 = F<T>::zero(), tn1 = F<T>::zero();
+    if (tpre && lane < nrn) {
+      if (k0 < nc) tn0 = z[(e + TB) * nrhs + k0];
+      if (k1 < nc) tn1 = z[(e + TB) * nrhs + k1];
+    }
+    // near terms: this wave's columns of the (up to) FKC preceding blocks
+    T part[RCH];
+#pragma unroll
+    for (int k = 0; k < RCH; ++k) part[k] = F<T>::zero();
+    if (c > 0) {
+      if (cpl) lt[FKC - 1][TPER - 1] = F<T>::zero();
+#pragma unroll
+      for (int q = 0; q < FKC; ++q) {
+        const int j = c - FKC + q;
+        if (j < 0) continue;
+        const T* yr = ring + (j % (FKC + 1)) * TB * RCH;
+#pragma unroll
+        for (int m = 0; m < TPER; ++m) {
+          const T* yp = yr + (wv + NWV * m) * RCH;
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) part[k] = part[k] + lt[q][m] * yp[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RCH; ++k) red[(wv * TB + lane) * RCH + k] = part[k];
+#pragma unroll
+    for (int q = 0; q < TPER; ++q) Ls[tid + q * SNT] = (dz && q == qz) ? F<T>::zero() : dv[q];  // Ld[p * TB + i]
+    if (c > 0 && c + 1 < nblk) load_tiles(c + 1);  // block 1's went out before the gather
+    if (c + 1 < nblk) load_diag(c + 1);
+    __syncthreads();
+    T a0 = F<T>::zero(), a1 = F<T>::zero();
+#pragma unroll
+    for (int q = 0; q < NWV; ++q) {
+      a0 = a0 + red[(q * TB + lane) * RCH + k0];
+      a1 = a1 + red[(q * TB + lane) * RCH + k1];
+    }
+    T y0 = t0 - a0, y1 = t1 - a1;
+#pragma unroll 8
+    for (int p = 0; p + 1 < nr; ++p) {
+      const T l = Ls[p * TB + lane];
+      const T q0 = lane_bcast(y0, p), q1 = lane_bcast(y1, p);
+      y0 = y0 - l * q0;
+      y1 = y1 - l * q1;
+    }
+    T* yr = ring + (c % (FKC + 1)) * TB * RCH;
+    const bool in = lane < nr;
+    yr[lane * RCH + k0] = (in && k0 < nc) ? y0 : F<T>::zero();
+    yr[lane * RCH + k1] = (in && k1 < nc) ? y1 : F<T>::zero();
+    if (in) {
+      if (k0 < nc) y[e * nrhs + k0] = y0;
+      if (k1 < nc) y[e * nrhs + k1] = y1;
+    }
+    wg_publish(counter(sy, s, CF_BLK), c + 1, false);
+    if (c + 1 < nblk) {
+      if (!tpre) {
+        // out of published helpers: wait for the next one, keep the count
+        if (tid == 0) {
+          *s_flag = wait_ge(counter(sy, s, CF_H), c + 1, sy.ctl);
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+        hseen = *s_flag;
+        if (lane < nrn) {
+          if (k0 < nc) tn0 = z[(e + TB) * nrhs + k0];
+          if (k1 < nc) tn1 = z[(e + TB) * nrhs + k1];
+        }
+      }
+      t0 = tn0;
+      t1 = tn1;
+    }
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, const int2* __restrict__ items,
                                                  SolveSync sy, int32_t nrhs, T* __restrict__ y,
-                                                 T* __restrict__ u) {
+                                                 T* __restrict__ u, T* __restrict__ z, int chain) {
+  // chain mode (real field): (s, -1) is the chain unit of s, (s, c) for
+  // 1 <= c < nblk its block-c helper (gather and the terms of blocks
+  // < c - FKC into z), (s, c >= nblk) the off-diagonal chunks as otherwise
+  constexpr bool CH = std::is_same<T, double>::value;
   __shared__ T Ls[TB * (TB + 1)];  // Ls[p * (TB+1) + r] = L[row0 + r][pj + p]; then the diagonal block
   __shared__ T vs[TB * RCH];       // vs[p * RCH + k] = y[pj + p][c0 + k]
-  __shared__ int s_t, s_ready;
+  __shared__ T ring[CH ? FRING : 1];
+  __shared__ T red[CH ? NWV * TB * RCH : 1];
+  __shared__ int s_t, s_ready, s_flag;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int2 it = items[take_ticket(sy.ctl, &s_t)];
   const int s = it.x, c = it.y;
+  if constexpr (CH) {
+    if (c < 0) {
+      fwd_chain<T>(S, Fc, sy, nrhs, y, u, z, s, Ls, ring, red, &s_flag);
+      return;
+    }
+  }
+  const bool helper = CH && chain != 0;
   const int w = S.sn_ncol[s], eo = S.e_off[s];
   const int64_t h = S.sn_nrow[s];
   const T* P = Fc.pool + S.panel_off[s];
@@ -878,7 +1083,7 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
   const bool own = c < nblk;
   const int64_t row0 = own ? (int64_t)c * TB : w + (int64_t)(c - nblk) * TB;
   const int nr = (int)min<int64_t>(TB, (own ? (int64_t)w : h) - row0);
-  const int jn = own ? c : nblk;  // own blocks feeding this unit
+  const int jn = own ? (helper ? max(0, c - FKC) : c) : nblk;  // own blocks feeding this unit
   const int k0 = wv, k1 = wv + NWV;
   // own block: the strict lower part of its diagonal block (stage_block_lower's
   // entries), loaded while the last product runs
@@ -907,8 +1112,8 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
   // a first block, before the last earlier block's wait otherwise (real
   // field; the complex field's registers load it after that wait)
   constexpr bool DEARLY = sizeof(T) == sizeof(double);
-  bool dpre = false;
-  if (DEARLY && own && jn == 0) {
+  bool dpre = helper;  // a helper solves no diagonal block
+  if (DEARLY && own && jn == 0 && !dpre) {
     load_diag();
     dpre = true;
   }
@@ -1025,7 +1230,12 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
         a1 = a1 + l * vs[p * RCH + k1];
       }
     }
-    if (own) {
+    if (own && helper) {
+      if (lane < nr) {
+        if (k0 < nc) z[e * nrhs + c0 + k0] = b0 - a0;
+        if (k1 < nc) z[e * nrhs + c0 + k1] = b1 - a1;
+      }
+    } else if (own) {
       T y0 = b0 - a0, y1 = b1 - a1;
       __syncthreads();  // the products are done with Ls
 #pragma unroll
@@ -1050,8 +1260,15 @@ __global__ __launch_bounds__(SNT, 2) void k_fwd_dep(DevSym S, DevFac<T> Fc, cons
     }
     __syncthreads();  // before the next right-hand-side chunk reuses Ls / vs
   }
-  if (own) wg_publish(counter(sy, s, CF_BLK), c + 1, false);
-  else wg_publish(counter(sy, s, CF_U), 1, true);
+  if (own && helper) {
+    // helpers publish in block order (the chain waits for CF_H >= c)
+    if (tid == 0 && c > 1) (void)wait_ge(counter(sy, s, CF_H), c - 1, sy.ctl);
+    wg_publish(counter(sy, s, CF_H), c, false);
+  } else if (own) {
+    wg_publish(counter(sy, s, CF_BLK), c + 1, false);
+  } else {
+    wg_publish(counter(sy, s, CF_U), 1, true);
+  }
 }
 
 template <class T>
@@ -1342,10 +1559,12 @@ void launch_sync_init(int32_t* cnt, const int32_t* fu_init, int32_t ns, hipStrea
 }
 template <class T>
 void launch_fwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count, const SolveSync& sy,
-                    int32_t nrhs, T* y, T* u, double bytes, hipStream_t st, LaunchProfiler* pf) {
+                    int32_t nrhs, T* y, T* u, T* z, int chain, double bytes, hipStream_t st, LaunchProfiler* pf) {
   if (count <= 0) return;
+  if (chain && !std::is_same<T, double>::value) throw std::runtime_error("solve: chain units are real-field only");
+  if (chain && nrhs > RCH) throw std::runtime_error("solve: chain units take at most 8 right-hand sides");
   ProfScope ps(pf, LK_FWD_TRI, 0.0, bytes, st);
-  hipLaunchKernelGGL(k_fwd_dep<T>, dim3((unsigned)count), dim3(SNT), 0, st, S, Fc, items, sy, nrhs, y, u);
+  hipLaunchKernelGGL(k_fwd_dep<T>, dim3((unsigned)count), dim3(SNT), 0, st, S, Fc, items, sy, nrhs, y, u, z, chain);
 }
 template <class T>
 void launch_bwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int count, const SolveSync& sy,
@@ -1378,7 +1597,7 @@ void launch_bwd_dep(const DevSym& S, const DevFac<T>& Fc, const int2* items, int
   template void launch_bwd_tri<T>(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, \
                                   T*, double, hipStream_t, LaunchProfiler*);                   \
   template void launch_fwd_dep<T>(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, \
-                                  int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);      \
+                                  int32_t, T*, T*, T*, int, double, hipStream_t, LaunchProfiler*);      \
   template void launch_bwd_dep<T>(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, \
                                   int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
 

@@ -345,6 +345,7 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
   // dependency-driven solve: every unit waits only on units taken before it
   // (updaters and ancestors sit in earlier / later levels, own blocks in order)
   P.dfwd_items.clear();
+  P.dfwd_chain.clear();
   P.dbwd_items.clear();
   // Split level: below it the solve runs level launches (many small
   // supernodes), from it up one dependency-driven launch per sweep. Automatic:
@@ -376,12 +377,19 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
     for (int32_t s : L.all) {
       const int32_t w = P.sn_ncol[s], off = P.sn_nrow[s] - w;
       for (int32_t c = 0; c < (w + 63) / 64; ++c) P.dfwd_items.push_back({s, c});
+      // a chain's helpers follow it: only the newest chain can wait on a
+      // helper not yet started, and every unit before it finishes without it
+      P.dfwd_chain.push_back({s, -1});
+      for (int32_t c = 1; c < (w + 63) / 64; ++c) P.dfwd_chain.push_back({s, c});
       P.dep_tri += 0.5 * (double)w * (w + 1);
       P.dep_off += (double)off * w;
     }
     for (int32_t s : L.all) {
       const int32_t w = P.sn_ncol[s], nblk = (w + 63) / 64, nuch = (P.sn_nrow[s] - w + 63) / 64;
-      for (int32_t c = nblk; c < nblk + nuch; ++c) P.dfwd_items.push_back({s, c});
+      for (int32_t c = nblk; c < nblk + nuch; ++c) {
+        P.dfwd_items.push_back({s, c});
+        P.dfwd_chain.push_back({s, c});
+      }
     }
   }
   for (size_t l = P.levels.size(); l-- > (size_t)P.solve_split;) {
@@ -428,6 +436,13 @@ void build_plan(Plan& P, const int64_t* colp, const int32_t* rowi, const PlanPar
       if (f_first[it.x] < 0) f_first[it.x] = i;
       fu_last[it.x] = i;
     }
+    // the chain list holds the same units per supernode in the same places
+    // (a chain and its helpers where the own blocks were)
+    if (P.dfwd_chain.size() != P.dfwd_items.size())
+      throw std::runtime_error("plan: chain and block solve lists differ in length");
+    for (size_t i = 0; i < P.dfwd_items.size(); ++i)
+      if (P.dfwd_chain[i].x != P.dfwd_items[i].x)
+        throw std::runtime_error("plan: chain and block solve lists order supernodes differently");
     for (int64_t i = 0; i < (int64_t)P.dbwd_items.size(); ++i) {
       const Int2 it = P.dbwd_items[i];
       if (b_first[it.x] < 0) b_first[it.x] = i;

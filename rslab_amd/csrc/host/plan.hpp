@@ -75,6 +75,8 @@ struct Plan {
   // dependency-driven solve: work units in the order they are taken
   std::vector<Int2> dfwd_items;  // (s, c): own 64-row blocks c < nblk of a level's supernodes,
                                  // then their off-diagonal 64-row chunks (c >= nblk); levels ascending
+  std::vector<Int2> dfwd_chain;  // real field: (s, -1) chain unit, (s, c) helpers 1 <= c < nblk,
+                                 // per supernode; then the off-diagonal chunks; levels ascending
   std::vector<Int2> dbwd_items;  // (s, -1 - (q + ng r)): off-diagonal rows of chunk r
                                  // (bwd_row_chunks) for own columns 16q..16q+15,
                                  // then (s, c): own 64-column blocks, last first; levels descending

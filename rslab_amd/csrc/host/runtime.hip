@@ -48,7 +48,7 @@ template <class T> void launch_dsolve(int32_t, int32_t, const T*, const T*, cons
 template <class T> void launch_bwd_update(const DevSym&, const DevFac<T>&, const int2*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_tri(const DevSym&, const DevFac<T>&, const int32_t*, int, int32_t, T*, double, hipStream_t, LaunchProfiler*);
 void launch_sync_init(int32_t*, const int32_t*, int32_t, hipStream_t);
-template <class T> void launch_fwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
+template <class T> void launch_fwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, T*, T*, int, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_bwd_dep(const DevSym&, const DevFac<T>&, const int2*, int, const SolveSync&, int32_t, T*, T*, double, hipStream_t, LaunchProfiler*);
 template <class T> void launch_expand(int64_t, const int64_t*, const T*, T*, hipStream_t);
 template <class T> void launch_residual(int32_t, int32_t, const int64_t*, const int32_t*, const T*, const T*, const T*, T*, hipStream_t, LaunchProfiler*);
@@ -209,6 +209,7 @@ void DeviceSymbolic::upload() {
   zitems.upload(P.zitems, stream);
   nb_eff.upload(P.nb_eff, stream);
   dfwd_items.upload(P.dfwd_items, stream);
+  dfwd_chain.upload(P.dfwd_chain, stream);
   dbwd_items.upload(P.dbwd_items, stream);
   fu_init.upload(P.fu_init, stream);
   anc_ptr.upload(P.anc_ptr, stream);
@@ -843,8 +844,11 @@ class DeviceFactor final : public DeviceFactorBase {
     if (split < P.levels.size()) {
       const double sweep_bytes = ts * (P.dep_tri + P.dep_off + 2.0 * P.n * nrhs);
       launch_sync_init(solve_sync.as<int32_t>(), sym->fu_init.as<int32_t>(), P.ns, st);
-      launch_fwd_dep<T>(S, fv, sym->dfwd_items.as<int2>(), (int)P.dfwd_items.size(), sy, (int32_t)nrhs, y, u,
-                        sweep_bytes, st, pf);
+      // real field: each supernode's own blocks as one chain unit and its
+      // helpers (z is free while the sweeps run)
+      constexpr int chain = FieldOf<T>::id == 0 ? 1 : 0;
+      launch_fwd_dep<T>(S, fv, chain ? sym->dfwd_chain.as<int2>() : sym->dfwd_items.as<int2>(),
+                        (int)P.dfwd_items.size(), sy, (int32_t)nrhs, y, u, z, chain, sweep_bytes, st, pf);
       launch_dsolve<T>(P.n, (int32_t)nrhs, fv.d, fv.ds, fv.two, y, st, pf);
       launch_bwd_dep<T>(S, fv, sym->dbwd_items.as<int2>(), (int)P.dbwd_items.size(), sy, (int32_t)nrhs, y, u,
                         sweep_bytes, st, pf);

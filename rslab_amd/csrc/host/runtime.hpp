@@ -72,7 +72,7 @@ struct DeviceSymbolic {
   DBuf sn_first, sn_ncol, sn_nrow, e_off, panel_off, rs_off, rs_idx, upd_ptr, upd_idx, sym_perm,
       amap, a_row, a_col, items, tasks, level_lists, u_off, gat_ptr, gat_idx,
       witems, nb_eff, gat_items, sitems,
-      fwd_items, bwd_items, ptiles, bitems, zitems, g_off, dfwd_items, dbwd_items, anc_ptr, anc_idx, fu_init;
+      fwd_items, bwd_items, ptiles, bitems, zitems, g_off, dfwd_items, dfwd_chain, dbwd_items, anc_ptr, anc_idx, fu_init;
   std::vector<int64_t> level_all_off, level_small_off, level_wide_off, level_snarrow_off;  // into level_lists
   // user lower CSC pattern (host) and, built on first refinement, the full
   // symmetric CSR of A with each entry's source index in the CSC values
