@@ -882,7 +882,7 @@ __global__ void k_sync_init(int32_t* __restrict__ cnt, const int32_t* __restrict
 // runs out of published helpers (keeping the highest count seen), loading a
 // block's t one block ahead otherwise.
 // ---------------------------------------------------------------------------
-constexpr int FKC = 3;
+constexpr int FKC = 2;
 constexpr int FRING = (FKC + 1) * TB * RCH;
 template <class T>
 __device__ void fwd_chain(const DevSym& S, const DevFac<T>& Fc, const SolveSync& sy, int32_t nrhs,
