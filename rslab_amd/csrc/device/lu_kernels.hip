@@ -681,21 +681,37 @@ __global__ __launch_bounds__(LU_NT) void k_lu_boff(LuSymDev S, LuFacDev<T> Fc, c
 #pragma unroll
     for (int c = 0; c < CW; ++c) acc[c] = F<T>::zero();
     if (p < w) {
-      // four columns of the slice per pass, their loads issued together
+      // QB columns of the slice per pass, their loads issued together; the
+      // next pass's row indices are loaded with them, so a pass waits on one
+      // round of loads (ascending columns: the sums keep their order)
+      constexpr int QB = 8;
       int64_t t = sl;
-      for (; t + 48 < nt; t += 64) {
-        T u[4];
-        const T* xr[4];
+      int32_t rn[QB];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < QB; ++q) rn[q] = t + 16 * q < nt ? rs[t + 16 * q] : 0;
+      for (; t + 16 * (QB - 1) < nt; t += 16 * QB) {
+        T u[QB];
+        const T* xr[QB];
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
           u[q] = U[p + (t + 16 * q) * w];
-          xr[q] = x + (int64_t)rs[t + 16 * q] * nrhs + c0;
+          xr[q] = x + (int64_t)rn[q] * nrhs + c0;
+        }
+        T xv[QB][CW];
+#pragma unroll
+        for (int q = 0; q < QB; ++q)
+#pragma unroll
+          for (int c = 0; c < CW; ++c) xv[q][c] = c < cw ? xr[q][c] : F<T>::zero();
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+          const int64_t tn = t + 16 * QB + 16 * q;
+          rn[q] = tn < nt ? rs[tn] : 0;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < QB; ++q)
 #pragma unroll
           for (int c = 0; c < CW; ++c)
-            if (c < cw) acc[c] = lu_fma(u[q], xr[q][c], acc[c]);
+            if (c < cw) acc[c] = lu_fma(u[q], xv[q][c], acc[c]);
       }
       for (; t < nt; t += 16) {
         const T u1 = U[p + t * w];
