@@ -758,9 +758,20 @@ __global__ __launch_bounds__(LU_NT) void k_lu_fstep(LuSymDev S, LuFacDev<T> Fc, 
   const int64_t h = S.sn_nrow[s];
   const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
   const T* L = Fc.l + S.l_off[s];
-  for (int e = threadIdx.x; e < BS * BS; e += LU_NT) {
-    const int r = e % BS, q = e / BS;
-    Lb[r][q] = (r < m && q < r) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
+  {
+    // the block's loads issued together, then their LDS stores
+    constexpr int PER = BS * BS / LU_NT;
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * LU_NT, r = e % BS, q = e / BS;
+      v[u] = (r < m && q < r) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * LU_NT;
+      Lb[e % BS][e / BS] = v[u];
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -828,9 +839,20 @@ __global__ __launch_bounds__(LU_NT) void k_lu_bstep(LuSymDev S, LuFacDev<T> Fc, 
   const int64_t h = S.sn_nrow[s];
   const int w = S.sn_ncol[s], first = S.sn_first[s], m = min(BS, w - j0);
   const T* L = Fc.l + S.l_off[s];
-  for (int e = threadIdx.x; e < BS * BS; e += LU_NT) {
-    const int r = e % BS, q = e / BS;
-    Ub[r][q] = (q < m && r <= q) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
+  {
+    // the block's loads issued together, then their LDS stores
+    constexpr int PER = BS * BS / LU_NT;
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * LU_NT, r = e % BS, q = e / BS;
+      v[u] = (q < m && r <= q) ? L[(int64_t)(j0 + q) * h + j0 + r] : F<T>::zero();
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * LU_NT;
+      Ub[e % BS][e / BS] = v[u];
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
