@@ -360,20 +360,26 @@ __global__ __launch_bounds__(NT) void k_cmod_mfma(DevSym S, DevFac<double> Fc, c
   double* P = Fc.pool + S.panel_off[s];
   const int32_t* rs = S.rs_idx + S.rs_off[s] + it.r0;
   const int mr = min<int64_t>(TM, h - it.r0), mc = min(TN, w - it.c0);
+  // the targets read as one batch, then updated (a store followed by a
+  // load of another column would wait for it: the columns' distance is a
+  // runtime stride)
+  double old[16];
+  int n = 0;
+  auto get = [&](int i, int j, double) {
+    old[n++] = (i < mr && j < mc && rs[i] >= gcol0 + j) ? P[(it.r0 + i) + (int64_t)(it.c0 + j) * h] : 0.0;
+  };
+  auto put = [&](int i, int j, double v) {
+    if (i < mr && j < mc && rs[i] >= gcol0 + j) P[(it.r0 + i) + (int64_t)(it.c0 + j) * h] = old[n] - v;
+    ++n;
+  };
   if constexpr (PR != TP_F64) {
-    tile_for_each_d<PR>(acc64, [&](int i, int j, double v) {
-      if (i < mr && j < mc && rs[i] >= gcol0 + j) {
-        double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * h;
-        *dst = *dst - v;
-      }
-    });
+    tile_for_each_d<PR>(acc64, get);
+    n = 0;
+    tile_for_each_d<PR>(acc64, put);
   } else {
-    tile_for_each<PR>(acc, [&](int i, int j, double v) {
-      if (i < mr && j < mc && rs[i] >= gcol0 + j) {
-        double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * h;
-        *dst = *dst - v;
-      }
-    });
+    tile_for_each<PR>(acc, get);
+    n = 0;
+    tile_for_each<PR>(acc, put);
   }
 }
 
@@ -601,12 +607,19 @@ __global__ __launch_bounds__(NT) void k_cmod_sum(DevSym S, DevFac<double> Fc, co
   double* P = Fc.pool + S.panel_off[s];
   const int32_t* rs = S.rs_idx + S.rs_off[s] + it.r0;
   const int mr = min<int64_t>(TM, h - it.r0), mc = min(TN, w - it.c0);
-  for (int e = tid; e < TM * TN; e += NT) {
-    const int i = e & 63, jj = e >> 6;
-    if (i < mr && jj < mc && rs[i] >= gcol0 + jj) {
-      double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + jj) * h;
-      *dst = *dst - tile[jj * (TM + 1) + i];
-    }
+  // the targets read as one batch, then updated
+  constexpr int PER = TM * TN / NT;
+  double old[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = tid + u * NT, i = e & 63, jj = e >> 6;
+    old[u] = (i < mr && jj < mc && rs[i] >= gcol0 + jj) ? P[(it.r0 + i) + (int64_t)(it.c0 + jj) * h] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = tid + u * NT, i = e & 63, jj = e >> 6;
+    if (i < mr && jj < mc && rs[i] >= gcol0 + jj)
+      P[(it.r0 + i) + (int64_t)(it.c0 + jj) * h] = old[u] - tile[jj * (TM + 1) + i];
   }
 }
 
@@ -1258,11 +1271,16 @@ __device__ void panel_schur_tile(const SnCtx<T>& c, int kb, int ke, int rr0, int
     tile_zero(acc);
     ldl_tile_mfma<PR>(c.P, c.h, kb, pw, ke + rr0, ma, ke + cc0, mb, c.d + kb, c.ds + kb, c.two + kb, acc,
                      reinterpret_cast<Sx*>(stage), reinterpret_cast<Sx*>(stage + STG));
+    // the targets read as one batch, then updated
+    double old[16];
+    int n = 0;
+    tile_for_each<PR>(acc, [&](int i, int j, double) {
+      old[n++] = (i < ma && j < mb && rr0 + i >= cc0 + j) ? c.P[(ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h] : 0.0;
+    });
+    n = 0;
     tile_for_each<PR>(acc, [&](int i, int j, double v) {
-      if (i < ma && j < mb && rr0 + i >= cc0 + j) {
-        double* dst = c.P + (ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h;
-        *dst = *dst - v;
-      }
+      if (i < ma && j < mb && rr0 + i >= cc0 + j) c.P[(ke + rr0 + i) + (int64_t)(ke + cc0 + j) * c.h] = old[n] - v;
+      ++n;
     });
   } else {
     // complex symmetric: four real FP64 matrix-core products per complex one

@@ -243,11 +243,16 @@ __global__ __launch_bounds__(LU_NT) void k_lu_cmod_mfma(LuSymDev S, LuFacDev<dou
     mr = w - it.r0;
     mc = h - w - it.c0;
   }
+  // the targets read as one batch, then updated
+  double old[16];
+  int n = 0;
+  tile_for_each<TP_F64>(acc, [&](int i, int j, double) {
+    old[n++] = (i < mr && j < mc) ? P[(it.r0 + i) + (int64_t)(it.c0 + j) * ld] : 0.0;
+  });
+  n = 0;
   tile_for_each<TP_F64>(acc, [&](int i, int j, double v) {
-    if (i < mr && j < mc) {
-      double* dst = P + (it.r0 + i) + (int64_t)(it.c0 + j) * ld;
-      *dst = *dst - v;
-    }
+    if (i < mr && j < mc) P[(it.r0 + i) + (int64_t)(it.c0 + j) * ld] = old[n] - v;
+    ++n;
   });
 }
 
@@ -604,15 +609,21 @@ __global__ __launch_bounds__(LU_NT) void k_lu_gemm(LuSymDev S, LuFacDev<T> Fc, c
     __syncthreads();
   }
   const int ty = tid >> 4, tx = tid & 15;
+  // the targets read as one batch, then updated
+  T old[4][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int i = ty * 4 + r, j = tx * 4 + q;
-      if (i < mr && j < mc) {
-        T* dst = P + (r0 + i) + (c0 + j) * ld;
-        *dst = *dst - acc.c[r][q];
-      }
+      old[r][q] = (i < mr && j < mc) ? P[(r0 + i) + (c0 + j) * ld] : F<T>::zero();
+    }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = ty * 4 + r, j = tx * 4 + q;
+      if (i < mr && j < mc) P[(r0 + i) + (c0 + j) * ld] = old[r][q] - acc.c[r][q];
     }
 }
 
